@@ -1,0 +1,243 @@
+#!/usr/bin/env python3
+"""Benchmark: Rayleigh–Sommerfeld far field, forward + adjoint, on one MI355X.
+
+Workload (the headline shape): 303 x 303 = 91,809 pillars at 0.35 um, lambda 0.633,
+f = 100 um hyperbolic phase with +-0.05 rad noise and |t| ~ U[0.7, 1.0], a 161 x 161
+focal grid over +-4 lambda/NA at z = f.  One step = the forward sum (complex field)
+plus the adjoint sum (gradient of -encircled-energy with respect to every pillar's
+complex transmission), FP32-compensated, inputs resident in HBM.
+
+Prints one JSON line.  Throughput = pillar-pixel pairs / (forward + adjoint time),
+in G pairs/s.  Roofline: 21 + 21 = 42 FLOP per pair (dx, dy, R^2, sqrt, 1/R, k R,
+sin, cos, complex multiply, 1/R scale, accumulate; same count transposed) against
+the MI355X FP32 vector peak of 157.3 TFLOP/s, using the two pair-sum kernels'
+durations measured with HIP events on their stream.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FLOP_PER_PAIR = 42.0
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--grid", type=int, default=161)
+    ap.add_argument("--mode", default="fp32c", choices=["fp32c", "fp64", "fp32"])
+    ap.add_argument("--no-compare", action="store_true", help="skip FP64/FP32 GPU comparison legs")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="approximate CPU time budget of the cpu_baseline sample")
+    return ap.parse_args()
+
+
+def cpu_model() -> str:
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, text=True).stdout
+        for ln in out.splitlines():
+            if ln.startswith("Model name"):
+                return ln.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline(lens, u_np, budget_s: float) -> dict:
+    """The oracle (FP64 restatement of focal.cpp:205-238 + its transposed adjoint) on
+    the host cores: rows striped over threads for the forward, pillars for the
+    adjoint, each on a bounded deterministic sample scaled by its pair count."""
+    import oracle
+
+    threads = oracle.num_cpus()
+    planes = lens.planes
+    n = planes[0].nx
+    # Calibrate on one row / a sparse pillar sample, then size the sample to the budget.
+    t0 = time.perf_counter()
+    p_cal = oracle.forward_pairs(lens.px, lens.py, lens.t, planes, threads, row_stride=n)
+    t_cal = time.perf_counter() - t0
+    # One row runs on one thread; the per-pair rate with all threads is ~threads x.
+    per_pair_1t = t_cal / max(p_cal, 1)
+    total_pairs = lens.npil * n * n
+    est_full = per_pair_1t * total_pairs / threads
+    frac = min(1.0, 0.5 * budget_s / max(est_full, 1e-9))
+    row_stride = max(1, int(round(1.0 / frac)))
+    t0 = time.perf_counter()
+    p_fwd = oracle.forward_pairs(lens.px, lens.py, lens.t, planes, threads, row_stride=row_stride)
+    t_fwd = time.perf_counter() - t0
+    pil_stride = row_stride
+    t0 = time.perf_counter()
+    _, p_adj = oracle.adjoint(lens.px, lens.py, u_np, planes, 1, threads, pillar_stride=pil_stride,
+                              return_pairs=True)
+    t_adj = time.perf_counter() - t0
+    tau = t_fwd / p_fwd + t_adj / p_adj  # seconds per pair, forward + adjoint
+    return {
+        "value": 1.0 / tau / 1e9,
+        "unit": "G pairs/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle FP64 forward on every {row_stride}th focal row ({p_fwd:.3e} pairs, "
+                   f"{t_fwd:.2f} s) + adjoint on every {pil_stride}th pillar ({p_adj:.3e} pairs, "
+                   f"{t_adj:.2f} s), {threads} threads, scaled per pair"),
+        "cpu_model": cpu_model(),
+        "fwd_s_per_pair": t_fwd / p_fwd,
+        "adj_s_per_pair": t_adj / p_adj,
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the forward shards over pixels and the adjoint over pillars with no exchange "
+              "beyond one gradient all-reduce, but no multi-GPU figure is measured here.",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from celeris_amd import Context, configs, kernel_name
+    from celeris_amd.autograd import encircled_energy
+
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    dev = torch.device("cuda:0")
+    lens = configs.config2(n=args.grid)
+    plane = lens.planes[0]
+    npix = plane.npix
+    pairs = lens.npil * npix
+    px = torch.from_numpy(lens.px).to(dev)
+    py = torch.from_numpy(lens.py).to(dev)
+    t = torch.from_numpy(lens.t).to(dev)
+    ctx = Context(0)
+    r0 = 1.22 * lens.wavelengths[0] / lens.na
+
+    # Cotangent of -EE at the initial design: u = dL/dRe E + i dL/dIm E = 2 w E.
+    E0, _ = ctx.forward(px, py, t, [plane], mode="fp64")
+    Ev = E0.clone().requires_grad_(True)
+    inten = Ev.real ** 2 + Ev.imag ** 2
+    loss = -encircled_energy(inten, plane, r0)
+    loss.backward()
+    u = Ev.grad.detach().contiguous()
+
+    def step(mode):
+        f, _ = ctx.forward(px, py, t, [plane], mode=mode, field=True)
+        g = ctx.adjoint(px, py, u, [plane], nsets=1, mode=mode)
+        return f, g
+
+    def time_steps(mode, steps, warmup):
+        for _ in range(warmup):
+            step(mode)
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(steps):
+            step(mode)
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / steps
+
+    def kernel_ms(mode, reps):
+        ctx.set_timing(True)
+        fw, ad = [], []
+        for _ in range(reps):
+            ctx.forward(px, py, t, [plane], mode=mode, field=True)
+            fw.append(ctx.last_kernel_ms(False))
+            ctx.adjoint(px, py, u, [plane], nsets=1, mode=mode)
+            ad.append(ctx.last_kernel_ms(True))
+        ctx.set_timing(False)
+        return float(np.mean(fw)), float(np.mean(ad))
+
+    ms = time_steps(args.mode, args.steps, args.warmup)
+    k_fwd, k_adj = kernel_ms(args.mode, max(3, min(args.steps, 10)))
+    value = pairs / (ms * 1e-3) / 1e9
+    achieved = pairs * FLOP_PER_PAIR / ((k_fwd + k_adj) * 1e-3) / 1e12
+    print(f"[bench] {args.mode}: {ms:.3f} ms/step, kernels fwd {k_fwd:.3f} ms adj {k_adj:.3f} ms, "
+          f"{value:.1f} G pairs/s", file=sys.stderr, flush=True)
+
+    extra = {}
+    f32c, g32c = step(args.mode)
+    torch.cuda.synchronize()
+    if not args.no_compare:
+        # Accuracy against the FP64 GPU kernel (itself pinned to the oracle by the tests),
+        # and the plain FP64 / FP32 GPU kernels' throughput on the same step.
+        f64, g64 = step("fp64")
+        torch.cuda.synchronize()
+        rel = lambda a, b: float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+        extra["rel_l2_field_vs_fp64"] = rel(f32c, f64)
+        extra["rel_l2_intensity_vs_fp64"] = rel(f32c.abs() ** 2, f64.abs() ** 2)
+        extra["max_abs_field_vs_fp64_over_peak"] = float((f32c - f64).abs().max() / f64.abs().max())
+        extra["rel_l2_grad_vs_fp64"] = rel(g32c, g64)
+        ms64 = time_steps("fp64", 2, 1)
+        ms32 = time_steps("fp32", 3, 1)
+        extra["fp64_plain_G_pairs_s"] = pairs / (ms64 * 1e-3) / 1e9
+        extra["fp32_plain_G_pairs_s"] = pairs / (ms32 * 1e-3) / 1e9
+        extra["speedup_vs_fp64_plain"] = ms64 / ms
+        print(f"[bench] compare: relL2 field {extra['rel_l2_field_vs_fp64']:.3e} grad "
+              f"{extra['rel_l2_grad_vs_fp64']:.3e}; fp64 {ms64:.2f} ms, fp32 {ms32:.2f} ms",
+              file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(lens, u.cpu().numpy(), args.cpu_seconds)
+        print(f"[bench] cpu baseline {cpu['value']:.3f} G pairs/s on {cpu['cores']} threads",
+              file=sys.stderr, flush=True)
+
+    out = {
+        "metric": "G pillar·pixel pairs/s fwd+adjoint, FP32-comp vs FP64 at ≤1e-5 rel L2",
+        "value": value,
+        "unit": "G pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.mode != "fp64" else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"RS far field fwd+adjoint, {lens.npil} pillars (303x303 @0.35um, "
+                         f"lambda 0.633, f 100) x {plane.nx}x{plane.ny} focal grid, mode {args.mode}"),
+            "pillars": lens.npil,
+            "grid": plane.nx,
+            "pairs_per_step": pairs,
+            "mode": args.mode,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": None,
+            "kernels": {kernel_name(args.mode): k_fwd, kernel_name(args.mode, True): k_adj},
+            "flop_per_pair": FLOP_PER_PAIR,
+        },
+        "cpu_baseline": cpu,
+        **extra,
+    }
+    print(json.dumps(out))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

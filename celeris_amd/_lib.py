@@ -1,0 +1,108 @@
+"""ctypes binding of libceleris_rs.so (the C ABI declared in include/celeris_rs.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (or
+``python -m celeris_amd.build``).  Loading fails loudly when it is missing: there
+is no CPU fallback anywhere in this package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_int, c_int32, c_int64, c_void_p
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libceleris_rs.so")
+
+MODE_FP32C = 0
+MODE_FP64 = 1
+MODE_FP32 = 2
+MODES = {"fp32c": MODE_FP32C, "fp64": MODE_FP64, "fp32": MODE_FP32}
+
+STATUS = {
+    0: "CRS_OK",
+    1: "CRS_ERR_NO_DEVICE",
+    2: "CRS_ERR_INVALID",
+    3: "CRS_ERR_ALLOC",
+    4: "CRS_ERR_LAUNCH",
+    5: "CRS_ERR_CAPTURE",
+}
+
+
+class crs_plane(ctypes.Structure):
+    """Mirror of ``crs_plane`` (include/celeris_rs.h)."""
+
+    _fields_ = [
+        ("k", c_double),
+        ("x0", c_double),
+        ("sx", c_double),
+        ("y0", c_double),
+        ("sy", c_double),
+        ("z0", c_double),
+        ("sz", c_double),
+        ("nx", c_int32),
+        ("ny", c_int32),
+        ("tset", c_int32),
+        ("reserved", c_int32),
+        ("offset", c_int64),
+    ]
+
+
+assert ctypes.sizeof(crs_plane) == 7 * 8 + 4 * 4 + 8
+
+_DP = POINTER(c_double)
+
+# (name, restype, argtypes) for every entry point of include/celeris_rs.h.
+SIGNATURES = [
+    ("crs_available", c_int, []),
+    ("crs_device_name", c_char_p, []),
+    ("crs_propagate_psf", c_int,
+     [_DP, _DP, _DP, c_int, c_double, c_double, c_double, c_double, c_int, c_double, _DP]),
+    ("crs_propagate_zx", c_int,
+     [_DP, _DP, _DP, c_int, c_double, c_double, c_int, c_double, c_double, c_int, c_double, _DP]),
+    ("crs_set_default_mode", c_int, [c_int]),
+    ("crs_context_create", c_int, [c_int, POINTER(c_void_p)]),
+    ("crs_context_destroy", None, [c_void_p]),
+    ("crs_last_error", c_char_p, []),
+    ("crs_forward", c_int,
+     [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_int,
+      POINTER(crs_plane), c_int, c_void_p, c_void_p]),
+    ("crs_adjoint", c_int,
+     [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_int,
+      POINTER(crs_plane), c_int, c_void_p, c_void_p]),
+    ("crs_set_tuning", c_int, [c_void_p, c_int, c_int, c_int]),
+    ("crs_set_timing", c_int, [c_void_p, c_int]),
+    ("crs_last_kernel_ms", c_int, [c_void_p, c_int, POINTER(ctypes.c_float)]),
+    ("crs_kernel_name", c_char_p, [c_int, c_int]),
+]
+
+_lib = None
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def load() -> ctypes.CDLL:
+    """Load the HIP library (once).  Raises LibraryMissing if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LibraryMissing(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+        )
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    return load().crs_last_error().decode()
+
+
+def check(status: int, what: str) -> None:
+    if status != 0:
+        raise RuntimeError(f"{what} failed: {STATUS.get(status, status)}: {last_error()}")

@@ -1,0 +1,78 @@
+"""Build recipes: the HIP library (product) and the oracle (test infrastructure).
+
+    python -m celeris_amd.build            # library + oracle (+ reference build if present)
+
+The library is compiled for gfx950 only.  Packed-FP32 instructions are disabled
+(``-packed-fp32-ops``): every float op is a plain VALU op.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "celeris_amd", "csrc")
+LIBDIR = os.path.join(ROOT, "celeris_amd", "_lib")
+LIB = os.path.join(LIBDIR, "libceleris_rs.so")
+ORACLE = os.path.join(ROOT, "oracle")
+REFERENCE = "/root/reference"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    "-Wall",
+    "-Wno-unused-value",
+    "-Wno-unused-result",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+SOURCES = ["rs_kernels.hip", "rs_api.cpp", "propagate_shim.cpp"]
+
+
+def _run(cmd, cwd=None):
+    res = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    lines = [ln for ln in res.stdout.splitlines()
+             if "packed-fp32-ops" not in ln and "hip-link" not in ln]
+    if res.returncode != 0:
+        sys.stderr.write("\n".join(lines) + "\n")
+        raise RuntimeError(f"command failed ({res.returncode}): {' '.join(cmd)}")
+    return lines
+
+
+def build_library() -> str:
+    os.makedirs(LIBDIR, exist_ok=True)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, *[os.path.join(CSRC, s) for s in SOURCES], "-o", tmp])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle() -> str:
+    _run(["make", "-s", "-C", ORACLE])
+    return os.path.join(ORACLE, "_build", "liboracle_rs.so")
+
+
+def build_reference() -> bool:
+    """Compile the reference's CPU path into oracle/_ref (only where its sources are)."""
+    if not os.path.isdir(os.path.join(REFERENCE, "src", "celeris")):
+        return False
+    if shutil.which("g++") is None:
+        return False
+    _run(["make", "-s", "-C", ORACLE, "ref"])
+    return True
+
+
+def build_all() -> None:
+    build_library()
+    build_oracle()
+    build_reference()
+
+
+if __name__ == "__main__":
+    build_all()
+    print("built", LIB)

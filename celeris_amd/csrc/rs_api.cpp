@@ -1,0 +1,719 @@
+// rs_api.cpp — host side of the C ABI (include/celeris_rs.h): argument checks, launch
+// planning, scratch management and the reference-compatible drop-in entry points.
+//
+// Reference behaviour kept by the drop-in layer (src/celeris/cuda/propagate.cu):
+//   - false when no device is visible (:137-138) or npil <= 0 || n <= 0 (:139);
+//     propagate_zx also rejects nx <= 1 || nz <= 1 (:201)
+//   - false on any allocation (:164-171) or launch/sync failure (:184), with `out`
+//     left untouched in every false case
+//   - grid step 2W/(n-1) (:143); n == 1 gives step = inf and NaN coordinates, which
+//     the reference propagates into NaN intensities; so does this library
+//   - synchronous; callable from several host threads at once (the reference's
+//     callers do so from std::async trials and GUI threads)
+#include "../../include/celeris_rs.h"
+#include "rs_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <atomic>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace crs {
+hipError_t launch_stage_pillars(int mode, const double* px, const double* py, const double* t,
+                                int npil, int nsets, void* out, hipStream_t s);
+hipError_t launch_forward(int mode, const void* pil, int npil, const DevPlane* planes,
+                          const FwdBlock* blocks, int nblocks, double2* partial, hipStream_t s);
+hipError_t launch_fwd_reduce(int restore_phase, const DevPlane* planes, const PixBlock* pblocks,
+                             int nblocks, const double2* partial, double2* field,
+                             double* intensity, hipStream_t s);
+hipError_t launch_stage_pixels(const DevPlane* planes, const PixBlock* pblocks, int nblocks,
+                               const double2* u, float4* pix, hipStream_t s);
+hipError_t launch_adjoint(int mode, const void* pil, int npil, const DevPlane* planes,
+                          const AdjRow* rows, const AdjBlock* blocks, int nblocks,
+                          const void* pix_or_u, double2* partial, hipStream_t s);
+hipError_t launch_adj_reduce(const double2* partial, int nslots, int64_t n, double2* g,
+                             hipStream_t s);
+}  // namespace crs
+
+using namespace crs;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+void set_error(const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+}
+
+// Fault injection for tests of the failure paths: CELERIS_RS_FAIL=alloc makes every
+// scratch allocation fail, CELERIS_RS_FAIL=launch makes every launch fail; in both
+// cases before any device work is enqueued.
+bool inject(const char* what) {
+    const char* v = std::getenv("CELERIS_RS_FAIL");
+    return v && std::strcmp(v, what) == 0;
+}
+
+constexpr double kInv2Pi = 0.15915494309189533577;
+
+int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+struct DevBuffer {
+    void* ptr = nullptr;
+    size_t cap = 0;
+};
+
+bool stream_capturing(hipStream_t s) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &st) != hipSuccess) return false;
+    return st != hipStreamCaptureStatusNone;
+}
+
+}  // namespace
+
+struct crs_context {
+    int device = 0;
+    DevBuffer pil, pix, tables, partial;
+    void* host_tables = nullptr;
+    size_t host_cap = 0;
+    hipEvent_t tables_ev = nullptr;
+    bool tables_pending = false;
+    int fwd_target[3] = {0, 0, 0};
+    int adj_target[3] = {0, 0, 0};
+    bool timing = false;
+    hipEvent_t ev[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [adjoint][start/stop]
+    bool ev_valid[2] = {false, false};
+};
+
+namespace {
+
+// Built-in launch targets: work-groups per forward/adjoint launch.  256 CUs x a few
+// resident work-groups, several waves deep so the tail of uneven splits is short.
+int default_target(int mode, bool adjoint) {
+    (void)adjoint;
+    return mode == CRS_MODE_FP32C ? 2048 : 4096;
+}
+
+crs_status ensure(DevBuffer& b, size_t bytes, hipStream_t stream, const char* what) {
+    if (bytes == 0) bytes = 256;
+    if (b.cap >= bytes) return CRS_OK;
+    if (stream_capturing(stream)) {
+        set_error("%s: %zu bytes of scratch needed while the stream is being captured; "
+                  "make one uncaptured call of the same shape first", what, bytes);
+        return CRS_ERR_CAPTURE;
+    }
+    if (inject("alloc")) {
+        set_error("%s: injected allocation failure", what);
+        return CRS_ERR_ALLOC;
+    }
+    if (b.ptr) {
+        // The old buffer may still be read by queued work on this stream.
+        hipStreamSynchronize(stream);
+        hipFree(b.ptr);
+        b.ptr = nullptr;
+        b.cap = 0;
+    }
+    const size_t want = bytes + bytes / 4;
+    if (hipMalloc(&b.ptr, want) != hipSuccess) {
+        hipGetLastError();
+        b.ptr = nullptr;
+        set_error("%s: hipMalloc of %zu bytes failed", what, want);
+        return CRS_ERR_ALLOC;
+    }
+    b.cap = want;
+    return CRS_OK;
+}
+
+// Host-built descriptor tables travel in one pinned buffer and one async copy.
+struct TableBuilder {
+    std::vector<unsigned char> bytes;
+    size_t add(const void* p, size_t n) {
+        size_t off = (bytes.size() + 255) & ~size_t(255);
+        bytes.resize(off + n);
+        if (n) std::memcpy(bytes.data() + off, p, n);
+        return off;
+    }
+};
+
+crs_status upload_tables(crs_context* ctx, const TableBuilder& tb, hipStream_t stream) {
+    const size_t n = tb.bytes.size();
+    crs_status st = ensure(ctx->tables, n, stream, "descriptor tables");
+    if (st != CRS_OK) return st;
+    if (ctx->host_cap < n) {
+        if (stream_capturing(stream)) {
+            set_error("descriptor tables: pinned staging must grow while capturing");
+            return CRS_ERR_CAPTURE;
+        }
+        if (ctx->tables_pending) { hipEventSynchronize(ctx->tables_ev); ctx->tables_pending = false; }
+        if (ctx->host_tables) hipHostFree(ctx->host_tables);
+        ctx->host_tables = nullptr;
+        ctx->host_cap = 0;
+        const size_t want = n + n / 4 + 4096;
+        if (hipHostMalloc(&ctx->host_tables, want, hipHostMallocDefault) != hipSuccess) {
+            hipGetLastError();
+            ctx->host_tables = nullptr;
+            set_error("descriptor tables: hipHostMalloc of %zu bytes failed", want);
+            return CRS_ERR_ALLOC;
+        }
+        ctx->host_cap = want;
+    }
+    // The previous call's copy may still be reading the pinned buffer.
+    if (ctx->tables_pending) { hipEventSynchronize(ctx->tables_ev); ctx->tables_pending = false; }
+    std::memcpy(ctx->host_tables, tb.bytes.data(), n);
+    if (hipMemcpyAsync(ctx->tables.ptr, ctx->host_tables, n, hipMemcpyHostToDevice, stream) !=
+        hipSuccess) {
+        hipGetLastError();
+        set_error("descriptor tables: hipMemcpyAsync failed");
+        return CRS_ERR_LAUNCH;
+    }
+    if (!stream_capturing(stream)) {
+        hipEventRecord(ctx->tables_ev, stream);
+        ctx->tables_pending = true;
+    }
+    return CRS_OK;
+}
+
+template <class T>
+const T* dev_at(crs_context* ctx, size_t off) {
+    return reinterpret_cast<const T*>(static_cast<unsigned char*>(ctx->tables.ptr) + off);
+}
+
+crs_status check_common(crs_context* ctx, int mode, int npil, int nsets, const crs_plane* planes,
+                        int nplanes) {
+    if (!ctx) { set_error("null context"); return CRS_ERR_INVALID; }
+    if (mode < 0 || mode > 2) { set_error("unknown mode %d", mode); return CRS_ERR_INVALID; }
+    if (npil <= 0) { set_error("npil must be positive (got %d)", npil); return CRS_ERR_INVALID; }
+    if (nsets <= 0) { set_error("nsets must be positive (got %d)", nsets); return CRS_ERR_INVALID; }
+    if (nplanes < 0 || (nplanes > 0 && !planes)) {
+        set_error("bad plane list");
+        return CRS_ERR_INVALID;
+    }
+    if (static_cast<int64_t>(npil) * nsets > (int64_t(1) << 31) - 1) {
+        set_error("nsets*npil exceeds 2^31-1");
+        return CRS_ERR_INVALID;
+    }
+    for (int p = 0; p < nplanes; ++p) {
+        const crs_plane& P = planes[p];
+        if (P.nx <= 0 || P.ny <= 0 || static_cast<int64_t>(P.nx) * P.ny > (int64_t(1) << 30)) {
+            set_error("plane %d: bad pixel counts %d x %d", p, P.nx, P.ny);
+            return CRS_ERR_INVALID;
+        }
+        if (P.tset < 0 || P.tset >= nsets) {
+            set_error("plane %d: tset %d outside [0, %d)", p, P.tset, nsets);
+            return CRS_ERR_INVALID;
+        }
+        if (P.offset < 0 || P.reserved != 0) {
+            set_error("plane %d: negative offset or nonzero reserved field", p);
+            return CRS_ERR_INVALID;
+        }
+    }
+    if (hipSetDevice(ctx->device) != hipSuccess) {
+        hipGetLastError();
+        set_error("hipSetDevice(%d) failed", ctx->device);
+        return CRS_ERR_NO_DEVICE;
+    }
+    return CRS_OK;
+}
+
+DevPlane make_dev_plane(const crs_plane& P, int px_per_thread, int py_per_thread) {
+    DevPlane d{};
+    d.k = P.k;
+    d.kturn = P.k * kInv2Pi;
+    d.x0 = P.x0; d.sx = P.sx;
+    d.y0 = P.y0; d.sy = P.sy;
+    d.z0 = P.z0; d.sz = P.sz;
+    d.nx = P.nx; d.ny = P.ny;
+    d.tx = static_cast<int32_t>(ceil_div(P.nx, px_per_thread));
+    d.ty = static_cast<int32_t>(ceil_div(P.ny, py_per_thread));
+    d.tset = P.tset;
+    d.offset = P.offset;
+    d.part = 0;
+    d.nslots = 1;
+    return d;
+}
+
+size_t pillar_stride_bytes(int mode) { return mode == CRS_MODE_FP64 ? sizeof(double4) : sizeof(float4); }
+
+crs_status launch_status(hipError_t e, const char* what) {
+    if (e == hipSuccess) return CRS_OK;
+    set_error("%s: %s", what, hipGetErrorString(e));
+    return CRS_ERR_LAUNCH;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------
+// Batched layer.
+
+extern "C" crs_status crs_context_create(int device, crs_context** out) {
+    if (!out) { set_error("null output pointer"); return CRS_ERR_INVALID; }
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        hipGetLastError();
+        set_error("no HIP device visible");
+        return CRS_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= count) {
+        set_error("device %d outside [0, %d)", device, count);
+        return CRS_ERR_INVALID;
+    }
+    if (hipSetDevice(device) != hipSuccess) {
+        hipGetLastError();
+        set_error("hipSetDevice(%d) failed", device);
+        return CRS_ERR_NO_DEVICE;
+    }
+    crs_context* ctx = new crs_context();
+    ctx->device = device;
+    if (hipEventCreateWithFlags(&ctx->tables_ev, hipEventDisableTiming) != hipSuccess) {
+        hipGetLastError();
+        delete ctx;
+        set_error("hipEventCreate failed");
+        return CRS_ERR_LAUNCH;
+    }
+    *out = ctx;
+    return CRS_OK;
+}
+
+extern "C" void crs_context_destroy(crs_context* ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    if (ctx->tables_pending) hipEventSynchronize(ctx->tables_ev);
+    hipDeviceSynchronize();
+    for (DevBuffer* b : {&ctx->pil, &ctx->pix, &ctx->tables, &ctx->partial})
+        if (b->ptr) hipFree(b->ptr);
+    for (auto& pair : ctx->ev)
+        for (hipEvent_t e : pair)
+            if (e) hipEventDestroy(e);
+    if (ctx->host_tables) hipHostFree(ctx->host_tables);
+    if (ctx->tables_ev) hipEventDestroy(ctx->tables_ev);
+    delete ctx;
+}
+
+extern "C" const char* crs_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" crs_status crs_set_tuning(crs_context* ctx, int mode, int fwd_target_blocks,
+                                     int adj_target_blocks) {
+    if (!ctx || mode < 0 || mode > 2 || fwd_target_blocks < 0 || adj_target_blocks < 0) {
+        set_error("bad tuning arguments");
+        return CRS_ERR_INVALID;
+    }
+    ctx->fwd_target[mode] = fwd_target_blocks;
+    ctx->adj_target[mode] = adj_target_blocks;
+    return CRS_OK;
+}
+
+extern "C" crs_status crs_set_timing(crs_context* ctx, int enable) {
+    if (!ctx) { set_error("null context"); return CRS_ERR_INVALID; }
+    if (enable && !ctx->ev[0][0]) {
+        hipSetDevice(ctx->device);
+        for (auto& pair : ctx->ev)
+            for (hipEvent_t& e : pair)
+                if (hipEventCreate(&e) != hipSuccess) {
+                    hipGetLastError();
+                    set_error("hipEventCreate failed");
+                    return CRS_ERR_LAUNCH;
+                }
+    }
+    ctx->timing = enable != 0;
+    ctx->ev_valid[0] = ctx->ev_valid[1] = false;
+    return CRS_OK;
+}
+
+extern "C" crs_status crs_last_kernel_ms(crs_context* ctx, int adjoint, float* ms) {
+    if (!ctx || !ms || adjoint < 0 || adjoint > 1) { set_error("bad arguments"); return CRS_ERR_INVALID; }
+    if (!ctx->ev_valid[adjoint]) { set_error("no timed launch recorded"); return CRS_ERR_INVALID; }
+    if (hipEventSynchronize(ctx->ev[adjoint][1]) != hipSuccess ||
+        hipEventElapsedTime(ms, ctx->ev[adjoint][0], ctx->ev[adjoint][1]) != hipSuccess) {
+        hipGetLastError();
+        set_error("hipEventElapsedTime failed");
+        return CRS_ERR_LAUNCH;
+    }
+    return CRS_OK;
+}
+
+extern "C" const char* crs_kernel_name(int mode, int adjoint) {
+    switch (mode) {
+        case CRS_MODE_FP32C: return adjoint ? "adj_fp32c" : "fwd_fp32c";
+        case CRS_MODE_FP64: return adjoint ? "adj_fp64" : "fwd_fp64";
+        case CRS_MODE_FP32: return adjoint ? "adj_fp32" : "fwd_fp32";
+        default: return "";
+    }
+}
+
+extern "C" crs_status crs_forward(crs_context* ctx, void* stream_, int mode, const double* px,
+                                  const double* py, const double* t, int npil, int nsets,
+                                  const crs_plane* planes, int nplanes, double* field,
+                                  double* intensity) {
+    g_last_error.clear();
+    crs_status st = check_common(ctx, mode, npil, nsets, planes, nplanes);
+    if (st != CRS_OK) return st;
+    if (!px || !py || !t) { set_error("null pillar array"); return CRS_ERR_INVALID; }
+    if (nplanes == 0 || (!field && !intensity)) return CRS_OK;
+    hipStream_t stream = static_cast<hipStream_t>(stream_);
+
+    const int PX = mode == CRS_MODE_FP32C ? kPX : 1;
+    const int PY = mode == CRS_MODE_FP32C ? kPY : 1;
+
+    // Split over pillar chunks: enough work-groups to fill the chip, chunks of at
+    // least one LDS tile.
+    std::vector<DevPlane> dp(nplanes);
+    int64_t base_blocks = 0;
+    for (int p = 0; p < nplanes; ++p) {
+        dp[p] = make_dev_plane(planes[p], PX, PY);
+        base_blocks += ceil_div(static_cast<int64_t>(dp[p].tx) * dp[p].ty, kThreads);
+    }
+    const int target = ctx->fwd_target[mode] > 0 ? ctx->fwd_target[mode] : default_target(mode, false);
+    int64_t nsplit = std::max<int64_t>(1, ceil_div(target, base_blocks));
+    nsplit = std::min<int64_t>(nsplit, std::max<int64_t>(1, npil / kThreads));
+    const int64_t chunk = ceil_div(ceil_div(npil, nsplit), 64) * 64;
+    nsplit = ceil_div(npil, chunk);
+    if (base_blocks * nsplit > (int64_t(1) << 31) - 1) {
+        set_error("launch too large (%lld work-groups)", (long long)(base_blocks * nsplit));
+        return CRS_ERR_INVALID;
+    }
+
+    int64_t part = 0;
+    for (int p = 0; p < nplanes; ++p) {
+        dp[p].part = part;
+        dp[p].nslots = static_cast<int32_t>(nsplit);
+        part += nsplit * static_cast<int64_t>(dp[p].nx) * dp[p].ny;
+    }
+    std::vector<FwdBlock> blocks;
+    blocks.reserve(static_cast<size_t>(base_blocks * nsplit));
+    for (int64_t s = 0; s < nsplit; ++s) {
+        const int32_t q0 = static_cast<int32_t>(s * chunk);
+        const int32_t q1 = static_cast<int32_t>(std::min<int64_t>(npil, (s + 1) * chunk));
+        for (int p = 0; p < nplanes; ++p) {
+            const int64_t tiles = static_cast<int64_t>(dp[p].tx) * dp[p].ty;
+            const int64_t npix = static_cast<int64_t>(dp[p].nx) * dp[p].ny;
+            for (int64_t b = 0; b < tiles; b += kThreads)
+                blocks.push_back({p, static_cast<int32_t>(b), q0, q1, dp[p].part + s * npix});
+        }
+    }
+    std::vector<PixBlock> pblocks;
+    for (int p = 0; p < nplanes; ++p) {
+        const int64_t npix = static_cast<int64_t>(dp[p].nx) * dp[p].ny;
+        for (int64_t b = 0; b < npix; b += kThreads) pblocks.push_back({p, static_cast<int32_t>(b)});
+    }
+
+    TableBuilder tb;
+    const size_t o_planes = tb.add(dp.data(), dp.size() * sizeof(DevPlane));
+    const size_t o_blocks = tb.add(blocks.data(), blocks.size() * sizeof(FwdBlock));
+    const size_t o_pblocks = tb.add(pblocks.data(), pblocks.size() * sizeof(PixBlock));
+
+    if ((st = ensure(ctx->pil, static_cast<size_t>(npil) * nsets * pillar_stride_bytes(mode), stream,
+                     "staged pillars")) != CRS_OK)
+        return st;
+    if ((st = ensure(ctx->partial, static_cast<size_t>(part) * sizeof(double2), stream,
+                     "forward partials")) != CRS_OK)
+        return st;
+    if ((st = upload_tables(ctx, tb, stream)) != CRS_OK) return st;
+    if (inject("launch")) { set_error("injected launch failure"); return CRS_ERR_LAUNCH; }
+
+    const DevPlane* d_planes = dev_at<DevPlane>(ctx, o_planes);
+    if ((st = launch_status(launch_stage_pillars(mode, px, py, t, npil, nsets, ctx->pil.ptr, stream),
+                            "stage pillars")) != CRS_OK)
+        return st;
+    const bool timed = ctx->timing && !stream_capturing(stream);
+    if (timed) hipEventRecord(ctx->ev[0][0], stream);
+    if ((st = launch_status(launch_forward(mode, ctx->pil.ptr, npil, d_planes,
+                                           dev_at<FwdBlock>(ctx, o_blocks),
+                                           static_cast<int>(blocks.size()),
+                                           static_cast<double2*>(ctx->partial.ptr), stream),
+                            "forward")) != CRS_OK)
+        return st;
+    if (timed) { hipEventRecord(ctx->ev[0][1], stream); ctx->ev_valid[0] = true; }
+    return launch_status(launch_fwd_reduce(mode == CRS_MODE_FP32C ? 1 : 0, d_planes,
+                                           dev_at<PixBlock>(ctx, o_pblocks),
+                                           static_cast<int>(pblocks.size()),
+                                           static_cast<const double2*>(ctx->partial.ptr),
+                                           reinterpret_cast<double2*>(field), intensity, stream),
+                         "forward reduce");
+}
+
+extern "C" crs_status crs_adjoint(crs_context* ctx, void* stream_, int mode, const double* px,
+                                  const double* py, int npil, int nsets, const crs_plane* planes,
+                                  int nplanes, const double* u, double* g) {
+    g_last_error.clear();
+    crs_status st = check_common(ctx, mode, npil, nsets, planes, nplanes);
+    if (st != CRS_OK) return st;
+    if (!px || !py || !g || (nplanes > 0 && !u)) { set_error("null array argument"); return CRS_ERR_INVALID; }
+    hipStream_t stream = static_cast<hipStream_t>(stream_);
+
+    const int PQ = mode == CRS_MODE_FP32C ? kPQ : 1;
+    std::vector<DevPlane> dp(nplanes);
+    for (int p = 0; p < nplanes; ++p) dp[p] = make_dev_plane(planes[p], 1, 1);
+
+    // Per-set row lists (planes in order, rows in order) and their pixel counts.
+    std::vector<AdjRow> rows;
+    std::vector<int64_t> set_row0(nsets + 1, 0);
+    std::vector<std::vector<int64_t>> row_pix_prefix(nsets);
+    for (int s = 0; s < nsets; ++s) {
+        set_row0[s] = static_cast<int64_t>(rows.size());
+        std::vector<int64_t>& pre = row_pix_prefix[s];
+        pre.push_back(0);
+        for (int p = 0; p < nplanes; ++p) {
+            if (dp[p].tset != s) continue;
+            for (int j = 0; j < dp[p].ny; ++j) {
+                rows.push_back({p, j});
+                pre.push_back(pre.back() + dp[p].nx);
+            }
+        }
+    }
+    set_row0[nsets] = static_cast<int64_t>(rows.size());
+    if (rows.size() > static_cast<size_t>(INT32_MAX)) { set_error("too many rows"); return CRS_ERR_INVALID; }
+
+    const int64_t groups = ceil_div(npil, static_cast<int64_t>(kThreads) * PQ);
+    const int64_t base_blocks = groups * nsets;
+    int64_t max_rows = 1;
+    for (int s = 0; s < nsets; ++s) max_rows = std::max<int64_t>(max_rows, set_row0[s + 1] - set_row0[s]);
+    const int target = ctx->adj_target[mode] > 0 ? ctx->adj_target[mode] : default_target(mode, true);
+    int64_t nsplit = std::max<int64_t>(1, ceil_div(target, base_blocks));
+    nsplit = std::min<int64_t>(nsplit, max_rows);
+    if (base_blocks * nsplit > (int64_t(1) << 31) - 1) { set_error("launch too large"); return CRS_ERR_INVALID; }
+
+    // Cut each set's rows into nsplit runs of about equal pixel count.
+    std::vector<AdjBlock> blocks;
+    blocks.reserve(static_cast<size_t>(base_blocks * nsplit));
+    const int64_t n_out = static_cast<int64_t>(nsets) * npil;
+    for (int64_t c = 0; c < nsplit; ++c) {
+        for (int s = 0; s < nsets; ++s) {
+            const std::vector<int64_t>& pre = row_pix_prefix[s];
+            const int64_t nrows = static_cast<int64_t>(pre.size()) - 1;
+            const int64_t total = pre.back();
+            auto cut = [&](int64_t k) -> int64_t {  // first row whose prefix >= k*total/nsplit
+                if (k >= nsplit) return nrows;
+                const int64_t want = (total * k) / nsplit;
+                return static_cast<int64_t>(std::lower_bound(pre.begin(), pre.end(), want) - pre.begin());
+            };
+            const int64_t r0 = std::min(cut(c), nrows), r1 = std::min(cut(c + 1), nrows);
+            for (int64_t gq = 0; gq < groups; ++gq)
+                blocks.push_back({s, static_cast<int32_t>(gq * kThreads * PQ),
+                                  static_cast<int32_t>(set_row0[s] + r0),
+                                  static_cast<int32_t>(set_row0[s] + std::max(r0, r1)),
+                                  c * n_out + static_cast<int64_t>(s) * npil});
+        }
+    }
+    std::vector<PixBlock> pblocks;
+    int64_t total_pix = 0;
+    for (int p = 0; p < nplanes; ++p) {
+        const int64_t npix = static_cast<int64_t>(dp[p].nx) * dp[p].ny;
+        total_pix = std::max(total_pix, dp[p].offset + npix);
+        for (int64_t b = 0; b < npix; b += kThreads) pblocks.push_back({p, static_cast<int32_t>(b)});
+    }
+
+    TableBuilder tb;
+    const size_t o_planes = tb.add(dp.data(), dp.size() * sizeof(DevPlane));
+    const size_t o_rows = tb.add(rows.data(), rows.size() * sizeof(AdjRow));
+    const size_t o_blocks = tb.add(blocks.data(), blocks.size() * sizeof(AdjBlock));
+    const size_t o_pblocks = tb.add(pblocks.data(), pblocks.size() * sizeof(PixBlock));
+
+    // Positions only: stage set 0 (the adjoint pair loop reads x, y).
+    if ((st = ensure(ctx->pil, static_cast<size_t>(npil) * pillar_stride_bytes(mode), stream,
+                     "staged pillars")) != CRS_OK)
+        return st;
+    if (mode == CRS_MODE_FP32C &&
+        (st = ensure(ctx->pix, static_cast<size_t>(total_pix) * sizeof(float4), stream,
+                     "staged pixels")) != CRS_OK)
+        return st;
+    if ((st = ensure(ctx->partial, static_cast<size_t>(nsplit * n_out) * sizeof(double2), stream,
+                     "adjoint partials")) != CRS_OK)
+        return st;
+    if ((st = upload_tables(ctx, tb, stream)) != CRS_OK) return st;
+    if (inject("launch")) { set_error("injected launch failure"); return CRS_ERR_LAUNCH; }
+
+    // Positions only (t = NULL stages zero transmissions): the adjoint pair loop
+    // reads x and y.
+    const DevPlane* d_planes = dev_at<DevPlane>(ctx, o_planes);
+    if ((st = launch_status(launch_stage_pillars(mode == CRS_MODE_FP64 ? 1 : 2, px, py, nullptr,
+                                                 npil, 1, ctx->pil.ptr, stream),
+                            "stage positions")) != CRS_OK)
+        return st;
+    const void* pix_or_u = u;
+    if (mode == CRS_MODE_FP32C) {
+        if ((st = launch_status(launch_stage_pixels(d_planes, dev_at<PixBlock>(ctx, o_pblocks),
+                                                    static_cast<int>(pblocks.size()),
+                                                    reinterpret_cast<const double2*>(u),
+                                                    static_cast<float4*>(ctx->pix.ptr), stream),
+                                "stage pixels")) != CRS_OK)
+            return st;
+        pix_or_u = ctx->pix.ptr;
+    }
+    const bool timed = ctx->timing && !stream_capturing(stream);
+    if (timed) hipEventRecord(ctx->ev[1][0], stream);
+    if ((st = launch_status(launch_adjoint(mode, ctx->pil.ptr, npil, d_planes,
+                                           dev_at<AdjRow>(ctx, o_rows),
+                                           dev_at<AdjBlock>(ctx, o_blocks),
+                                           static_cast<int>(blocks.size()), pix_or_u,
+                                           static_cast<double2*>(ctx->partial.ptr), stream),
+                            "adjoint")) != CRS_OK)
+        return st;
+    if (timed) { hipEventRecord(ctx->ev[1][1], stream); ctx->ev_valid[1] = true; }
+    return launch_status(launch_adj_reduce(static_cast<const double2*>(ctx->partial.ptr),
+                                           static_cast<int>(nsplit), n_out,
+                                           reinterpret_cast<double2*>(g), stream),
+                         "adjoint reduce");
+}
+
+// ------------------------------------------------------------------------------------
+// Drop-in layer: the reference's propagate.hpp semantics on top of crs_forward.
+
+namespace {
+
+std::atomic<int> g_default_mode{CRS_MODE_FP32C};
+
+// One drop-in call's device state.  States live in a process-wide pool: a call takes
+// one, uses it on its own stream and returns it, so concurrent callers never share
+// buffers and nothing is torn down at thread or process exit.
+struct DropinState {
+    crs_context* ctx = nullptr;
+    hipStream_t stream = nullptr;
+    DevBuffer in, out;
+    std::vector<double> host_out;
+};
+
+std::mutex g_pool_mu;
+std::vector<DropinState*> g_pool;
+
+DropinState* acquire_state() {
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        if (!g_pool.empty()) {
+            DropinState* s = g_pool.back();
+            g_pool.pop_back();
+            return s;
+        }
+    }
+    DropinState* s = new DropinState();
+    if (crs_context_create(0, &s->ctx) != CRS_OK) { delete s; return nullptr; }
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        hipGetLastError();
+        crs_context_destroy(s->ctx);
+        delete s;
+        set_error("hipStreamCreate failed");
+        return nullptr;
+    }
+    return s;
+}
+
+void release_state(DropinState* s) {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    g_pool.push_back(s);
+}
+
+// Runs one plane through crs_forward with host inputs; on success copies |E|^2 into
+// `out` (n_out doubles).  `out` is written only when every step succeeded.
+int dropin_run(const double* px, const double* py, const double* pt, int npil,
+               const crs_plane& plane, double* out) {
+    DropinState* s = acquire_state();
+    if (!s) return 0;
+    const size_t n_out = static_cast<size_t>(plane.nx) * plane.ny;
+    const size_t pil_bytes = static_cast<size_t>(npil) * sizeof(double);
+    int ok = 0;
+    if (ensure(s->in, 4 * pil_bytes, s->stream, "drop-in inputs") == CRS_OK &&
+        ensure(s->out, n_out * sizeof(double), s->stream, "drop-in output") == CRS_OK) {
+        double* d_px = static_cast<double*>(s->in.ptr);
+        double* d_py = d_px + npil;
+        double* d_t = d_py + npil;
+        if (hipMemcpyAsync(d_px, px, pil_bytes, hipMemcpyHostToDevice, s->stream) == hipSuccess &&
+            hipMemcpyAsync(d_py, py, pil_bytes, hipMemcpyHostToDevice, s->stream) == hipSuccess &&
+            hipMemcpyAsync(d_t, pt, 2 * pil_bytes, hipMemcpyHostToDevice, s->stream) == hipSuccess &&
+            crs_forward(s->ctx, s->stream, g_default_mode.load(), d_px, d_py, d_t, npil, 1, &plane, 1,
+                        nullptr, static_cast<double*>(s->out.ptr)) == CRS_OK) {
+            s->host_out.resize(n_out);
+            if (hipMemcpyAsync(s->host_out.data(), s->out.ptr, n_out * sizeof(double),
+                               hipMemcpyDeviceToHost, s->stream) == hipSuccess &&
+                hipStreamSynchronize(s->stream) == hipSuccess) {
+                std::memcpy(out, s->host_out.data(), n_out * sizeof(double));
+                ok = 1;
+            }
+        }
+    }
+    if (!ok) {
+        hipGetLastError();
+        hipStreamSynchronize(s->stream);
+    }
+    release_state(s);
+    return ok;
+}
+
+}  // namespace
+
+extern "C" int crs_available(void) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess) {
+        hipGetLastError();
+        return 0;
+    }
+    return count > 0 ? 1 : 0;
+}
+
+extern "C" const char* crs_device_name(void) {
+    static std::once_flag once;
+    static char name[256] = {0};
+    std::call_once(once, [] {
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) { hipGetLastError(); return; }
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, 0) != hipSuccess) { hipGetLastError(); return; }
+        std::snprintf(name, sizeof name, "%s", prop.name);
+    });
+    return name;
+}
+
+extern "C" int crs_set_default_mode(int mode) {
+    if (mode < 0 || mode > 2) return -1;
+    return g_default_mode.exchange(mode);
+}
+
+extern "C" int crs_propagate_psf(const double* px, const double* py, const double* pt, int npil,
+                                 double cx, double cy, double z, double k, int n,
+                                 double half_window, double* out) {
+    if (!crs_available()) return 0;
+    if (npil <= 0 || n <= 0) return 0;
+    if (!px || !py || !pt || !out) return 0;
+    const double step = 2.0 * half_window / (n - 1);
+    crs_plane plane{};
+    plane.k = k;
+    plane.x0 = cx - half_window;
+    plane.sx = step;
+    plane.y0 = cy - half_window;
+    plane.sy = step;
+    plane.z0 = z;
+    plane.sz = 0.0;
+    plane.nx = n;
+    plane.ny = n;
+    return dropin_run(px, py, pt, npil, plane, out);
+}
+
+extern "C" int crs_propagate_zx(const double* px, const double* py, const double* pt, int npil,
+                                double xmin, double xmax, int nx, double zmin, double zmax, int nz,
+                                double k, double* out) {
+    if (!crs_available()) return 0;
+    if (npil <= 0 || nx <= 1 || nz <= 1) return 0;
+    if (!px || !py || !pt || !out) return 0;
+    crs_plane plane{};
+    plane.k = k;
+    plane.x0 = xmin;
+    plane.sx = (xmax - xmin) / (nx - 1);
+    plane.y0 = 0.0;
+    plane.sy = 0.0;
+    plane.z0 = zmin;
+    plane.sz = (zmax - zmin) / (nz - 1);
+    plane.nx = nx;
+    plane.ny = nz;
+    return dropin_run(px, py, pt, npil, plane, out);
+}

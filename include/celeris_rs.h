@@ -1,0 +1,169 @@
+/*
+ * celeris_rs.h — C ABI of the MI355X (gfx950) Rayleigh–Sommerfeld far-field library.
+ *
+ * Computes, for metalens pillars q at (x_q, y_q) with complex transmission t_q and
+ * focal-plane pixels p at (fx_p, fy_p, z_p),
+ *
+ *     E_p = sum_q t_q * exp(i k R_pq) / R_pq,   R_pq = sqrt((fx_p-x_q)^2 + (fy_p-y_q)^2 + z_p^2)
+ *
+ * (the kernel of src/celeris/cuda/propagate.hpp:4-5 and analysis/focal.cpp:229-233 in
+ * the reference) and its adjoint g_q = sum_p conj(G_pq) u_p.  Units are microns; k is
+ * 2*pi/lambda in rad/um.
+ *
+ * Two layers:
+ *   1. Drop-in entry points with the reference's exact semantics (host arrays in,
+ *      host |E|^2 out, synchronous, thread-safe, nonzero = success / 0 = "fall back to
+ *      CPU"): crs_available, crs_device_name, crs_propagate_psf, crs_propagate_zx.
+ *      include/celeris/cuda/propagate.hpp wraps them in the reference's C++ names.
+ *   2. Batched device-pointer entry points (no reference counterpart): many planes,
+ *      wavelengths and transmission sets in one launch, complex-field output, an
+ *      adjoint, three arithmetic modes, an explicit stream.  These return a crs_status.
+ *
+ * Every pointer argument is a plain pointer; no HIP or torch types appear here.
+ * Complex arrays are interleaved (re, im) float64 pairs, i.e. std::complex<double>.
+ */
+#ifndef CELERIS_RS_H
+#define CELERIS_RS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes of the batched entry points. */
+typedef enum crs_status {
+    CRS_OK = 0,
+    CRS_ERR_NO_DEVICE = 1, /* no HIP device visible */
+    CRS_ERR_INVALID = 2,   /* bad argument (sizes, plane descriptors, mode) */
+    CRS_ERR_ALLOC = 3,     /* device or pinned allocation failed */
+    CRS_ERR_LAUNCH = 4,    /* kernel launch or HIP runtime call failed */
+    CRS_ERR_CAPTURE = 5    /* the call would allocate while the stream is being captured */
+} crs_status;
+
+/* Arithmetic modes. */
+typedef enum crs_mode {
+    /* FP32 pair arithmetic; the phase argument k*(R - z) is formed and reduced mod
+     * 2*pi in FP64; pillar-tile partial sums are Kahan-compensated; partials over
+     * pillar chunks are combined in FP64.  Default and fastest. */
+    CRS_MODE_FP32C = 0,
+    /* The reference algorithm in FP64 throughout (sqrt, sincos, 1/R, complex MAC). */
+    CRS_MODE_FP64 = 1,
+    /* The reference GPU kernel's arithmetic: FP32 R, FP32 k*R, FP32 sincos, FP32
+     * accumulation (propagate.cu:68-78). */
+    CRS_MODE_FP32 = 2
+} crs_mode;
+
+/* One focal plane (an "instance": one wavelength, one pixel lattice).
+ * Pixel (i, j), 0 <= i < nx, 0 <= j < ny, sits at
+ *     fx = x0 + i*sx,   fy = y0 + j*sy,   z = z0 + j*sz
+ * so a psf window is (x0 = cx-W, sx = 2W/(n-1), y0 = cy-W, sy = sx, sz = 0) and a
+ * y=0 caustic slice is (x0 = xmin, sx = xstep, y0 = sy = 0, z0 = zmin, sz = zstep).
+ * Its field occupies elements [offset, offset + nx*ny) of the output arrays, row
+ * j at offset + j*nx.  tset selects the transmission set (0 <= tset < nsets). */
+typedef struct crs_plane {
+    double k;
+    double x0, sx;
+    double y0, sy;
+    double z0, sz;
+    int32_t nx, ny;
+    int32_t tset;
+    int32_t reserved; /* must be 0 */
+    int64_t offset;
+} crs_plane;
+
+typedef struct crs_context crs_context;
+
+/* ---- drop-in layer (reference: src/celeris/cuda/propagate.hpp) ------------------ */
+
+/* Replaces celeris::cuda::available() (propagate.hpp:20, propagate.cu:18-21).
+ * Nonzero when at least one HIP device is visible. */
+int crs_available(void);
+
+/* Replaces celeris::cuda::device_name() (propagate.hpp:21, propagate.cu:23-31).
+ * Name of device 0, or "" when none.  The string stays valid for the process. */
+const char *crs_device_name(void);
+
+/* Replaces celeris::cuda::propagate_psf (propagate.hpp:27-30, propagate.cu:133-193).
+ * px, py: npil pillar centres (um); pt: npil complex transmissions (interleaved);
+ * out: caller-owned n*n doubles, row-major (row j = y), receiving |E|^2 on the grid
+ * fx = cx - W + i*2W/(n-1), fy = cy - W + j*2W/(n-1) at distance z.
+ * Returns nonzero on success; 0 (out untouched) when no device is visible,
+ * npil <= 0 or n <= 0, or any HIP call fails, so the caller can fall back to its
+ * CPU loop.  Arithmetic: the mode from crs_set_default_mode (CRS_MODE_FP32C unless
+ * changed).  Synchronous; safe to call from several host threads at once (each
+ * thread gets its own stream and buffers). */
+int crs_propagate_psf(const double *px, const double *py, const double *pt, int npil,
+                      double cx, double cy, double z, double k, int n,
+                      double half_window, double *out);
+
+/* Replaces celeris::cuda::propagate_zx (propagate.hpp:35-38, propagate.cu:195-253).
+ * |E|^2 on an nz x nx grid in the plane y = 0: fx = xmin + ix*(xmax-xmin)/(nx-1),
+ * z = zmin + iz*(zmax-zmin)/(nz-1); out row-major nz*nx (row = z index).
+ * Returns 0 when no device, npil <= 0, nx <= 1 or nz <= 1, or on any HIP failure. */
+int crs_propagate_zx(const double *px, const double *py, const double *pt, int npil,
+                     double xmin, double xmax, int nx, double zmin, double zmax, int nz,
+                     double k, double *out);
+
+/* Mode used by crs_propagate_psf / crs_propagate_zx in this process (no reference
+ * counterpart; the reference always used FP32).  Returns the previous mode, or -1
+ * for an unknown mode (then nothing changes). */
+int crs_set_default_mode(int mode);
+
+/* ---- batched device-pointer layer (no reference counterpart) --------------------- */
+
+/* Create a context bound to HIP device `device`.  A context owns device scratch
+ * (staged pillars, split-sum partials, descriptor tables) that grows on demand and
+ * is reused; one context must not be used by two host threads at once. */
+crs_status crs_context_create(int device, crs_context **ctx);
+void crs_context_destroy(crs_context *ctx);
+
+/* Message describing the last failure on the calling host thread ("" if none). */
+const char *crs_last_error(void);
+
+/* Forward sum.  All array arguments are device pointers:
+ *   px, py : npil pillar centres, float64
+ *   t      : nsets*npil complex transmissions (set-major, interleaved float64)
+ *   field  : complex output, total pixels (max over planes of offset + nx*ny), or NULL
+ *   intensity : |E|^2 output of the same length, or NULL
+ * planes is a HOST array of nplanes descriptors.  Work is enqueued on `stream`
+ * (a hipStream_t; NULL = the null stream) and the call returns without waiting.
+ * Inputs must stay valid and unmodified until the stream reaches this work. */
+crs_status crs_forward(crs_context *ctx, void *stream, int mode,
+                       const double *px, const double *py, const double *t,
+                       int npil, int nsets, const crs_plane *planes, int nplanes,
+                       double *field, double *intensity);
+
+/* Adjoint (vector-Jacobian product) of crs_forward with respect to t:
+ *   g[s*npil + q] = sum over planes with tset == s, over their pixels p, of conj(G_pq) u_p
+ * u: complex cotangent per pixel (device, laid out like crs_forward's field);
+ * g: nsets*npil complex output (device).  With u = dL/dRe(E) + i dL/dIm(E) for a
+ * real loss L, g = dL/dRe(t) + i dL/dIm(t).  Same stream semantics as crs_forward. */
+crs_status crs_adjoint(crs_context *ctx, void *stream, int mode,
+                       const double *px, const double *py, int npil, int nsets,
+                       const crs_plane *planes, int nplanes,
+                       const double *u, double *g);
+
+/* Launch-shape tuning (no reference counterpart).  target_blocks: work-groups a
+ * forward/adjoint launch aims for through its split over pillar chunks (forward)
+ * or pixel rows (adjoint); 0 restores the built-in default.  Applies to the
+ * context's later calls. */
+crs_status crs_set_tuning(crs_context *ctx, int mode, int fwd_target_blocks,
+                          int adj_target_blocks);
+
+/* Kernel timing (no reference counterpart).  With timing enabled, each crs_forward /
+ * crs_adjoint records HIP events on its stream immediately before and after the
+ * kernel that does the pair sums; crs_last_kernel_ms waits for the last such event
+ * and returns that kernel's duration in milliseconds (adjoint = 0: forward). */
+crs_status crs_set_timing(crs_context *ctx, int enable);
+crs_status crs_last_kernel_ms(crs_context *ctx, int adjoint, float *ms);
+
+/* Name (as a profiler lists it, demangled prefix) of the kernel that does the pair
+ * sums for `mode`, forward (adjoint = 0) or adjoint (adjoint = 1); "" if unknown. */
+const char *crs_kernel_name(int mode, int adjoint);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CELERIS_RS_H */

@@ -1,0 +1,157 @@
+// ref_driver.cpp — runs the reference's own CPU propagation and analysis code.
+//
+// TEST INFRASTRUCTURE ONLY.  Built by oracle/ref.mk from this file plus the
+// reference sources where they lie (src/celeris/analysis/{focal,throughfocus,
+// chromatic,mtf}.cpp, src/celeris/design/{metalens,phase_profile}.cpp); the
+// binary lands in oracle/_ref/ and is used to pin the FP64 restatement in
+// oracle/rs_oracle.c and to write golden vectors.  Nothing here is shipped.
+//
+// Protocol: `ref_driver <command> <in.f64> <out.f64>`.  Both files are flat
+// little-endian float64 arrays; integers travel as float64.
+//
+//   pillars      : npil, px[npil], py[npil], (re,im)[npil], cx, cy, z, lambda, n, W
+//                  -> intensity[n*n]       (celeris::propagate_pillars, focal.cpp:205)
+//   lens commands share a lens block:
+//                  n_cells, period, fill_map[n_cells^2], nlib, fill[nlib],
+//                  phase[nlib], amplitude[nlib]
+//   focus        : lens, f, lambda, D      -> strehl, phase_strehl, fwhm, dl, ee
+//                                             (celeris::analyze_focus, focal.cpp:32)
+//   psf          : lens, f, lambda, D, n, W -> intensity[n*n]  (compute_psf, focal.cpp:148)
+//   throughfocus : lens, f, lambda, D      -> N, z[N], I[N], z_peak, dof,
+//                                             cnx, cnz, caustic[cnz*cnx]
+//                                             (analyze_through_focus, throughfocus.cpp:15)
+//   chromatic    : lens, f, lambda, D, lmin, lmax, n -> n x (lambda, f, rel)
+//                                             (analyze_chromatic, chromatic.cpp:9)
+//   mtf          : lens, f, lambda, D      -> m, freq[m], mtf[m], ideal[m], cutoff
+//                                             (analyze_mtf, mtf.cpp:10)
+#include "celeris/analysis/chromatic.hpp"
+#include "celeris/analysis/focal.hpp"
+#include "celeris/analysis/mtf.hpp"
+#include "celeris/analysis/throughfocus.hpp"
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace celeris;
+
+namespace {
+
+struct Reader {
+    std::vector<double> v;
+    std::size_t i = 0;
+    double next() {
+        if (i >= v.size()) { std::fprintf(stderr, "ref_driver: input too short\n"); std::exit(2); }
+        return v[i++];
+    }
+    int next_int() { return static_cast<int>(next()); }
+};
+
+std::vector<double> read_all(const char* path) {
+    std::FILE* f = std::fopen(path, "rb");
+    if (!f) { std::perror(path); std::exit(2); }
+    std::vector<double> out;
+    double buf[4096];
+    std::size_t got;
+    while ((got = std::fread(buf, sizeof(double), 4096, f)) > 0) out.insert(out.end(), buf, buf + got);
+    std::fclose(f);
+    return out;
+}
+
+void write_all(const char* path, const std::vector<double>& v) {
+    std::FILE* f = std::fopen(path, "wb");
+    if (!f) { std::perror(path); std::exit(2); }
+    std::fwrite(v.data(), sizeof(double), v.size(), f);
+    std::fclose(f);
+}
+
+void read_lens(Reader& r, MetalensDesign& lens, UnitCellLibrary& lib) {
+    lens.n_cells = r.next_int();
+    lens.period_um = r.next();
+    lens.fill_map.resize(static_cast<std::size_t>(lens.n_cells) * lens.n_cells);
+    for (double& x : lens.fill_map) x = r.next();
+    lens.rms_phase_error_deg = 0.0;
+    lens.mean_amplitude = 0.0;
+    const int nlib = r.next_int();
+    lib.period_um = lens.period_um;
+    lib.wavelength_um = 0.0;
+    lib.thickness_um = 0.0;
+    lib.fill.resize(nlib);
+    lib.phase.resize(nlib);
+    lib.amplitude.resize(nlib);
+    for (double& x : lib.fill) x = r.next();
+    for (double& x : lib.phase) x = r.next();
+    for (double& x : lib.amplitude) x = r.next();
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc != 4) {
+        std::fprintf(stderr, "usage: ref_driver <command> <in.f64> <out.f64>\n");
+        return 2;
+    }
+    const std::string cmd = argv[1];
+    Reader r{read_all(argv[2])};
+    std::vector<double> out;
+
+    if (cmd == "pillars") {
+        const int npil = r.next_int();
+        std::vector<double> px(npil), py(npil);
+        std::vector<cdouble> t(npil);
+        for (double& x : px) x = r.next();
+        for (double& y : py) y = r.next();
+        for (cdouble& c : t) { const double re = r.next(); const double im = r.next(); c = {re, im}; }
+        const double cx = r.next(), cy = r.next(), z = r.next(), lam = r.next();
+        const int n = r.next_int();
+        const double W = r.next();
+        PsfMap m = propagate_pillars(px, py, t, cx, cy, z, lam, n, W);
+        out = m.intensity;
+    } else {
+        MetalensDesign lens;
+        UnitCellLibrary lib;
+        read_lens(r, lens, lib);
+        const double f = r.next(), lam = r.next(), D = r.next();
+        if (cmd == "focus") {
+            FocalAnalysis a = analyze_focus(lens, lib, f, lam, D);
+            out = {a.strehl, a.phase_strehl, a.fwhm_um, a.diffraction_limit_um, a.encircled_energy};
+        } else if (cmd == "psf") {
+            const int n = r.next_int();
+            const double W = r.next();
+            PsfMap m = compute_psf(lens, lib, f, lam, D, n, W);
+            out = m.intensity;
+        } else if (cmd == "throughfocus") {
+            ThroughFocus tf = analyze_through_focus(lens, lib, f, lam, D);
+            out.push_back(static_cast<double>(tf.z_um.size()));
+            for (float z : tf.z_um) out.push_back(z);
+            for (float v : tf.intensity) out.push_back(v);
+            out.push_back(tf.z_peak_um);
+            out.push_back(tf.dof_um);
+            out.push_back(tf.caustic_nx);
+            out.push_back(tf.caustic_nz);
+            for (float v : tf.caustic) out.push_back(v);
+        } else if (cmd == "chromatic") {
+            const double lmin = r.next(), lmax = r.next();
+            const int n = r.next_int();
+            auto pts = analyze_chromatic(lens, lib, f, lam, D, lmin, lmax, n);
+            for (const auto& p : pts) {
+                out.push_back(p.wavelength_um);
+                out.push_back(p.focal_length_um);
+                out.push_back(p.rel_peak);
+            }
+        } else if (cmd == "mtf") {
+            MtfCurve c = analyze_mtf(lens, lib, f, lam, D);
+            out.push_back(static_cast<double>(c.mtf.size()));
+            for (float v : c.freq_cyc_mm) out.push_back(v);
+            for (float v : c.mtf) out.push_back(v);
+            for (float v : c.mtf_ideal) out.push_back(v);
+            out.push_back(c.cutoff_cyc_mm);
+        } else {
+            std::fprintf(stderr, "ref_driver: unknown command %s\n", cmd.c_str());
+            return 2;
+        }
+    }
+    write_all(argv[3], out);
+    return 0;
+}

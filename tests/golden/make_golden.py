@@ -1,0 +1,96 @@
+#!/usr/bin/env python3
+"""Writes tests/golden/*.npz: inputs and outputs of the REFERENCE's own CPU code.
+
+Run where the reference sources are present (they are compiled by oracle/ref.mk into
+oracle/_ref/ref_driver, from src/celeris/analysis/{focal,throughfocus,chromatic,mtf}.cpp
+and src/celeris/design/{metalens,phase_profile}.cpp).  The fixtures are data only:
+seeded inputs and the reference's outputs, used by tests/test_oracle.py to pin the
+FP64 restatement in oracle/rs_oracle.c where the reference build is absent.
+
+    python tests/golden/make_golden.py
+"""
+import math
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+from celeris_amd import configs  # noqa: E402
+
+
+def pillars_cases():
+    """(name, px, py, t, cx, cy, z, lam, n, W) cases for celeris::propagate_pillars."""
+    cases = []
+    px, py = configs.square_grid(12, 0.4)
+    a = 0.8 + 0.2 * configs.uniform(configs.SEED, configs.STREAM_AMP, px.size)
+    t = a * np.exp(1j * configs.hyperbolic_phase(px, py, 0.532, 20.0))
+    cases.append(("lens12_center", px, py, t, 0.0, 0.0, 20.0, 0.532, 9, 2.0))
+    cases.append(("lens12_offaxis", px, py, t, 1.3, -0.7, 18.5, 0.532, 6, 1.5))
+    px2, py2 = configs.square_grid(7, 0.35)
+    u = configs.uniform(configs.SEED, configs.STREAM_U, 2 * px2.size)
+    t2 = (0.5 + 0.5 * u[: px2.size]) * np.exp(2j * np.pi * u[px2.size:])
+    cases.append(("random49_two_pixel", px2, py2, t2, 0.1, 0.2, 7.0, 0.633, 2, 0.9))
+    cases.append(("random49_wide", px2, py2, t2, 0.0, 0.0, 3.0, 0.45, 11, 6.0))
+    return cases
+
+
+def lens_case():
+    """A small metalens described the reference's way: a fill map over a square cell
+    lattice plus a 16-entry unit-cell library (phase ramp, |t| in [0.75, 0.95]);
+    pillars inside a circle of diameter D (focal.cpp:45-57)."""
+    n_cells, period, lam, f, D = 21, 0.4, 0.532, 15.0, 8.0
+    nlib = 16
+    lib_fill = np.linspace(0.1, 0.85, nlib)
+    lib_phase = np.linspace(-math.pi, math.pi, nlib, endpoint=False) + 0.5 * math.pi / nlib
+    lib_amp = 0.75 + 0.2 * configs.uniform(configs.SEED, 11, nlib)
+    c = (n_cells - 1) / 2.0
+    iy, ix = np.meshgrid(np.arange(n_cells), np.arange(n_cells), indexing="ij")
+    x = (ix - c) * period
+    y = (iy - c) * period
+    k = 2 * math.pi / lam
+    phi = -k * (np.sqrt(x * x + y * y + f * f) - f)
+    wrapped = np.angle(np.exp(1j * phi))
+    # nearest library phase (circular), then that entry's fill
+    d = np.angle(np.exp(1j * (wrapped[..., None] - lib_phase[None, None, :])))
+    entry = np.argmin(np.abs(d), axis=-1)
+    fill_map = lib_fill[entry]
+    return dict(n_cells=n_cells, period=period, lam=lam, f=f, D=D, lib_fill=lib_fill,
+                lib_phase=lib_phase, lib_amp=lib_amp, fill_map=fill_map)
+
+
+def main():
+    if not oracle.ref_available():
+        sys.exit("oracle/_ref/ref_driver is not built (make -C oracle ref)")
+    out = {}
+    for name, px, py, t, cx, cy, z, lam, n, W in pillars_cases():
+        I = oracle.ref_propagate_pillars(px, py, t, cx, cy, z, lam, n, W)
+        out[f"{name}__px"] = px
+        out[f"{name}__py"] = py
+        out[f"{name}__t"] = t
+        out[f"{name}__args"] = np.array([cx, cy, z, lam, n, W])
+        out[f"{name}__I"] = I
+    np.savez_compressed(os.path.join(HERE, "propagate_pillars.npz"), **out)
+
+    L = lens_case()
+    blk = oracle.lens_block(L["n_cells"], L["period"], L["fill_map"], L["lib_fill"], L["lib_phase"],
+                            L["lib_amp"])
+    base = np.concatenate([blk, [L["f"], L["lam"], L["D"]]])
+    lens_out = {k: np.asarray(v) for k, v in L.items()}
+    lens_out["focus"] = oracle.ref_run("focus", base)
+    lens_out["psf_n"] = np.array([33, 3.0])
+    lens_out["psf"] = oracle.ref_run("psf", np.concatenate([base, [33, 3.0]]))
+    lens_out["throughfocus"] = oracle.ref_run("throughfocus", base)
+    lens_out["chromatic_args"] = np.array([0.45, 0.65, 5])
+    lens_out["chromatic"] = oracle.ref_run("chromatic", np.concatenate([base, [0.45, 0.65, 5]]))
+    lens_out["mtf"] = oracle.ref_run("mtf", base)
+    np.savez_compressed(os.path.join(HERE, "lens_analyses.npz"), **lens_out)
+    print("wrote", sorted(os.listdir(HERE)))
+
+
+if __name__ == "__main__":
+    main()

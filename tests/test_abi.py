@@ -1,0 +1,107 @@
+"""CPU checks of the C ABI: the library loads, exports what include/*.h declares, lays
+out crs_plane as declared, and keeps the reference's no-device behaviour."""
+import ctypes
+import math
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from celeris_amd import Plane, _lib
+from celeris_amd import rs as crs
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_c_functions():
+    hdr = open(os.path.join(ROOT, "include", "celeris_rs.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    return sorted(set(re.findall(r"\b(crs_[a-z_]+)\s*\(", hdr)))
+
+
+def test_library_exports_every_declared_function():
+    lib = _lib.load()
+    names = declared_c_functions()
+    assert len(names) >= 14
+    for name in names:
+        assert hasattr(lib, name), name
+    bound = {n for n, _, _ in _lib.SIGNATURES}
+    assert set(names) == bound
+
+
+def test_library_exports_reference_cpp_names():
+    """include/celeris/cuda/propagate.hpp: the reference's celeris::cuda names."""
+    out = subprocess.run(["nm", "-DC", _lib.LIB_PATH], stdout=subprocess.PIPE, text=True).stdout
+    for sig in ("celeris::cuda::available()", "celeris::cuda::device_name()",
+                "celeris::cuda::propagate_psf(double const*, double const*, std::complex<double> const*, int, double, double, double, double, int, double, double*)",
+                "celeris::cuda::propagate_zx(double const*, double const*, std::complex<double> const*, int, double, double, int, double, double, int, double, double*)"):
+        assert sig in out, sig
+
+
+def test_dropin_header_matches_reference_signatures():
+    hdr = open(os.path.join(ROOT, "include", "celeris", "cuda", "propagate.hpp")).read()
+    flat = " ".join(hdr.split())
+    assert "namespace celeris::cuda" in flat
+    assert ("bool propagate_psf(const double* px, const double* py, const std::complex<double>* pt, "
+            "int npil, double cx, double cy, double z, double k, int n, double half_window, double* out);") in flat
+    assert ("bool propagate_zx(const double* px, const double* py, const std::complex<double>* pt, "
+            "int npil, double xmin, double xmax, int nx, double zmin, double zmax, int nz, double k, "
+            "double* out);") in flat
+
+
+def test_plane_struct_layout():
+    P = _lib.crs_plane
+    assert ctypes.sizeof(P) == 80
+    assert P.k.offset == 0 and P.nx.offset == 56 and P.tset.offset == 64 and P.offset.offset == 72
+
+
+def test_plane_geometry_follows_reference_grid():
+    p = Plane.psf(0.5, -0.25, 100.0, 9.9, 161, 5.4)
+    assert p.x0 == 0.5 - 5.4 and p.sx == 2 * 5.4 / 160 and p.sz == 0.0
+    # n == 1: step = 2W/0 = inf, as propagate.cu:143 and focal.cpp:223 compute it
+    assert math.isinf(Plane.psf(0, 0, 10, 10, 1, 1.0).sx)
+    zx = Plane.zx(-3, 3, 201, 50, 150, 101, 10.0)
+    assert zx.sy == 0 and zx.y0 == 0 and zx.sz == 1.0 and zx.sx == 6 / 200
+
+
+def test_kernel_names():
+    assert crs.kernel_name("fp32c") == "fwd_fp32c"
+    assert crs.kernel_name("fp32c", adjoint=True) == "adj_fp32c"
+    assert crs.kernel_name("fp64") == "fwd_fp64"
+
+
+def _has_gpu():
+    try:
+        return bool(_lib.load().crs_available())
+    except Exception:
+        return False
+
+
+@pytest.mark.skipif(_has_gpu(), reason="checks the no-device path")
+def test_no_device_behaviour():
+    """No device: available() false, device_name() "", propagate_* false with out
+    untouched (propagate.cu:18-31, :137-138, :199-200)."""
+    assert crs.available() is False
+    assert crs.device_name() == ""
+    px = np.zeros(3)
+    t = np.ones(3, complex)
+    assert crs.propagate_psf(px, px, t, 0, 0, 10, 10, 5, 1.0) is None
+    assert crs.propagate_zx(px, px, t, -1, 1, 5, 5, 10, 5, 10) is None
+    lib = _lib.load()
+    out = (ctypes.c_double * 4)(7, 7, 7, 7)
+    dp = ctypes.POINTER(ctypes.c_double)
+    pxa = (ctypes.c_double * 3)()
+    ta = (ctypes.c_double * 6)(1, 0, 1, 0, 1, 0)
+    assert lib.crs_propagate_psf(pxa, pxa, ta, 3, 0.0, 0.0, 10.0, 10.0, 2, 1.0, out) == 0
+    assert list(out) == [7, 7, 7, 7]
+    h = ctypes.c_void_p()
+    assert lib.crs_context_create(0, ctypes.byref(h)) == 1  # CRS_ERR_NO_DEVICE
+    assert "device" in _lib.last_error()
+
+
+def test_set_default_mode_roundtrip():
+    prev = crs.set_default_mode("fp64")
+    assert crs.set_default_mode(prev) == "fp64"
+    assert _lib.load().crs_set_default_mode(7) == -1

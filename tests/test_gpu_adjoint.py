@@ -1,0 +1,171 @@
+"""GPU parity of the adjoint kernel and the autograd bridge.
+
+The adjoint has no reference counterpart; it is pinned by the FP64 oracle's adjoint
+(itself pinned by the dot-product identity), by the dot-product identity on the GPU,
+and by central finite differences of the encircled-energy loss.
+Tolerances: FP64 <= 1e-12, FP32C <= 1e-5 relative L2 of the gradient vector.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from celeris_amd import Plane, configs
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def gpu_adjoint(torch, ctx, px, py, u, planes, nsets, mode):
+    print(f"[gpu_adjoint] mode={mode} npil={np.size(px)} planes={len(planes)}", flush=True)
+    g = ctx.adjoint(dev(torch, px), dev(torch, py), dev(torch, u), planes, nsets=nsets, mode=mode)
+    torch.cuda.synchronize()
+    return g.cpu().numpy()
+
+
+@pytest.mark.parametrize("mode,tol", [("fp64", 1e-12), ("fp32c", 1e-5)])
+def test_adjoint_small_vs_oracle(torch_cuda, rsctx, mode, tol):
+    L = configs.small_lens(n_side=21, n=15)
+    rng = np.random.default_rng(11)
+    u = rng.standard_normal(225) + 1j * rng.standard_normal(225)
+    ref = oracle.adjoint(L.px, L.py, u, L.planes)
+    g = gpu_adjoint(torch_cuda, rsctx, L.px, L.py, u, L.planes, 1, mode)
+    e = rel_l2(g, ref)
+    print(f"adjoint {mode} relL2 {e:.3e}")
+    assert e <= tol
+
+
+@pytest.mark.parametrize("mode,tol", [("fp64", 1e-12), ("fp32c", 1e-5)])
+def test_adjoint_multiset_vs_oracle(torch_cuda, rsctx, mode, tol):
+    px, py = configs.square_grid(40, 0.3)  # 1,600 pillars
+    planes = [
+        Plane.psf(0.0, 0.0, 25.0, 2 * math.pi / 0.5, 21, 2.0, tset=0),
+        Plane.psf(0.1, 0.0, 27.0, 2 * math.pi / 0.6, 9, 1.0, tset=2),
+        Plane.zx(-2.0, 2.0, 13, 15.0, 35.0, 7, 2 * math.pi / 0.6, tset=2),
+        Plane.axis(10.0, 40.0, 17, 2 * math.pi / 0.5, tset=0),
+    ]
+    total = sum(p.npix for p in planes)
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal(total) + 1j * rng.standard_normal(total)
+    ref = oracle.adjoint(px, py, u, planes, nsets=3)
+    g = gpu_adjoint(torch_cuda, rsctx, px, py, u, planes, 3, mode)
+    assert np.all(g[1] == 0)  # no plane uses set 1
+    for s in (0, 2):
+        e = rel_l2(g[s], ref[s])
+        print(mode, "set", s, e)
+        assert e <= tol
+
+
+def test_dot_product_identity(torch_cuda, rsctx):
+    """Re<G t, u> = Re<t, G^H u> with the GPU forward and adjoint (config 1 size)."""
+    L = configs.config1()
+    rng = np.random.default_rng(2)
+    u = rng.standard_normal(81 * 81) + 1j * rng.standard_normal(81 * 81)
+    for mode, tol in (("fp64", 1e-12), ("fp32c", 1e-5)):
+        f, _ = rsctx.forward(dev(torch_cuda, L.px), dev(torch_cuda, L.py), dev(torch_cuda, L.t),
+                             L.planes, mode=mode)
+        g = rsctx.adjoint(dev(torch_cuda, L.px), dev(torch_cuda, L.py), dev(torch_cuda, u),
+                          L.planes, mode=mode)
+        lhs = np.vdot(u, f.cpu().numpy())
+        rhs = np.vdot(g.cpu().numpy()[0], L.t[0])
+        err = abs(lhs - rhs) / abs(lhs)
+        print(mode, "dot identity", err)
+        assert err <= tol
+
+
+def _ee_loss(torch, field, plane, r0):
+    from celeris_amd.autograd import encircled_energy
+
+    inten = field.real ** 2 + field.imag ** 2
+    return -encircled_energy(inten, plane, r0)
+
+
+def test_autograd_fd_encircled_energy(torch_cuda):
+    """d(-EE)/d(phase) and d(-EE)/d(amplitude) through the adjoint kernel vs central
+    finite differences (h = 1e-4 rad / 1e-4) computed with the FP64 forward."""
+    torch = torch_cuda
+    from celeris_amd.autograd import rs_field
+
+    L = configs.small_lens(n_side=24, n=31, focal=25.0)
+    plane = L.planes[0]
+    r0 = 1.22 * L.wavelengths[0] / L.na
+    px, py = dev(torch, L.px), dev(torch, L.py)
+    amp0 = np.abs(L.t[0])
+    phi0 = np.angle(L.t[0])
+
+    def loss_np(phi, amp, mode="fp64"):
+        t = torch.polar(dev(torch, amp), dev(torch, phi))[None]
+        return float(_ee_loss(torch, rs_field(t, px, py, [plane], mode=mode), plane, r0))
+
+    phi = dev(torch, phi0).requires_grad_(True)
+    amp = dev(torch, amp0).requires_grad_(True)
+    for mode, tol in (("fp64", 1e-7), ("fp32c", 1e-4)):
+        phi.grad = None
+        amp.grad = None
+        t = torch.polar(amp, phi)[None]
+        loss = _ee_loss(torch, rs_field(t, px, py, [plane], mode=mode), plane, r0)
+        loss.backward()
+        gphi = phi.grad.cpu().numpy()
+        gamp = amp.grad.cpu().numpy()
+        pick = np.random.default_rng(1).choice(L.npil, 8, replace=False)
+        h = 1e-4
+        for q in pick:
+            e = np.zeros(L.npil)
+            e[q] = h
+            fd_phi = (loss_np(phi0 + e, amp0) - loss_np(phi0 - e, amp0)) / (2 * h)
+            fd_amp = (loss_np(phi0, amp0 + e) - loss_np(phi0, amp0 - e)) / (2 * h)
+            scale_p = np.max(np.abs(gphi))
+            scale_a = np.max(np.abs(gamp))
+            print(mode, q, gphi[q], fd_phi, gamp[q], fd_amp)
+            assert abs(gphi[q] - fd_phi) <= tol * scale_p
+            assert abs(gamp[q] - fd_amp) <= tol * scale_a
+
+
+def test_config4_gradient_fp32c_vs_fp64(torch_cuda):
+    """Config 4: -EE within 1.22 lambda/NA at z = f, 91,809 pillars, 161^2 grid; the
+    FP32C adjoint gradient against the FP64 one, and against FD on 16 seeded pillars
+    (FD exploits linearity: E(phi_q +- h) = E + t_q (e^{+-ih} - 1) G_q)."""
+    torch = torch_cuda
+    from celeris_amd.autograd import rs_field
+
+    L = configs.config2(n=161)
+    plane = L.planes[0]
+    r0 = 1.22 * L.wavelengths[0] / L.na
+    px, py = dev(torch, L.px), dev(torch, L.py)
+    grads = {}
+    for mode in ("fp64", "fp32c"):
+        phi = dev(torch, L.meta["phase"]).requires_grad_(True)
+        amp = dev(torch, L.meta["amp"]).requires_grad_(True)
+        t = torch.polar(amp, phi)[None]
+        loss = _ee_loss(torch, rs_field(t, px, py, [plane], mode=mode), plane, r0)
+        loss.backward()
+        grads[mode] = (phi.grad.cpu().numpy(), amp.grad.cpu().numpy(), float(loss))
+    ep = rel_l2(grads["fp32c"][0], grads["fp64"][0])
+    ea = rel_l2(grads["fp32c"][1], grads["fp64"][1])
+    print(f"config4 grad relL2 phase {ep:.3e} amp {ea:.3e} loss {grads['fp64'][2]:.6f}")
+    assert ep <= 1e-5 and ea <= 1e-5
+
+    # FD on seeded pillars with exact column updates (FP64 forward).
+    from celeris_amd import Context
+
+    ctx = Context(0)
+    E, _ = ctx.forward(px, py, dev(torch, L.t), [plane], mode="fp64")
+    pick = (configs.uniform(configs.SEED, configs.STREAM_FD_PILLARS, 16) * L.npil).astype(int)
+    h = 1e-4
+    gphi = grads["fp64"][0]
+    for q in pick:
+        one = torch.ones((1, 1), dtype=torch.complex128, device="cuda")
+        Gq, _ = ctx.forward(px[q:q + 1].contiguous(), py[q:q + 1].contiguous(), one, [plane],
+                            mode="fp64")
+        tq = complex(L.t[0, q])
+        lp = float(_ee_loss(torch, E + tq * (np.exp(1j * h) - 1) * Gq, plane, r0))
+        lm = float(_ee_loss(torch, E + tq * (np.exp(-1j * h) - 1) * Gq, plane, r0))
+        fd = (lp - lm) / (2 * h)
+        print(q, gphi[q], fd)
+        assert abs(gphi[q] - fd) <= 1e-6 * np.max(np.abs(gphi)) + 1e-9
+    ctx.close()

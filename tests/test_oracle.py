@@ -1,0 +1,190 @@
+"""Pins the FP64 oracle (oracle/rs_oracle.c) before anything is compared against it.
+
+Checks, each chosen so that a plausible slip (a dropped 1/R, a sign of the phase, a
+transposed index, a wrong pixel step, a missing conjugate) fails at least one:
+  * the reference's own CPU code (oracle/_ref, compiled from its sources), directly
+    where it is built and through the committed golden vectors everywhere;
+  * closed forms: one pillar, the in-phase focus of a hyperbolic lens;
+  * long-double brute force on tiny inputs;
+  * linearity and mirror symmetry;
+  * the adjoint: dot-product identity and column-by-column construction.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from celeris_amd import Plane, configs
+from celeris_amd.design import MetalensDesign, UnitCellLibrary, aperture_pillars
+from conftest import rel_l2
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def plane_for(cx, cy, z, lam, n, W):
+    return Plane.psf(cx, cy, z, 2 * math.pi / lam, int(n), W)
+
+
+def load_pillar_golden():
+    d = np.load(os.path.join(GOLDEN, "propagate_pillars.npz"))
+    names = sorted({k.split("__")[0] for k in d.files})
+    return [(nm, d[f"{nm}__px"], d[f"{nm}__py"], d[f"{nm}__t"], d[f"{nm}__args"], d[f"{nm}__I"])
+            for nm in names]
+
+
+@pytest.mark.parametrize("case", load_pillar_golden(), ids=lambda c: c[0])
+def test_oracle_matches_reference_golden(case):
+    """Golden |E|^2 written by celeris::propagate_pillars (focal.cpp:205-238)."""
+    name, px, py, t, args, I_ref = case
+    cx, cy, z, lam, n, W = args
+    f, I = oracle.forward(px, py, t[None], [plane_for(cx, cy, z, lam, n, W)], intensity=True)
+    err = np.max(np.abs(I - I_ref.ravel())) / np.max(np.abs(I_ref))
+    assert err <= 1e-13, err
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference build absent")
+def test_oracle_matches_reference_build_random():
+    rng = np.random.default_rng(123)
+    for trial in range(4):
+        npil = int(rng.integers(1, 60))
+        px = rng.uniform(-5, 5, npil)
+        py = rng.uniform(-5, 5, npil)
+        t = rng.uniform(0.2, 1, npil) * np.exp(1j * rng.uniform(-4, 4, npil))
+        cx, cy = rng.uniform(-2, 2, 2)
+        z = rng.uniform(2, 40)
+        lam = rng.uniform(0.4, 0.8)
+        n = int(rng.integers(2, 12))
+        W = rng.uniform(0.5, 4)
+        I_ref = oracle.ref_propagate_pillars(px, py, t, cx, cy, z, lam, n, W)
+        _, I = oracle.forward(px, py, t[None], [plane_for(cx, cy, z, lam, n, W)], intensity=True)
+        assert np.max(np.abs(I - I_ref.ravel())) / np.max(I_ref) <= 1e-13
+
+
+def test_oracle_psf_of_lens_matches_reference_compute_psf():
+    """compute_psf (focal.cpp:148-203) of a fill-map lens, through aperture_pillars."""
+    d = np.load(os.path.join(GOLDEN, "lens_analyses.npz"))
+    lens = MetalensDesign(int(d["n_cells"]), float(d["period"]), d["fill_map"])
+    lib = UnitCellLibrary(float(d["period"]), d["lib_fill"], d["lib_phase"], d["lib_amp"])
+    px, py, t = aperture_pillars(lens, lib, float(d["D"]))
+    n, W = int(d["psf_n"][0]), float(d["psf_n"][1])
+    _, I = oracle.forward(px, py, t[None], [plane_for(0, 0, float(d["f"]), float(d["lam"]), n, W)],
+                          intensity=True)
+    assert np.max(np.abs(I - d["psf"])) / np.max(d["psf"]) <= 1e-13
+
+
+def test_oracle_zx_and_axis_match_reference_through_focus():
+    """analyze_through_focus (throughfocus.cpp:15-132): the 161-point on-axis scan and
+    the 81x81 CPU caustic, both normalized to their peak and stored as float."""
+    d = np.load(os.path.join(GOLDEN, "lens_analyses.npz"))
+    lens = MetalensDesign(int(d["n_cells"]), float(d["period"]), d["fill_map"])
+    lib = UnitCellLibrary(float(d["period"]), d["lib_fill"], d["lib_phase"], d["lib_amp"])
+    f, lam, D = float(d["f"]), float(d["lam"]), float(d["D"])
+    px, py, t = aperture_pillars(lens, lib, D)
+    tf = d["throughfocus"]
+    N = int(tf[0])
+    z_ref, I_ref = tf[1:1 + N], tf[1 + N:1 + 2 * N]
+    cnx, cnz = int(tf[3 + 2 * N]), int(tf[4 + 2 * N])
+    caustic_ref = tf[5 + 2 * N:5 + 2 * N + cnx * cnz]
+    k = 2 * math.pi / lam
+    na = math.sin(math.atan((D / 2) / f))
+    half = max(2.0 * lam / (na * na + 1e-6), 0.25 * f)
+    zlo, zhi = max(0.1 * f, f - half), f + half
+    axis = oracle.forward(px, py, t[None], [Plane.axis(zlo, zhi, N, k)])
+    Ia = np.abs(axis) ** 2
+    assert np.max(np.abs(Ia / Ia.max() - I_ref)) <= 2e-7  # float storage
+    assert np.max(np.abs(zlo + (zhi - zlo) * np.arange(N) / (N - 1) - z_ref)) <= 1e-5
+    Wx = max(8.0 * lam * f / D, 3.0)
+    zx = oracle.forward(px, py, t[None], [Plane.zx(-Wx, Wx, cnx, zlo, zhi, cnz, k)])
+    Iz = np.abs(zx) ** 2
+    assert np.max(np.abs(Iz / Iz.max() - caustic_ref)) <= 2e-7
+
+
+def test_single_pillar_closed_form():
+    k = 2 * math.pi / 0.55
+    plane = Plane.psf(0.3, -0.1, 12.0, k, 7, 2.0)
+    a, phi, x, y = 0.7, -2.1, 1.5, 0.4
+    f = oracle.forward(np.array([x]), np.array([y]), np.array([[a * np.exp(1j * phi)]]), [plane])
+    i = np.arange(7)
+    fx = plane.x0 + i * plane.sx
+    fy = plane.y0 + i * plane.sy
+    R = np.sqrt((fx[None, :] - x) ** 2 + (fy[:, None] - y) ** 2 + 144.0)
+    exact = a * np.exp(1j * (phi + k * R)) / R
+    assert rel_l2(f, exact.ravel()) <= 1e-14
+
+
+def test_inphase_focus_identity():
+    L = configs.config1()
+    k = 2 * math.pi / 0.532
+    f = oracle.forward(L.px, L.py, L.t, [Plane.psf(0, 0, L.focal, k, 3, 1.0)])
+    R = np.sqrt(L.px ** 2 + L.py ** 2 + L.focal ** 2)
+    exact = np.exp(1j * k * L.focal) * np.sum(np.abs(L.t[0]) / R)
+    assert abs(f[4] - exact) / abs(exact) <= 1e-12
+
+
+def test_long_double_brute_force():
+    L = configs.small_lens(n_side=9, n=5)
+    f = oracle.forward(L.px, L.py, L.t, L.planes)
+    fl = oracle.forward_ld(L.px, L.py, L.t, L.planes)
+    assert rel_l2(f, fl) <= 1e-12
+
+
+def test_linearity_and_mirror_symmetry():
+    px, py = configs.square_grid(8, 0.5)
+    rng = np.random.default_rng(9)
+    t1 = rng.standard_normal(px.size) + 1j * rng.standard_normal(px.size)
+    t2 = rng.standard_normal(px.size) + 1j * rng.standard_normal(px.size)
+    k = 2 * math.pi / 0.6
+    plane = Plane.psf(0, 0, 9.0, k, 9, 2.0)
+    e1 = oracle.forward(px, py, t1[None], [plane])
+    e2 = oracle.forward(px, py, t2[None], [plane])
+    e12 = oracle.forward(px, py, (2 * t1 - 1j * t2)[None], [plane])
+    assert rel_l2(e12, 2 * e1 - 1j * e2) <= 1e-13
+    # Mirror the aperture in x (square grid is x-symmetric): the field mirrors in x.
+    order = np.lexsort((-px, py))  # pillar that sits at (-x, y)
+    tm = t1[order]
+    em = oracle.forward(px, py, tm[None], [plane]).reshape(9, 9)
+    assert rel_l2(em, e1.reshape(9, 9)[:, ::-1]) <= 1e-12
+
+
+def test_adjoint_dot_product_and_columns():
+    px, py = configs.square_grid(6, 0.4)
+    k = 2 * math.pi / 0.5
+    planes = [Plane.psf(0, 0, 8.0, k, 5, 1.0), Plane.zx(-1, 1, 4, 6, 10, 3, k)]
+    npix = 25 + 12
+    rng = np.random.default_rng(4)
+    u = rng.standard_normal(npix) + 1j * rng.standard_normal(npix)
+    g = oracle.adjoint(px, py, u, planes)[0]
+    # column q of G is the field of a unit transmission at pillar q
+    for q in (0, 7, 35):
+        e = np.zeros(px.size, complex)
+        e[q] = 1.0
+        col = oracle.forward(px, py, e[None], planes)
+        assert abs(g[q] - np.vdot(col, u)) <= 1e-12 * np.abs(g).max()
+    t = rng.standard_normal(px.size) + 1j * rng.standard_normal(px.size)
+    lhs = np.vdot(u, oracle.forward(px, py, t[None], planes))
+    rhs = np.vdot(g, t)
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def test_adjoint_sets_follow_planes():
+    px, py = configs.square_grid(5, 0.4)
+    k = 2 * math.pi / 0.5
+    planes = [Plane.psf(0, 0, 8.0, k, 3, 1.0, tset=1), Plane.psf(0, 0, 9.0, k, 3, 1.0, tset=1)]
+    u = np.ones(18, complex)
+    g = oracle.adjoint(px, py, u, planes, nsets=2)
+    assert np.all(g[0] == 0) and np.any(g[1] != 0)
+    g_a = oracle.adjoint(px, py, u[:9], planes[:1], nsets=2)[1]
+    g_b = oracle.adjoint(px, py, u[9:], [Plane.psf(0, 0, 9.0, k, 3, 1.0, tset=1)], nsets=2)[1]
+    assert rel_l2(g[1], g_a + g_b) <= 1e-14
+
+
+def test_row_stride_sample_matches_full():
+    L = configs.small_lens(n_side=10, n=9)
+    full = oracle.forward(L.px, L.py, L.t, L.planes)
+    part = oracle.forward(L.px, L.py, L.t, L.planes, row_stride=4)
+    rows = np.zeros((9, 9), bool)
+    rows[::4] = True
+    assert np.array_equal(part[rows.ravel()], full[rows.ravel()])
+    assert np.all(part[~rows.ravel()] == 0)
