@@ -69,6 +69,14 @@ SIGNATURES = [
      [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_int,
       POINTER(crs_plane), c_int, c_void_p, c_void_p]),
     ("crs_set_tuning", c_int, [c_void_p, c_int, c_int, c_int]),
+    ("crs_graph_forward", c_int,
+     [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_int, POINTER(crs_plane), c_int,
+      c_void_p, c_void_p, POINTER(c_void_p)]),
+    ("crs_graph_adjoint", c_int,
+     [c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, POINTER(crs_plane), c_int, c_void_p,
+      c_void_p, POINTER(c_void_p)]),
+    ("crs_graph_launch", c_int, [c_void_p, c_void_p]),
+    ("crs_graph_destroy", None, [c_void_p]),
     ("crs_set_timing", c_int, [c_void_p, c_int]),
     ("crs_last_kernel_ms", c_int, [c_void_p, c_int, POINTER(ctypes.c_float)]),
     ("crs_kernel_name", c_char_p, [c_int, c_int]),

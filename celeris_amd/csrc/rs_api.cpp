@@ -244,7 +244,12 @@ DevPlane make_dev_plane(const crs_plane& P, int px_per_thread, int py_per_thread
     return d;
 }
 
-size_t pillar_stride_bytes(int mode) { return mode == CRS_MODE_FP64 ? sizeof(double4) : sizeof(float4); }
+// Bytes of staged pillar data: FP32C keeps FP64 positions once plus {|t|, arg t} per
+// set; FP32 and FP64 keep one {x, y, Re t, Im t} record per pillar and set.
+size_t staged_pillar_bytes(int mode, int64_t npil, int64_t nsets) {
+    if (mode == CRS_MODE_FP32C) return npil * sizeof(double2) + npil * nsets * sizeof(float2);
+    return npil * nsets * (mode == CRS_MODE_FP64 ? sizeof(double4) : sizeof(float4));
+}
 
 crs_status launch_status(hipError_t e, const char* what) {
     if (e == hipSuccess) return CRS_OK;
@@ -353,20 +358,34 @@ extern "C" const char* crs_kernel_name(int mode, int adjoint) {
     }
 }
 
-extern "C" crs_status crs_forward(crs_context* ctx, void* stream_, int mode, const double* px,
-                                  const double* py, const double* t, int npil, int nsets,
-                                  const crs_plane* planes, int nplanes, double* field,
-                                  double* intensity) {
-    g_last_error.clear();
-    crs_status st = check_common(ctx, mode, npil, nsets, planes, nplanes);
-    if (st != CRS_OK) return st;
-    if (!px || !py || !t) { set_error("null pillar array"); return CRS_ERR_INVALID; }
-    if (nplanes == 0 || (!field && !intensity)) return CRS_OK;
-    hipStream_t stream = static_cast<hipStream_t>(stream_);
+// ------------------------------------------------------------------------------------
+// Planning: host-built descriptor tables for one forward / adjoint call.
 
+namespace {
+
+struct Scratch {  // where one call's device data lives (a context's or a graph's)
+    DevBuffer* pil;
+    DevBuffer* pix;
+    DevBuffer* partial;
+    unsigned char* tables;  // device copy of the plan's table bytes
+};
+
+struct FwdPlan {
+    int mode = 0;
+    int npil = 0, nsets = 0;
+    int64_t part_elems = 0;
+    int nblocks = 0, npblocks = 0;
+    TableBuilder tb;
+    size_t o_planes = 0, o_blocks = 0, o_pblocks = 0;
+};
+
+crs_status plan_forward(const crs_context* ctx, int mode, int npil, int nsets,
+                        const crs_plane* planes, int nplanes, FwdPlan& plan) {
     const int PX = mode == CRS_MODE_FP32C ? kPX : 1;
     const int PY = mode == CRS_MODE_FP32C ? kPY : 1;
-
+    plan.mode = mode;
+    plan.npil = npil;
+    plan.nsets = nsets;
     // Split over pillar chunks: enough work-groups to fill the chip, chunks of at
     // least one LDS tile.
     std::vector<DevPlane> dp(nplanes);
@@ -384,7 +403,6 @@ extern "C" crs_status crs_forward(crs_context* ctx, void* stream_, int mode, con
         set_error("launch too large (%lld work-groups)", (long long)(base_blocks * nsplit));
         return CRS_ERR_INVALID;
     }
-
     int64_t part = 0;
     for (int p = 0; p < nplanes; ++p) {
         dp[p].part = part;
@@ -408,52 +426,64 @@ extern "C" crs_status crs_forward(crs_context* ctx, void* stream_, int mode, con
         const int64_t npix = static_cast<int64_t>(dp[p].nx) * dp[p].ny;
         for (int64_t b = 0; b < npix; b += kThreads) pblocks.push_back({p, static_cast<int32_t>(b)});
     }
+    plan.part_elems = part;
+    plan.nblocks = static_cast<int>(blocks.size());
+    plan.npblocks = static_cast<int>(pblocks.size());
+    plan.o_planes = plan.tb.add(dp.data(), dp.size() * sizeof(DevPlane));
+    plan.o_blocks = plan.tb.add(blocks.data(), blocks.size() * sizeof(FwdBlock));
+    plan.o_pblocks = plan.tb.add(pblocks.data(), pblocks.size() * sizeof(PixBlock));
+    return CRS_OK;
+}
 
-    TableBuilder tb;
-    const size_t o_planes = tb.add(dp.data(), dp.size() * sizeof(DevPlane));
-    const size_t o_blocks = tb.add(blocks.data(), blocks.size() * sizeof(FwdBlock));
-    const size_t o_pblocks = tb.add(pblocks.data(), pblocks.size() * sizeof(PixBlock));
+crs_status size_forward(const FwdPlan& plan, const Scratch& sc, hipStream_t stream) {
+    crs_status st = ensure(*sc.pil, staged_pillar_bytes(plan.mode, plan.npil, plan.nsets), stream,
+                           "staged pillars");
+    if (st != CRS_OK) return st;
+    return ensure(*sc.partial, static_cast<size_t>(plan.part_elems) * sizeof(double2), stream,
+                  "forward partials");
+}
 
-    if ((st = ensure(ctx->pil, static_cast<size_t>(npil) * nsets * pillar_stride_bytes(mode), stream,
-                     "staged pillars")) != CRS_OK)
-        return st;
-    if ((st = ensure(ctx->partial, static_cast<size_t>(part) * sizeof(double2), stream,
-                     "forward partials")) != CRS_OK)
-        return st;
-    if ((st = upload_tables(ctx, tb, stream)) != CRS_OK) return st;
-    if (inject("launch")) { set_error("injected launch failure"); return CRS_ERR_LAUNCH; }
-
-    const DevPlane* d_planes = dev_at<DevPlane>(ctx, o_planes);
-    if ((st = launch_status(launch_stage_pillars(mode, px, py, t, npil, nsets, ctx->pil.ptr, stream),
+crs_status enqueue_forward(const FwdPlan& plan, const Scratch& sc, const double* px,
+                           const double* py, const double* t, double* field, double* intensity,
+                           hipStream_t stream, hipEvent_t ev0, hipEvent_t ev1) {
+    crs_status st;
+    const int mode = plan.mode;
+    const DevPlane* d_planes = reinterpret_cast<const DevPlane*>(sc.tables + plan.o_planes);
+    if ((st = launch_status(launch_stage_pillars(mode, px, py, t, plan.npil, plan.nsets,
+                                                 sc.pil->ptr, stream),
                             "stage pillars")) != CRS_OK)
         return st;
-    const bool timed = ctx->timing && !stream_capturing(stream);
-    if (timed) hipEventRecord(ctx->ev[0][0], stream);
-    if ((st = launch_status(launch_forward(mode, ctx->pil.ptr, npil, d_planes,
-                                           dev_at<FwdBlock>(ctx, o_blocks),
-                                           static_cast<int>(blocks.size()),
-                                           static_cast<double2*>(ctx->partial.ptr), stream),
+    if (ev0) hipEventRecord(ev0, stream);
+    if ((st = launch_status(launch_forward(mode, sc.pil->ptr, plan.npil, d_planes,
+                                           reinterpret_cast<const FwdBlock*>(sc.tables + plan.o_blocks),
+                                           plan.nblocks, static_cast<double2*>(sc.partial->ptr),
+                                           stream),
                             "forward")) != CRS_OK)
         return st;
-    if (timed) { hipEventRecord(ctx->ev[0][1], stream); ctx->ev_valid[0] = true; }
+    if (ev1) hipEventRecord(ev1, stream);
     return launch_status(launch_fwd_reduce(mode == CRS_MODE_FP32C ? 1 : 0, d_planes,
-                                           dev_at<PixBlock>(ctx, o_pblocks),
-                                           static_cast<int>(pblocks.size()),
-                                           static_cast<const double2*>(ctx->partial.ptr),
+                                           reinterpret_cast<const PixBlock*>(sc.tables + plan.o_pblocks),
+                                           plan.npblocks,
+                                           static_cast<const double2*>(sc.partial->ptr),
                                            reinterpret_cast<double2*>(field), intensity, stream),
                          "forward reduce");
 }
 
-extern "C" crs_status crs_adjoint(crs_context* ctx, void* stream_, int mode, const double* px,
-                                  const double* py, int npil, int nsets, const crs_plane* planes,
-                                  int nplanes, const double* u, double* g) {
-    g_last_error.clear();
-    crs_status st = check_common(ctx, mode, npil, nsets, planes, nplanes);
-    if (st != CRS_OK) return st;
-    if (!px || !py || !g || (nplanes > 0 && !u)) { set_error("null array argument"); return CRS_ERR_INVALID; }
-    hipStream_t stream = static_cast<hipStream_t>(stream_);
+struct AdjPlan {
+    int mode = 0;
+    int npil = 0, nsets = 0;
+    int64_t nsplit = 1, n_out = 0, total_pix = 0;
+    int nblocks = 0, npblocks = 0;
+    TableBuilder tb;
+    size_t o_planes = 0, o_rows = 0, o_blocks = 0, o_pblocks = 0;
+};
 
+crs_status plan_adjoint(const crs_context* ctx, int mode, int npil, int nsets,
+                        const crs_plane* planes, int nplanes, AdjPlan& plan) {
     const int PQ = mode == CRS_MODE_FP32C ? kPQ : 1;
+    plan.mode = mode;
+    plan.npil = npil;
+    plan.nsets = nsets;
     std::vector<DevPlane> dp(nplanes);
     for (int p = 0; p < nplanes; ++p) dp[p] = make_dev_plane(planes[p], 1, 1);
 
@@ -514,58 +544,255 @@ extern "C" crs_status crs_adjoint(crs_context* ctx, void* stream_, int mode, con
         total_pix = std::max(total_pix, dp[p].offset + npix);
         for (int64_t b = 0; b < npix; b += kThreads) pblocks.push_back({p, static_cast<int32_t>(b)});
     }
+    plan.nsplit = nsplit;
+    plan.n_out = n_out;
+    plan.total_pix = total_pix;
+    plan.nblocks = static_cast<int>(blocks.size());
+    plan.npblocks = static_cast<int>(pblocks.size());
+    plan.o_planes = plan.tb.add(dp.data(), dp.size() * sizeof(DevPlane));
+    plan.o_rows = plan.tb.add(rows.data(), rows.size() * sizeof(AdjRow));
+    plan.o_blocks = plan.tb.add(blocks.data(), blocks.size() * sizeof(AdjBlock));
+    plan.o_pblocks = plan.tb.add(pblocks.data(), pblocks.size() * sizeof(PixBlock));
+    return CRS_OK;
+}
 
-    TableBuilder tb;
-    const size_t o_planes = tb.add(dp.data(), dp.size() * sizeof(DevPlane));
-    const size_t o_rows = tb.add(rows.data(), rows.size() * sizeof(AdjRow));
-    const size_t o_blocks = tb.add(blocks.data(), blocks.size() * sizeof(AdjBlock));
-    const size_t o_pblocks = tb.add(pblocks.data(), pblocks.size() * sizeof(PixBlock));
-
-    // Positions only: stage set 0 (the adjoint pair loop reads x, y).
-    if ((st = ensure(ctx->pil, static_cast<size_t>(npil) * pillar_stride_bytes(mode), stream,
+crs_status size_adjoint(const AdjPlan& plan, const Scratch& sc, hipStream_t stream) {
+    crs_status st;
+    // Positions only: the adjoint pair loop reads x and y.
+    if ((st = ensure(*sc.pil, staged_pillar_bytes(plan.mode, plan.npil, 1), stream,
                      "staged pillars")) != CRS_OK)
         return st;
-    if (mode == CRS_MODE_FP32C &&
-        (st = ensure(ctx->pix, static_cast<size_t>(total_pix) * sizeof(float4), stream,
+    if (plan.mode == CRS_MODE_FP32C &&
+        (st = ensure(*sc.pix, static_cast<size_t>(plan.total_pix) * sizeof(float4), stream,
                      "staged pixels")) != CRS_OK)
         return st;
-    if ((st = ensure(ctx->partial, static_cast<size_t>(nsplit * n_out) * sizeof(double2), stream,
-                     "adjoint partials")) != CRS_OK)
-        return st;
-    if ((st = upload_tables(ctx, tb, stream)) != CRS_OK) return st;
-    if (inject("launch")) { set_error("injected launch failure"); return CRS_ERR_LAUNCH; }
+    return ensure(*sc.partial, static_cast<size_t>(plan.nsplit * plan.n_out) * sizeof(double2),
+                  stream, "adjoint partials");
+}
 
-    // Positions only (t = NULL stages zero transmissions): the adjoint pair loop
-    // reads x and y.
-    const DevPlane* d_planes = dev_at<DevPlane>(ctx, o_planes);
-    if ((st = launch_status(launch_stage_pillars(mode == CRS_MODE_FP64 ? 1 : 2, px, py, nullptr,
-                                                 npil, 1, ctx->pil.ptr, stream),
+crs_status enqueue_adjoint(const AdjPlan& plan, const Scratch& sc, const double* px,
+                           const double* py, const double* u, double* g, hipStream_t stream,
+                           hipEvent_t ev0, hipEvent_t ev1) {
+    crs_status st;
+    const int mode = plan.mode;
+    const DevPlane* d_planes = reinterpret_cast<const DevPlane*>(sc.tables + plan.o_planes);
+    // t = NULL stages positions with zero transmissions.
+    if ((st = launch_status(launch_stage_pillars(mode, px, py, nullptr, plan.npil, 1, sc.pil->ptr,
+                                                 stream),
                             "stage positions")) != CRS_OK)
         return st;
     const void* pix_or_u = u;
     if (mode == CRS_MODE_FP32C) {
-        if ((st = launch_status(launch_stage_pixels(d_planes, dev_at<PixBlock>(ctx, o_pblocks),
-                                                    static_cast<int>(pblocks.size()),
-                                                    reinterpret_cast<const double2*>(u),
-                                                    static_cast<float4*>(ctx->pix.ptr), stream),
-                                "stage pixels")) != CRS_OK)
+        if ((st = launch_status(
+                 launch_stage_pixels(d_planes,
+                                     reinterpret_cast<const PixBlock*>(sc.tables + plan.o_pblocks),
+                                     plan.npblocks, reinterpret_cast<const double2*>(u),
+                                     static_cast<float4*>(sc.pix->ptr), stream),
+                 "stage pixels")) != CRS_OK)
             return st;
-        pix_or_u = ctx->pix.ptr;
+        pix_or_u = sc.pix->ptr;
     }
-    const bool timed = ctx->timing && !stream_capturing(stream);
-    if (timed) hipEventRecord(ctx->ev[1][0], stream);
-    if ((st = launch_status(launch_adjoint(mode, ctx->pil.ptr, npil, d_planes,
-                                           dev_at<AdjRow>(ctx, o_rows),
-                                           dev_at<AdjBlock>(ctx, o_blocks),
-                                           static_cast<int>(blocks.size()), pix_or_u,
-                                           static_cast<double2*>(ctx->partial.ptr), stream),
+    if (ev0) hipEventRecord(ev0, stream);
+    if ((st = launch_status(launch_adjoint(mode, sc.pil->ptr, plan.npil, d_planes,
+                                           reinterpret_cast<const AdjRow*>(sc.tables + plan.o_rows),
+                                           reinterpret_cast<const AdjBlock*>(sc.tables + plan.o_blocks),
+                                           plan.nblocks, pix_or_u,
+                                           static_cast<double2*>(sc.partial->ptr), stream),
                             "adjoint")) != CRS_OK)
         return st;
-    if (timed) { hipEventRecord(ctx->ev[1][1], stream); ctx->ev_valid[1] = true; }
-    return launch_status(launch_adj_reduce(static_cast<const double2*>(ctx->partial.ptr),
-                                           static_cast<int>(nsplit), n_out,
+    if (ev1) hipEventRecord(ev1, stream);
+    return launch_status(launch_adj_reduce(static_cast<const double2*>(sc.partial->ptr),
+                                           static_cast<int>(plan.nsplit), plan.n_out,
                                            reinterpret_cast<double2*>(g), stream),
                          "adjoint reduce");
+}
+
+}  // namespace
+
+extern "C" crs_status crs_forward(crs_context* ctx, void* stream_, int mode, const double* px,
+                                  const double* py, const double* t, int npil, int nsets,
+                                  const crs_plane* planes, int nplanes, double* field,
+                                  double* intensity) {
+    g_last_error.clear();
+    crs_status st = check_common(ctx, mode, npil, nsets, planes, nplanes);
+    if (st != CRS_OK) return st;
+    if (!px || !py || !t) { set_error("null pillar array"); return CRS_ERR_INVALID; }
+    if (nplanes == 0 || (!field && !intensity)) return CRS_OK;
+    hipStream_t stream = static_cast<hipStream_t>(stream_);
+    FwdPlan plan;
+    if ((st = plan_forward(ctx, mode, npil, nsets, planes, nplanes, plan)) != CRS_OK) return st;
+    Scratch sc{&ctx->pil, &ctx->pix, &ctx->partial, nullptr};
+    if ((st = size_forward(plan, sc, stream)) != CRS_OK) return st;
+    if ((st = upload_tables(ctx, plan.tb, stream)) != CRS_OK) return st;
+    if (inject("launch")) { set_error("injected launch failure"); return CRS_ERR_LAUNCH; }
+    sc.tables = static_cast<unsigned char*>(ctx->tables.ptr);
+    const bool timed = ctx->timing && !stream_capturing(stream);
+    st = enqueue_forward(plan, sc, px, py, t, field, intensity, stream,
+                         timed ? ctx->ev[0][0] : nullptr, timed ? ctx->ev[0][1] : nullptr);
+    if (st == CRS_OK && timed) ctx->ev_valid[0] = true;
+    return st;
+}
+
+extern "C" crs_status crs_adjoint(crs_context* ctx, void* stream_, int mode, const double* px,
+                                  const double* py, int npil, int nsets, const crs_plane* planes,
+                                  int nplanes, const double* u, double* g) {
+    g_last_error.clear();
+    crs_status st = check_common(ctx, mode, npil, nsets, planes, nplanes);
+    if (st != CRS_OK) return st;
+    if (!px || !py || !g || (nplanes > 0 && !u)) { set_error("null array argument"); return CRS_ERR_INVALID; }
+    hipStream_t stream = static_cast<hipStream_t>(stream_);
+    AdjPlan plan;
+    if ((st = plan_adjoint(ctx, mode, npil, nsets, planes, nplanes, plan)) != CRS_OK) return st;
+    Scratch sc{&ctx->pil, &ctx->pix, &ctx->partial, nullptr};
+    if ((st = size_adjoint(plan, sc, stream)) != CRS_OK) return st;
+    if ((st = upload_tables(ctx, plan.tb, stream)) != CRS_OK) return st;
+    if (inject("launch")) { set_error("injected launch failure"); return CRS_ERR_LAUNCH; }
+    sc.tables = static_cast<unsigned char*>(ctx->tables.ptr);
+    const bool timed = ctx->timing && !stream_capturing(stream);
+    st = enqueue_adjoint(plan, sc, px, py, u, g, stream, timed ? ctx->ev[1][0] : nullptr,
+                         timed ? ctx->ev[1][1] : nullptr);
+    if (st == CRS_OK && timed) ctx->ev_valid[1] = true;
+    return st;
+}
+
+// ------------------------------------------------------------------------------------
+// HIP graphs: one forward or adjoint, captured with scratch the graph owns.
+
+struct crs_graph {
+    int device = 0;
+    DevBuffer pil, pix, tables, partial;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+template <class Plan, class SizeFn, class EnqueueFn>
+crs_status build_graph(crs_context* ctx, Plan& plan, SizeFn size_fn, EnqueueFn enqueue_fn,
+                       crs_graph** out) {
+    crs_graph* gr = new crs_graph();
+    gr->device = ctx->device;
+    Scratch sc{&gr->pil, &gr->pix, &gr->partial, nullptr};
+    hipStream_t cap = nullptr;
+    crs_status st = CRS_OK;
+    auto fail = [&](crs_status s) {
+        if (cap) hipStreamDestroy(cap);
+        if (gr->exec) hipGraphExecDestroy(gr->exec);
+        if (gr->graph) hipGraphDestroy(gr->graph);
+        for (DevBuffer* b : {&gr->pil, &gr->pix, &gr->tables, &gr->partial})
+            if (b->ptr) hipFree(b->ptr);
+        delete gr;
+        return s;
+    };
+    if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) {
+        hipGetLastError();
+        set_error("hipStreamCreate failed");
+        return fail(CRS_ERR_LAUNCH);
+    }
+    if ((st = size_fn(plan, sc, cap)) != CRS_OK) return fail(st);
+    const size_t nt = plan.tb.bytes.size();
+    if ((st = ensure(gr->tables, nt, cap, "graph tables")) != CRS_OK) return fail(st);
+    // Tables are copied once, before capture: the graph holds kernel nodes only.
+    if (hipMemcpy(gr->tables.ptr, plan.tb.bytes.data(), nt, hipMemcpyHostToDevice) != hipSuccess) {
+        hipGetLastError();
+        set_error("graph tables: hipMemcpy failed");
+        return fail(CRS_ERR_LAUNCH);
+    }
+    sc.tables = static_cast<unsigned char*>(gr->tables.ptr);
+    if (inject("launch")) { set_error("injected launch failure"); return fail(CRS_ERR_LAUNCH); }
+    if (hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        hipGetLastError();
+        set_error("hipStreamBeginCapture failed");
+        return fail(CRS_ERR_LAUNCH);
+    }
+    st = enqueue_fn(plan, sc, cap);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(cap, &g);
+    gr->graph = g;
+    if (st != CRS_OK) return fail(st);
+    if (ec != hipSuccess || !g) {
+        hipGetLastError();
+        set_error("hipStreamEndCapture failed");
+        return fail(CRS_ERR_LAUNCH);
+    }
+    if (hipGraphInstantiate(&gr->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        hipGetLastError();
+        set_error("hipGraphInstantiate failed");
+        return fail(CRS_ERR_LAUNCH);
+    }
+    hipStreamDestroy(cap);
+    *out = gr;
+    return CRS_OK;
+}
+
+}  // namespace
+
+extern "C" crs_status crs_graph_forward(crs_context* ctx, int mode, const double* px,
+                                        const double* py, const double* t, int npil, int nsets,
+                                        const crs_plane* planes, int nplanes, double* field,
+                                        double* intensity, crs_graph** out) {
+    g_last_error.clear();
+    if (!out) { set_error("null output pointer"); return CRS_ERR_INVALID; }
+    *out = nullptr;
+    crs_status st = check_common(ctx, mode, npil, nsets, planes, nplanes);
+    if (st != CRS_OK) return st;
+    if (!px || !py || !t || nplanes == 0 || (!field && !intensity)) {
+        set_error("graph needs pillars, at least one plane and an output");
+        return CRS_ERR_INVALID;
+    }
+    FwdPlan plan;
+    if ((st = plan_forward(ctx, mode, npil, nsets, planes, nplanes, plan)) != CRS_OK) return st;
+    return build_graph(
+        ctx, plan, size_forward,
+        [&](const FwdPlan& p, const Scratch& sc, hipStream_t s) {
+            return enqueue_forward(p, sc, px, py, t, field, intensity, s, nullptr, nullptr);
+        },
+        out);
+}
+
+extern "C" crs_status crs_graph_adjoint(crs_context* ctx, int mode, const double* px,
+                                        const double* py, int npil, int nsets,
+                                        const crs_plane* planes, int nplanes, const double* u,
+                                        double* g, crs_graph** out) {
+    g_last_error.clear();
+    if (!out) { set_error("null output pointer"); return CRS_ERR_INVALID; }
+    *out = nullptr;
+    crs_status st = check_common(ctx, mode, npil, nsets, planes, nplanes);
+    if (st != CRS_OK) return st;
+    if (!px || !py || !g || !u || nplanes == 0) {
+        set_error("graph needs pillars, at least one plane, u and g");
+        return CRS_ERR_INVALID;
+    }
+    AdjPlan plan;
+    if ((st = plan_adjoint(ctx, mode, npil, nsets, planes, nplanes, plan)) != CRS_OK) return st;
+    return build_graph(
+        ctx, plan, size_adjoint,
+        [&](const AdjPlan& p, const Scratch& sc, hipStream_t s) {
+            return enqueue_adjoint(p, sc, px, py, u, g, s, nullptr, nullptr);
+        },
+        out);
+}
+
+extern "C" crs_status crs_graph_launch(crs_graph* g, void* stream) {
+    if (!g || !g->exec) { set_error("null graph"); return CRS_ERR_INVALID; }
+    if (hipSetDevice(g->device) != hipSuccess ||
+        hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)) != hipSuccess) {
+        hipGetLastError();
+        set_error("hipGraphLaunch failed");
+        return CRS_ERR_LAUNCH;
+    }
+    return CRS_OK;
+}
+
+extern "C" void crs_graph_destroy(crs_graph* g) {
+    if (!g) return;
+    hipSetDevice(g->device);
+    hipDeviceSynchronize();
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    for (DevBuffer* b : {&g->pil, &g->pix, &g->tables, &g->partial})
+        if (b->ptr) hipFree(b->ptr);
+    delete g;
 }
 
 // ------------------------------------------------------------------------------------

@@ -14,16 +14,21 @@
 // writes a float64 partial and a second kernel adds the partials in a fixed order,
 // so results are bitwise repeatable run to run.
 //
-// FP32C ("FP32-compensated") pair arithmetic, per pillar-pixel pair:
-//   R2  = dx^2 + [(fy-y)^2 + z^2]          z^2 carried as a float hi+lo pair
-//   iR  = rsq(R2), R = R2*iR               v_rsq_f32
-//   d   = (R - z_hi) - (R*R - R2)/(2R)     one Newton step for R, so that d = R - z_hi
-//                                          is good to a few float ulps of d itself
-//   f   = fract(k/(2 pi) * d)              FP64: k*(R - z) reduced mod 2 pi, in turns
-//   th  = f + arg(t)/(2 pi)                transmission phase folded into the angle
-//   E  += |t|*iR * (cos 2pi th, sin 2pi th)   v_cos_f32 / v_sin_f32 take turns
-// The dropped factor exp(i k z_hi) is applied in FP64 by the reduction kernel.  Each
-// pillar tile's float partial is added into a Kahan-compensated running sum.
+// FP32C ("FP32-compensated") pair arithmetic.  Every phase-carrying coordinate is
+// formed in FP64 and scaled to turns, x' = k x / (2 pi) (likewise y, z, fx, fy), then
+// rounded to float once; z'^2 is carried as a float hi+lo pair.  Per pillar-pixel pair:
+//   R2  = dx'^2 + [(fy'-y')^2 + z'^2]       R' = sqrt(R2) is k R / (2 pi): the phase in turns
+//   iR  = rsq(R2), R = R2*iR                 v_rsq_f32
+//   th  = (R - (z'_hi - arg t/2pi)) - (R*R - R2)/(2R)
+//                                            one Newton step for R (removes any bias of
+//                                            the hardware root), with z'_hi subtracted
+//                                            exactly (Sterbenz), so th = k(R - z)/2pi + arg t/2pi
+//                                            carries float rounding of th itself, not of kR
+//   th  = fract(th)                          v_fract_f32, exact
+//   E  += |t| k/(2 pi) * iR * (cos 2pi th, sin 2pi th)   v_cos_f32 / v_sin_f32 take turns
+// The dropped factor exp(2 pi i z'_hi) is applied in FP64 by the reduction kernel.  Each
+// pillar tile's float partial is added into a Kahan-compensated running sum, and the
+// partials of the pillar-chunk split are added in FP64.
 #include "rs_internal.h"
 
 #include <hip/hip_runtime.h>
@@ -35,26 +40,26 @@ namespace crs {
 namespace {
 
 constexpr double kInv2Pi = 0.15915494309189533577;  // 1 / (2 pi)
+constexpr double kTwoPi = 6.28318530717958647693;
 
-// z^2 in FP64, split into the float nearest it and the float nearest the rest.
-__device__ inline void split_z2(double z, float& hi, float& lo) {
-    const double z2 = z * z;
-    hi = static_cast<float>(z2);
-    lo = static_cast<float>(z2 - static_cast<double>(hi));
+// w^2 in FP64, split into the float nearest it and the float nearest the rest.
+__device__ inline void split_sq(double w, float& hi, float& lo) {
+    const double w2 = w * w;
+    hi = static_cast<float>(w2);
+    lo = static_cast<float>(w2 - static_cast<double>(hi));
 }
 
-// Fractional part of an FP64 phase in turns, as a float in [0, 1).
-__device__ inline float frac_turns(double ph) {
-    return static_cast<float>(ph - floor(ph));
-}
-
-// d = R - zh for R = sqrt(R2), R2 = a float, with one Newton correction of the
-// hardware rsq-based root; also returns 1/R.
-__device__ inline float path_excess(float R2, float zh, float& iR) {
+// Phase (turns) of the pair: R - zeta with one Newton correction of R = sqrt(R2);
+// also returns 1/R.  `sign` = +1 gives (R_true - zeta), -1 gives (zeta - R_true).
+template <int SIGN>
+__device__ inline float pair_turns(float R2, float zeta, float& iR) {
     iR = __builtin_amdgcn_rsqf(R2);
     const float R = R2 * iR;
     const float resh = 0.5f * fmaf(R, R, -R2);
-    return fmaf(-resh, iR, R - zh);
+    if constexpr (SIGN > 0)
+        return fmaf(-resh, iR, R - zeta);
+    else
+        return fmaf(resh, iR, zeta - R);
 }
 
 }  // namespace
@@ -62,17 +67,20 @@ __device__ inline float path_excess(float R2, float zh, float& iR) {
 // ---------------------------------------------------------------------------------
 // Staging: float64 pillar lists -> the layout each mode's pair loop reads.
 
-// FP32C: {x, y, |t|, arg(t)/(2 pi)} per pillar, set-major.
+// FP32C: positions stay FP64 (pos[npil], shared by all sets; each plane scales them
+// by its own k/(2 pi) when it loads a tile) and {|t|, arg(t)/(2 pi)} per pillar and set.
 __global__ void stage_pillars_fp32c(const double* __restrict__ px, const double* __restrict__ py,
                                     const double2* __restrict__ t, int npil, int64_t total,
-                                    float4* __restrict__ out) {
+                                    double2* __restrict__ pos, float2* __restrict__ ap) {
     const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= total) return;
     const int q = static_cast<int>(idx % npil);
-    const double2 v = t[idx];
-    out[idx] = make_float4(static_cast<float>(px[q]), static_cast<float>(py[q]),
-                           static_cast<float>(hypot(v.x, v.y)),
-                           static_cast<float>(atan2(v.y, v.x) * kInv2Pi));
+    if (idx < npil) pos[q] = make_double2(px[q], py[q]);
+    if (t) {
+        const double2 v = t[idx];
+        ap[idx] = make_float2(static_cast<float>(hypot(v.x, v.y)),
+                              static_cast<float>(atan2(v.y, v.x) * kInv2Pi));
+    }
 }
 
 // FP32: {x, y, Re t, Im t} in float (the reference's host staging, propagate.cu:145-152).
@@ -100,12 +108,14 @@ __global__ void stage_pillars_fp64(const double* __restrict__ px, const double* 
 }
 
 // ---------------------------------------------------------------------------------
-// Forward, FP32C.  Thread owns a PX x PY pixel block; pillars stream through LDS.
+// Forward, FP32C.  Thread owns a PX x PY pixel block; pillars stream through LDS,
+// scaled to turns by the plane's k/(2 pi) as they are loaded.
 
 template <int PX, int PY>
 __global__ __launch_bounds__(kThreads, 2) void fwd_fp32c(
-    const float4* __restrict__ pil, int npil, const DevPlane* __restrict__ planes,
-    const FwdBlock* __restrict__ blocks, double2* __restrict__ partial) {
+    const double2* __restrict__ pos, const float2* __restrict__ ap, int npil,
+    const DevPlane* __restrict__ planes, const FwdBlock* __restrict__ blocks,
+    double2* __restrict__ partial) {
     __shared__ float4 tile[kThreads];
     const FwdBlock B = blocks[blockIdx.x];
     const DevPlane& P = planes[B.plane];
@@ -115,24 +125,25 @@ __global__ __launch_bounds__(kThreads, 2) void fwd_fp32c(
     const bool active = my_tile < P.tx * P.ty;
     const int tt = active ? my_tile : 0;
     const int ti = tt % P.tx, tj = tt / P.tx;
+    const double kt = P.kturn;
 
     float fx[PX];
 #pragma unroll
     for (int c = 0; c < PX; ++c) {
         const int i = min(ti * PX + c, nx - 1);
-        fx[c] = static_cast<float>(P.x0 + static_cast<double>(i) * P.sx);
+        fx[c] = static_cast<float>(kt * (P.x0 + static_cast<double>(i) * P.sx));
     }
     float fy[PY], zh[PY], z2h[PY], z2l[PY];
 #pragma unroll
     for (int r = 0; r < PY; ++r) {
         const int j = min(tj * PY + r, ny - 1);
-        fy[r] = static_cast<float>(P.y0 + static_cast<double>(j) * P.sy);
-        const double z = P.z0 + static_cast<double>(j) * P.sz;
-        zh[r] = static_cast<float>(z);
-        split_z2(z, z2h[r], z2l[r]);
+        fy[r] = static_cast<float>(kt * (P.y0 + static_cast<double>(j) * P.sy));
+        const double zt = kt * (P.z0 + static_cast<double>(j) * P.sz);
+        zh[r] = static_cast<float>(zt);
+        split_sq(zt, z2h[r], z2l[r]);
     }
-    const double kt = P.kturn;
-    const float4* __restrict__ src = pil + static_cast<int64_t>(P.tset) * npil;
+    const float amp_scale = static_cast<float>(kt);  // 1/R = (k/2pi) / R'
+    const float2* __restrict__ aps = ap + static_cast<int64_t>(P.tset) * npil;
 
     float sr[PX * PY], si[PX * PY], cr[PX * PY], ci[PX * PY];
 #pragma unroll
@@ -141,7 +152,12 @@ __global__ __launch_bounds__(kThreads, 2) void fwd_fp32c(
     for (int base = B.q0; base < B.q1; base += kThreads) {
         const int n = min(kThreads, B.q1 - base);
         __syncthreads();
-        if (tid < n) tile[tid] = src[base + tid];
+        if (tid < n) {
+            const double2 xy = pos[base + tid];
+            const float2 a = aps[base + tid];
+            tile[tid] = make_float4(static_cast<float>(kt * xy.x), static_cast<float>(kt * xy.y),
+                                    a.x * amp_scale, a.y);
+        }
         __syncthreads();
         float ar[PX * PY], ai[PX * PY];
 #pragma unroll
@@ -155,12 +171,12 @@ __global__ __launch_bounds__(kThreads, 2) void fwd_fp32c(
             for (int r = 0; r < PY; ++r) {
                 const float dy = fy[r] - Q.y;
                 const float dyz2 = fmaf(dy, dy, z2l[r]) + z2h[r];
+                const float zeta = zh[r] - Q.w;
 #pragma unroll
                 for (int c = 0; c < PX; ++c) {
                     const float R2 = fmaf(dx[c], dx[c], dyz2);
                     float iR;
-                    const float d = path_excess(R2, zh[r], iR);
-                    const float th = frac_turns(static_cast<double>(d) * kt) + Q.w;
+                    const float th = __builtin_amdgcn_fractf(pair_turns<1>(R2, zeta, iR));
                     const float A = Q.z * iR;
                     const int e = r * PX + c;
                     ar[e] = fmaf(A, __builtin_amdgcn_cosf(th), ar[e]);
@@ -284,7 +300,7 @@ __global__ __launch_bounds__(kThreads) void fwd_fp64(
 
 // ---------------------------------------------------------------------------------
 // Forward reduction: add the plane's pillar-chunk partials in slot order, restore
-// the factor exp(i k z_hi) that FP32C leaves out, write field and/or |E|^2.
+// the factor exp(2 pi i z'_hi) that FP32C leaves out, write field and/or |E|^2.
 
 __global__ __launch_bounds__(kThreads) void fwd_reduce(
     const DevPlane* __restrict__ planes, const PixBlock* __restrict__ pblocks,
@@ -302,10 +318,11 @@ __global__ __launch_bounds__(kThreads) void fwd_reduce(
         ei += v.y;
     }
     if (restore_phase) {
+        // exp(2 pi i z'_hi), z'_hi = float(k z / 2pi) exactly as the pair kernel used it.
         const int j = p / P.nx;
-        const double zh = static_cast<double>(static_cast<float>(P.z0 + j * P.sz));
+        const double zh = static_cast<double>(static_cast<float>(P.kturn * (P.z0 + j * P.sz)));
         double sn, cs;
-        sincos(P.k * zh, &sn, &cs);
+        sincos(kTwoPi * (zh - floor(zh)), &sn, &cs);
         const double r = er * cs - ei * sn;
         ei = er * sn + ei * cs;
         er = r;
@@ -316,9 +333,9 @@ __global__ __launch_bounds__(kThreads) void fwd_reduce(
 }
 
 // ---------------------------------------------------------------------------------
-// Adjoint staging (FP32C): per pixel {fx, |u|, arg(u)/(2 pi) - k z_hi/(2 pi) mod 1, 0}.
-// Folding exp(-i k z_hi) into u leaves the pair loop the same path excess d = R - z_hi
-// as the forward.
+// Adjoint staging (FP32C): per pixel {fx', |u| k/(2 pi), Z, 0} with fx' = k fx/(2 pi) and
+// Z = z'_hi + (arg(u)/(2 pi) - z'_hi mod 1), so that Z - R' is the phase (turns) of
+// conj(G) u up to whole turns: the pair loop then needs no per-row phase offset.
 
 __global__ __launch_bounds__(kThreads) void stage_pixels_fp32c(
     const DevPlane* __restrict__ planes, const PixBlock* __restrict__ pblocks,
@@ -330,30 +347,32 @@ __global__ __launch_bounds__(kThreads) void stage_pixels_fp32c(
     if (p >= npix) return;
     const int i = p % P.nx, j = p / P.nx;
     const int64_t o = P.offset + p;
+    const double kt = P.kturn;
     const double2 v = u[o];
-    const double zh = static_cast<double>(static_cast<float>(P.z0 + j * P.sz));
-    double chi = atan2(v.y, v.x) * kInv2Pi - P.kturn * zh;
-    chi -= rint(chi);
-    pix[o] = make_float4(static_cast<float>(P.x0 + static_cast<double>(i) * P.sx),
-                         static_cast<float>(hypot(v.x, v.y)), static_cast<float>(chi), 0.0f);
+    const double zh = static_cast<double>(static_cast<float>(kt * (P.z0 + j * P.sz)));
+    double zeta = atan2(v.y, v.x) * kInv2Pi - zh;
+    zeta -= rint(zeta);
+    pix[o] = make_float4(static_cast<float>(kt * (P.x0 + static_cast<double>(i) * P.sx)),
+                         static_cast<float>(hypot(v.x, v.y) * kt), static_cast<float>(zeta + zh),
+                         0.0f);
 }
 
 // ---------------------------------------------------------------------------------
-// Adjoint, FP32C.  Thread owns PQ pillars; each row's pixels stream through LDS.
+// Adjoint, FP32C.  Thread owns PQ pillars (FP64 positions in registers, scaled to
+// turns by each row's plane); each row's pixels stream through LDS.
 
 template <int PQ>
 __global__ __launch_bounds__(kThreads, 2) void adj_fp32c(
-    const float4* __restrict__ pil, int npil, const DevPlane* __restrict__ planes,
+    const double2* __restrict__ pos, int npil, const DevPlane* __restrict__ planes,
     const AdjRow* __restrict__ rows, const AdjBlock* __restrict__ blocks,
     const float4* __restrict__ pix, double2* __restrict__ partial) {
     __shared__ float4 tile[kThreads];
     const AdjBlock B = blocks[blockIdx.x];
     const int tid = threadIdx.x;
-    float qx[PQ], qy[PQ];
+    double qx[PQ], qy[PQ];
 #pragma unroll
     for (int k = 0; k < PQ; ++k) {
-        const int q = min(B.q0 + tid + k * kThreads, npil - 1);
-        const float4 v = pil[q];
+        const double2 v = pos[min(B.q0 + tid + k * kThreads, npil - 1)];
         qx[k] = v.x;
         qy[k] = v.y;
     }
@@ -365,16 +384,15 @@ __global__ __launch_bounds__(kThreads, 2) void adj_fp32c(
         const AdjRow rw = rows[r];
         const DevPlane& P = planes[rw.plane];
         const int nx = P.nx;
-        const float fy = static_cast<float>(P.y0 + static_cast<double>(rw.j) * P.sy);
-        const double z = P.z0 + static_cast<double>(rw.j) * P.sz;
-        const float zh = static_cast<float>(z);
-        float z2h, z2l;
-        split_z2(z, z2h, z2l);
         const double kt = P.kturn;
-        float dyz2[PQ];
+        const double fy = P.y0 + static_cast<double>(rw.j) * P.sy;
+        float z2h, z2l;
+        split_sq(kt * (P.z0 + static_cast<double>(rw.j) * P.sz), z2h, z2l);
+        float xq[PQ], dyz2[PQ];
 #pragma unroll
         for (int k = 0; k < PQ; ++k) {
-            const float dy = fy - qy[k];
+            xq[k] = static_cast<float>(kt * qx[k]);
+            const float dy = static_cast<float>(kt * fy) - static_cast<float>(kt * qy[k]);
             dyz2[k] = fmaf(dy, dy, z2l) + z2h;
         }
         const float4* __restrict__ prow = pix + P.offset + static_cast<int64_t>(rw.j) * nx;
@@ -390,11 +408,10 @@ __global__ __launch_bounds__(kThreads, 2) void adj_fp32c(
                 const float4 X = tile[p];
 #pragma unroll
                 for (int k = 0; k < PQ; ++k) {
-                    const float dx = X.x - qx[k];
+                    const float dx = X.x - xq[k];
                     const float R2 = fmaf(dx, dx, dyz2[k]);
                     float iR;
-                    const float d = path_excess(R2, zh, iR);
-                    const float th = X.z - frac_turns(static_cast<double>(d) * kt);
+                    const float th = __builtin_amdgcn_fractf(pair_turns<-1>(R2, X.z, iR));
                     const float A = X.y * iR;
                     ar[k] = fmaf(A, __builtin_amdgcn_cosf(th), ar[k]);
                     ai[k] = fmaf(A, __builtin_amdgcn_sinf(th), ai[k]);
@@ -537,9 +554,12 @@ hipError_t launch_stage_pillars(int mode, const double* px, const double* py, co
                                 int npil, int nsets, void* out, hipStream_t s) {
     const int64_t total = static_cast<int64_t>(npil) * nsets;
     const double2* tt = reinterpret_cast<const double2*>(t);
-    if (mode == 0)
-        stage_pillars_fp32c<<<grid_for(total), kThreads, 0, s>>>(px, py, tt, npil, total,
-                                                                  static_cast<float4*>(out));
+    if (mode == 0) {
+        // Layout: pos (double2 x npil) then ap (float2 x nsets*npil).
+        double2* pos = static_cast<double2*>(out);
+        float2* ap = reinterpret_cast<float2*>(pos + npil);
+        stage_pillars_fp32c<<<grid_for(total), kThreads, 0, s>>>(px, py, tt, npil, total, pos, ap);
+    }
     else if (mode == 2)
         stage_pillars_fp32<<<grid_for(total), kThreads, 0, s>>>(px, py, tt, npil, total,
                                                                  static_cast<float4*>(out));
@@ -553,8 +573,10 @@ hipError_t launch_forward(int mode, const void* pil, int npil, const DevPlane* p
                           const FwdBlock* blocks, int nblocks, double2* partial, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
     if (mode == 0)
-        fwd_fp32c<kPX, kPY><<<nblocks, kThreads, 0, s>>>(static_cast<const float4*>(pil), npil,
-                                                         planes, blocks, partial);
+        fwd_fp32c<kPX, kPY><<<nblocks, kThreads, 0, s>>>(
+            static_cast<const double2*>(pil),
+            reinterpret_cast<const float2*>(static_cast<const double2*>(pil) + npil), npil, planes,
+            blocks, partial);
     else if (mode == 2)
         fwd_fp32<<<nblocks, kThreads, 0, s>>>(static_cast<const float4*>(pil), npil, planes,
                                               blocks, partial);
@@ -585,7 +607,7 @@ hipError_t launch_adjoint(int mode, const void* pil, int npil, const DevPlane* p
                           const void* pix_or_u, double2* partial, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
     if (mode == 0)
-        adj_fp32c<kPQ><<<nblocks, kThreads, 0, s>>>(static_cast<const float4*>(pil), npil, planes,
+        adj_fp32c<kPQ><<<nblocks, kThreads, 0, s>>>(static_cast<const double2*>(pil), npil, planes,
                                                     rows, blocks,
                                                     static_cast<const float4*>(pix_or_u), partial);
     else if (mode == 2)

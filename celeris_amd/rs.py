@@ -270,5 +270,57 @@ class Context:
         return g
 
 
+class Graph:
+    """An instantiated HIP graph of one forward or adjoint (crs_graph_*)."""
+
+    def __init__(self, handle, keep):
+        self._h = handle
+        self._keep = keep  # tensors bound by pointer: keep them alive
+
+    def launch(self, stream=None) -> None:
+        torch = _torch()
+        s = stream if stream is not None else torch.cuda.current_stream()
+        _lib.check(_lib.load().crs_graph_launch(self._h, ctypes.c_void_p(s.cuda_stream)),
+                   "crs_graph_launch")
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.load().crs_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def graph_forward(ctx: "Context", px, py, t, planes: Sequence[Plane], field, intensity=None,
+                  mode="fp32c") -> Graph:
+    """Capture ctx.forward(px, py, t, planes) writing into the given output tensors."""
+    arr, _ = pack_planes(planes)
+    nsets = t.numel() // max(px.numel(), 1)
+    h = ctypes.c_void_p()
+    _lib.check(_lib.load().crs_graph_forward(
+        ctx._h, _mode(mode), ctypes.c_void_p(px.data_ptr()), ctypes.c_void_p(py.data_ptr()),
+        ctypes.c_void_p(t.data_ptr()), px.numel(), nsets, arr, len(planes),
+        ctypes.c_void_p(field.data_ptr() if field is not None else None),
+        ctypes.c_void_p(intensity.data_ptr() if intensity is not None else None),
+        ctypes.byref(h)), "crs_graph_forward")
+    return Graph(h, (px, py, t, field, intensity))
+
+
+def graph_adjoint(ctx: "Context", px, py, u, planes: Sequence[Plane], g, nsets: int = 1,
+                  mode="fp32c") -> Graph:
+    """Capture ctx.adjoint(px, py, u, planes) writing into g (complex128 [nsets, npil])."""
+    arr, _ = pack_planes(planes)
+    h = ctypes.c_void_p()
+    _lib.check(_lib.load().crs_graph_adjoint(
+        ctx._h, _mode(mode), ctypes.c_void_p(px.data_ptr()), ctypes.c_void_p(py.data_ptr()),
+        px.numel(), int(nsets), arr, len(planes), ctypes.c_void_p(u.data_ptr()),
+        ctypes.c_void_p(g.data_ptr()), ctypes.byref(h)), "crs_graph_adjoint")
+    return Graph(h, (px, py, u, g))
+
+
 def kernel_name(mode, adjoint: bool = False) -> str:
     return _lib.load().crs_kernel_name(_mode(mode), int(bool(adjoint))).decode()

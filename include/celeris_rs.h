@@ -151,6 +151,21 @@ crs_status crs_adjoint(crs_context *ctx, void *stream, int mode,
 crs_status crs_set_tuning(crs_context *ctx, int mode, int fwd_target_blocks,
                           int adj_target_blocks);
 
+/* HIP graphs (no reference counterpart).  Capture one crs_forward / crs_adjoint of
+ * this exact shape and these exact device pointers into an instantiated graph.  The
+ * graph owns its scratch and descriptor tables (copied to the device before capture),
+ * so it holds kernel nodes only and later calls on `ctx` do not disturb it.  Replays
+ * with crs_graph_launch recompute from the current contents of the bound inputs. */
+typedef struct crs_graph crs_graph;
+crs_status crs_graph_forward(crs_context *ctx, int mode, const double *px, const double *py,
+                             const double *t, int npil, int nsets, const crs_plane *planes,
+                             int nplanes, double *field, double *intensity, crs_graph **graph);
+crs_status crs_graph_adjoint(crs_context *ctx, int mode, const double *px, const double *py,
+                             int npil, int nsets, const crs_plane *planes, int nplanes,
+                             const double *u, double *g, crs_graph **graph);
+crs_status crs_graph_launch(crs_graph *graph, void *stream);
+void crs_graph_destroy(crs_graph *graph);
+
 /* Kernel timing (no reference counterpart).  With timing enabled, each crs_forward /
  * crs_adjoint records HIP events on its stream immediately before and after the
  * kernel that does the pair sums; crs_last_kernel_ms waits for the last such event

@@ -1,0 +1,131 @@
+"""HIP-graph capture of the batched forward/adjoint, and the failure paths.
+
+Graphs: each replay is checked against the eager result, with eager work of the
+same context and torch ops between replays, and after the bound inputs change in
+place.  Failures are forced from the host (CELERIS_RS_FAIL=alloc|launch) so every
+kernel still runs normally; the reference's contract is that a failed call returns
+false and leaves the caller's output untouched (propagate.cu:164-171, :184-192).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from celeris_amd import Context, Plane, configs
+from celeris_amd import rs as crs
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def batch():
+    px, py = configs.square_grid(30, 0.35)
+    rng = np.random.default_rng(21)
+    t = rng.uniform(0.5, 1, (2, px.size)) * np.exp(1j * rng.uniform(-np.pi, np.pi, (2, px.size)))
+    planes = [Plane.psf(0, 0, 30.0, 2 * math.pi / 0.5, 17, 2.0, tset=0),
+              Plane.psf(0, 0, 32.0, 2 * math.pi / 0.6, 9, 2.0, tset=1),
+              Plane.zx(-2, 2, 11, 20, 40, 5, 2 * math.pi / 0.5, tset=0)]
+    return px, py, t, planes
+
+
+@pytest.mark.parametrize("mode", ["fp32c", "fp64"])
+def test_graph_forward_replays_match_eager(torch_cuda, mode):
+    torch = torch_cuda
+    px, py, t, planes = batch()
+    ctx = Context(0)
+    dpx, dpy, dt = dev(torch, px), dev(torch, py), dev(torch, t)
+    total = sum(p.npix for p in planes)
+    field = torch.empty(total, dtype=torch.complex128, device="cuda")
+    inten = torch.empty(total, dtype=torch.float64, device="cuda")
+    print(f"[graph] capture forward {mode}", flush=True)
+    g = crs.graph_forward(ctx, dpx, dpy, dt, planes, field, inten, mode=mode)
+    for rep in range(3):
+        ref_f, ref_i = ctx.forward(dpx, dpy, dt, planes, mode=mode, intensity=True)
+        field.zero_()
+        inten.zero_()
+        g.launch()
+        torch.cuda.synchronize()
+        assert torch.equal(field, ref_f), rep
+        assert torch.equal(inten, ref_i), rep
+        # eager work in between, then change the bound input in place
+        _ = (dt * 2).sum().item()
+        ctx.forward(dpx, dpy, dt * 0.5, planes[:1], mode=mode)
+        dt.mul_(torch.exp(1j * torch.tensor(0.3, dtype=torch.float64, device="cuda")))
+    g.close()
+    ctx.close()
+
+
+def test_graph_adjoint_replays_match_eager(torch_cuda):
+    torch = torch_cuda
+    px, py, t, planes = batch()
+    ctx = Context(0)
+    dpx, dpy = dev(torch, px), dev(torch, py)
+    total = sum(p.npix for p in planes)
+    rng = np.random.default_rng(8)
+    du = dev(torch, rng.standard_normal(total) + 1j * rng.standard_normal(total))
+    g_out = torch.empty((2, px.size), dtype=torch.complex128, device="cuda")
+    print("[graph] capture adjoint", flush=True)
+    gr = crs.graph_adjoint(ctx, dpx, dpy, du, planes, g_out, nsets=2)
+    for rep in range(3):
+        ref = ctx.adjoint(dpx, dpy, du, planes, nsets=2)
+        g_out.zero_()
+        gr.launch()
+        torch.cuda.synchronize()
+        assert torch.equal(g_out, ref), rep
+        du.mul_(1.5 - 0.5j)
+    gr.close()
+    ctx.close()
+
+
+def test_injected_failures(torch_cuda, monkeypatch):
+    torch = torch_cuda
+    px, py, t, planes = batch()
+    dpx, dpy, dt = dev(torch, px), dev(torch, py), dev(torch, t)
+    # Drop-in: false with out untouched.
+    L = configs.small_lens(n_side=10, n=5)
+    out = np.full((5, 5), 7.0)
+    monkeypatch.setenv("CELERIS_RS_FAIL", "launch")
+    import ctypes
+
+    from celeris_amd import _lib
+
+    lib = _lib.load()
+    dp = ctypes.POINTER(ctypes.c_double)
+    t0 = np.ascontiguousarray(L.t[0])
+    ok = lib.crs_propagate_psf(L.px.ctypes.data_as(dp), L.py.ctypes.data_as(dp),
+                               t0.ctypes.data_as(dp), L.npil, 0.0, 0.0, L.focal,
+                               L.planes[0].k, 5, 1.0, out.ctypes.data_as(dp))
+    assert ok == 0 and np.all(out == 7.0)
+    # Batched: status codes, output untouched.
+    ctx = Context(0)
+    field = torch.full((sum(p.npix for p in planes),), 3.0 + 0j, dtype=torch.complex128,
+                       device="cuda")
+    with pytest.raises(RuntimeError, match="CRS_ERR_LAUNCH"):
+        ctx.forward(dpx, dpy, dt, planes, out_field=field)
+    monkeypatch.setenv("CELERIS_RS_FAIL", "alloc")
+    fresh = Context(0)
+    with pytest.raises(RuntimeError, match="CRS_ERR_ALLOC"):
+        fresh.forward(dpx, dpy, dt, planes, out_field=field)
+    torch.cuda.synchronize()
+    assert torch.all(field == 3.0)
+    monkeypatch.delenv("CELERIS_RS_FAIL")
+    f, _ = fresh.forward(dpx, dpy, dt, planes)  # recovers once the fault is gone
+    torch.cuda.synchronize()
+    assert torch.isfinite(f.real).all()
+    fresh.close()
+    ctx.close()
+
+
+def test_invalid_arguments(torch_cuda, rsctx):
+    torch = torch_cuda
+    px, py, t, planes = batch()
+    dpx, dpy, dt = dev(torch, px), dev(torch, py), dev(torch, t)
+    bad = [Plane.psf(0, 0, 30.0, 10.0, 9, 2.0, tset=5)]
+    with pytest.raises(RuntimeError, match="CRS_ERR_INVALID"):
+        rsctx.forward(dpx, dpy, dt, bad)
+    with pytest.raises(RuntimeError, match="CRS_ERR_INVALID"):
+        rsctx.forward(dpx, dpy, dt, [Plane(10.0, 0, 0.1, 0, 0.1, 5, 0, 0, 3)])
