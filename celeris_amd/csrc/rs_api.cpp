@@ -29,15 +29,15 @@
 namespace crs {
 hipError_t launch_stage_pillars(int mode, const double* px, const double* py, const double* t,
                                 int npil, int nsets, void* out, hipStream_t s);
-hipError_t launch_forward(int mode, const void* pil, int npil, const DevPlane* planes,
+hipError_t launch_forward(int mode, int order, const void* pil, int npil, const DevPlane* planes,
                           const FwdBlock* blocks, int nblocks, double2* partial, hipStream_t s);
 hipError_t launch_fwd_reduce(int restore_phase, const DevPlane* planes, const PixBlock* pblocks,
                              int nblocks, const double2* partial, double2* field,
                              double* intensity, hipStream_t s);
 hipError_t launch_stage_pixels(const DevPlane* planes, const PixBlock* pblocks, int nblocks,
                                const double2* u, float4* pix, hipStream_t s);
-hipError_t launch_adjoint(int mode, const void* pil, int npil, const DevPlane* planes,
-                          const AdjRow* rows, const AdjBlock* blocks, int nblocks,
+hipError_t launch_adjoint(int mode, int order, const void* pil, int npil, const DevPlane* planes,
+                          const AdjQuad* quads, const AdjBlock* blocks, int nblocks,
                           const void* pix_or_u, double2* partial, hipStream_t s);
 hipError_t launch_adj_reduce(const double2* partial, int nslots, int64_t n, double2* g,
                              hipStream_t s);
@@ -234,14 +234,44 @@ DevPlane make_dev_plane(const crs_plane& P, int px_per_thread, int py_per_thread
     d.x0 = P.x0; d.sx = P.sx;
     d.y0 = P.y0; d.sy = P.sy;
     d.z0 = P.z0; d.sz = P.sz;
+    d.zref = d.kturn * P.z0;
     d.nx = P.nx; d.ny = P.ny;
     d.tx = static_cast<int32_t>(ceil_div(P.nx, px_per_thread));
     d.ty = static_cast<int32_t>(ceil_div(P.ny, py_per_thread));
     d.tset = P.tset;
+    d.nslots = 1;
     d.offset = P.offset;
     d.part = 0;
-    d.nslots = 1;
+    d.sbase = 0;
+    d.nxp = static_cast<int32_t>(ceil_div(P.nx, kBlk) * kBlk);
+    d.nquads = static_cast<int32_t>(ceil_div(P.ny, kBlk));
     return d;
+}
+
+// FP32C series order for a set of planes.  A 4 x 4 pixel block spans at most e turns
+// around its centre; at distance >= z' the expansion parameter is r = e / z', and the
+// neglected terms are ~ e r^2 / 2 turns of phase and 1.5 r^2 of amplitude (ORDER 1),
+// ~ 0.6 e r^3 and 2.5 r^3 (ORDER 2).  Returns 1 or 2, or 0 when even ORDER 2 would
+// not stay below 1e-6 (then FP32C is refused for the call).
+int fp32c_order(const std::vector<DevPlane>& dp) {
+    int order = 1;
+    for (const DevPlane& d : dp) {
+        const double ex = 1.5 * std::fabs(d.sx) * d.kturn;
+        const double ey = 1.5 * std::fabs(d.sy) * d.kturn;
+        const double ez = 1.5 * std::fabs(d.sz) * d.kturn;
+        const double e = std::sqrt(ex * ex + ey * ey) + ez;
+        const double zlast = d.z0 + (d.ny - 1) * d.sz;
+        const double zmin = d.kturn * std::min(std::fabs(d.z0), std::fabs(zlast));
+        if (!(std::isfinite(e) && zmin > 0.0)) {
+            if (std::isfinite(e)) return 0;  // z = 0 plane: no distance bound
+            continue;                        // NaN/inf geometry (n == 1 windows): NaN out
+        }
+        const double r = e / zmin;
+        if (1.5 * r * r <= 1e-6 && 0.5 * e * r * r <= 1e-7) continue;
+        if (2.5 * r * r * r <= 1e-6 && 0.625 * e * r * r * r <= 1e-7) { order = 2; continue; }
+        return 0;
+    }
+    return order;
 }
 
 // Bytes of staged pillar data: FP32C keeps FP64 positions once plus {|t|, arg t} per
@@ -372,6 +402,7 @@ struct Scratch {  // where one call's device data lives (a context's or a graph'
 
 struct FwdPlan {
     int mode = 0;
+    int order = 1;
     int npil = 0, nsets = 0;
     int64_t part_elems = 0;
     int nblocks = 0, npblocks = 0;
@@ -381,8 +412,8 @@ struct FwdPlan {
 
 crs_status plan_forward(const crs_context* ctx, int mode, int npil, int nsets,
                         const crs_plane* planes, int nplanes, FwdPlan& plan) {
-    const int PX = mode == CRS_MODE_FP32C ? kPX : 1;
-    const int PY = mode == CRS_MODE_FP32C ? kPY : 1;
+    const int PX = mode == CRS_MODE_FP32C ? kBlk : 1;
+    const int PY = mode == CRS_MODE_FP32C ? kBlk : 1;
     plan.mode = mode;
     plan.npil = npil;
     plan.nsets = nsets;
@@ -393,6 +424,11 @@ crs_status plan_forward(const crs_context* ctx, int mode, int npil, int nsets,
     for (int p = 0; p < nplanes; ++p) {
         dp[p] = make_dev_plane(planes[p], PX, PY);
         base_blocks += ceil_div(static_cast<int64_t>(dp[p].tx) * dp[p].ty, kThreads);
+    }
+    if (mode == CRS_MODE_FP32C && (plan.order = fp32c_order(dp)) == 0) {
+        set_error("FP32C: a plane lies too close to the aperture for its pixel pitch "
+                  "(block size / distance > 2%%); use CRS_MODE_FP64");
+        return CRS_ERR_INVALID;
     }
     const int target = ctx->fwd_target[mode] > 0 ? ctx->fwd_target[mode] : default_target(mode, false);
     int64_t nsplit = std::max<int64_t>(1, ceil_div(target, base_blocks));
@@ -454,7 +490,7 @@ crs_status enqueue_forward(const FwdPlan& plan, const Scratch& sc, const double*
                             "stage pillars")) != CRS_OK)
         return st;
     if (ev0) hipEventRecord(ev0, stream);
-    if ((st = launch_status(launch_forward(mode, sc.pil->ptr, plan.npil, d_planes,
+    if ((st = launch_status(launch_forward(mode, plan.order, sc.pil->ptr, plan.npil, d_planes,
                                            reinterpret_cast<const FwdBlock*>(sc.tables + plan.o_blocks),
                                            plan.nblocks, static_cast<double2*>(sc.partial->ptr),
                                            stream),
@@ -471,11 +507,12 @@ crs_status enqueue_forward(const FwdPlan& plan, const Scratch& sc, const double*
 
 struct AdjPlan {
     int mode = 0;
+    int order = 1;
     int npil = 0, nsets = 0;
-    int64_t nsplit = 1, n_out = 0, total_pix = 0;
+    int64_t nsplit = 1, n_out = 0, staged_records = 0;
     int nblocks = 0, npblocks = 0;
     TableBuilder tb;
-    size_t o_planes = 0, o_rows = 0, o_blocks = 0, o_pblocks = 0;
+    size_t o_planes = 0, o_quads = 0, o_blocks = 0, o_pblocks = 0;
 };
 
 crs_status plan_adjoint(const crs_context* ctx, int mode, int npil, int nsets,
@@ -485,72 +522,86 @@ crs_status plan_adjoint(const crs_context* ctx, int mode, int npil, int nsets,
     plan.npil = npil;
     plan.nsets = nsets;
     std::vector<DevPlane> dp(nplanes);
-    for (int p = 0; p < nplanes; ++p) dp[p] = make_dev_plane(planes[p], 1, 1);
+    int64_t srec = 0;
+    for (int p = 0; p < nplanes; ++p) {
+        dp[p] = make_dev_plane(planes[p], 1, 1);
+        dp[p].sbase = srec;
+        srec += static_cast<int64_t>(dp[p].nquads) * dp[p].nxp * kBlk;
+    }
+    if (srec > (int64_t(1) << 31)) { set_error("too many pixels"); return CRS_ERR_INVALID; }
+    if (mode == CRS_MODE_FP32C && (plan.order = fp32c_order(dp)) == 0) {
+        set_error("FP32C: a plane lies too close to the aperture for its pixel pitch "
+                  "(block size / distance > 2%%); use CRS_MODE_FP64");
+        return CRS_ERR_INVALID;
+    }
 
-    // Per-set row lists (planes in order, rows in order) and their pixel counts.
-    std::vector<AdjRow> rows;
-    std::vector<int64_t> set_row0(nsets + 1, 0);
-    std::vector<std::vector<int64_t>> row_pix_prefix(nsets);
+    // Per-set quad lists (planes in order, quads in order) and their work, in pixel
+    // records (FP32C computes the padded 4 x nxp records of a quad).
+    std::vector<AdjQuad> quads;
+    std::vector<int64_t> set_q0(nsets + 1, 0);
+    std::vector<std::vector<int64_t>> work_prefix(nsets);
     for (int s = 0; s < nsets; ++s) {
-        set_row0[s] = static_cast<int64_t>(rows.size());
-        std::vector<int64_t>& pre = row_pix_prefix[s];
+        set_q0[s] = static_cast<int64_t>(quads.size());
+        std::vector<int64_t>& pre = work_prefix[s];
         pre.push_back(0);
         for (int p = 0; p < nplanes; ++p) {
             if (dp[p].tset != s) continue;
-            for (int j = 0; j < dp[p].ny; ++j) {
-                rows.push_back({p, j});
-                pre.push_back(pre.back() + dp[p].nx);
+            for (int j = 0; j < dp[p].ny; j += kBlk) {
+                quads.push_back({p, j});
+                const int64_t w = mode == CRS_MODE_FP32C
+                                      ? static_cast<int64_t>(kBlk) * dp[p].nxp
+                                      : static_cast<int64_t>(std::min(kBlk, dp[p].ny - j)) * dp[p].nx;
+                pre.push_back(pre.back() + w);
             }
         }
     }
-    set_row0[nsets] = static_cast<int64_t>(rows.size());
-    if (rows.size() > static_cast<size_t>(INT32_MAX)) { set_error("too many rows"); return CRS_ERR_INVALID; }
+    set_q0[nsets] = static_cast<int64_t>(quads.size());
+    if (quads.size() > static_cast<size_t>(INT32_MAX)) { set_error("too many rows"); return CRS_ERR_INVALID; }
 
     const int64_t groups = ceil_div(npil, static_cast<int64_t>(kThreads) * PQ);
     const int64_t base_blocks = groups * nsets;
-    int64_t max_rows = 1;
-    for (int s = 0; s < nsets; ++s) max_rows = std::max<int64_t>(max_rows, set_row0[s + 1] - set_row0[s]);
+    int64_t max_quads = 1;
+    for (int s = 0; s < nsets; ++s) max_quads = std::max<int64_t>(max_quads, set_q0[s + 1] - set_q0[s]);
     const int target = ctx->adj_target[mode] > 0 ? ctx->adj_target[mode] : default_target(mode, true);
     int64_t nsplit = std::max<int64_t>(1, ceil_div(target, base_blocks));
-    nsplit = std::min<int64_t>(nsplit, max_rows);
+    nsplit = std::min<int64_t>(nsplit, max_quads);
     if (base_blocks * nsplit > (int64_t(1) << 31) - 1) { set_error("launch too large"); return CRS_ERR_INVALID; }
 
-    // Cut each set's rows into nsplit runs of about equal pixel count.
+    // Cut each set's quads into nsplit runs of about equal work.
     std::vector<AdjBlock> blocks;
     blocks.reserve(static_cast<size_t>(base_blocks * nsplit));
     const int64_t n_out = static_cast<int64_t>(nsets) * npil;
     for (int64_t c = 0; c < nsplit; ++c) {
         for (int s = 0; s < nsets; ++s) {
-            const std::vector<int64_t>& pre = row_pix_prefix[s];
-            const int64_t nrows = static_cast<int64_t>(pre.size()) - 1;
+            const std::vector<int64_t>& pre = work_prefix[s];
+            const int64_t nq = static_cast<int64_t>(pre.size()) - 1;
             const int64_t total = pre.back();
-            auto cut = [&](int64_t k) -> int64_t {  // first row whose prefix >= k*total/nsplit
-                if (k >= nsplit) return nrows;
+            auto cut = [&](int64_t k) -> int64_t {  // first quad whose prefix >= k*total/nsplit
+                if (k >= nsplit) return nq;
                 const int64_t want = (total * k) / nsplit;
                 return static_cast<int64_t>(std::lower_bound(pre.begin(), pre.end(), want) - pre.begin());
             };
-            const int64_t r0 = std::min(cut(c), nrows), r1 = std::min(cut(c + 1), nrows);
+            const int64_t u0 = std::min(cut(c), nq), u1 = std::min(cut(c + 1), nq);
             for (int64_t gq = 0; gq < groups; ++gq)
                 blocks.push_back({s, static_cast<int32_t>(gq * kThreads * PQ),
-                                  static_cast<int32_t>(set_row0[s] + r0),
-                                  static_cast<int32_t>(set_row0[s] + std::max(r0, r1)),
+                                  static_cast<int32_t>(set_q0[s] + u0),
+                                  static_cast<int32_t>(set_q0[s] + std::max(u0, u1)),
                                   c * n_out + static_cast<int64_t>(s) * npil});
         }
     }
+    // Staging blocks walk each plane's staged records.
     std::vector<PixBlock> pblocks;
-    int64_t total_pix = 0;
     for (int p = 0; p < nplanes; ++p) {
-        const int64_t npix = static_cast<int64_t>(dp[p].nx) * dp[p].ny;
-        total_pix = std::max(total_pix, dp[p].offset + npix);
-        for (int64_t b = 0; b < npix; b += kThreads) pblocks.push_back({p, static_cast<int32_t>(b)});
+        const int64_t nrec = static_cast<int64_t>(dp[p].nquads) * dp[p].nxp * kBlk;
+        for (int64_t b = 0; b < nrec; b += kThreads) pblocks.push_back({p, static_cast<int32_t>(b)});
     }
     plan.nsplit = nsplit;
     plan.n_out = n_out;
-    plan.total_pix = total_pix;
+    plan.staged_records = srec;
     plan.nblocks = static_cast<int>(blocks.size());
     plan.npblocks = static_cast<int>(pblocks.size());
     plan.o_planes = plan.tb.add(dp.data(), dp.size() * sizeof(DevPlane));
-    plan.o_rows = plan.tb.add(rows.data(), rows.size() * sizeof(AdjRow));
+    plan.o_quads = plan.tb.add(quads.data(), quads.size() * sizeof(AdjQuad));
     plan.o_blocks = plan.tb.add(blocks.data(), blocks.size() * sizeof(AdjBlock));
     plan.o_pblocks = plan.tb.add(pblocks.data(), pblocks.size() * sizeof(PixBlock));
     return CRS_OK;
@@ -563,7 +614,7 @@ crs_status size_adjoint(const AdjPlan& plan, const Scratch& sc, hipStream_t stre
                      "staged pillars")) != CRS_OK)
         return st;
     if (plan.mode == CRS_MODE_FP32C &&
-        (st = ensure(*sc.pix, static_cast<size_t>(plan.total_pix) * sizeof(float4), stream,
+        (st = ensure(*sc.pix, static_cast<size_t>(plan.staged_records) * sizeof(float4), stream,
                      "staged pixels")) != CRS_OK)
         return st;
     return ensure(*sc.partial, static_cast<size_t>(plan.nsplit * plan.n_out) * sizeof(double2),
@@ -593,8 +644,8 @@ crs_status enqueue_adjoint(const AdjPlan& plan, const Scratch& sc, const double*
         pix_or_u = sc.pix->ptr;
     }
     if (ev0) hipEventRecord(ev0, stream);
-    if ((st = launch_status(launch_adjoint(mode, sc.pil->ptr, plan.npil, d_planes,
-                                           reinterpret_cast<const AdjRow*>(sc.tables + plan.o_rows),
+    if ((st = launch_status(launch_adjoint(mode, plan.order, sc.pil->ptr, plan.npil, d_planes,
+                                           reinterpret_cast<const AdjQuad*>(sc.tables + plan.o_quads),
                                            reinterpret_cast<const AdjBlock*>(sc.tables + plan.o_blocks),
                                            plan.nblocks, pix_or_u,
                                            static_cast<double2*>(sc.partial->ptr), stream),

@@ -10,12 +10,13 @@ namespace crs {
 
 // Work-group size of every pair kernel: 4 wave64s, one per SIMD of a CU.
 constexpr int kThreads = 256;
-// FP32C forward: each thread owns a kPX x kPY block of pixels (columns x rows);
-// the pillar's dx is shared down a column and its (fy-y)^2+z^2 along a row.
-constexpr int kPX = 4;
-constexpr int kPY = 4;
-// FP32C adjoint: each thread owns kPQ pillars (q0 + tid + k*kThreads).
-constexpr int kPQ = 8;
+// FP32C: pixels are handled in 4 x 4 blocks (4 columns x 4 rows).  The forward gives
+// each thread one block; the adjoint walks blocks of a 4-row strip ("quad") of a plane.
+constexpr int kBlk = 4;
+// FP32C adjoint: pillars per thread (q0 + tid + k*kThreads, k < kPQ).
+constexpr int kPQ = 4;
+// FP32C adjoint: columns per LDS tile of a quad (kAdjCols x 4 pixel records).
+constexpr int kAdjCols = kThreads / kBlk;
 
 // One plane as the kernels see it (host computes the derived fields).
 struct DevPlane {
@@ -24,13 +25,17 @@ struct DevPlane {
     double x0, sx;  // fx = x0 + i*sx
     double y0, sy;  // fy = y0 + j*sy
     double z0, sz;  // z  = z0 + j*sz
+    double zref;    // kturn * z0: FP32C phase reference, restored as exp(2 pi i zref)
     int32_t nx, ny;
-    int32_t tx;     // thread tiles across a row: ceil(nx / px_per_thread)
+    int32_t tx;     // forward thread tiles across a row: ceil(nx / px_per_thread)
+    int32_t ty;     // forward thread tiles down a column: ceil(ny / py_per_thread)
     int32_t tset;
-    int64_t offset; // first output element of the plane
-    int64_t part;   // first partial element of slot 0 (forward)
     int32_t nslots; // pillar chunks the forward sum is split into
-    int32_t ty;     // thread tiles down a column: ceil(ny / py_per_thread)
+    int64_t offset; // first output element of the plane
+    int64_t part;   // first forward partial element of slot 0
+    int64_t sbase;  // FP32C adjoint: first staged pixel record of the plane
+    int32_t nxp;    // FP32C adjoint: nx rounded up to a multiple of 4
+    int32_t nquads; // ceil(ny / 4)
 };
 
 // Forward work-group: thread tiles [tile0, tile0+kThreads) of `plane`, summed over
@@ -40,21 +45,21 @@ struct FwdBlock {
     int64_t part;
 };
 
-// A run of <= kThreads consecutive pixels of one plane (staging and reductions).
+// A run of <= kThreads consecutive elements of one plane (staging and reductions).
 struct PixBlock {
     int32_t plane, pix0;
 };
 
-// One pixel row in the adjoint's per-set row list.
-struct AdjRow {
-    int32_t plane, j;
+// Four consecutive rows j0..j0+3 of one plane (rows >= ny are padding).
+struct AdjQuad {
+    int32_t plane, j0;
 };
 
 // Adjoint work-group: pillars q0 + tid + k*kThreads (k < pillars per thread) of
-// transmission set `tset`, summed over rows [r0, r1) of the row list; results go
+// transmission set `tset`, summed over quads [u0, u1) of the quad list; results go
 // to partial[part + q].
 struct AdjBlock {
-    int32_t tset, q0, r0, r1;
+    int32_t tset, q0, u0, u1;
     int64_t part;
 };
 

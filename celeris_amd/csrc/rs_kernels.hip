@@ -8,27 +8,31 @@
 //
 // Three arithmetic modes share one work decomposition: a work-group of 256 threads
 // (4 wave64s) owns a set of pixels (forward) or pillars (adjoint) and streams the
-// other side through LDS in 256-entry tiles that every lane reads by broadcast.
-// Sums are split over pillar chunks (forward) or pixel rows (adjoint) so that small
-// focal grids still launch thousands of work-groups across the 256 CUs; each split
-// writes a float64 partial and a second kernel adds the partials in a fixed order,
-// so results are bitwise repeatable run to run.
+// other side through LDS in tiles that every lane reads by broadcast.  Sums are split
+// over pillar chunks (forward) or pixel quads (adjoint) so that small focal grids
+// still launch thousands of work-groups across the 256 CUs; each split writes a
+// float64 partial and a second kernel adds the partials in a fixed order, so results
+// are bitwise repeatable run to run.
 //
-// FP32C ("FP32-compensated") pair arithmetic.  Every phase-carrying coordinate is
-// formed in FP64 and scaled to turns, x' = k x / (2 pi) (likewise y, z, fx, fy), then
-// rounded to float once; z'^2 is carried as a float hi+lo pair.  Per pillar-pixel pair:
-//   R2  = dx'^2 + [(fy'-y')^2 + z'^2]       R' = sqrt(R2) is k R / (2 pi): the phase in turns
-//   iR  = rsq(R2), R = R2*iR                 v_rsq_f32
-//   th  = (R - (z'_hi - arg t/2pi)) - (R*R - R2)/(2R)
-//                                            one Newton step for R (removes any bias of
-//                                            the hardware root), with z'_hi subtracted
-//                                            exactly (Sterbenz), so th = k(R - z)/2pi + arg t/2pi
-//                                            carries float rounding of th itself, not of kR
-//   th  = fract(th)                          v_fract_f32, exact
-//   E  += |t| k/(2 pi) * iR * (cos 2pi th, sin 2pi th)   v_cos_f32 / v_sin_f32 take turns
-// The dropped factor exp(2 pi i z'_hi) is applied in FP64 by the reduction kernel.  Each
-// pillar tile's float partial is added into a Kahan-compensated running sum, and the
-// partials of the pillar-chunk split are added in FP64.
+// FP32C ("FP32-compensated").  Lengths are measured in turns (x' = k x / 2 pi), so a
+// distance is directly a phase.  Pixels are taken in 4 x 4 blocks.  For each pillar q
+// and block b the distance R0 from q to the block centre and the phase
+// frac(R0 - z'_ref + arg t_q / 2 pi) are formed in FP64 (one v_rsq_f32 plus one FP64
+// Newton step).  Every pixel p of the block sits at a small, exactly known offset e
+// from the centre, so
+//     R^2 = R0^2 + D,   D = e_x (2 dx0 + e_x) + e_y (2 dy0 + e_y) + (z_p'^2 - z_c'^2)
+// where D is tiny next to R0^2 and is formed in float without cancellation (each
+// term is one fma against per-block constants).  Then, with h = 1/(2 R0),
+// x1 = D h and y = x1 h = D / (4 R0^2):
+//     R - R0 = x1 (1 - y + 2 y^2 - ...)          (ORDER 1 keeps 1 - y)
+//     R0 / R = 1 - 2 y + 6 y^2 - ...              (ORDER 1 keeps 1 - 2 y)
+// and the pair is
+//     E += |t| (k/2 pi) / R0 * (R0/R) * (cos 2 pi th, sin 2 pi th),  th = frac(...) + (R - R0)
+// with v_cos_f32 / v_sin_f32 (which take turns).  No float ever holds kR (~10^3 rad)
+// or R^2 (~10^4 turns^2): the phase is carried to ~1e-7 turns.  The host picks ORDER
+// per launch from the block size relative to the distance.  The factor exp(2 pi i
+// z'_ref) is restored in FP64 by the reduction.  Each pillar tile's float partial is
+// folded into a Kahan-compensated running sum; split partials are added in FP64.
 #include "rs_internal.h"
 
 #include <hip/hip_runtime.h>
@@ -42,25 +46,46 @@ namespace {
 constexpr double kInv2Pi = 0.15915494309189533577;  // 1 / (2 pi)
 constexpr double kTwoPi = 6.28318530717958647693;
 
-// w^2 in FP64, split into the float nearest it and the float nearest the rest.
-__device__ inline void split_sq(double w, float& hi, float& lo) {
-    const double w2 = w * w;
-    hi = static_cast<float>(w2);
-    lo = static_cast<float>(w2 - static_cast<double>(hi));
+// Distance from a pillar to a block centre, R0 = sqrt(R0sq), to FP64 accuracy: the
+// float rsq seeds one FP64 Newton step.  Returns R0; rf = 1/R0 as a float.
+__device__ inline double block_distance(double R0sq, float& rf) {
+    const float s = static_cast<float>(R0sq);
+    rf = __builtin_amdgcn_rsqf(s);
+    const double R0f = static_cast<double>(s * rf);
+    const double res = fma(-R0f, R0f, R0sq);
+    return fma(res, static_cast<double>(0.5f * rf), R0f);
 }
 
-// Phase (turns) of the pair: R - zeta with one Newton correction of R = sqrt(R2);
-// also returns 1/R.  `sign` = +1 gives (R_true - zeta), -1 gives (zeta - R_true).
-template <int SIGN>
-__device__ inline float pair_turns(float R2, float zeta, float& iR) {
-    iR = __builtin_amdgcn_rsqf(R2);
-    const float R = R2 * iR;
-    const float resh = 0.5f * fmaf(R, R, -R2);
-    if constexpr (SIGN > 0)
-        return fmaf(-resh, iR, R - zeta);
-    else
-        return fmaf(resh, iR, zeta - R);
+// Offsets of a block's 4 columns (or rows) from its centre, in turns: e = step*(c-1.5).
+__device__ inline void block_offsets(double step_turns, float (&te)[kBlk], float (&e2)[kBlk]) {
+#pragma unroll
+    for (int c = 0; c < kBlk; ++c) {
+        const float e = static_cast<float>(step_turns * (c - 1.5));
+        te[c] = 2.0f * e;
+        e2[c] = e * e;
+    }
 }
+
+// One pair's path excess R - R0 (turns) and amplitude A = a0 R0/R, from x1 = D h and
+// y = x1 h (y = D / (4 R0^2)); a0 is the amplitude at R0 and a0n = -2 a0.  ORDER 1
+// keeps R - R0 = x1 (1 - y), R0/R = 1 - 2y; ORDER 2 adds the next term of each.
+template <int ORDER>
+__device__ inline void pair_terms(float x1, float y, float a0, float a0n, float& dR, float& A) {
+    if constexpr (ORDER == 1) {
+        dR = fmaf(-y, x1, x1);
+        A = fmaf(a0n, y, a0);
+    } else {
+        dR = fmaf(x1 * y, fmaf(2.0f, y, -1.0f), x1);
+        A = fmaf(a0 * y, fmaf(6.0f, y, -2.0f), a0);
+    }
+}
+
+// LDS record of one pillar for the FP32C forward: FP64 turn-scaled position, the
+// phase offset z'_ref - arg(t)/2pi, and |t| k/2pi.
+struct alignas(16) PillarRec {
+    double x, y, zeta;
+    float amp, pad;
+};
 
 }  // namespace
 
@@ -69,6 +94,7 @@ __device__ inline float pair_turns(float R2, float zeta, float& iR) {
 
 // FP32C: positions stay FP64 (pos[npil], shared by all sets; each plane scales them
 // by its own k/(2 pi) when it loads a tile) and {|t|, arg(t)/(2 pi)} per pillar and set.
+// t == NULL stages positions only.
 __global__ void stage_pillars_fp32c(const double* __restrict__ px, const double* __restrict__ py,
                                     const double2* __restrict__ t, int npil, int64_t total,
                                     double2* __restrict__ pos, float2* __restrict__ ap) {
@@ -108,15 +134,15 @@ __global__ void stage_pillars_fp64(const double* __restrict__ px, const double* 
 }
 
 // ---------------------------------------------------------------------------------
-// Forward, FP32C.  Thread owns a PX x PY pixel block; pillars stream through LDS,
+// Forward, FP32C.  Each thread owns one 4 x 4 pixel block; pillars stream through LDS,
 // scaled to turns by the plane's k/(2 pi) as they are loaded.
 
-template <int PX, int PY>
+template <int ORDER>
 __global__ __launch_bounds__(kThreads, 2) void fwd_fp32c(
     const double2* __restrict__ pos, const float2* __restrict__ ap, int npil,
     const DevPlane* __restrict__ planes, const FwdBlock* __restrict__ blocks,
     double2* __restrict__ partial) {
-    __shared__ float4 tile[kThreads];
+    __shared__ PillarRec tile[kThreads];
     const FwdBlock B = blocks[blockIdx.x];
     const DevPlane& P = planes[B.plane];
     const int nx = P.nx, ny = P.ny;
@@ -126,28 +152,29 @@ __global__ __launch_bounds__(kThreads, 2) void fwd_fp32c(
     const int tt = active ? my_tile : 0;
     const int ti = tt % P.tx, tj = tt / P.tx;
     const double kt = P.kturn;
+    const double zref = P.zref;
 
-    float fx[PX];
+    // Block centre and the offsets of its pixels (turns).
+    const double ic = ti * kBlk + 1.5, jc = tj * kBlk + 1.5;
+    const double FXc = kt * (P.x0 + ic * P.sx);
+    const double FYc = kt * (P.y0 + jc * P.sy);
+    const double zc = kt * (P.z0 + jc * P.sz);
+    const double ZC2 = zc * zc;
+    float tex[kBlk], ex2[kBlk], tey[kBlk], eyz[kBlk];
+    block_offsets(kt * P.sx, tex, ex2);
+    block_offsets(kt * P.sy, tey, eyz);
 #pragma unroll
-    for (int c = 0; c < PX; ++c) {
-        const int i = min(ti * PX + c, nx - 1);
-        fx[c] = static_cast<float>(kt * (P.x0 + static_cast<double>(i) * P.sx));
-    }
-    float fy[PY], zh[PY], z2h[PY], z2l[PY];
-#pragma unroll
-    for (int r = 0; r < PY; ++r) {
-        const int j = min(tj * PY + r, ny - 1);
-        fy[r] = static_cast<float>(kt * (P.y0 + static_cast<double>(j) * P.sy));
-        const double zt = kt * (P.z0 + static_cast<double>(j) * P.sz);
-        zh[r] = static_cast<float>(zt);
-        split_sq(zt, z2h[r], z2l[r]);
+    for (int r = 0; r < kBlk; ++r) {
+        const double zr = kt * (P.z0 + (tj * kBlk + r) * P.sz);
+        eyz[r] += static_cast<float>(zr * zr - ZC2);
     }
     const float amp_scale = static_cast<float>(kt);  // 1/R = (k/2pi) / R'
     const float2* __restrict__ aps = ap + static_cast<int64_t>(P.tset) * npil;
 
-    float sr[PX * PY], si[PX * PY], cr[PX * PY], ci[PX * PY];
+    constexpr int NPIX = kBlk * kBlk;
+    float sr[NPIX], si[NPIX], cr[NPIX], ci[NPIX];
 #pragma unroll
-    for (int e = 0; e < PX * PY; ++e) sr[e] = si[e] = cr[e] = ci[e] = 0.0f;
+    for (int e = 0; e < NPIX; ++e) sr[e] = si[e] = cr[e] = ci[e] = 0.0f;
 
     for (int base = B.q0; base < B.q1; base += kThreads) {
         const int n = min(kThreads, B.q1 - base);
@@ -155,30 +182,43 @@ __global__ __launch_bounds__(kThreads, 2) void fwd_fp32c(
         if (tid < n) {
             const double2 xy = pos[base + tid];
             const float2 a = aps[base + tid];
-            tile[tid] = make_float4(static_cast<float>(kt * xy.x), static_cast<float>(kt * xy.y),
-                                    a.x * amp_scale, a.y);
+            PillarRec rec;
+            rec.x = kt * xy.x;
+            rec.y = kt * xy.y;
+            rec.zeta = zref - static_cast<double>(a.y);
+            rec.amp = a.x * amp_scale;
+            rec.pad = 0.0f;
+            tile[tid] = rec;
         }
         __syncthreads();
-        float ar[PX * PY], ai[PX * PY];
+        float ar[NPIX], ai[NPIX];
 #pragma unroll
-        for (int e = 0; e < PX * PY; ++e) ar[e] = ai[e] = 0.0f;
+        for (int e = 0; e < NPIX; ++e) ar[e] = ai[e] = 0.0f;
         for (int q = 0; q < n; ++q) {
-            const float4 Q = tile[q];
-            float dx[PX];
+            const PillarRec T = tile[q];
+            const double dX0 = FXc - T.x, dY0 = FYc - T.y;
+            float rf;
+            const double R0 = block_distance(fma(dX0, dX0, fma(dY0, dY0, ZC2)), rf);
+            const float th0 = static_cast<float>(__builtin_amdgcn_fract(R0 - T.zeta));
+            const float h = 0.5f * rf;
+            const float c1 = T.amp * rf;
+            const float c1n = -2.0f * c1;
+            const float dx0 = static_cast<float>(dX0), dy0 = static_cast<float>(dY0);
+            float a[kBlk], b[kBlk];
 #pragma unroll
-            for (int c = 0; c < PX; ++c) dx[c] = fx[c] - Q.x;
+            for (int c = 0; c < kBlk; ++c) a[c] = fmaf(tex[c], dx0, ex2[c]);
 #pragma unroll
-            for (int r = 0; r < PY; ++r) {
-                const float dy = fy[r] - Q.y;
-                const float dyz2 = fmaf(dy, dy, z2l[r]) + z2h[r];
-                const float zeta = zh[r] - Q.w;
+            for (int r = 0; r < kBlk; ++r) b[r] = fmaf(tey[r], dy0, eyz[r]);
 #pragma unroll
-                for (int c = 0; c < PX; ++c) {
-                    const float R2 = fmaf(dx[c], dx[c], dyz2);
-                    float iR;
-                    const float th = __builtin_amdgcn_fractf(pair_turns<1>(R2, zeta, iR));
-                    const float A = Q.z * iR;
-                    const int e = r * PX + c;
+            for (int r = 0; r < kBlk; ++r) {
+#pragma unroll
+                for (int c = 0; c < kBlk; ++c) {
+                    const float x1 = (a[c] + b[r]) * h;
+                    const float y = x1 * h;
+                    float dR, A;
+                    pair_terms<ORDER>(x1, y, c1, c1n, dR, A);
+                    const float th = th0 + dR;
+                    const int e = r * kBlk + c;
                     ar[e] = fmaf(A, __builtin_amdgcn_cosf(th), ar[e]);
                     ai[e] = fmaf(A, __builtin_amdgcn_sinf(th), ai[e]);
                 }
@@ -186,7 +226,7 @@ __global__ __launch_bounds__(kThreads, 2) void fwd_fp32c(
         }
         // Kahan: fold this pillar tile's partial into the running sum.
 #pragma unroll
-        for (int e = 0; e < PX * PY; ++e) {
+        for (int e = 0; e < NPIX; ++e) {
             const float yr = ar[e] - cr[e];
             const float tr = sr[e] + yr;
             cr[e] = (tr - sr[e]) - yr;
@@ -200,14 +240,14 @@ __global__ __launch_bounds__(kThreads, 2) void fwd_fp32c(
 
     if (!active) return;
 #pragma unroll
-    for (int r = 0; r < PY; ++r) {
-        const int j = tj * PY + r;
+    for (int r = 0; r < kBlk; ++r) {
+        const int j = tj * kBlk + r;
         if (j >= ny) break;
 #pragma unroll
-        for (int c = 0; c < PX; ++c) {
-            const int i = ti * PX + c;
+        for (int c = 0; c < kBlk; ++c) {
+            const int i = ti * kBlk + c;
             if (i >= nx) break;
-            const int e = r * PX + c;
+            const int e = r * kBlk + c;
             partial[B.part + static_cast<int64_t>(j) * nx + i] =
                 make_double2(static_cast<double>(sr[e]) - static_cast<double>(cr[e]),
                              static_cast<double>(si[e]) - static_cast<double>(ci[e]));
@@ -300,7 +340,7 @@ __global__ __launch_bounds__(kThreads) void fwd_fp64(
 
 // ---------------------------------------------------------------------------------
 // Forward reduction: add the plane's pillar-chunk partials in slot order, restore
-// the factor exp(2 pi i z'_hi) that FP32C leaves out, write field and/or |E|^2.
+// the factor exp(2 pi i z'_ref) that FP32C leaves out, write field and/or |E|^2.
 
 __global__ __launch_bounds__(kThreads) void fwd_reduce(
     const DevPlane* __restrict__ planes, const PixBlock* __restrict__ pblocks,
@@ -318,11 +358,8 @@ __global__ __launch_bounds__(kThreads) void fwd_reduce(
         ei += v.y;
     }
     if (restore_phase) {
-        // exp(2 pi i z'_hi), z'_hi = float(k z / 2pi) exactly as the pair kernel used it.
-        const int j = p / P.nx;
-        const double zh = static_cast<double>(static_cast<float>(P.kturn * (P.z0 + j * P.sz)));
         double sn, cs;
-        sincos(kTwoPi * (zh - floor(zh)), &sn, &cs);
+        sincos(kTwoPi * (P.zref - floor(P.zref)), &sn, &cs);
         const double r = er * cs - ei * sn;
         ei = er * sn + ei * cs;
         er = r;
@@ -333,38 +370,44 @@ __global__ __launch_bounds__(kThreads) void fwd_reduce(
 }
 
 // ---------------------------------------------------------------------------------
-// Adjoint staging (FP32C): per pixel {fx', |u| k/(2 pi), Z, 0} with fx' = k fx/(2 pi) and
-// Z = z'_hi + (arg(u)/(2 pi) - z'_hi mod 1), so that Z - R' is the phase (turns) of
-// conj(G) u up to whole turns: the pair loop then needs no per-row phase offset.
+// Adjoint staging (FP32C).  Pixel records are laid out quad by quad (4 rows), column
+// by column, 4 records per column: record (quad u, column i, row r) sits at
+// sbase + (u*nxp + i)*4 + r, so an LDS tile of 64 columns x 4 rows is one contiguous
+// run.  Record = {|u| k/2pi, -2|u| k/2pi, arg(u)/2pi - z'_ref mod 1, 0}; padding (rows >= ny,
+// columns >= nx) has |u| = 0 and contributes nothing.
 
 __global__ __launch_bounds__(kThreads) void stage_pixels_fp32c(
     const DevPlane* __restrict__ planes, const PixBlock* __restrict__ pblocks,
     const double2* __restrict__ u, float4* __restrict__ pix) {
     const PixBlock B = pblocks[blockIdx.x];
     const DevPlane& P = planes[B.plane];
-    const int npix = P.nx * P.ny;
-    const int p = B.pix0 + static_cast<int>(threadIdx.x);
-    if (p >= npix) return;
-    const int i = p % P.nx, j = p / P.nx;
-    const int64_t o = P.offset + p;
-    const double kt = P.kturn;
-    const double2 v = u[o];
-    const double zh = static_cast<double>(static_cast<float>(kt * (P.z0 + j * P.sz)));
-    double zeta = atan2(v.y, v.x) * kInv2Pi - zh;
-    zeta -= rint(zeta);
-    pix[o] = make_float4(static_cast<float>(kt * (P.x0 + static_cast<double>(i) * P.sx)),
-                         static_cast<float>(hypot(v.x, v.y) * kt), static_cast<float>(zeta + zh),
-                         0.0f);
+    const int nrec = P.nquads * P.nxp * kBlk;
+    const int rix = B.pix0 + static_cast<int>(threadIdx.x);
+    if (rix >= nrec) return;
+    const int quad = rix / (P.nxp * kBlk);
+    const int rem = rix - quad * P.nxp * kBlk;
+    const int i = rem / kBlk, j = quad * kBlk + rem % kBlk;
+    float4 rec = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (i < P.nx && j < P.ny) {
+        const double2 v = u[P.offset + static_cast<int64_t>(j) * P.nx + i];
+        double chi = atan2(v.y, v.x) * kInv2Pi - P.zref;
+        chi -= rint(chi);
+        const float U = static_cast<float>(hypot(v.x, v.y) * P.kturn);
+        rec = make_float4(U, -2.0f * U, static_cast<float>(chi), 0.0f);
+    }
+    pix[P.sbase + rix] = rec;
 }
 
 // ---------------------------------------------------------------------------------
 // Adjoint, FP32C.  Thread owns PQ pillars (FP64 positions in registers, scaled to
-// turns by each row's plane); each row's pixels stream through LDS.
+// turns by each quad's plane); a quad's pixel records stream through LDS 64 columns
+// at a time and are consumed in 4 x 4 blocks, each with its own reference distance.
+// Per pair: conj(G) u = |u| (k/2pi)/R0 * (R0/R) * exp(2 pi i (chi - th0 - (R - R0))).
 
-template <int PQ>
+template <int PQ, int ORDER>
 __global__ __launch_bounds__(kThreads, 2) void adj_fp32c(
     const double2* __restrict__ pos, int npil, const DevPlane* __restrict__ planes,
-    const AdjRow* __restrict__ rows, const AdjBlock* __restrict__ blocks,
+    const AdjQuad* __restrict__ quads, const AdjBlock* __restrict__ blocks,
     const float4* __restrict__ pix, double2* __restrict__ partial) {
     __shared__ float4 tile[kThreads];
     const AdjBlock B = blocks[blockIdx.x];
@@ -380,41 +423,71 @@ __global__ __launch_bounds__(kThreads, 2) void adj_fp32c(
 #pragma unroll
     for (int k = 0; k < PQ; ++k) sr[k] = si[k] = cr[k] = ci[k] = 0.0f;
 
-    for (int r = B.r0; r < B.r1; ++r) {
-        const AdjRow rw = rows[r];
-        const DevPlane& P = planes[rw.plane];
-        const int nx = P.nx;
+    for (int u = B.u0; u < B.u1; ++u) {
+        const AdjQuad Q = quads[u];
+        const DevPlane& P = planes[Q.plane];
         const double kt = P.kturn;
-        const double fy = P.y0 + static_cast<double>(rw.j) * P.sy;
-        float z2h, z2l;
-        split_sq(kt * (P.z0 + static_cast<double>(rw.j) * P.sz), z2h, z2l);
-        float xq[PQ], dyz2[PQ];
+        const double zref = P.zref;
+        const double jc = Q.j0 + 1.5;
+        const double FYc = kt * (P.y0 + jc * P.sy);
+        const double zc = kt * (P.z0 + jc * P.sz);
+        const double ZC2 = zc * zc;
+        float tex[kBlk], ex2[kBlk], tey[kBlk], eyz[kBlk];
+        block_offsets(kt * P.sx, tex, ex2);
+        block_offsets(kt * P.sy, tey, eyz);
+#pragma unroll
+        for (int r = 0; r < kBlk; ++r) {
+            const double zr = kt * (P.z0 + (Q.j0 + r) * P.sz);
+            eyz[r] += static_cast<float>(zr * zr - ZC2);
+        }
+        double X[PQ], dY0[PQ];
 #pragma unroll
         for (int k = 0; k < PQ; ++k) {
-            xq[k] = static_cast<float>(kt * qx[k]);
-            const float dy = static_cast<float>(kt * fy) - static_cast<float>(kt * qy[k]);
-            dyz2[k] = fmaf(dy, dy, z2l) + z2h;
+            X[k] = kt * qx[k];
+            dY0[k] = FYc - kt * qy[k];
         }
-        const float4* __restrict__ prow = pix + P.offset + static_cast<int64_t>(rw.j) * nx;
-        for (int base = 0; base < nx; base += kThreads) {
-            const int n = min(kThreads, nx - base);
+        const float4* __restrict__ prow =
+            pix + P.sbase + static_cast<int64_t>(Q.j0 / kBlk) * P.nxp * kBlk;
+        for (int cb = 0; cb < P.nxp; cb += kAdjCols) {
+            const int ncol = min(kAdjCols, P.nxp - cb);
             __syncthreads();
-            if (tid < n) tile[tid] = prow[base + tid];
+            if (tid < ncol * kBlk) tile[tid] = prow[cb * kBlk + tid];
             __syncthreads();
             float ar[PQ], ai[PQ];
 #pragma unroll
             for (int k = 0; k < PQ; ++k) ar[k] = ai[k] = 0.0f;
-            for (int p = 0; p < n; ++p) {
-                const float4 X = tile[p];
+            for (int g = 0; g < ncol; g += kBlk) {
+                const double FXc = kt * (P.x0 + (cb + g + 1.5) * P.sx);
 #pragma unroll
                 for (int k = 0; k < PQ; ++k) {
-                    const float dx = X.x - xq[k];
-                    const float R2 = fmaf(dx, dx, dyz2[k]);
-                    float iR;
-                    const float th = __builtin_amdgcn_fractf(pair_turns<-1>(R2, X.z, iR));
-                    const float A = X.y * iR;
-                    ar[k] = fmaf(A, __builtin_amdgcn_cosf(th), ar[k]);
-                    ai[k] = fmaf(A, __builtin_amdgcn_sinf(th), ai[k]);
+                    const double dX0 = FXc - X[k];
+                    float rf;
+                    const double R0 = block_distance(fma(dX0, dX0, fma(dY0[k], dY0[k], ZC2)), rf);
+                    const float th0 = static_cast<float>(__builtin_amdgcn_fract(R0 - zref));
+                    const float h = 0.5f * rf;
+                    const float dx0 = static_cast<float>(dX0), dy0 = static_cast<float>(dY0[k]);
+                    float a[kBlk], b[kBlk];
+#pragma unroll
+                    for (int c = 0; c < kBlk; ++c) a[c] = fmaf(tex[c], dx0, ex2[c]);
+#pragma unroll
+                    for (int r = 0; r < kBlk; ++r) b[r] = fmaf(tey[r], dy0, eyz[r]);
+                    float br = 0.0f, bi = 0.0f;
+#pragma unroll
+                    for (int c = 0; c < kBlk; ++c) {
+#pragma unroll
+                        for (int r = 0; r < kBlk; ++r) {
+                            const float4 W = tile[(g + c) * kBlk + r];
+                            const float x1 = (a[c] + b[r]) * h;
+                            const float y = x1 * h;
+                            float dR, A;
+                            pair_terms<ORDER>(x1, y, W.x, W.y, dR, A);
+                            const float th = (W.z - th0) - dR;
+                            br = fmaf(A, __builtin_amdgcn_cosf(th), br);
+                            bi = fmaf(A, __builtin_amdgcn_sinf(th), bi);
+                        }
+                    }
+                    ar[k] = fmaf(rf, br, ar[k]);
+                    ai[k] = fmaf(rf, bi, ai[k]);
                 }
             }
 #pragma unroll
@@ -442,11 +515,11 @@ __global__ __launch_bounds__(kThreads, 2) void adj_fp32c(
 
 // ---------------------------------------------------------------------------------
 // Adjoint, FP32 and FP64 (reference pair arithmetic, transposed).  One pillar per
-// thread; u rows stream through LDS.
+// thread; each row's u streams through LDS.
 
 __global__ __launch_bounds__(kThreads) void adj_fp32(
     const float4* __restrict__ pil, int npil, const DevPlane* __restrict__ planes,
-    const AdjRow* __restrict__ rows, const AdjBlock* __restrict__ blocks,
+    const AdjQuad* __restrict__ quads, const AdjBlock* __restrict__ blocks,
     const double2* __restrict__ u, double2* __restrict__ partial) {
     __shared__ float2 tile[kThreads];
     const AdjBlock B = blocks[blockIdx.x];
@@ -454,34 +527,36 @@ __global__ __launch_bounds__(kThreads) void adj_fp32(
     const int q = B.q0 + tid;
     const float4 v = pil[min(q, npil - 1)];
     float gr = 0.0f, gi = 0.0f;
-    for (int r = B.r0; r < B.r1; ++r) {
-        const AdjRow rw = rows[r];
-        const DevPlane& P = planes[rw.plane];
+    for (int uq = B.u0; uq < B.u1; ++uq) {
+        const AdjQuad Q = quads[uq];
+        const DevPlane& P = planes[Q.plane];
         const int nx = P.nx;
-        const float fy = static_cast<float>(P.y0) + static_cast<float>(rw.j) * static_cast<float>(P.sy);
-        const float z = static_cast<float>(P.z0) + static_cast<float>(rw.j) * static_cast<float>(P.sz);
         const float k = static_cast<float>(P.k);
-        const float dy = fy - v.y;
-        const double2* __restrict__ urow = u + P.offset + static_cast<int64_t>(rw.j) * nx;
-        for (int base = 0; base < nx; base += kThreads) {
-            const int n = min(kThreads, nx - base);
-            __syncthreads();
-            if (tid < n) {
-                const double2 w = urow[base + tid];
-                tile[tid] = make_float2(static_cast<float>(w.x), static_cast<float>(w.y));
-            }
-            __syncthreads();
-            for (int p = 0; p < n; ++p) {
-                const float2 U = tile[p];
-                const float fx = static_cast<float>(P.x0) +
-                                 static_cast<float>(base + p) * static_cast<float>(P.sx);
-                const float dx = fx - v.x;
-                const float R = sqrtf(dx * dx + dy * dy + z * z);
-                float s, c;
-                sincosf(k * R, &s, &c);
-                const float inv = 1.0f / R;
-                gr += (c * U.x + s * U.y) * inv;
-                gi += (c * U.y - s * U.x) * inv;
+        for (int j = Q.j0; j < min(Q.j0 + kBlk, P.ny); ++j) {
+            const float fy = static_cast<float>(P.y0) + static_cast<float>(j) * static_cast<float>(P.sy);
+            const float z = static_cast<float>(P.z0) + static_cast<float>(j) * static_cast<float>(P.sz);
+            const float dy = fy - v.y;
+            const double2* __restrict__ urow = u + P.offset + static_cast<int64_t>(j) * nx;
+            for (int base = 0; base < nx; base += kThreads) {
+                const int n = min(kThreads, nx - base);
+                __syncthreads();
+                if (tid < n) {
+                    const double2 w = urow[base + tid];
+                    tile[tid] = make_float2(static_cast<float>(w.x), static_cast<float>(w.y));
+                }
+                __syncthreads();
+                for (int p = 0; p < n; ++p) {
+                    const float2 U = tile[p];
+                    const float fx = static_cast<float>(P.x0) +
+                                     static_cast<float>(base + p) * static_cast<float>(P.sx);
+                    const float dx = fx - v.x;
+                    const float R = sqrtf(dx * dx + dy * dy + z * z);
+                    float s, c;
+                    sincosf(k * R, &s, &c);
+                    const float inv = 1.0f / R;
+                    gr += (c * U.x + s * U.y) * inv;
+                    gi += (c * U.y - s * U.x) * inv;
+                }
             }
         }
     }
@@ -490,7 +565,7 @@ __global__ __launch_bounds__(kThreads) void adj_fp32(
 
 __global__ __launch_bounds__(kThreads) void adj_fp64(
     const double4* __restrict__ pil, int npil, const DevPlane* __restrict__ planes,
-    const AdjRow* __restrict__ rows, const AdjBlock* __restrict__ blocks,
+    const AdjQuad* __restrict__ quads, const AdjBlock* __restrict__ blocks,
     const double2* __restrict__ u, double2* __restrict__ partial) {
     __shared__ double2 tile[kThreads];
     const AdjBlock B = blocks[blockIdx.x];
@@ -498,30 +573,32 @@ __global__ __launch_bounds__(kThreads) void adj_fp64(
     const int q = B.q0 + tid;
     const double4 v = pil[min(q, npil - 1)];
     double gr = 0.0, gi = 0.0;
-    for (int r = B.r0; r < B.r1; ++r) {
-        const AdjRow rw = rows[r];
-        const DevPlane& P = planes[rw.plane];
+    for (int uq = B.u0; uq < B.u1; ++uq) {
+        const AdjQuad Q = quads[uq];
+        const DevPlane& P = planes[Q.plane];
         const int nx = P.nx;
-        const double fy = P.y0 + rw.j * P.sy;
-        const double z = P.z0 + rw.j * P.sz;
-        const double dy = fy - v.y;
-        const double2* __restrict__ urow = u + P.offset + static_cast<int64_t>(rw.j) * nx;
-        for (int base = 0; base < nx; base += kThreads) {
-            const int n = min(kThreads, nx - base);
-            __syncthreads();
-            if (tid < n) tile[tid] = urow[base + tid];
-            __syncthreads();
-            for (int p = 0; p < n; ++p) {
-                const double2 U = tile[p];
-                const double fx = P.x0 + (base + p) * P.sx;
-                const double dx = fx - v.x;
-                const double R = sqrt(dx * dx + dy * dy + z * z);
-                double s, c;
-                sincos(P.k * R, &s, &c);
-                const double rho = 1.0 / R;
-                const double pr = rho * c, pim = rho * s;
-                gr += pr * U.x + pim * U.y;
-                gi += pr * U.y - pim * U.x;
+        for (int j = Q.j0; j < min(Q.j0 + kBlk, P.ny); ++j) {
+            const double fy = P.y0 + j * P.sy;
+            const double z = P.z0 + j * P.sz;
+            const double dy = fy - v.y;
+            const double2* __restrict__ urow = u + P.offset + static_cast<int64_t>(j) * nx;
+            for (int base = 0; base < nx; base += kThreads) {
+                const int n = min(kThreads, nx - base);
+                __syncthreads();
+                if (tid < n) tile[tid] = urow[base + tid];
+                __syncthreads();
+                for (int p = 0; p < n; ++p) {
+                    const double2 U = tile[p];
+                    const double fx = P.x0 + (base + p) * P.sx;
+                    const double dx = fx - v.x;
+                    const double R = sqrt(dx * dx + dy * dy + z * z);
+                    double s, c;
+                    sincos(P.k * R, &s, &c);
+                    const double rho = 1.0 / R;
+                    const double pr = rho * c, pim = rho * s;
+                    gr += pr * U.x + pim * U.y;
+                    gi += pr * U.y - pim * U.x;
+                }
             }
         }
     }
@@ -559,30 +636,33 @@ hipError_t launch_stage_pillars(int mode, const double* px, const double* py, co
         double2* pos = static_cast<double2*>(out);
         float2* ap = reinterpret_cast<float2*>(pos + npil);
         stage_pillars_fp32c<<<grid_for(total), kThreads, 0, s>>>(px, py, tt, npil, total, pos, ap);
-    }
-    else if (mode == 2)
+    } else if (mode == 2) {
         stage_pillars_fp32<<<grid_for(total), kThreads, 0, s>>>(px, py, tt, npil, total,
                                                                  static_cast<float4*>(out));
-    else
+    } else {
         stage_pillars_fp64<<<grid_for(total), kThreads, 0, s>>>(px, py, tt, npil, total,
                                                                  static_cast<double4*>(out));
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_forward(int mode, const void* pil, int npil, const DevPlane* planes,
+hipError_t launch_forward(int mode, int order, const void* pil, int npil, const DevPlane* planes,
                           const FwdBlock* blocks, int nblocks, double2* partial, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
-    if (mode == 0)
-        fwd_fp32c<kPX, kPY><<<nblocks, kThreads, 0, s>>>(
-            static_cast<const double2*>(pil),
-            reinterpret_cast<const float2*>(static_cast<const double2*>(pil) + npil), npil, planes,
-            blocks, partial);
-    else if (mode == 2)
+    if (mode == 0) {
+        const double2* pos = static_cast<const double2*>(pil);
+        const float2* ap = reinterpret_cast<const float2*>(pos + npil);
+        if (order == 1)
+            fwd_fp32c<1><<<nblocks, kThreads, 0, s>>>(pos, ap, npil, planes, blocks, partial);
+        else
+            fwd_fp32c<2><<<nblocks, kThreads, 0, s>>>(pos, ap, npil, planes, blocks, partial);
+    } else if (mode == 2) {
         fwd_fp32<<<nblocks, kThreads, 0, s>>>(static_cast<const float4*>(pil), npil, planes,
                                               blocks, partial);
-    else
+    } else {
         fwd_fp64<<<nblocks, kThreads, 0, s>>>(static_cast<const double4*>(pil), npil, planes,
                                               blocks, partial);
+    }
     return hipGetLastError();
 }
 
@@ -602,22 +682,28 @@ hipError_t launch_stage_pixels(const DevPlane* planes, const PixBlock* pblocks, 
     return hipGetLastError();
 }
 
-hipError_t launch_adjoint(int mode, const void* pil, int npil, const DevPlane* planes,
-                          const AdjRow* rows, const AdjBlock* blocks, int nblocks,
+hipError_t launch_adjoint(int mode, int order, const void* pil, int npil, const DevPlane* planes,
+                          const AdjQuad* quads, const AdjBlock* blocks, int nblocks,
                           const void* pix_or_u, double2* partial, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
-    if (mode == 0)
-        adj_fp32c<kPQ><<<nblocks, kThreads, 0, s>>>(static_cast<const double2*>(pil), npil, planes,
-                                                    rows, blocks,
-                                                    static_cast<const float4*>(pix_or_u), partial);
-    else if (mode == 2)
-        adj_fp32<<<nblocks, kThreads, 0, s>>>(static_cast<const float4*>(pil), npil, planes, rows,
+    if (mode == 0) {
+        const double2* pos = static_cast<const double2*>(pil);
+        const float4* pix = static_cast<const float4*>(pix_or_u);
+        if (order == 1)
+            adj_fp32c<kPQ, 1><<<nblocks, kThreads, 0, s>>>(pos, npil, planes, quads, blocks, pix,
+                                                           partial);
+        else
+            adj_fp32c<kPQ, 2><<<nblocks, kThreads, 0, s>>>(pos, npil, planes, quads, blocks, pix,
+                                                           partial);
+    } else if (mode == 2) {
+        adj_fp32<<<nblocks, kThreads, 0, s>>>(static_cast<const float4*>(pil), npil, planes, quads,
                                               blocks, static_cast<const double2*>(pix_or_u),
                                               partial);
-    else
-        adj_fp64<<<nblocks, kThreads, 0, s>>>(static_cast<const double4*>(pil), npil, planes, rows,
+    } else {
+        adj_fp64<<<nblocks, kThreads, 0, s>>>(static_cast<const double4*>(pil), npil, planes, quads,
                                               blocks, static_cast<const double2*>(pix_or_u),
                                               partial);
+    }
     return hipGetLastError();
 }
 
