@@ -83,13 +83,64 @@ bool stream_capturing(hipStream_t s) {
 
 }  // namespace
 
+namespace {
+
+// Host-built descriptor tables travel in one pinned buffer and one async copy.
+struct TableBuilder {
+    std::vector<unsigned char> bytes;
+    size_t add(const void* p, size_t n) {
+        size_t off = (bytes.size() + 255) & ~size_t(255);
+        bytes.resize(off + n);
+        if (n) std::memcpy(bytes.data() + off, p, n);
+        return off;
+    }
+};
+
+struct FwdPlan {
+    int mode = 0;
+    int order = 1;
+    int npil = 0, nsets = 0;
+    int64_t part_elems = 0;
+    int nblocks = 0, npblocks = 0;
+    TableBuilder tb;
+    size_t o_planes = 0, o_blocks = 0, o_pblocks = 0;
+};
+
+struct AdjPlan {
+    int mode = 0;
+    int order = 1;
+    int npil = 0, nsets = 0;
+    int64_t nsplit = 1, n_out = 0, staged_records = 0;
+    int nblocks = 0, npblocks = 0;
+    TableBuilder tb;
+    size_t o_planes = 0, o_quads = 0, o_blocks = 0, o_pblocks = 0;
+};
+
+// A context remembers the last plan of each direction; a call with the same shape
+// (mode, sizes, planes, tuning) reuses it and its device tables without host work.
+struct PlanSlot {
+    std::vector<unsigned char> key;
+    DevBuffer tables;
+    bool valid = false;
+};
+
+constexpr int kPinnedRing = 4;
+
+}  // namespace
+
 struct crs_context {
     int device = 0;
-    DevBuffer pil, pix, tables, partial;
-    void* host_tables = nullptr;
-    size_t host_cap = 0;
-    hipEvent_t tables_ev = nullptr;
-    bool tables_pending = false;
+    DevBuffer pil, pix, partial;
+    PlanSlot slot[2];  // [0] forward, [1] adjoint
+    FwdPlan fwd_plan;
+    AdjPlan adj_plan;
+    // Pinned staging for table uploads: a ring, so a call waits only for the copy
+    // issued kPinnedRing calls earlier.
+    void* host_tables[kPinnedRing] = {nullptr, nullptr, nullptr, nullptr};
+    size_t host_cap[kPinnedRing] = {0, 0, 0, 0};
+    hipEvent_t tables_ev[kPinnedRing] = {nullptr, nullptr, nullptr, nullptr};
+    bool tables_pending[kPinnedRing] = {false, false, false, false};
+    int ring_next = 0;
     int fwd_target[3] = {0, 0, 0};
     int adj_target[3] = {0, 0, 0};
     bool timing = false;
@@ -136,59 +187,57 @@ crs_status ensure(DevBuffer& b, size_t bytes, hipStream_t stream, const char* wh
     return CRS_OK;
 }
 
-// Host-built descriptor tables travel in one pinned buffer and one async copy.
-struct TableBuilder {
-    std::vector<unsigned char> bytes;
-    size_t add(const void* p, size_t n) {
-        size_t off = (bytes.size() + 255) & ~size_t(255);
-        bytes.resize(off + n);
-        if (n) std::memcpy(bytes.data() + off, p, n);
-        return off;
-    }
-};
 
-crs_status upload_tables(crs_context* ctx, const TableBuilder& tb, hipStream_t stream) {
+crs_status upload_tables(crs_context* ctx, DevBuffer& dst, const TableBuilder& tb,
+                         hipStream_t stream) {
     const size_t n = tb.bytes.size();
-    crs_status st = ensure(ctx->tables, n, stream, "descriptor tables");
+    crs_status st = ensure(dst, n, stream, "descriptor tables");
     if (st != CRS_OK) return st;
-    if (ctx->host_cap < n) {
+    const int r = ctx->ring_next;
+    ctx->ring_next = (r + 1) % kPinnedRing;
+    // The copy issued kPinnedRing calls ago may still be reading this pinned buffer.
+    if (ctx->tables_pending[r]) { hipEventSynchronize(ctx->tables_ev[r]); ctx->tables_pending[r] = false; }
+    if (ctx->host_cap[r] < n) {
         if (stream_capturing(stream)) {
             set_error("descriptor tables: pinned staging must grow while capturing");
             return CRS_ERR_CAPTURE;
         }
-        if (ctx->tables_pending) { hipEventSynchronize(ctx->tables_ev); ctx->tables_pending = false; }
-        if (ctx->host_tables) hipHostFree(ctx->host_tables);
-        ctx->host_tables = nullptr;
-        ctx->host_cap = 0;
+        if (ctx->host_tables[r]) hipHostFree(ctx->host_tables[r]);
+        ctx->host_tables[r] = nullptr;
+        ctx->host_cap[r] = 0;
         const size_t want = n + n / 4 + 4096;
-        if (hipHostMalloc(&ctx->host_tables, want, hipHostMallocDefault) != hipSuccess) {
+        if (hipHostMalloc(&ctx->host_tables[r], want, hipHostMallocDefault) != hipSuccess) {
             hipGetLastError();
-            ctx->host_tables = nullptr;
+            ctx->host_tables[r] = nullptr;
             set_error("descriptor tables: hipHostMalloc of %zu bytes failed", want);
             return CRS_ERR_ALLOC;
         }
-        ctx->host_cap = want;
+        ctx->host_cap[r] = want;
     }
-    // The previous call's copy may still be reading the pinned buffer.
-    if (ctx->tables_pending) { hipEventSynchronize(ctx->tables_ev); ctx->tables_pending = false; }
-    std::memcpy(ctx->host_tables, tb.bytes.data(), n);
-    if (hipMemcpyAsync(ctx->tables.ptr, ctx->host_tables, n, hipMemcpyHostToDevice, stream) !=
+    std::memcpy(ctx->host_tables[r], tb.bytes.data(), n);
+    if (hipMemcpyAsync(dst.ptr, ctx->host_tables[r], n, hipMemcpyHostToDevice, stream) !=
         hipSuccess) {
         hipGetLastError();
         set_error("descriptor tables: hipMemcpyAsync failed");
         return CRS_ERR_LAUNCH;
     }
     if (!stream_capturing(stream)) {
-        hipEventRecord(ctx->tables_ev, stream);
-        ctx->tables_pending = true;
+        hipEventRecord(ctx->tables_ev[r], stream);
+        ctx->tables_pending[r] = true;
     }
     return CRS_OK;
 }
 
-template <class T>
-const T* dev_at(crs_context* ctx, size_t off) {
-    return reinterpret_cast<const T*>(static_cast<unsigned char*>(ctx->tables.ptr) + off);
+// Key of a call's plan: everything the tables depend on (not the data pointers).
+std::vector<unsigned char> plan_key(int mode, int npil, int nsets, const crs_plane* planes,
+                                    int nplanes, int target) {
+    std::vector<unsigned char> key(5 * sizeof(int) + static_cast<size_t>(nplanes) * sizeof(crs_plane));
+    const int head[5] = {mode, npil, nsets, nplanes, target};
+    std::memcpy(key.data(), head, sizeof head);
+    if (nplanes) std::memcpy(key.data() + sizeof head, planes, static_cast<size_t>(nplanes) * sizeof(crs_plane));
+    return key;
 }
+
 
 crs_status check_common(crs_context* ctx, int mode, int npil, int nsets, const crs_plane* planes,
                         int nplanes) {
@@ -312,11 +361,13 @@ extern "C" crs_status crs_context_create(int device, crs_context** out) {
     }
     crs_context* ctx = new crs_context();
     ctx->device = device;
-    if (hipEventCreateWithFlags(&ctx->tables_ev, hipEventDisableTiming) != hipSuccess) {
-        hipGetLastError();
-        delete ctx;
-        set_error("hipEventCreate failed");
-        return CRS_ERR_LAUNCH;
+    for (hipEvent_t& e : ctx->tables_ev) {
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            hipGetLastError();
+            crs_context_destroy(ctx);
+            set_error("hipEventCreate failed");
+            return CRS_ERR_LAUNCH;
+        }
     }
     *out = ctx;
     return CRS_OK;
@@ -325,15 +376,17 @@ extern "C" crs_status crs_context_create(int device, crs_context** out) {
 extern "C" void crs_context_destroy(crs_context* ctx) {
     if (!ctx) return;
     hipSetDevice(ctx->device);
-    if (ctx->tables_pending) hipEventSynchronize(ctx->tables_ev);
     hipDeviceSynchronize();
-    for (DevBuffer* b : {&ctx->pil, &ctx->pix, &ctx->tables, &ctx->partial})
+    for (DevBuffer* b : {&ctx->pil, &ctx->pix, &ctx->partial, &ctx->slot[0].tables,
+                         &ctx->slot[1].tables})
         if (b->ptr) hipFree(b->ptr);
     for (auto& pair : ctx->ev)
         for (hipEvent_t e : pair)
             if (e) hipEventDestroy(e);
-    if (ctx->host_tables) hipHostFree(ctx->host_tables);
-    if (ctx->tables_ev) hipEventDestroy(ctx->tables_ev);
+    for (int r = 0; r < kPinnedRing; ++r) {
+        if (ctx->host_tables[r]) hipHostFree(ctx->host_tables[r]);
+        if (ctx->tables_ev[r]) hipEventDestroy(ctx->tables_ev[r]);
+    }
     delete ctx;
 }
 
@@ -400,15 +453,6 @@ struct Scratch {  // where one call's device data lives (a context's or a graph'
     unsigned char* tables;  // device copy of the plan's table bytes
 };
 
-struct FwdPlan {
-    int mode = 0;
-    int order = 1;
-    int npil = 0, nsets = 0;
-    int64_t part_elems = 0;
-    int nblocks = 0, npblocks = 0;
-    TableBuilder tb;
-    size_t o_planes = 0, o_blocks = 0, o_pblocks = 0;
-};
 
 crs_status plan_forward(const crs_context* ctx, int mode, int npil, int nsets,
                         const crs_plane* planes, int nplanes, FwdPlan& plan) {
@@ -505,15 +549,6 @@ crs_status enqueue_forward(const FwdPlan& plan, const Scratch& sc, const double*
                          "forward reduce");
 }
 
-struct AdjPlan {
-    int mode = 0;
-    int order = 1;
-    int npil = 0, nsets = 0;
-    int64_t nsplit = 1, n_out = 0, staged_records = 0;
-    int nblocks = 0, npblocks = 0;
-    TableBuilder tb;
-    size_t o_planes = 0, o_quads = 0, o_blocks = 0, o_pblocks = 0;
-};
 
 crs_status plan_adjoint(const crs_context* ctx, int mode, int npil, int nsets,
                         const crs_plane* planes, int nplanes, AdjPlan& plan) {
@@ -670,13 +705,25 @@ extern "C" crs_status crs_forward(crs_context* ctx, void* stream_, int mode, con
     if (!px || !py || !t) { set_error("null pillar array"); return CRS_ERR_INVALID; }
     if (nplanes == 0 || (!field && !intensity)) return CRS_OK;
     hipStream_t stream = static_cast<hipStream_t>(stream_);
-    FwdPlan plan;
-    if ((st = plan_forward(ctx, mode, npil, nsets, planes, nplanes, plan)) != CRS_OK) return st;
+    PlanSlot& slot = ctx->slot[0];
+    std::vector<unsigned char> key = plan_key(mode, npil, nsets, planes, nplanes, ctx->fwd_target[mode]);
+    const bool reuse = slot.valid && slot.key == key;
+    if (!reuse) {
+        slot.valid = false;
+        ctx->fwd_plan = FwdPlan();
+        if ((st = plan_forward(ctx, mode, npil, nsets, planes, nplanes, ctx->fwd_plan)) != CRS_OK)
+            return st;
+    }
+    const FwdPlan& plan = ctx->fwd_plan;
     Scratch sc{&ctx->pil, &ctx->pix, &ctx->partial, nullptr};
     if ((st = size_forward(plan, sc, stream)) != CRS_OK) return st;
-    if ((st = upload_tables(ctx, plan.tb, stream)) != CRS_OK) return st;
+    if (!reuse) {
+        if ((st = upload_tables(ctx, slot.tables, plan.tb, stream)) != CRS_OK) return st;
+        slot.key = std::move(key);
+        slot.valid = true;
+    }
     if (inject("launch")) { set_error("injected launch failure"); return CRS_ERR_LAUNCH; }
-    sc.tables = static_cast<unsigned char*>(ctx->tables.ptr);
+    sc.tables = static_cast<unsigned char*>(slot.tables.ptr);
     const bool timed = ctx->timing && !stream_capturing(stream);
     st = enqueue_forward(plan, sc, px, py, t, field, intensity, stream,
                          timed ? ctx->ev[0][0] : nullptr, timed ? ctx->ev[0][1] : nullptr);
@@ -692,13 +739,25 @@ extern "C" crs_status crs_adjoint(crs_context* ctx, void* stream_, int mode, con
     if (st != CRS_OK) return st;
     if (!px || !py || !g || (nplanes > 0 && !u)) { set_error("null array argument"); return CRS_ERR_INVALID; }
     hipStream_t stream = static_cast<hipStream_t>(stream_);
-    AdjPlan plan;
-    if ((st = plan_adjoint(ctx, mode, npil, nsets, planes, nplanes, plan)) != CRS_OK) return st;
+    PlanSlot& slot = ctx->slot[1];
+    std::vector<unsigned char> key = plan_key(mode, npil, nsets, planes, nplanes, ctx->adj_target[mode]);
+    const bool reuse = slot.valid && slot.key == key;
+    if (!reuse) {
+        slot.valid = false;
+        ctx->adj_plan = AdjPlan();
+        if ((st = plan_adjoint(ctx, mode, npil, nsets, planes, nplanes, ctx->adj_plan)) != CRS_OK)
+            return st;
+    }
+    const AdjPlan& plan = ctx->adj_plan;
     Scratch sc{&ctx->pil, &ctx->pix, &ctx->partial, nullptr};
     if ((st = size_adjoint(plan, sc, stream)) != CRS_OK) return st;
-    if ((st = upload_tables(ctx, plan.tb, stream)) != CRS_OK) return st;
+    if (!reuse) {
+        if ((st = upload_tables(ctx, slot.tables, plan.tb, stream)) != CRS_OK) return st;
+        slot.key = std::move(key);
+        slot.valid = true;
+    }
     if (inject("launch")) { set_error("injected launch failure"); return CRS_ERR_LAUNCH; }
-    sc.tables = static_cast<unsigned char*>(ctx->tables.ptr);
+    sc.tables = static_cast<unsigned char*>(slot.tables.ptr);
     const bool timed = ctx->timing && !stream_capturing(stream);
     st = enqueue_adjoint(plan, sc, px, py, u, g, stream, timed ? ctx->ev[1][0] : nullptr,
                          timed ? ctx->ev[1][1] : nullptr);
