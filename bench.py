@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU time budget of the cpu_baseline sample")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="also run a plain device copy of BYTES (reference for HBM counters)")
     return ap.parse_args()
 
 
@@ -199,6 +201,14 @@ def main():
         cpu = cpu_baseline(lens, u.cpu().numpy(), args.cpu_seconds)
         print(f"[bench] cpu baseline {cpu['value']:.3f} G pairs/s on {cpu['cores']} threads",
               file=sys.stderr, flush=True)
+
+    if args.calibrate_copy > 0:
+        n = args.calibrate_copy // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
 
     out = {
         "metric": "G pillar·pixel pairs/s fwd+adjoint, FP32-comp vs FP64 at ≤1e-5 rel L2",

@@ -77,6 +77,16 @@ SIGNATURES = [
       c_void_p, POINTER(c_void_p)]),
     ("crs_graph_launch", c_int, [c_void_p, c_void_p]),
     ("crs_graph_destroy", None, [c_void_p]),
+    ("crs_forward_host", c_int,
+     [c_int, _DP, _DP, _DP, c_int, c_int, POINTER(crs_plane), c_int, _DP, _DP]),
+    ("crs_analyze_focus", c_int,
+     [_DP, _DP, _DP, _DP, c_int, c_double, c_double, c_double, c_int, _DP, _DP]),
+    ("crs_analyze_through_focus", c_int,
+     [_DP, _DP, _DP, c_int, c_double, c_double, c_double, c_int, c_int, _DP, _DP, _DP, _DP]),
+    ("crs_analyze_chromatic", c_int,
+     [_DP, _DP, _DP, c_int, c_double, c_double, c_double, c_double, c_int, c_int, _DP]),
+    ("crs_analyze_field_grid", c_int,
+     [_DP, _DP, _DP, c_int, c_double, c_double, c_double, c_double, c_int, c_int, c_int, _DP]),
     ("crs_set_timing", c_int, [c_void_p, c_int]),
     ("crs_last_kernel_ms", c_int, [c_void_p, c_int, POINTER(ctypes.c_float)]),
     ("crs_kernel_name", c_char_p, [c_int, c_int]),
@@ -98,6 +108,13 @@ def load() -> ctypes.CDLL:
         raise LibraryMissing(
             f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
         )
+    # One HIP runtime per process: torch carries its own libamdhip64 with the same
+    # soname as ROCm's, and whichever is loaded first serves every later user.  Load
+    # torch's first (when torch is installed) so torch and this library share it.
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     for name, res, args in SIGNATURES:
         fn = getattr(lib, name)

@@ -1,0 +1,305 @@
+// rs_analysis.cpp — the reference's focal-plane analyses on the MI355X propagation,
+// each as ONE batched launch (crs_forward_host) instead of a GPU grid plus CPU loops.
+//
+//   crs_analyze_focus          analysis/focal.cpp:32-146   (analyze_focus)
+//   crs_analyze_through_focus  analysis/throughfocus.cpp:15-132
+//   crs_analyze_chromatic      analysis/chromatic.cpp:9-58
+//   crs_analyze_field_grid     analysis/field.cpp:150-233
+//
+// Inputs are the aperture pillar list the reference builds from a MetalensDesign and
+// a UnitCellLibrary (focal.cpp:42-57): positions, complex transmissions and, for the
+// focus metrics, each cell's ideal focusing phase.  The scalar post-processing
+// (peaks, FWHM, encircled energy, normalisation, argmax) follows the reference line
+// by line; only the sums move to the GPU.  In the reference the on-axis scans of
+// through-focus and chromatic, and analyze_focus's ideal peak, were FP64 CPU loops.
+#include "../../include/celeris_rs.h"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <vector>
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+crs_plane psf_plane(double cx, double cy, double z, double k, int n, double W, int tset,
+                    int64_t offset) {
+    crs_plane p{};
+    p.k = k;
+    p.x0 = cx - W;
+    p.sx = 2.0 * W / (n - 1);
+    p.y0 = cy - W;
+    p.sy = p.sx;
+    p.z0 = z;
+    p.sz = 0.0;
+    p.nx = n;
+    p.ny = n;
+    p.tset = tset;
+    p.offset = offset;
+    return p;
+}
+
+crs_plane axis_plane(double zlo, double zhi, int nz, double k, int tset, int64_t offset) {
+    crs_plane p{};
+    p.k = k;
+    p.z0 = zlo;
+    p.sz = nz > 1 ? (zhi - zlo) / (nz - 1) : 0.0;
+    p.nx = 1;
+    p.ny = nz;
+    p.tset = tset;
+    p.offset = offset;
+    return p;
+}
+
+// FWHM along a line of samples: (hi - lo) * step over the samples >= half, or 0
+// (focal.cpp:117-124).
+double fwhm_line(const double* v, int n, int stride, double half, double step) {
+    int lo = -1, hi = -1;
+    for (int s = 0; s < n; ++s) {
+        if (v[static_cast<int64_t>(s) * stride] >= half) {
+            if (lo < 0) lo = s;
+            hi = s;
+        }
+    }
+    return (hi > lo && lo >= 0) ? (hi - lo) * step : 0.0;
+}
+
+}  // namespace
+
+extern "C" crs_status crs_analyze_focus(const double* px, const double* py, const double* t,
+                                        const double* phi_ideal, int npil, double focal_um,
+                                        double wavelength_um, double diameter_um, int mode,
+                                        double* out5, double* intensity) {
+    if (!px || !py || !t || !phi_ideal || !out5 || npil <= 0) return CRS_ERR_INVALID;
+    const double f = focal_um;
+    const double k = 2.0 * kPi / wavelength_um;
+    const double dl = wavelength_um * focal_um / diameter_um;
+    const double W = std::max(6.0 * dl, 4.0);  // focal.cpp:61-64
+    const int NG = 121;
+    const double step = 2.0 * W / (NG - 1);
+    const int64_t ng2 = static_cast<int64_t>(NG) * NG;
+
+    // Set 0: the design; set 1: the ideal lens, polar(1, phi_ideal) (focal.cpp:72).
+    std::vector<double> tt(4 * static_cast<size_t>(npil));
+    for (int q = 0; q < npil; ++q) {
+        tt[2 * q] = t[2 * q];
+        tt[2 * q + 1] = t[2 * q + 1];
+        tt[2 * (npil + q)] = std::cos(phi_ideal[q]);
+        tt[2 * (npil + q) + 1] = std::sin(phi_ideal[q]);
+    }
+    crs_plane planes[2] = {psf_plane(0.0, 0.0, f, k, NG, W, 0, 0),
+                           axis_plane(f, f, 1, k, 1, ng2)};
+    std::vector<double> field(2 * (ng2 + 1)), I(ng2 + 1);
+    crs_status st = crs_forward_host(mode, px, py, tt.data(), npil, 2, planes, 2, field.data(),
+                                     I.data());
+    if (st != CRS_OK) return st;
+
+    const double peak = *std::max_element(I.begin(), I.begin() + ng2);
+    const double peak_ideal = I[ng2];
+    double amp_inphase = 0.0;  // focal.cpp:109-115
+    for (int q = 0; q < npil; ++q) {
+        const double Rr = std::sqrt(px[q] * px[q] + py[q] * py[q] + f * f);
+        amp_inphase += std::abs(std::complex<double>(t[2 * q], t[2 * q + 1])) / Rr;
+    }
+    const double peak_same_amp = amp_inphase * amp_inphase;
+    const int center_row = NG / 2;
+    const double fwhm = fwhm_line(I.data() + static_cast<int64_t>(center_row) * NG, NG, 1,
+                                  peak / 2.0, step);
+    const double r_null = 1.22 * dl;  // focal.cpp:126-137
+    double in = 0.0, total = 0.0;
+    for (int j = 0; j < NG; ++j) {
+        const double fy = -W + j * step;
+        for (int i = 0; i < NG; ++i) {
+            const double fx = -W + i * step;
+            const double v = I[static_cast<int64_t>(j) * NG + i];
+            total += v;
+            if (std::sqrt(fx * fx + fy * fy) <= r_null) in += v;
+        }
+    }
+    out5[0] = peak_ideal > 0 ? peak / peak_ideal : 0.0;
+    out5[1] = peak_same_amp > 0 ? peak / peak_same_amp : 0.0;
+    out5[2] = fwhm;
+    out5[3] = dl;
+    out5[4] = total > 0 ? in / total : 0.0;
+    if (intensity) std::copy(I.begin(), I.begin() + ng2, intensity);
+    return CRS_OK;
+}
+
+extern "C" crs_status crs_analyze_through_focus(const double* px, const double* py,
+                                                const double* t, int npil, double focal_um,
+                                                double wavelength_um, double diameter_um,
+                                                int caustic_n, int mode, double* z161,
+                                                double* I161, double* scalars6,
+                                                double* caustic) {
+    if (!px || !py || !t || !z161 || !I161 || !scalars6 || npil <= 0 || caustic_n < 2)
+        return CRS_ERR_INVALID;
+    const double f = focal_um;
+    const double k = 2.0 * kPi / wavelength_um;
+    const double R = diameter_um / 2.0;
+    // throughfocus.cpp:78-82
+    const double na = std::sin(std::atan(R / f));
+    const double half = std::max(2.0 * wavelength_um / (na * na + 1e-6), 0.25 * f);
+    const double zlo = std::max(0.1 * f, f - half);
+    const double zhi = f + half;
+    const int N = 161;
+    // throughfocus.cpp:113-116
+    const double dl = wavelength_um * f / std::max(diameter_um, 1e-6);
+    const double Wx = std::max(8.0 * dl, 3.0);
+    const int cn = caustic_n;
+
+    crs_plane planes[2] = {axis_plane(zlo, zhi, N, k, 0, 0), crs_plane{}};
+    planes[1].k = k;
+    planes[1].x0 = -Wx;
+    planes[1].sx = (2.0 * Wx) / (cn - 1);
+    planes[1].z0 = zlo;
+    planes[1].sz = (zhi - zlo) / (cn - 1);
+    planes[1].nx = cn;
+    planes[1].ny = cn;
+    planes[1].offset = N;
+    const int64_t total = N + static_cast<int64_t>(cn) * cn;
+    std::vector<double> I(total);
+    crs_status st = crs_forward_host(mode, px, py, t, npil, 1, planes, 2, nullptr, I.data());
+    if (st != CRS_OK) return st;
+
+    double peak = 0.0;
+    for (int i = 0; i < N; ++i) {
+        z161[i] = static_cast<float>(zlo + (zhi - zlo) * i / (N - 1));  // stored as float
+        peak = std::max(peak, I[i]);
+    }
+    if (peak <= 0) peak = 1;
+    int ipk = 0;
+    for (int i = 0; i < N; ++i) {
+        I161[i] = static_cast<float>(I[i] / peak);
+        if (I[i] > I[ipk]) ipk = i;
+    }
+    int lo = -1, hi = -1;
+    for (int i = 0; i < N; ++i)
+        if (I[i] >= peak / 2.0) { if (lo < 0) lo = i; hi = i; }
+    scalars6[0] = z161[ipk];
+    scalars6[1] = (hi > lo && lo >= 0) ? (z161[hi] - z161[lo]) : 0.0;
+    scalars6[2] = -Wx;
+    scalars6[3] = Wx;
+    scalars6[4] = zlo;
+    scalars6[5] = zhi;
+    if (caustic) {
+        double cpeak = 0.0;
+        for (int64_t i = N; i < total; ++i) cpeak = std::max(cpeak, I[i]);
+        if (cpeak <= 0) cpeak = 1.0;
+        for (int64_t i = N; i < total; ++i) caustic[i - N] = static_cast<float>(I[i] / cpeak);
+    }
+    return CRS_OK;
+}
+
+extern "C" crs_status crs_analyze_chromatic(const double* px, const double* py, const double* t,
+                                            int npil, double design_focal, double design_wavelength,
+                                            double lambda_min, double lambda_max, int n, int mode,
+                                            double* out3n) {
+    if (!px || !py || !t || !out3n || npil <= 0 || n < 2) return CRS_ERR_INVALID;
+    const int NZ = 120;  // chromatic.cpp:48
+    std::vector<crs_plane> planes;
+    planes.reserve(n + 1);
+    // chromatic.cpp:40: the design-wavelength peak at the design focus.
+    planes.push_back(axis_plane(design_focal, design_focal, 1, 2.0 * kPi / design_wavelength, 0, 0));
+    std::vector<double> lam(n), zlo(n), zhi(n);
+    for (int i = 0; i < n; ++i) {
+        lam[i] = lambda_min + (lambda_max - lambda_min) * i / (n - 1);
+        const double expected = design_focal * design_wavelength / lam[i];
+        zlo[i] = 0.55 * expected;
+        zhi[i] = 1.6 * expected;
+        planes.push_back(axis_plane(zlo[i], zhi[i], NZ, 2.0 * kPi / lam[i], 0,
+                                    1 + static_cast<int64_t>(i) * NZ));
+    }
+    std::vector<double> I(1 + static_cast<size_t>(n) * NZ);
+    crs_status st = crs_forward_host(mode, px, py, t, npil, 1, planes.data(),
+                                     static_cast<int>(planes.size()), nullptr, I.data());
+    if (st != CRS_OK) return st;
+    const double design_peak = I[0];
+    for (int i = 0; i < n; ++i) {
+        const double expected = design_focal * design_wavelength / lam[i];
+        double best_z = expected, best_I = -1.0;
+        for (int j = 0; j < NZ; ++j) {
+            const double z = zlo[i] + (zhi[i] - zlo[i]) * j / (NZ - 1);
+            const double v = I[1 + static_cast<size_t>(i) * NZ + j];
+            if (v > best_I) { best_I = v; best_z = z; }
+        }
+        out3n[3 * i] = lam[i];
+        out3n[3 * i + 1] = best_z;
+        out3n[3 * i + 2] = design_peak > 0 ? best_I / design_peak : 0.0;
+    }
+    return CRS_OK;
+}
+
+extern "C" crs_status crs_analyze_field_grid(const double* px, const double* py, const double* t,
+                                             int npil, double focal_um, double wavelength_um,
+                                             double diameter_um, double max_angle_deg, int n_half,
+                                             int psf_n, int mode, double* out7) {
+    if (!px || !py || !t || !out7 || npil <= 0 || n_half < 0 || psf_n < 2) return CRS_ERR_INVALID;
+    const double k = 2.0 * kPi / wavelength_um;
+    const double dl = wavelength_um * focal_um / diameter_um;  // field.cpp:176-178
+    const double W = std::max(6.0 * dl, 4.0);
+    const double step = 2.0 * W / (psf_n - 1);
+    const int g = 2 * n_half + 1;
+    const int npts = 1 + g * g;  // the on-axis reference spot, then the grid
+    const int64_t npix = static_cast<int64_t>(psf_n) * psf_n;
+
+    // One transmission set per field point: t * polar(1, k (sx x + sy y)) (field.cpp:186-187).
+    std::vector<double> tt(2 * static_cast<size_t>(npil) * npts);
+    std::vector<crs_plane> planes(npts);
+    std::vector<double> ang_x(npts), ang_y(npts), cxs(npts), cys(npts);
+    for (int p = 0; p < npts; ++p) {
+        double ax = 0.0, ay = 0.0;
+        if (p > 0) {
+            const int jy = (p - 1) / g - n_half, jx = (p - 1) % g - n_half;
+            ay = n_half > 0 ? max_angle_deg * jy / n_half : 0.0;
+            ax = n_half > 0 ? max_angle_deg * jx / n_half : 0.0;
+        }
+        const double tx = ax * kPi / 180.0, ty = ay * kPi / 180.0;
+        const double cx = focal_um * std::tan(tx), cy = focal_um * std::tan(ty);
+        const double sx = std::sin(tx), sy = std::sin(ty);
+        double* ts = tt.data() + 2 * static_cast<size_t>(npil) * p;
+        for (int q = 0; q < npil; ++q) {
+            const std::complex<double> v = std::complex<double>(t[2 * q], t[2 * q + 1]) *
+                                           std::polar(1.0, k * (sx * px[q] + sy * py[q]));
+            ts[2 * q] = v.real();
+            ts[2 * q + 1] = v.imag();
+        }
+        planes[p] = psf_plane(cx, cy, focal_um, k, psf_n, W, p, npix * p);
+        ang_x[p] = ax;
+        ang_y[p] = ay;
+        cxs[p] = cx;
+        cys[p] = cy;
+    }
+    std::vector<double> I(npix * npts);
+    crs_status st = crs_forward_host(mode, px, py, tt.data(), npil, npts, planes.data(), npts,
+                                     nullptr, I.data());
+    if (st != CRS_OK) return st;
+
+    // Peak pixel and tangential / sagittal FWHM of each spot (field.cpp:191-208).
+    std::vector<double> peak(npts), fwx(npts), fwy(npts);
+    for (int p = 0; p < npts; ++p) {
+        const double* m = I.data() + npix * p;
+        int pix = 0, piy = 0;
+        double best = -1.0;
+        for (int j = 0; j < psf_n; ++j)
+            for (int i = 0; i < psf_n; ++i) {
+                const double v = m[static_cast<int64_t>(j) * psf_n + i];
+                if (v > best) { best = v; pix = i; piy = j; }
+            }
+        peak[p] = best;
+        fwx[p] = fwhm_line(m + static_cast<int64_t>(piy) * psf_n, psf_n, 1, best / 2.0, step);
+        fwy[p] = fwhm_line(m + pix, psf_n, psf_n, best / 2.0, step);
+    }
+    const double ref = peak[0] > 0 ? peak[0] : 1.0;  // field.cpp:210-211
+    for (int p = 1; p < npts; ++p) {
+        double* o = out7 + 7 * static_cast<size_t>(p - 1);
+        o[0] = ang_x[p];
+        o[1] = ang_y[p];
+        o[2] = peak[p] / ref;
+        o[3] = fwx[p];
+        o[4] = fwy[p];
+        o[5] = cxs[p];
+        o[6] = cys[p];
+    }
+    return CRS_OK;
+}

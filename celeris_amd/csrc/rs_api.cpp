@@ -951,40 +951,67 @@ void release_state(DropinState* s) {
     g_pool.push_back(s);
 }
 
-// Runs one plane through crs_forward with host inputs; on success copies |E|^2 into
-// `out` (n_out doubles).  `out` is written only when every step succeeded.
-int dropin_run(const double* px, const double* py, const double* pt, int npil,
-               const crs_plane& plane, double* out) {
+// Runs planes through crs_forward with host inputs and host outputs (complex field
+// and/or |E|^2, each `total` elements).  Outputs are written only when every step
+// succeeded.
+crs_status host_run(int mode, const double* px, const double* py, const double* t, int npil,
+                    int nsets, const crs_plane* planes, int nplanes, double* field,
+                    double* intensity) {
+    int64_t total = 0;
+    for (int p = 0; p < nplanes; ++p)
+        total = std::max<int64_t>(total, planes[p].offset + static_cast<int64_t>(planes[p].nx) * planes[p].ny);
     DropinState* s = acquire_state();
-    if (!s) return 0;
-    const size_t n_out = static_cast<size_t>(plane.nx) * plane.ny;
+    if (!s) return CRS_ERR_NO_DEVICE;
     const size_t pil_bytes = static_cast<size_t>(npil) * sizeof(double);
-    int ok = 0;
-    if (ensure(s->in, 4 * pil_bytes, s->stream, "drop-in inputs") == CRS_OK &&
-        ensure(s->out, n_out * sizeof(double), s->stream, "drop-in output") == CRS_OK) {
+    const size_t t_bytes = 2 * pil_bytes * static_cast<size_t>(nsets);
+    const size_t f_elems = field ? 2 * static_cast<size_t>(total) : 0;
+    const size_t i_elems = intensity ? static_cast<size_t>(total) : 0;
+    crs_status st = ensure(s->in, 2 * pil_bytes + t_bytes, s->stream, "host inputs");
+    if (st == CRS_OK)
+        st = ensure(s->out, (f_elems + i_elems) * sizeof(double), s->stream, "host outputs");
+    if (st == CRS_OK) {
         double* d_px = static_cast<double*>(s->in.ptr);
         double* d_py = d_px + npil;
         double* d_t = d_py + npil;
-        if (hipMemcpyAsync(d_px, px, pil_bytes, hipMemcpyHostToDevice, s->stream) == hipSuccess &&
-            hipMemcpyAsync(d_py, py, pil_bytes, hipMemcpyHostToDevice, s->stream) == hipSuccess &&
-            hipMemcpyAsync(d_t, pt, 2 * pil_bytes, hipMemcpyHostToDevice, s->stream) == hipSuccess &&
-            crs_forward(s->ctx, s->stream, g_default_mode.load(), d_px, d_py, d_t, npil, 1, &plane, 1,
-                        nullptr, static_cast<double*>(s->out.ptr)) == CRS_OK) {
-            s->host_out.resize(n_out);
-            if (hipMemcpyAsync(s->host_out.data(), s->out.ptr, n_out * sizeof(double),
-                               hipMemcpyDeviceToHost, s->stream) == hipSuccess &&
-                hipStreamSynchronize(s->stream) == hipSuccess) {
-                std::memcpy(out, s->host_out.data(), n_out * sizeof(double));
-                ok = 1;
+        double* d_f = field ? static_cast<double*>(s->out.ptr) : nullptr;
+        double* d_i = intensity ? static_cast<double*>(s->out.ptr) + f_elems : nullptr;
+        if (hipMemcpyAsync(d_px, px, pil_bytes, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+            hipMemcpyAsync(d_py, py, pil_bytes, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+            hipMemcpyAsync(d_t, t, t_bytes, hipMemcpyHostToDevice, s->stream) != hipSuccess) {
+            hipGetLastError();
+            set_error("host inputs: hipMemcpyAsync failed");
+            st = CRS_ERR_LAUNCH;
+        }
+        if (st == CRS_OK)
+            st = crs_forward(s->ctx, s->stream, mode, d_px, d_py, d_t, npil, nsets, planes, nplanes,
+                             d_f, d_i);
+        if (st == CRS_OK) {
+            s->host_out.resize(f_elems + i_elems);
+            if (hipMemcpyAsync(s->host_out.data(), s->out.ptr, (f_elems + i_elems) * sizeof(double),
+                               hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+                hipStreamSynchronize(s->stream) != hipSuccess) {
+                hipGetLastError();
+                set_error("host outputs: copy or synchronize failed");
+                st = CRS_ERR_LAUNCH;
+            } else {
+                if (field) std::memcpy(field, s->host_out.data(), f_elems * sizeof(double));
+                if (intensity)
+                    std::memcpy(intensity, s->host_out.data() + f_elems, i_elems * sizeof(double));
             }
         }
     }
-    if (!ok) {
+    if (st != CRS_OK) {
         hipGetLastError();
         hipStreamSynchronize(s->stream);
     }
     release_state(s);
-    return ok;
+    return st;
+}
+
+// One plane of the drop-in layer: |E|^2 into `out`, nonzero on success.
+int dropin_run(const double* px, const double* py, const double* pt, int npil,
+               const crs_plane& plane, double* out) {
+    return host_run(g_default_mode.load(), px, py, pt, npil, 1, &plane, 1, nullptr, out) == CRS_OK;
 }
 
 }  // namespace
@@ -1009,6 +1036,17 @@ extern "C" const char* crs_device_name(void) {
         std::snprintf(name, sizeof name, "%s", prop.name);
     });
     return name;
+}
+
+extern "C" crs_status crs_forward_host(int mode, const double* px, const double* py,
+                                       const double* t, int npil, int nsets,
+                                       const crs_plane* planes, int nplanes, double* field,
+                                       double* intensity) {
+    g_last_error.clear();
+    if (!crs_available()) { set_error("no HIP device visible"); return CRS_ERR_NO_DEVICE; }
+    if (!px || !py || !t || (!field && !intensity)) { set_error("null array argument"); return CRS_ERR_INVALID; }
+    if (nplanes <= 0 || !planes) { set_error("no planes"); return CRS_ERR_INVALID; }
+    return host_run(mode, px, py, t, npil, nsets, planes, nplanes, field, intensity);
 }
 
 extern "C" int crs_set_default_mode(int mode) {

@@ -144,6 +144,56 @@ crs_status crs_adjoint(crs_context *ctx, void *stream, int mode,
                        const crs_plane *planes, int nplanes,
                        const double *u, double *g);
 
+/* crs_forward on HOST arrays (no reference counterpart): inputs are copied to the
+ * device, the planes run in one launch, and the outputs (field: complex, and/or
+ * intensity, total-pixel length) are copied back before the call returns; they are
+ * written only on success.  Synchronous and thread-safe like crs_propagate_psf.  The
+ * analysis entry points below are built on it. */
+crs_status crs_forward_host(int mode, const double *px, const double *py, const double *t,
+                            int npil, int nsets, const crs_plane *planes, int nplanes,
+                            double *field, double *intensity);
+
+/* ---- analyses on the propagation (reference: src/celeris/analysis/) -------------------
+ * Each takes the aperture pillar list the reference builds from a MetalensDesign and a
+ * UnitCellLibrary (focal.cpp:42-57: cells inside the circular aperture, t from the
+ * library), runs all of its sums in one batched launch, and does the reference's scalar
+ * post-processing on the host.  `mode` as for crs_forward.  Synchronous. */
+
+/* analyze_focus (focal.cpp:32-146).  phi_ideal: each cell's ideal focusing phase
+ * (focal.cpp:52-54).  out5 = {strehl, phase_strehl, fwhm_um, diffraction_limit_um,
+ * encircled_energy}; intensity (optional) receives the 121 x 121 |E|^2 grid. */
+crs_status crs_analyze_focus(const double *px, const double *py, const double *t,
+                             const double *phi_ideal, int npil, double focal_um,
+                             double wavelength_um, double diameter_um, int mode, double *out5,
+                             double *intensity);
+
+/* analyze_through_focus (throughfocus.cpp:15-132): the 161-point on-axis scan and a
+ * caustic_n x caustic_n y=0 slice (201 on the reference's GPU path, 81 on its CPU path).
+ * z161, I161: axial positions and peak-normalised intensity (float-rounded, as stored
+ * by the reference); scalars6 = {z_peak, dof, xmin, xmax, zmin, zmax}; caustic
+ * (optional, caustic_n^2, row = z) peak-normalised. */
+crs_status crs_analyze_through_focus(const double *px, const double *py, const double *t,
+                                     int npil, double focal_um, double wavelength_um,
+                                     double diameter_um, int caustic_n, int mode, double *z161,
+                                     double *I161, double *scalars6, double *caustic);
+
+/* analyze_chromatic (chromatic.cpp:9-58): n wavelengths x 120 on-axis z samples plus
+ * the design-wavelength peak.  out3n = n x {lambda, best focus z, peak / design peak}. */
+crs_status crs_analyze_chromatic(const double *px, const double *py, const double *t, int npil,
+                                 double design_focal, double design_wavelength,
+                                 double lambda_min, double lambda_max, int n, int mode,
+                                 double *out3n);
+
+/* analyze_field_grid (field.cpp:150-233): tilted-illumination spots on a
+ * (2 n_half + 1)^2 grid of field angles up to max_angle_deg, each psf_n^2, plus the
+ * on-axis reference spot, all in one launch (one transmission set per field point).
+ * out7 = per point {angle_x, angle_y, rel_strehl, fwhm_x, fwhm_y, cx, cy}, row-major
+ * with angle_y outer. */
+crs_status crs_analyze_field_grid(const double *px, const double *py, const double *t, int npil,
+                                  double focal_um, double wavelength_um, double diameter_um,
+                                  double max_angle_deg, int n_half, int psf_n, int mode,
+                                  double *out7);
+
 /* Launch-shape tuning (no reference counterpart).  target_blocks: work-groups a
  * forward/adjoint launch aims for through its split over pillar chunks (forward)
  * or pixel rows (adjoint); 0 restores the built-in default.  Applies to the
