@@ -1,0 +1,237 @@
+"""The reference's focal-plane analyses, on the MI355X propagation.
+
+Same names, arguments and result fields as the reference's C++ API and its Python
+module (src/celeris/analysis/*.hpp; bindings/python/celeris_module.cpp:415-440):
+
+* :func:`analyze_focus`         focal.cpp:32-146      -> :class:`FocalAnalysis`
+* :func:`compute_psf`           focal.cpp:148-203     -> :class:`PsfMap`
+* :func:`propagate_pillars`     focal.cpp:205-238     -> :class:`PsfMap`
+* :func:`compute_psf_field`     focal.cpp:240-311     -> :class:`FieldPsf`
+* :func:`analyze_through_focus` throughfocus.cpp:15-132 -> :class:`ThroughFocus`
+* :func:`analyze_chromatic`     chromatic.cpp:9-58    -> list of :class:`ChromaticPoint`
+* :func:`analyze_field_grid`    field.cpp:150-233     -> :class:`FieldGrid`
+
+Each runs its sums in one launch through the C ABI (crs_analyze_*, crs_forward_host);
+where the reference looped on the CPU (on-axis scans, the ideal peak) the sums are on
+the GPU too.  ``mode`` selects the arithmetic ("fp32c" default, "fp64", "fp32").
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+from typing import List
+
+import numpy as np
+
+from . import _lib
+from .design import MetalensDesign, UnitCellLibrary, aperture_pillars
+from .rs import _mode, crs_plane
+
+_DP = ctypes.POINTER(ctypes.c_double)
+
+
+def _p(a):
+    return a.ctypes.data_as(_DP)
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+@dataclass
+class FocalAnalysis:
+    strehl: float
+    phase_strehl: float
+    fwhm_um: float
+    diffraction_limit_um: float
+    encircled_energy: float
+    intensity: np.ndarray = None  # the 121 x 121 |E|^2 grid the metrics come from
+
+
+@dataclass
+class PsfMap:
+    n: int
+    half_window_um: float
+    intensity: np.ndarray  # (n, n), |E|^2, row j = y
+
+
+@dataclass
+class FieldPsf:
+    psf: PsfMap
+    angle_deg: float
+    cx_um: float
+    rel_strehl: float
+
+
+@dataclass
+class ThroughFocus:
+    z_um: np.ndarray          # float32, 161 axial positions
+    intensity: np.ndarray     # float32, on-axis |E|^2 / peak
+    z_peak_um: float
+    dof_um: float
+    caustic_nx: int = 0
+    caustic_nz: int = 0
+    caustic_xmin: float = 0.0
+    caustic_xmax: float = 0.0
+    caustic_zmin: float = 0.0
+    caustic_zmax: float = 0.0
+    caustic: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))
+
+
+@dataclass
+class ChromaticPoint:
+    wavelength_um: float
+    focal_length_um: float
+    rel_peak: float
+
+
+@dataclass
+class FieldGridPoint:
+    theta_x_deg: float
+    theta_y_deg: float
+    rel_strehl: float
+    fwhm_x_um: float
+    fwhm_y_um: float
+    cx_um: float
+    cy_um: float
+
+
+@dataclass
+class FieldGrid:
+    n: int
+    max_angle_deg: float
+    points: List[FieldGridPoint]
+
+
+def _check(status, what):
+    _lib.check(status, what)
+
+
+def analyze_focus(lens: MetalensDesign, lib: UnitCellLibrary, focal_length_um: float,
+                  wavelength_um: float, diameter_um: float, mode="fp32c") -> FocalAnalysis:
+    px, py, t, phi = aperture_pillars(lens, lib, diameter_um, focal_length_um, wavelength_um)
+    return analyze_focus_pillars(px, py, t, phi, focal_length_um, wavelength_um, diameter_um, mode)
+
+
+def analyze_focus_pillars(px, py, t, phi_ideal, focal_length_um, wavelength_um, diameter_um,
+                          mode="fp32c") -> FocalAnalysis:
+    px, py, t, phi = _d(px), _d(py), _c(t), _d(phi_ideal)
+    out = np.zeros(5)
+    grid = np.zeros((121, 121))
+    _check(_lib.load().crs_analyze_focus(_p(px), _p(py), _p(t), _p(phi), px.size,
+                                         focal_length_um, wavelength_um, diameter_um,
+                                         _mode(mode), _p(out), _p(grid)), "crs_analyze_focus")
+    return FocalAnalysis(*map(float, out), intensity=grid)
+
+
+def propagate_pillars(px, py, t, cx, cy, z, wavelength_um, n, half_window_um,
+                      mode="fp32c") -> PsfMap:
+    """focal.cpp:205-238: |E|^2 on an n x n window; an empty list or n <= 0 gives zeros."""
+    px, py, t = _d(px), _d(py), _c(t)
+    n = int(n)
+    m = PsfMap(n, half_window_um, np.zeros((max(n, 0), max(n, 0))))
+    if px.size == 0 or n <= 0:  # focal.cpp:216
+        return m
+    k = 2.0 * math.pi / wavelength_um
+    with np.errstate(divide="ignore", invalid="ignore"):
+        step = float(np.float64(2.0 * half_window_um) / np.float64(n - 1))
+    plane = crs_plane(k, cx - half_window_um, step, cy - half_window_um, step, z, 0.0, n, n, 0,
+                      0, 0)
+    _check(_lib.load().crs_forward_host(_mode(mode), _p(px), _p(py), _p(t), px.size, 1,
+                                        ctypes.byref(plane), 1, None, _p(m.intensity)),
+           "crs_forward_host")
+    return m
+
+
+def compute_psf(lens: MetalensDesign, lib: UnitCellLibrary, focal_length_um: float,
+                wavelength_um: float, diameter_um: float, n: int, half_window_um: float,
+                mode="fp32c") -> PsfMap:
+    """focal.cpp:148-203: the design's PSF at its focal plane."""
+    px, py, t = aperture_pillars(lens, lib, diameter_um)
+    return propagate_pillars(px, py, t, 0.0, 0.0, focal_length_um, wavelength_um, n,
+                             half_window_um, mode)
+
+
+def compute_psf_field(lens: MetalensDesign, lib: UnitCellLibrary, focal_length_um: float,
+                      wavelength_um: float, diameter_um: float, angle_deg: float, n: int,
+                      half_window_um: float, on_axis_peak: float = 0.0,
+                      mode="fp32c") -> FieldPsf:
+    """focal.cpp:240-311: tilted plane wave exp(i k sin(theta) x), window centred on the
+    chief ray x = f tan(theta)."""
+    px, py, t = aperture_pillars(lens, lib, diameter_um)
+    k = 2.0 * math.pi / wavelength_um
+    theta = angle_deg * math.pi / 180.0
+    tt = t * (np.cos(k * math.sin(theta) * px) + 1j * np.sin(k * math.sin(theta) * px))
+    cx = focal_length_um * math.tan(theta)
+    psf = propagate_pillars(px, py, tt, cx, 0.0, focal_length_um, wavelength_um, n,
+                            half_window_um, mode)
+    peak = float(psf.intensity.max()) if psf.intensity.size else 0.0
+    rel = peak / on_axis_peak if on_axis_peak > 0 else 1.0
+    return FieldPsf(psf, angle_deg, cx, rel)
+
+
+def analyze_through_focus(lens: MetalensDesign, lib: UnitCellLibrary, focal_length_um: float,
+                          wavelength_um: float, diameter_um: float, caustic_n: int = 201,
+                          mode="fp32c") -> ThroughFocus:
+    """throughfocus.cpp:15-132.  caustic_n = 201 is the reference's GPU-path caustic
+    (throughfocus.cpp:128), 81 its CPU-path one (:140)."""
+    px, py, t = aperture_pillars(lens, lib, diameter_um)
+    return analyze_through_focus_pillars(px, py, t, focal_length_um, wavelength_um, diameter_um,
+                                         caustic_n, mode)
+
+
+def analyze_through_focus_pillars(px, py, t, focal_length_um, wavelength_um, diameter_um,
+                                  caustic_n=201, mode="fp32c") -> ThroughFocus:
+    px, py, t = _d(px), _d(py), _c(t)
+    z = np.zeros(161)
+    inten = np.zeros(161)
+    sc = np.zeros(6)
+    caustic = np.zeros(caustic_n * caustic_n)
+    _check(_lib.load().crs_analyze_through_focus(_p(px), _p(py), _p(t), px.size, focal_length_um,
+                                                 wavelength_um, diameter_um, int(caustic_n),
+                                                 _mode(mode), _p(z), _p(inten), _p(sc),
+                                                 _p(caustic)), "crs_analyze_through_focus")
+    return ThroughFocus(z.astype(np.float32), inten.astype(np.float32), float(sc[0]), float(sc[1]),
+                        caustic_n, caustic_n, float(sc[2]), float(sc[3]), float(sc[4]),
+                        float(sc[5]), caustic.astype(np.float32))
+
+
+def analyze_chromatic(lens: MetalensDesign, lib: UnitCellLibrary, design_focal: float,
+                      design_wavelength: float, diameter: float, lambda_min: float,
+                      lambda_max: float, n: int, mode="fp32c") -> List[ChromaticPoint]:
+    """chromatic.cpp:9-58 (fixed transmission, geometric-phase model)."""
+    px, py, t = aperture_pillars(lens, lib, diameter)
+    return analyze_chromatic_pillars(px, py, t, design_focal, design_wavelength, lambda_min,
+                                     lambda_max, n, mode)
+
+
+def analyze_chromatic_pillars(px, py, t, design_focal, design_wavelength, lambda_min,
+                              lambda_max, n, mode="fp32c") -> List[ChromaticPoint]:
+    px, py, t = _d(px), _d(py), _c(t)
+    out = np.zeros(3 * n)
+    _check(_lib.load().crs_analyze_chromatic(_p(px), _p(py), _p(t), px.size, design_focal,
+                                             design_wavelength, lambda_min, lambda_max, int(n),
+                                             _mode(mode), _p(out)), "crs_analyze_chromatic")
+    return [ChromaticPoint(*map(float, out[3 * i:3 * i + 3])) for i in range(n)]
+
+
+def analyze_field_grid(lens: MetalensDesign, lib: UnitCellLibrary, focal_length_um: float,
+                       wavelength_um: float, diameter_um: float, max_angle_deg: float,
+                       n_half: int, psf_n: int, mode="fp32c") -> FieldGrid:
+    """field.cpp:150-233: all (2 n_half + 1)^2 field points plus the on-axis reference
+    in one launch."""
+    px, py, t = aperture_pillars(lens, lib, diameter_um)
+    px, py, t = _d(px), _d(py), _c(t)
+    g = 2 * n_half + 1
+    out = np.zeros(7 * g * g)
+    _check(_lib.load().crs_analyze_field_grid(_p(px), _p(py), _p(t), px.size, focal_length_um,
+                                              wavelength_um, diameter_um, max_angle_deg,
+                                              int(n_half), int(psf_n), _mode(mode), _p(out)),
+           "crs_analyze_field_grid")
+    pts = [FieldGridPoint(*map(float, out[7 * i:7 * i + 7])) for i in range(g * g)]
+    return FieldGrid(g, max_angle_deg, pts)

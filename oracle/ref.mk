@@ -2,7 +2,8 @@
 # compiled from its sources where they lie (read-only) plus ref_driver.cpp.
 # Test infrastructure only; outputs go to oracle/_ref/ and nowhere else.
 #
-# Only the translation units on the propagation path are compiled.  They need
+# Only the translation units on the propagation path are compiled (analysis/{focal,
+# throughfocus,chromatic,mtf,field}.cpp, design/{metalens,phase_profile}.cpp).  They need
 # no Eigen, RCWA or CUDA code for the functions the driver calls; the linker's
 # --gc-sections drops the unreferenced functions of those units (for example
 # the library builders that would pull in the RCWA solver).  No stand-in for
@@ -14,7 +15,7 @@ CXX     ?= g++
 # The reference's Release flags: -O3 (CMakeLists.txt:12-14), -mavx2 -mfma (:35-47), C++20.
 FLAGS   := -std=c++20 -O3 -mavx2 -mfma -ffunction-sections -fdata-sections -I$(REF)/src
 OUT     := _ref
-UNITS   := analysis/focal analysis/throughfocus analysis/chromatic analysis/mtf \
+UNITS   := analysis/focal analysis/throughfocus analysis/chromatic analysis/mtf analysis/field \
            design/metalens design/phase_profile
 OBJS    := $(addprefix $(OUT)/,$(addsuffix .o,$(notdir $(UNITS))))
 
@@ -35,6 +36,9 @@ $(OUT)/chromatic.o: $(SRC)/analysis/chromatic.cpp
 	@mkdir -p $(OUT)
 	$(CXX) $(FLAGS) -c -o $@ $<
 $(OUT)/mtf.o: $(SRC)/analysis/mtf.cpp
+	@mkdir -p $(OUT)
+	$(CXX) $(FLAGS) -c -o $@ $<
+$(OUT)/field.o: $(SRC)/analysis/field.cpp
 	@mkdir -p $(OUT)
 	$(CXX) $(FLAGS) -c -o $@ $<
 $(OUT)/metalens.o: $(SRC)/design/metalens.cpp

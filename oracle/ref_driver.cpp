@@ -2,7 +2,7 @@
 //
 // TEST INFRASTRUCTURE ONLY.  Built by oracle/ref.mk from this file plus the
 // reference sources where they lie (src/celeris/analysis/{focal,throughfocus,
-// chromatic,mtf}.cpp, src/celeris/design/{metalens,phase_profile}.cpp); the
+// chromatic,mtf,field}.cpp, src/celeris/design/{metalens,phase_profile}.cpp); the
 // binary lands in oracle/_ref/ and is used to pin the FP64 restatement in
 // oracle/rs_oracle.c and to write golden vectors.  Nothing here is shipped.
 //
@@ -24,7 +24,13 @@
 //                                             (analyze_chromatic, chromatic.cpp:9)
 //   mtf          : lens, f, lambda, D      -> m, freq[m], mtf[m], ideal[m], cutoff
 //                                             (analyze_mtf, mtf.cpp:10)
+//   fieldgrid    : lens, f, lambda, D, max_angle, n_half, psf_n -> n, then n*n x
+//                  (theta_x, theta_y, rel_strehl, fwhm_x, fwhm_y, cx, cy)
+//                                             (analyze_field_grid, field.cpp:150)
+//   psffield     : lens, f, lambda, D, angle, n, W, on_axis_peak -> angle, cx, rel_strehl,
+//                  intensity[n*n]             (compute_psf_field, focal.cpp:240)
 #include "celeris/analysis/chromatic.hpp"
+#include "celeris/analysis/field.hpp"
 #include "celeris/analysis/focal.hpp"
 #include "celeris/analysis/mtf.hpp"
 #include "celeris/analysis/throughfocus.hpp"
@@ -147,6 +153,22 @@ int main(int argc, char** argv) {
             for (float v : c.mtf) out.push_back(v);
             for (float v : c.mtf_ideal) out.push_back(v);
             out.push_back(c.cutoff_cyc_mm);
+        } else if (cmd == "fieldgrid") {
+            const double max_angle = r.next();
+            const int n_half = r.next_int(), psf_n = r.next_int();
+            FieldGrid g = analyze_field_grid(lens, lib, f, lam, D, max_angle, n_half, psf_n);
+            out.push_back(g.n);
+            for (const auto& p : g.points) {
+                out.insert(out.end(), {p.theta_x_deg, p.theta_y_deg, p.rel_strehl, p.fwhm_x_um,
+                                       p.fwhm_y_um, p.cx_um, p.cy_um});
+            }
+        } else if (cmd == "psffield") {
+            const double angle = r.next();
+            const int n = r.next_int();
+            const double W = r.next(), on_axis_peak = r.next();
+            FieldPsf p = compute_psf_field(lens, lib, f, lam, D, angle, n, W, on_axis_peak);
+            out = {p.angle_deg, p.cx_um, p.rel_strehl};
+            out.insert(out.end(), p.psf.intensity.begin(), p.psf.intensity.end());
         } else {
             std::fprintf(stderr, "ref_driver: unknown command %s\n", cmd.c_str());
             return 2;

@@ -2,7 +2,7 @@
 """Writes tests/golden/*.npz: inputs and outputs of the REFERENCE's own CPU code.
 
 Run where the reference sources are present (they are compiled by oracle/ref.mk into
-oracle/_ref/ref_driver, from src/celeris/analysis/{focal,throughfocus,chromatic,mtf}.cpp
+oracle/_ref/ref_driver, from src/celeris/analysis/{focal,throughfocus,chromatic,mtf,field}.cpp
 and src/celeris/design/{metalens,phase_profile}.cpp).  The fixtures are data only:
 seeded inputs and the reference's outputs, used by tests/test_oracle.py to pin the
 FP64 restatement in oracle/rs_oracle.c where the reference build is absent.
@@ -88,6 +88,11 @@ def main():
     lens_out["chromatic_args"] = np.array([0.45, 0.65, 5])
     lens_out["chromatic"] = oracle.ref_run("chromatic", np.concatenate([base, [0.45, 0.65, 5]]))
     lens_out["mtf"] = oracle.ref_run("mtf", base)
+    lens_out["fieldgrid_args"] = np.array([8.0, 1, 41])
+    lens_out["fieldgrid"] = oracle.ref_run("fieldgrid", np.concatenate([base, [8.0, 1, 41]]))
+    on_axis_peak = float(np.max(lens_out["psf"]))
+    lens_out["psffield_args"] = np.array([5.0, 33, 3.0, on_axis_peak])
+    lens_out["psffield"] = oracle.ref_run("psffield", np.concatenate([base, [5.0, 33, 3.0, on_axis_peak]]))
     np.savez_compressed(os.path.join(HERE, "lens_analyses.npz"), **lens_out)
     print("wrote", sorted(os.listdir(HERE)))
 

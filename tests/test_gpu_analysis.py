@@ -1,0 +1,124 @@
+"""The analysis layer (celeris_amd.analysis -> crs_analyze_*) against golden vectors
+written by the reference's own analysis code (tests/golden/lens_analyses.npz, from
+oracle/_ref/ref_driver: focal.cpp, throughfocus.cpp, chromatic.cpp, field.cpp).
+
+In FP64 mode every sum matches the reference's FP64 CPU loops to roundoff, so the
+derived metrics match to 1e-9 relative; discrete outputs (FWHM in whole pixel steps,
+argmax positions, z grids) must match exactly.  FP32C metrics match to 1e-5.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from celeris_amd import analysis
+from celeris_amd.design import MetalensDesign, UnitCellLibrary
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "lens_analyses.npz")
+
+
+@pytest.fixture(scope="module")
+def lens(torch_cuda):
+    d = np.load(GOLDEN)
+    L = MetalensDesign(int(d["n_cells"]), float(d["period"]), d["fill_map"])
+    lib = UnitCellLibrary(float(d["period"]), d["lib_fill"], d["lib_phase"], d["lib_amp"])
+    return d, L, lib, float(d["f"]), float(d["lam"]), float(d["D"])
+
+
+@pytest.mark.parametrize("mode,tol", [("fp64", 1e-9), ("fp32c", 1e-5)])
+def test_analyze_focus(lens, mode, tol):
+    d, L, lib, f, lam, D = lens
+    print(f"[analysis] analyze_focus {mode}", flush=True)
+    a = analysis.analyze_focus(L, lib, f, lam, D, mode=mode)
+    ref = d["focus"]
+    got = np.array([a.strehl, a.phase_strehl, a.fwhm_um, a.diffraction_limit_um, a.encircled_energy])
+    print(mode, got, ref)
+    assert got[3] == ref[3]  # lambda f / D
+    W = max(6 * ref[3], 4.0)
+    slack = 0 if mode == "fp64" else 2 * W / 120 * 1.0001  # one pixel step (see field grid)
+    assert abs(got[2] - ref[2]) <= slack
+    assert np.allclose(got[[0, 1, 4]], ref[[0, 1, 4]], rtol=tol, atol=0)
+
+
+@pytest.mark.parametrize("mode,tol", [("fp64", 1e-12), ("fp32c", 1e-5)])
+def test_compute_psf(lens, mode, tol):
+    d, L, lib, f, lam, D = lens
+    n, W = int(d["psf_n"][0]), float(d["psf_n"][1])
+    m = analysis.compute_psf(L, lib, f, lam, D, n, W, mode=mode)
+    ref = d["psf"].reshape(n, n)
+    assert np.linalg.norm(m.intensity - ref) / np.linalg.norm(ref) <= tol
+
+
+@pytest.mark.parametrize("mode,tol", [("fp64", 2e-7), ("fp32c", 1e-5)])
+def test_analyze_through_focus(lens, mode, tol):
+    d, L, lib, f, lam, D = lens
+    tf_ref = d["throughfocus"]
+    N = int(tf_ref[0])
+    z_ref, I_ref = tf_ref[1:1 + N], tf_ref[1 + N:1 + 2 * N]
+    zpk_ref, dof_ref = tf_ref[1 + 2 * N], tf_ref[2 + 2 * N]
+    cnx = int(tf_ref[3 + 2 * N])
+    caustic_ref = tf_ref[5 + 2 * N:]
+    print(f"[analysis] through focus {mode}", flush=True)
+    tf = analysis.analyze_through_focus(L, lib, f, lam, D, caustic_n=cnx, mode=mode)
+    assert np.array_equal(tf.z_um.astype(np.float64), z_ref)
+    assert np.max(np.abs(tf.intensity - I_ref)) <= tol
+    assert tf.z_peak_um == pytest.approx(zpk_ref, abs=1e-6)
+    assert tf.dof_um == pytest.approx(dof_ref, abs=1e-5)
+    assert np.max(np.abs(tf.caustic - caustic_ref)) <= tol
+    # the reference's GPU path samples the caustic on 201 x 201
+    tf201 = analysis.analyze_through_focus(L, lib, f, lam, D, mode=mode)
+    assert tf201.caustic.size == 201 * 201 and tf201.caustic.max() == pytest.approx(1.0)
+
+
+@pytest.mark.parametrize("mode,tol", [("fp64", 1e-9), ("fp32c", 1e-5)])
+def test_analyze_chromatic(lens, mode, tol):
+    d, L, lib, f, lam, D = lens
+    lmin, lmax, n = d["chromatic_args"]
+    pts = analysis.analyze_chromatic(L, lib, f, lam, D, lmin, lmax, int(n), mode=mode)
+    ref = d["chromatic"].reshape(-1, 3)
+    got = np.array([[p.wavelength_um, p.focal_length_um, p.rel_peak] for p in pts])
+    print(mode, got, ref)
+    assert np.array_equal(got[:, 0], ref[:, 0])
+    assert np.allclose(got[:, 1], ref[:, 1], rtol=1e-12, atol=0)  # argmax on the same z grid
+    assert np.allclose(got[:, 2], ref[:, 2], rtol=tol, atol=0)
+
+
+@pytest.mark.parametrize("mode,tol", [("fp64", 1e-9), ("fp32c", 1e-5)])
+def test_analyze_field_grid(lens, mode, tol):
+    d, L, lib, f, lam, D = lens
+    max_angle, n_half, psf_n = d["fieldgrid_args"]
+    g = analysis.analyze_field_grid(L, lib, f, lam, D, max_angle, int(n_half), int(psf_n), mode=mode)
+    ref = d["fieldgrid"]
+    assert g.n == int(ref[0])
+    ref = ref[1:].reshape(-1, 7)
+    got = np.array([[p.theta_x_deg, p.theta_y_deg, p.rel_strehl, p.fwhm_x_um, p.fwhm_y_um,
+                     p.cx_um, p.cy_um] for p in g.points])
+    print(mode, got[:, 2], ref[:, 2])
+    assert np.array_equal(got[:, [0, 1, 5, 6]], ref[:, [0, 1, 5, 6]])
+    # FWHM counts whole pixel steps above half maximum; in FP32C a pixel lying within
+    # roundoff of the threshold may fall on the other side (one step, 2W/(psf_n-1)).
+    W = max(6 * lam * f / D, 4.0)
+    step = 2 * W / (psf_n - 1)
+    slack = 0 if mode == "fp64" else step * 1.0001
+    assert np.max(np.abs(got[:, [3, 4]] - ref[:, [3, 4]])) <= slack
+    assert np.allclose(got[:, 2], ref[:, 2], rtol=tol, atol=0)
+    # selftest [21] invariants (selftest.cpp:908-929): centre node is the reference,
+    # 4-corner symmetry of a rotationally symmetric lens
+    c = g.n // 2
+    assert got[c * g.n + c, 2] == pytest.approx(1.0, abs=1e-9)
+    corners = got[[0, g.n - 1, (g.n - 1) * g.n, g.n * g.n - 1], 2]
+    assert corners.max() - corners.min() < 1e-3
+
+
+@pytest.mark.parametrize("mode,tol", [("fp64", 1e-12), ("fp32c", 1e-5)])
+def test_compute_psf_field(lens, mode, tol):
+    d, L, lib, f, lam, D = lens
+    angle, n, W, on_axis_peak = d["psffield_args"]
+    p = analysis.compute_psf_field(L, lib, f, lam, D, angle, int(n), W, on_axis_peak, mode=mode)
+    ref = d["psffield"]
+    assert p.angle_deg == ref[0] and p.cx_um == pytest.approx(ref[1], rel=1e-15)
+    assert p.rel_strehl == pytest.approx(ref[2], rel=tol)
+    I_ref = ref[3:].reshape(int(n), int(n))
+    assert np.linalg.norm(p.psf.intensity - I_ref) / np.linalg.norm(I_ref) <= tol
