@@ -98,7 +98,7 @@ struct TableBuilder {
 
 struct FwdPlan {
     int mode = 0;
-    int order = 1;
+    int order = 0;
     int npil = 0, nsets = 0;
     int64_t part_elems = 0;
     int nblocks = 0, npblocks = 0;
@@ -292,18 +292,21 @@ DevPlane make_dev_plane(const crs_plane& P, int px_per_thread, int py_per_thread
     d.offset = P.offset;
     d.part = 0;
     d.sbase = 0;
-    d.nxp = static_cast<int32_t>(ceil_div(P.nx, kBlk) * kBlk);
-    d.nquads = static_cast<int32_t>(ceil_div(P.ny, kBlk));
+    d.nxp = static_cast<int32_t>(ceil_div(P.nx, kQuad) * kQuad);
+    d.nquads = static_cast<int32_t>(ceil_div(P.ny, kQuad));
     return d;
 }
 
-// FP32C series order for a set of planes.  A 4 x 4 pixel block spans at most e turns
-// around its centre; at distance >= z' the expansion parameter is r = e / z', and the
-// neglected terms are ~ e r^2 / 2 turns of phase and 1.5 r^2 of amplitude (ORDER 1),
-// ~ 0.6 e r^3 and 2.5 r^3 (ORDER 2).  Returns 1 or 2, or 0 when even ORDER 2 would
-// not stay below 1e-6 (then FP32C is refused for the call).
+// FP32C series orders for a set of planes.  A 4 x 4 pixel block spans at most e turns
+// around its centre; at distance >= z' the expansion parameter is r = e / z' (y <= r/2).
+// Dropped terms, at worst (a pillar seen at grazing angle): phase order 1 leaves
+// 2 x1 y^2 <= e r^2 / 2 turns, order 2 leaves ~5 x1 y^3 <= 0.625 e r^3; amplitude order
+// 1 leaves 6 y^2 <= 1.5 r^2, order 2 leaves ~20 y^3 <= 2.5 r^3.  Each order is the
+// lowest that keeps its term below 1.6e-7 turns (1e-6 rad) / 4e-6 of amplitude.
+// Returns 2 * (phase order - 1) + (amplitude order - 1), or kExactOrder when even
+// order 2 would not (then every pair is evaluated at its own reference distance).
 int fp32c_order(const std::vector<DevPlane>& dp) {
-    int order = 1;
+    int po = 1, ao = 1;
     for (const DevPlane& d : dp) {
         const double ex = 1.5 * std::fabs(d.sx) * d.kturn;
         const double ey = 1.5 * std::fabs(d.sy) * d.kturn;
@@ -312,15 +315,20 @@ int fp32c_order(const std::vector<DevPlane>& dp) {
         const double zlast = d.z0 + (d.ny - 1) * d.sz;
         const double zmin = d.kturn * std::min(std::fabs(d.z0), std::fabs(zlast));
         if (!(std::isfinite(e) && zmin > 0.0)) {
-            if (std::isfinite(e)) return 0;  // z = 0 plane: no distance bound
-            continue;                        // NaN/inf geometry (n == 1 windows): NaN out
+            if (std::isfinite(e)) return kExactOrder;  // a z = 0 row: no distance bound
+            continue;  // NaN/inf geometry (n == 1 windows): NaN out, as the reference
         }
         const double r = e / zmin;
-        if (1.5 * r * r <= 1e-6 && 0.5 * e * r * r <= 1e-7) continue;
-        if (2.5 * r * r * r <= 1e-6 && 0.625 * e * r * r * r <= 1e-7) { order = 2; continue; }
-        return 0;
+        if (0.5 * e * r * r > 1.6e-7) {
+            if (0.625 * e * r * r * r > 1.6e-7) return kExactOrder;
+            po = 2;
+        }
+        if (1.5 * r * r > 4e-6) {
+            if (2.5 * r * r * r > 4e-6) return kExactOrder;
+            ao = 2;
+        }
     }
-    return order;
+    return 2 * (po - 1) + (ao - 1);
 }
 
 // Bytes of staged pillar data: FP32C keeps FP64 positions once plus {|t|, arg t} per
@@ -456,23 +464,22 @@ struct Scratch {  // where one call's device data lives (a context's or a graph'
 
 crs_status plan_forward(const crs_context* ctx, int mode, int npil, int nsets,
                         const crs_plane* planes, int nplanes, FwdPlan& plan) {
-    const int PX = mode == CRS_MODE_FP32C ? kBlk : 1;
-    const int PY = mode == CRS_MODE_FP32C ? kBlk : 1;
     plan.mode = mode;
     plan.npil = npil;
     plan.nsets = nsets;
+    std::vector<DevPlane> dp(nplanes);
+    for (int p = 0; p < nplanes; ++p) dp[p] = make_dev_plane(planes[p], 1, 1);
+    int blk = 1;
+    if (mode == CRS_MODE_FP32C) {
+        plan.order = fp32c_order(dp);
+        blk = plan.order == kExactOrder ? 1 : kBlk;
+    }
     // Split over pillar chunks: enough work-groups to fill the chip, chunks of at
     // least one LDS tile.
-    std::vector<DevPlane> dp(nplanes);
     int64_t base_blocks = 0;
     for (int p = 0; p < nplanes; ++p) {
-        dp[p] = make_dev_plane(planes[p], PX, PY);
+        dp[p] = make_dev_plane(planes[p], blk, blk);
         base_blocks += ceil_div(static_cast<int64_t>(dp[p].tx) * dp[p].ty, kThreads);
-    }
-    if (mode == CRS_MODE_FP32C && (plan.order = fp32c_order(dp)) == 0) {
-        set_error("FP32C: a plane lies too close to the aperture for its pixel pitch "
-                  "(block size / distance > 2%%); use CRS_MODE_FP64");
-        return CRS_ERR_INVALID;
     }
     const int target = ctx->fwd_target[mode] > 0 ? ctx->fwd_target[mode] : default_target(mode, false);
     int64_t nsplit = std::max<int64_t>(1, ceil_div(target, base_blocks));
@@ -561,14 +568,10 @@ crs_status plan_adjoint(const crs_context* ctx, int mode, int npil, int nsets,
     for (int p = 0; p < nplanes; ++p) {
         dp[p] = make_dev_plane(planes[p], 1, 1);
         dp[p].sbase = srec;
-        srec += static_cast<int64_t>(dp[p].nquads) * dp[p].nxp * kBlk;
+        srec += static_cast<int64_t>(dp[p].nquads) * dp[p].nxp * kQuad;
     }
     if (srec > (int64_t(1) << 31)) { set_error("too many pixels"); return CRS_ERR_INVALID; }
-    if (mode == CRS_MODE_FP32C && (plan.order = fp32c_order(dp)) == 0) {
-        set_error("FP32C: a plane lies too close to the aperture for its pixel pitch "
-                  "(block size / distance > 2%%); use CRS_MODE_FP64");
-        return CRS_ERR_INVALID;
-    }
+    if (mode == CRS_MODE_FP32C) plan.order = fp32c_order(dp);
 
     // Per-set quad lists (planes in order, quads in order) and their work, in pixel
     // records (FP32C computes the padded 4 x nxp records of a quad).
@@ -581,11 +584,11 @@ crs_status plan_adjoint(const crs_context* ctx, int mode, int npil, int nsets,
         pre.push_back(0);
         for (int p = 0; p < nplanes; ++p) {
             if (dp[p].tset != s) continue;
-            for (int j = 0; j < dp[p].ny; j += kBlk) {
+            for (int j = 0; j < dp[p].ny; j += kQuad) {
                 quads.push_back({p, j});
                 const int64_t w = mode == CRS_MODE_FP32C
-                                      ? static_cast<int64_t>(kBlk) * dp[p].nxp
-                                      : static_cast<int64_t>(std::min(kBlk, dp[p].ny - j)) * dp[p].nx;
+                                      ? static_cast<int64_t>(kQuad) * dp[p].nxp
+                                      : static_cast<int64_t>(std::min(kQuad, dp[p].ny - j)) * dp[p].nx;
                 pre.push_back(pre.back() + w);
             }
         }
@@ -627,7 +630,7 @@ crs_status plan_adjoint(const crs_context* ctx, int mode, int npil, int nsets,
     // Staging blocks walk each plane's staged records.
     std::vector<PixBlock> pblocks;
     for (int p = 0; p < nplanes; ++p) {
-        const int64_t nrec = static_cast<int64_t>(dp[p].nquads) * dp[p].nxp * kBlk;
+        const int64_t nrec = static_cast<int64_t>(dp[p].nquads) * dp[p].nxp * kQuad;
         for (int64_t b = 0; b < nrec; b += kThreads) pblocks.push_back({p, static_cast<int32_t>(b)});
     }
     plan.nsplit = nsplit;

@@ -10,13 +10,19 @@ namespace crs {
 
 // Work-group size of every pair kernel: 4 wave64s, one per SIMD of a CU.
 constexpr int kThreads = 256;
-// FP32C: pixels are handled in 4 x 4 blocks (4 columns x 4 rows).  The forward gives
-// each thread one block; the adjoint walks blocks of a 4-row strip ("quad") of a plane.
+// FP32C: pixels are handled in blocks of 4 x 4 (4 columns x 4 rows) that share one
+// reference distance, or 1 x 1 where a plane's pixel pitch is too coarse for the
+// block expansion.  The forward gives each thread one block; the adjoint walks a
+// plane in strips of kQuad rows ("quads") and each quad in 4-column groups.
 constexpr int kBlk = 4;
+constexpr int kQuad = 4;
+// Launch order code of the 1 x 1 (unexpanded) FP32C kernels; 0..3 are the 4 x 4
+// kernels' series orders, 2 * (phase order - 1) + (amplitude order - 1).
+constexpr int kExactOrder = 4;
 // FP32C adjoint: pillars per thread (q0 + tid + k*kThreads, k < kPQ).
 constexpr int kPQ = 4;
 // FP32C adjoint: columns per LDS tile of a quad (kAdjCols x 4 pixel records).
-constexpr int kAdjCols = kThreads / kBlk;
+constexpr int kAdjCols = kThreads / kQuad;
 
 // One plane as the kernels see it (host computes the derived fields).
 struct DevPlane {

@@ -24,13 +24,13 @@
 // where D is tiny next to R0^2 and is formed in float without cancellation (each
 // term is one fma against per-block constants).  Then, with h = 1/(2 R0),
 // x1 = D h and y = x1 h = D / (4 R0^2):
-//     R - R0 = x1 (1 - y + 2 y^2 - ...)          (ORDER 1 keeps 1 - y)
-//     R0 / R = 1 - 2 y + 6 y^2 - ...              (ORDER 1 keeps 1 - 2 y)
+//     R - R0 = x1 (1 - y + 2 y^2 - ...)          (phase order 1 keeps 1 - y)
+//     R0 / R = 1 - 2 y + 6 y^2 - ...              (amplitude order 1 keeps 1 - 2 y)
 // and the pair is
 //     E += |t| (k/2 pi) / R0 * (R0/R) * (cos 2 pi th, sin 2 pi th),  th = frac(...) + (R - R0)
 // with v_cos_f32 / v_sin_f32 (which take turns).  No float ever holds kR (~10^3 rad)
-// or R^2 (~10^4 turns^2): the phase is carried to ~1e-7 turns.  The host picks ORDER
-// per launch from the block size relative to the distance.  The factor exp(2 pi i
+// or R^2 (~10^4 turns^2): the phase is carried to ~1e-7 turns.  The host picks both
+// orders per launch from the block size relative to the distance.  The factor exp(2 pi i
 // z'_ref) is restored in FP64 by the reduction.  Each pillar tile's float partial is
 // folded into a Kahan-compensated running sum; split partials are added in FP64.
 #include "rs_internal.h"
@@ -56,28 +56,32 @@ __device__ inline double block_distance(double R0sq, float& rf) {
     return fma(res, static_cast<double>(0.5f * rf), R0f);
 }
 
-// Offsets of a block's 4 columns (or rows) from its centre, in turns: e = step*(c-1.5).
-__device__ inline void block_offsets(double step_turns, float (&te)[kBlk], float (&e2)[kBlk]) {
+// Offsets of a block's B columns (or rows) from its centre, in turns:
+// e = step * (c - (B-1)/2).
+template <int B>
+__device__ inline void block_offsets(double step_turns, float (&te)[B], float (&e2)[B]) {
 #pragma unroll
-    for (int c = 0; c < kBlk; ++c) {
-        const float e = static_cast<float>(step_turns * (c - 1.5));
+    for (int c = 0; c < B; ++c) {
+        const float e = static_cast<float>(step_turns * (c - (B - 1) * 0.5));
         te[c] = 2.0f * e;
         e2[c] = e * e;
     }
 }
 
 // One pair's path excess R - R0 (turns) and amplitude A = a0 R0/R, from x1 = D h and
-// y = x1 h (y = D / (4 R0^2)); a0 is the amplitude at R0 and a0n = -2 a0.  ORDER 1
-// keeps R - R0 = x1 (1 - y), R0/R = 1 - 2y; ORDER 2 adds the next term of each.
-template <int ORDER>
+// y = x1 h (y = D / (4 R0^2)); a0 is the amplitude at R0 and a0n = -2 a0.  Phase order
+// PO = 1 keeps R - R0 = x1 (1 - y), PO = 2 adds 2 x1 y^2; amplitude order AO = 1 keeps
+// R0/R = 1 - 2y, AO = 2 adds 6 y^2.
+template <int PO, int AO>
 __device__ inline void pair_terms(float x1, float y, float a0, float a0n, float& dR, float& A) {
-    if constexpr (ORDER == 1) {
+    if constexpr (PO == 1)
         dR = fmaf(-y, x1, x1);
-        A = fmaf(a0n, y, a0);
-    } else {
+    else
         dR = fmaf(x1 * y, fmaf(2.0f, y, -1.0f), x1);
+    if constexpr (AO == 1)
+        A = fmaf(a0n, y, a0);
+    else
         A = fmaf(a0 * y, fmaf(6.0f, y, -2.0f), a0);
-    }
 }
 
 // LDS record of one pillar for the FP32C forward: FP64 turn-scaled position, the
@@ -134,10 +138,11 @@ __global__ void stage_pillars_fp64(const double* __restrict__ px, const double* 
 }
 
 // ---------------------------------------------------------------------------------
-// Forward, FP32C.  Each thread owns one 4 x 4 pixel block; pillars stream through LDS,
-// scaled to turns by the plane's k/(2 pi) as they are loaded.
+// Forward, FP32C.  Each thread owns one BLK x BLK pixel block; pillars stream through
+// LDS, scaled to turns by the plane's k/(2 pi) as they are loaded.  BLK = 1 evaluates
+// every pair at its own FP64 reference distance (no expansion).
 
-template <int ORDER>
+template <int BLK, int PO, int AO>
 __global__ __launch_bounds__(kThreads, 2) void fwd_fp32c(
     const double2* __restrict__ pos, const float2* __restrict__ ap, int npil,
     const DevPlane* __restrict__ planes, const FwdBlock* __restrict__ blocks,
@@ -155,23 +160,23 @@ __global__ __launch_bounds__(kThreads, 2) void fwd_fp32c(
     const double zref = P.zref;
 
     // Block centre and the offsets of its pixels (turns).
-    const double ic = ti * kBlk + 1.5, jc = tj * kBlk + 1.5;
+    const double ic = ti * BLK + (BLK - 1) * 0.5, jc = tj * BLK + (BLK - 1) * 0.5;
     const double FXc = kt * (P.x0 + ic * P.sx);
     const double FYc = kt * (P.y0 + jc * P.sy);
     const double zc = kt * (P.z0 + jc * P.sz);
     const double ZC2 = zc * zc;
-    float tex[kBlk], ex2[kBlk], tey[kBlk], eyz[kBlk];
-    block_offsets(kt * P.sx, tex, ex2);
-    block_offsets(kt * P.sy, tey, eyz);
+    float tex[BLK], ex2[BLK], tey[BLK], eyz[BLK];
+    block_offsets<BLK>(kt * P.sx, tex, ex2);
+    block_offsets<BLK>(kt * P.sy, tey, eyz);
 #pragma unroll
-    for (int r = 0; r < kBlk; ++r) {
-        const double zr = kt * (P.z0 + (tj * kBlk + r) * P.sz);
+    for (int r = 0; r < BLK; ++r) {
+        const double zr = kt * (P.z0 + (tj * BLK + r) * P.sz);
         eyz[r] += static_cast<float>(zr * zr - ZC2);
     }
     const float amp_scale = static_cast<float>(kt);  // 1/R = (k/2pi) / R'
     const float2* __restrict__ aps = ap + static_cast<int64_t>(P.tset) * npil;
 
-    constexpr int NPIX = kBlk * kBlk;
+    constexpr int NPIX = BLK * BLK;
     float sr[NPIX], si[NPIX], cr[NPIX], ci[NPIX];
 #pragma unroll
     for (int e = 0; e < NPIX; ++e) sr[e] = si[e] = cr[e] = ci[e] = 0.0f;
@@ -200,27 +205,32 @@ __global__ __launch_bounds__(kThreads, 2) void fwd_fp32c(
             float rf;
             const double R0 = block_distance(fma(dX0, dX0, fma(dY0, dY0, ZC2)), rf);
             const float th0 = static_cast<float>(__builtin_amdgcn_fract(R0 - T.zeta));
-            const float h = 0.5f * rf;
             const float c1 = T.amp * rf;
-            const float c1n = -2.0f * c1;
-            const float dx0 = static_cast<float>(dX0), dy0 = static_cast<float>(dY0);
-            float a[kBlk], b[kBlk];
+            if constexpr (BLK == 1) {
+                ar[0] = fmaf(c1, __builtin_amdgcn_cosf(th0), ar[0]);
+                ai[0] = fmaf(c1, __builtin_amdgcn_sinf(th0), ai[0]);
+            } else {
+                const float h = 0.5f * rf;
+                const float c1n = -2.0f * c1;
+                const float dx0 = static_cast<float>(dX0), dy0 = static_cast<float>(dY0);
+                float a[BLK], b[BLK];
 #pragma unroll
-            for (int c = 0; c < kBlk; ++c) a[c] = fmaf(tex[c], dx0, ex2[c]);
+                for (int c = 0; c < BLK; ++c) a[c] = fmaf(tex[c], dx0, ex2[c]);
 #pragma unroll
-            for (int r = 0; r < kBlk; ++r) b[r] = fmaf(tey[r], dy0, eyz[r]);
+                for (int r = 0; r < BLK; ++r) b[r] = fmaf(tey[r], dy0, eyz[r]);
 #pragma unroll
-            for (int r = 0; r < kBlk; ++r) {
+                for (int r = 0; r < BLK; ++r) {
 #pragma unroll
-                for (int c = 0; c < kBlk; ++c) {
-                    const float x1 = (a[c] + b[r]) * h;
-                    const float y = x1 * h;
-                    float dR, A;
-                    pair_terms<ORDER>(x1, y, c1, c1n, dR, A);
-                    const float th = th0 + dR;
-                    const int e = r * kBlk + c;
-                    ar[e] = fmaf(A, __builtin_amdgcn_cosf(th), ar[e]);
-                    ai[e] = fmaf(A, __builtin_amdgcn_sinf(th), ai[e]);
+                    for (int c = 0; c < BLK; ++c) {
+                        const float x1 = (a[c] + b[r]) * h;
+                        const float y = x1 * h;
+                        float dR, A;
+                        pair_terms<PO, AO>(x1, y, c1, c1n, dR, A);
+                        const float th = th0 + dR;
+                        const int e = r * BLK + c;
+                        ar[e] = fmaf(A, __builtin_amdgcn_cosf(th), ar[e]);
+                        ai[e] = fmaf(A, __builtin_amdgcn_sinf(th), ai[e]);
+                    }
                 }
             }
         }
@@ -240,14 +250,14 @@ __global__ __launch_bounds__(kThreads, 2) void fwd_fp32c(
 
     if (!active) return;
 #pragma unroll
-    for (int r = 0; r < kBlk; ++r) {
-        const int j = tj * kBlk + r;
+    for (int r = 0; r < BLK; ++r) {
+        const int j = tj * BLK + r;
         if (j >= ny) break;
 #pragma unroll
-        for (int c = 0; c < kBlk; ++c) {
-            const int i = ti * kBlk + c;
+        for (int c = 0; c < BLK; ++c) {
+            const int i = ti * BLK + c;
             if (i >= nx) break;
-            const int e = r * kBlk + c;
+            const int e = r * BLK + c;
             partial[B.part + static_cast<int64_t>(j) * nx + i] =
                 make_double2(static_cast<double>(sr[e]) - static_cast<double>(cr[e]),
                              static_cast<double>(si[e]) - static_cast<double>(ci[e]));
@@ -381,12 +391,12 @@ __global__ __launch_bounds__(kThreads) void stage_pixels_fp32c(
     const double2* __restrict__ u, float4* __restrict__ pix) {
     const PixBlock B = pblocks[blockIdx.x];
     const DevPlane& P = planes[B.plane];
-    const int nrec = P.nquads * P.nxp * kBlk;
+    const int nrec = P.nquads * P.nxp * kQuad;
     const int rix = B.pix0 + static_cast<int>(threadIdx.x);
     if (rix >= nrec) return;
-    const int quad = rix / (P.nxp * kBlk);
-    const int rem = rix - quad * P.nxp * kBlk;
-    const int i = rem / kBlk, j = quad * kBlk + rem % kBlk;
+    const int quad = rix / (P.nxp * kQuad);
+    const int rem = rix - quad * P.nxp * kQuad;
+    const int i = rem / kQuad, j = quad * kQuad + rem % kQuad;
     float4 rec = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (i < P.nx && j < P.ny) {
         const double2 v = u[P.offset + static_cast<int64_t>(j) * P.nx + i];
@@ -401,10 +411,11 @@ __global__ __launch_bounds__(kThreads) void stage_pixels_fp32c(
 // ---------------------------------------------------------------------------------
 // Adjoint, FP32C.  Thread owns PQ pillars (FP64 positions in registers, scaled to
 // turns by each quad's plane); a quad's pixel records stream through LDS 64 columns
-// at a time and are consumed in 4 x 4 blocks, each with its own reference distance.
+// at a time and are consumed in BLK x BLK blocks, each with its own reference distance
+// (BLK = 1: every pair at its own).
 // Per pair: conj(G) u = |u| (k/2pi)/R0 * (R0/R) * exp(2 pi i (chi - th0 - (R - R0))).
 
-template <int PQ, int ORDER>
+template <int PQ, int BLK, int PO, int AO>
 __global__ __launch_bounds__(kThreads, 2) void adj_fp32c(
     const double2* __restrict__ pos, int npil, const DevPlane* __restrict__ planes,
     const AdjQuad* __restrict__ quads, const AdjBlock* __restrict__ blocks,
@@ -428,66 +439,80 @@ __global__ __launch_bounds__(kThreads, 2) void adj_fp32c(
         const DevPlane& P = planes[Q.plane];
         const double kt = P.kturn;
         const double zref = P.zref;
-        const double jc = Q.j0 + 1.5;
-        const double FYc = kt * (P.y0 + jc * P.sy);
-        const double zc = kt * (P.z0 + jc * P.sz);
-        const double ZC2 = zc * zc;
-        float tex[kBlk], ex2[kBlk], tey[kBlk], eyz[kBlk];
-        block_offsets(kt * P.sx, tex, ex2);
-        block_offsets(kt * P.sy, tey, eyz);
+        float tex[BLK], ex2[BLK];
+        block_offsets<BLK>(kt * P.sx, tex, ex2);
+        double X[PQ];
 #pragma unroll
-        for (int r = 0; r < kBlk; ++r) {
-            const double zr = kt * (P.z0 + (Q.j0 + r) * P.sz);
-            eyz[r] += static_cast<float>(zr * zr - ZC2);
-        }
-        double X[PQ], dY0[PQ];
-#pragma unroll
-        for (int k = 0; k < PQ; ++k) {
-            X[k] = kt * qx[k];
-            dY0[k] = FYc - kt * qy[k];
-        }
+        for (int k = 0; k < PQ; ++k) X[k] = kt * qx[k];
         const float4* __restrict__ prow =
-            pix + P.sbase + static_cast<int64_t>(Q.j0 / kBlk) * P.nxp * kBlk;
+            pix + P.sbase + static_cast<int64_t>(Q.j0 / kQuad) * P.nxp * kQuad;
         for (int cb = 0; cb < P.nxp; cb += kAdjCols) {
             const int ncol = min(kAdjCols, P.nxp - cb);
             __syncthreads();
-            if (tid < ncol * kBlk) tile[tid] = prow[cb * kBlk + tid];
+            if (tid < ncol * kQuad) tile[tid] = prow[cb * kQuad + tid];
             __syncthreads();
             float ar[PQ], ai[PQ];
 #pragma unroll
             for (int k = 0; k < PQ; ++k) ar[k] = ai[k] = 0.0f;
-            for (int g = 0; g < ncol; g += kBlk) {
-                const double FXc = kt * (P.x0 + (cb + g + 1.5) * P.sx);
 #pragma unroll
-                for (int k = 0; k < PQ; ++k) {
-                    const double dX0 = FXc - X[k];
-                    float rf;
-                    const double R0 = block_distance(fma(dX0, dX0, fma(dY0[k], dY0[k], ZC2)), rf);
-                    const float th0 = static_cast<float>(__builtin_amdgcn_fract(R0 - zref));
-                    const float h = 0.5f * rf;
-                    const float dx0 = static_cast<float>(dX0), dy0 = static_cast<float>(dY0[k]);
-                    float a[kBlk], b[kBlk];
+            for (int sr0 = 0; sr0 < kQuad; sr0 += BLK) {
+                // Block rows sr0 .. sr0+BLK-1 of the quad.
+                const double jc = Q.j0 + sr0 + (BLK - 1) * 0.5;
+                const double FYc = kt * (P.y0 + jc * P.sy);
+                const double zc = kt * (P.z0 + jc * P.sz);
+                const double ZC2 = zc * zc;
+                float tey[BLK], eyz[BLK];
+                block_offsets<BLK>(kt * P.sy, tey, eyz);
 #pragma unroll
-                    for (int c = 0; c < kBlk; ++c) a[c] = fmaf(tex[c], dx0, ex2[c]);
+                for (int r = 0; r < BLK; ++r) {
+                    const double zr = kt * (P.z0 + (Q.j0 + sr0 + r) * P.sz);
+                    eyz[r] += static_cast<float>(zr * zr - ZC2);
+                }
+                double dY0[PQ];
 #pragma unroll
-                    for (int r = 0; r < kBlk; ++r) b[r] = fmaf(tey[r], dy0, eyz[r]);
-                    float br = 0.0f, bi = 0.0f;
+                for (int k = 0; k < PQ; ++k) dY0[k] = FYc - kt * qy[k];
+                for (int g = 0; g < ncol; g += BLK) {
+                    const double FXc = kt * (P.x0 + (cb + g + (BLK - 1) * 0.5) * P.sx);
 #pragma unroll
-                    for (int c = 0; c < kBlk; ++c) {
+                    for (int k = 0; k < PQ; ++k) {
+                        const double dX0 = FXc - X[k];
+                        float rf;
+                        const double R0 =
+                            block_distance(fma(dX0, dX0, fma(dY0[k], dY0[k], ZC2)), rf);
+                        const float th0 = static_cast<float>(__builtin_amdgcn_fract(R0 - zref));
+                        float br = 0.0f, bi = 0.0f;
+                        if constexpr (BLK == 1) {
+                            const float4 W = tile[g * kQuad + sr0];
+                            const float th = W.z - th0;
+                            br = W.x * __builtin_amdgcn_cosf(th);
+                            bi = W.x * __builtin_amdgcn_sinf(th);
+                        } else {
+                            const float h = 0.5f * rf;
+                            const float dx0 = static_cast<float>(dX0);
+                            const float dy0 = static_cast<float>(dY0[k]);
+                            float a[BLK], b[BLK];
 #pragma unroll
-                        for (int r = 0; r < kBlk; ++r) {
-                            const float4 W = tile[(g + c) * kBlk + r];
-                            const float x1 = (a[c] + b[r]) * h;
-                            const float y = x1 * h;
-                            float dR, A;
-                            pair_terms<ORDER>(x1, y, W.x, W.y, dR, A);
-                            const float th = (W.z - th0) - dR;
-                            br = fmaf(A, __builtin_amdgcn_cosf(th), br);
-                            bi = fmaf(A, __builtin_amdgcn_sinf(th), bi);
+                            for (int c = 0; c < BLK; ++c) a[c] = fmaf(tex[c], dx0, ex2[c]);
+#pragma unroll
+                            for (int r = 0; r < BLK; ++r) b[r] = fmaf(tey[r], dy0, eyz[r]);
+#pragma unroll
+                            for (int c = 0; c < BLK; ++c) {
+#pragma unroll
+                                for (int r = 0; r < BLK; ++r) {
+                                    const float4 W = tile[(g + c) * kQuad + sr0 + r];
+                                    const float x1 = (a[c] + b[r]) * h;
+                                    const float y = x1 * h;
+                                    float dR, A;
+                                    pair_terms<PO, AO>(x1, y, W.x, W.y, dR, A);
+                                    const float th = (W.z - th0) - dR;
+                                    br = fmaf(A, __builtin_amdgcn_cosf(th), br);
+                                    bi = fmaf(A, __builtin_amdgcn_sinf(th), bi);
+                                }
+                            }
                         }
+                        ar[k] = fmaf(rf, br, ar[k]);
+                        ai[k] = fmaf(rf, bi, ai[k]);
                     }
-                    ar[k] = fmaf(rf, br, ar[k]);
-                    ai[k] = fmaf(rf, bi, ai[k]);
                 }
             }
 #pragma unroll
@@ -532,7 +557,7 @@ __global__ __launch_bounds__(kThreads) void adj_fp32(
         const DevPlane& P = planes[Q.plane];
         const int nx = P.nx;
         const float k = static_cast<float>(P.k);
-        for (int j = Q.j0; j < min(Q.j0 + kBlk, P.ny); ++j) {
+        for (int j = Q.j0; j < min(Q.j0 + kQuad, P.ny); ++j) {
             const float fy = static_cast<float>(P.y0) + static_cast<float>(j) * static_cast<float>(P.sy);
             const float z = static_cast<float>(P.z0) + static_cast<float>(j) * static_cast<float>(P.sz);
             const float dy = fy - v.y;
@@ -577,7 +602,7 @@ __global__ __launch_bounds__(kThreads) void adj_fp64(
         const AdjQuad Q = quads[uq];
         const DevPlane& P = planes[Q.plane];
         const int nx = P.nx;
-        for (int j = Q.j0; j < min(Q.j0 + kBlk, P.ny); ++j) {
+        for (int j = Q.j0; j < min(Q.j0 + kQuad, P.ny); ++j) {
             const double fy = P.y0 + j * P.sy;
             const double z = P.z0 + j * P.sz;
             const double dy = fy - v.y;
@@ -652,10 +677,15 @@ hipError_t launch_forward(int mode, int order, const void* pil, int npil, const 
     if (mode == 0) {
         const double2* pos = static_cast<const double2*>(pil);
         const float2* ap = reinterpret_cast<const float2*>(pos + npil);
-        if (order == 1)
-            fwd_fp32c<1><<<nblocks, kThreads, 0, s>>>(pos, ap, npil, planes, blocks, partial);
-        else
-            fwd_fp32c<2><<<nblocks, kThreads, 0, s>>>(pos, ap, npil, planes, blocks, partial);
+        // order = 2 * (phase order - 1) + (amplitude order - 1) for 4 x 4 blocks;
+        // kExactOrder: 1 x 1 blocks (every pair at its own reference distance).
+        switch (order) {
+            case 0: fwd_fp32c<kBlk, 1, 1><<<nblocks, kThreads, 0, s>>>(pos, ap, npil, planes, blocks, partial); break;
+            case 1: fwd_fp32c<kBlk, 1, 2><<<nblocks, kThreads, 0, s>>>(pos, ap, npil, planes, blocks, partial); break;
+            case 2: fwd_fp32c<kBlk, 2, 1><<<nblocks, kThreads, 0, s>>>(pos, ap, npil, planes, blocks, partial); break;
+            case 3: fwd_fp32c<kBlk, 2, 2><<<nblocks, kThreads, 0, s>>>(pos, ap, npil, planes, blocks, partial); break;
+            default: fwd_fp32c<1, 1, 1><<<nblocks, kThreads, 0, s>>>(pos, ap, npil, planes, blocks, partial); break;
+        }
     } else if (mode == 2) {
         fwd_fp32<<<nblocks, kThreads, 0, s>>>(static_cast<const float4*>(pil), npil, planes,
                                               blocks, partial);
@@ -689,12 +719,13 @@ hipError_t launch_adjoint(int mode, int order, const void* pil, int npil, const 
     if (mode == 0) {
         const double2* pos = static_cast<const double2*>(pil);
         const float4* pix = static_cast<const float4*>(pix_or_u);
-        if (order == 1)
-            adj_fp32c<kPQ, 1><<<nblocks, kThreads, 0, s>>>(pos, npil, planes, quads, blocks, pix,
-                                                           partial);
-        else
-            adj_fp32c<kPQ, 2><<<nblocks, kThreads, 0, s>>>(pos, npil, planes, quads, blocks, pix,
-                                                           partial);
+        switch (order) {
+            case 0: adj_fp32c<kPQ, kBlk, 1, 1><<<nblocks, kThreads, 0, s>>>(pos, npil, planes, quads, blocks, pix, partial); break;
+            case 1: adj_fp32c<kPQ, kBlk, 1, 2><<<nblocks, kThreads, 0, s>>>(pos, npil, planes, quads, blocks, pix, partial); break;
+            case 2: adj_fp32c<kPQ, kBlk, 2, 1><<<nblocks, kThreads, 0, s>>>(pos, npil, planes, quads, blocks, pix, partial); break;
+            case 3: adj_fp32c<kPQ, kBlk, 2, 2><<<nblocks, kThreads, 0, s>>>(pos, npil, planes, quads, blocks, pix, partial); break;
+            default: adj_fp32c<kPQ, 1, 1, 1><<<nblocks, kThreads, 0, s>>>(pos, npil, planes, quads, blocks, pix, partial); break;
+        }
     } else if (mode == 2) {
         adj_fp32<<<nblocks, kThreads, 0, s>>>(static_cast<const float4*>(pil), npil, planes, quads,
                                               blocks, static_cast<const double2*>(pix_or_u),
