@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU time budget of the cpu_baseline sample")
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch-split autotuner")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="also run a plain device copy of BYTES (reference for HBM counters)")
     return ap.parse_args()
@@ -137,6 +138,14 @@ def main():
     loss = -encircled_energy(inten, plane, r0)
     loss.backward()
     u = Ev.grad.detach().contiguous()
+
+    tuning = None
+    if not args.no_tune:
+        from celeris_amd.tune import autotune
+
+        tuning = autotune(ctx, px, py, t, [plane], u, mode=args.mode, use_cache=False)
+        print(f"[bench] tuned work-groups: forward {tuning['fwd']}, adjoint {tuning.get('adj')}",
+              file=sys.stderr, flush=True)
 
     def step(mode):
         f, _ = ctx.forward(px, py, t, [plane], mode=mode, field=True)
@@ -231,6 +240,8 @@ def main():
             "pairs_per_step": pairs,
             "mode": args.mode,
             "parallelism": "single GPU",
+            "tuning": tuning and {"fwd_target_blocks": tuning["fwd"],
+                                  "adj_target_blocks": tuning.get("adj")},
         },
         "roofline": {
             "bound": "valu",
