@@ -29,8 +29,9 @@
 namespace crs {
 hipError_t launch_stage_pillars(int mode, const double* px, const double* py, const double* t,
                                 int npil, int nsets, void* out, hipStream_t s);
-hipError_t launch_forward(int mode, int order, const void* pil, int npil, const DevPlane* planes,
-                          const FwdBlock* blocks, int nblocks, double2* partial, hipStream_t s);
+hipError_t launch_forward(int mode, int order, const void* pil, int npil, int nsets,
+                          const DevPlane* planes, const FwdBlock* blocks, int nblocks,
+                          double2* partial, hipStream_t s);
 hipError_t launch_fwd_reduce(int restore_phase, const DevPlane* planes, const PixBlock* pblocks,
                              int nblocks, const double2* partial, double2* field,
                              double* intensity, hipStream_t s);
@@ -334,7 +335,7 @@ int fp32c_order(const std::vector<DevPlane>& dp) {
 // Bytes of staged pillar data: FP32C keeps FP64 positions once plus {|t|, arg t} per
 // set; FP32 and FP64 keep one {x, y, Re t, Im t} record per pillar and set.
 size_t staged_pillar_bytes(int mode, int64_t npil, int64_t nsets) {
-    if (mode == CRS_MODE_FP32C) return npil * sizeof(double2) + npil * nsets * sizeof(float2);
+    if (mode == CRS_MODE_FP32C) return fp32c_staged_layout(npil, nsets).bytes;
     return npil * nsets * (mode == CRS_MODE_FP64 ? sizeof(double4) : sizeof(float4));
 }
 
@@ -464,6 +465,18 @@ struct Scratch {  // where one call's device data lives (a context's or a graph'
 
 crs_status plan_forward(const crs_context* ctx, int mode, int npil, int nsets,
                         const crs_plane* planes, int nplanes, FwdPlan& plan) {
+    // Output ranges [offset, offset + nx*ny) must not overlap: each element has one writer.
+    {
+        std::vector<std::pair<int64_t, int64_t>> r(nplanes);
+        for (int p = 0; p < nplanes; ++p)
+            r[p] = {planes[p].offset, planes[p].offset + static_cast<int64_t>(planes[p].nx) * planes[p].ny};
+        std::sort(r.begin(), r.end());
+        for (int p = 1; p < nplanes; ++p)
+            if (r[p].first < r[p - 1].second) {
+                set_error("planes' output ranges overlap at element %lld", (long long)r[p].first);
+                return CRS_ERR_INVALID;
+            }
+    }
     plan.mode = mode;
     plan.npil = npil;
     plan.nsets = nsets;
@@ -474,17 +487,20 @@ crs_status plan_forward(const crs_context* ctx, int mode, int npil, int nsets,
         plan.order = fp32c_order(dp);
         blk = plan.order == kExactOrder ? 1 : kBlk;
     }
-    // Split over pillar chunks: enough work-groups to fill the chip, chunks of at
-    // least one LDS tile.
+    // Split over pillar chunks: enough work-groups to fill the chip, chunks of whole
+    // LDS tiles (FP32C: no longer than kMaxChunk).  FP32C work-groups take kFwdTiles
+    // thread tiles of a plane, FP32 / FP64 ones kThreads pixels.
+    const int per_wg = mode == CRS_MODE_FP32C ? kFwdTiles : kThreads;
     int64_t base_blocks = 0;
     for (int p = 0; p < nplanes; ++p) {
         dp[p] = make_dev_plane(planes[p], blk, blk);
-        base_blocks += ceil_div(static_cast<int64_t>(dp[p].tx) * dp[p].ty, kThreads);
+        base_blocks += ceil_div(static_cast<int64_t>(dp[p].tx) * dp[p].ty, per_wg);
     }
     const int target = ctx->fwd_target[mode] > 0 ? ctx->fwd_target[mode] : default_target(mode, false);
     int64_t nsplit = std::max<int64_t>(1, ceil_div(target, base_blocks));
+    if (mode == CRS_MODE_FP32C) nsplit = std::max<int64_t>(nsplit, ceil_div(npil, kMaxChunk));
     nsplit = std::min<int64_t>(nsplit, std::max<int64_t>(1, npil / kThreads));
-    const int64_t chunk = ceil_div(ceil_div(npil, nsplit), 64) * 64;
+    const int64_t chunk = ceil_div(ceil_div(npil, nsplit), kThreads) * kThreads;
     nsplit = ceil_div(npil, chunk);
     if (base_blocks * nsplit > (int64_t(1) << 31) - 1) {
         set_error("launch too large (%lld work-groups)", (long long)(base_blocks * nsplit));
@@ -504,7 +520,7 @@ crs_status plan_forward(const crs_context* ctx, int mode, int npil, int nsets,
         for (int p = 0; p < nplanes; ++p) {
             const int64_t tiles = static_cast<int64_t>(dp[p].tx) * dp[p].ty;
             const int64_t npix = static_cast<int64_t>(dp[p].nx) * dp[p].ny;
-            for (int64_t b = 0; b < tiles; b += kThreads)
+            for (int64_t b = 0; b < tiles; b += per_wg)
                 blocks.push_back({p, static_cast<int32_t>(b), q0, q1, dp[p].part + s * npix});
         }
     }
@@ -541,7 +557,7 @@ crs_status enqueue_forward(const FwdPlan& plan, const Scratch& sc, const double*
                             "stage pillars")) != CRS_OK)
         return st;
     if (ev0) hipEventRecord(ev0, stream);
-    if ((st = launch_status(launch_forward(mode, plan.order, sc.pil->ptr, plan.npil, d_planes,
+    if ((st = launch_status(launch_forward(mode, plan.order, sc.pil->ptr, plan.npil, plan.nsets, d_planes,
                                            reinterpret_cast<const FwdBlock*>(sc.tables + plan.o_blocks),
                                            plan.nblocks, static_cast<double2*>(sc.partial->ptr),
                                            stream),

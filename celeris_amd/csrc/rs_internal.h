@@ -10,12 +10,19 @@ namespace crs {
 
 // Work-group size of every pair kernel: 4 wave64s, one per SIMD of a CU.
 constexpr int kThreads = 256;
+// Waves (wave64) per work-group.
+constexpr int kWaves = kThreads / 64;
 // FP32C: pixels are handled in blocks of 4 x 4 (4 columns x 4 rows) that share one
 // reference distance, or 1 x 1 where a plane's pixel pitch is too coarse for the
-// block expansion.  The forward gives each thread one block; the adjoint walks a
-// plane in strips of kQuad rows ("quads") and each quad in 4-column groups.
+// block expansion.  The forward gives each lane one block ("thread tile") and a
+// work-group kFwdTiles tiles, its 4 waves splitting the pillar chunk; the adjoint
+// walks a plane in strips of kQuad rows ("quads") and each quad in 4-column groups.
 constexpr int kBlk = 4;
 constexpr int kQuad = 4;
+constexpr int kFwdTiles = 64;
+// Longest pillar chunk one FP32C forward work-group sums (each wave a quarter of it,
+// in float, folded every 64 pillars): bounds the float accumulation error.
+constexpr int kMaxChunk = 16384;
 // Launch order code of the 1 x 1 (unexpanded) FP32C kernels; 0..3 are the 4 x 4
 // kernels' series orders, 2 * (phase order - 1) + (amplitude order - 1).
 constexpr int kExactOrder = 4;
@@ -23,6 +30,25 @@ constexpr int kExactOrder = 4;
 constexpr int kPQ = 4;
 // FP32C adjoint: columns per LDS tile of a quad (kAdjCols x 4 pixel records).
 constexpr int kAdjCols = kThreads / kQuad;
+
+// FP32C staged pillar buffer: positions (double2, npil), {|t|, arg t / 2 pi} (float2,
+// nsets x npil), the staging blocks' maxima of |x|, |y| (double2 per block of
+// kThreads staged elements) and their overall maximum (one double2, um).
+struct StagedLayout {
+    size_t pos, ap, bmax, ext, bytes;
+    int64_t nbmax;
+};
+inline StagedLayout fp32c_staged_layout(int64_t npil, int64_t nsets) {
+    auto align16 = [](size_t v) { return (v + 15) & ~static_cast<size_t>(15); };
+    StagedLayout L;
+    L.nbmax = (npil * nsets + kThreads - 1) / kThreads;
+    L.pos = 0;
+    L.ap = static_cast<size_t>(npil) * 16;
+    L.bmax = align16(L.ap + static_cast<size_t>(npil * nsets) * 8);
+    L.ext = L.bmax + static_cast<size_t>(L.nbmax) * 16;
+    L.bytes = L.ext + 16;
+    return L;
+}
 
 // One plane as the kernels see it (host computes the derived fields).
 struct DevPlane {
@@ -44,8 +70,9 @@ struct DevPlane {
     int32_t nquads; // ceil(ny / 4)
 };
 
-// Forward work-group: thread tiles [tile0, tile0+kThreads) of `plane`, summed over
-// pillars [q0, q1); results go to partial[part + pixel].
+// Forward work-group: thread tiles [tile0, tile0 + kFwdTiles) (FP32C) or pixels
+// [tile0, tile0 + kThreads) (FP32, FP64) of `plane`, summed over pillars [q0, q1);
+// results go to partial[part + pixel].
 struct FwdBlock {
     int32_t plane, tile0, q0, q1;
     int64_t part;

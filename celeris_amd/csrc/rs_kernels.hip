@@ -98,18 +98,72 @@ struct alignas(16) PillarRec {
 
 // FP32C: positions stay FP64 (pos[npil], shared by all sets; each plane scales them
 // by its own k/(2 pi) when it loads a tile) and {|t|, arg(t)/(2 pi)} per pillar and set.
-// t == NULL stages positions only.
-__global__ void stage_pillars_fp32c(const double* __restrict__ px, const double* __restrict__ py,
-                                    const double2* __restrict__ t, int npil, int64_t total,
-                                    double2* __restrict__ pos, float2* __restrict__ ap) {
+// t == NULL stages positions only.  Each block also writes the maximum |x|, |y| of the
+// positions it staged (0 when it staged none) for extent_reduce.
+__global__ __launch_bounds__(kThreads) void stage_pillars_fp32c(
+    const double* __restrict__ px, const double* __restrict__ py, const double2* __restrict__ t,
+    int npil, int64_t total, double2* __restrict__ pos, float2* __restrict__ ap,
+    double2* __restrict__ bmax) {
+    __shared__ double2 red[kWaves];
     const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (idx >= total) return;
-    const int q = static_cast<int>(idx % npil);
-    if (idx < npil) pos[q] = make_double2(px[q], py[q]);
-    if (t) {
-        const double2 v = t[idx];
-        ap[idx] = make_float2(static_cast<float>(hypot(v.x, v.y)),
-                              static_cast<float>(atan2(v.y, v.x) * kInv2Pi));
+    double mx = 0.0, my = 0.0;
+    if (idx < total) {
+        const int q = static_cast<int>(idx % npil);
+        if (idx < npil) {
+            const double x = px[q], y = py[q];
+            pos[q] = make_double2(x, y);
+            mx = fabs(x);
+            my = fabs(y);
+        }
+        if (t) {
+            const double2 v = t[idx];
+            ap[idx] = make_float2(static_cast<float>(hypot(v.x, v.y)),
+                                  static_cast<float>(atan2(v.y, v.x) * kInv2Pi));
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        mx = fmax(mx, __shfl_xor(mx, o));
+        my = fmax(my, __shfl_xor(my, o));
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = make_double2(mx, my);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double2 m = red[0];
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w) {
+            m.x = fmax(m.x, red[w].x);
+            m.y = fmax(m.y, red[w].y);
+        }
+        bmax[blockIdx.x] = m;
+    }
+}
+
+// One work-group: ext = max over the staging blocks' maxima (the pillars' extent, um).
+__global__ __launch_bounds__(kThreads) void extent_reduce(const double2* __restrict__ bmax,
+                                                          int n, double2* __restrict__ ext) {
+    __shared__ double2 red[kWaves];
+    double mx = 0.0, my = 0.0;
+    for (int i = threadIdx.x; i < n; i += kThreads) {
+        const double2 v = bmax[i];
+        mx = fmax(mx, v.x);
+        my = fmax(my, v.y);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        mx = fmax(mx, __shfl_xor(mx, o));
+        my = fmax(my, __shfl_xor(my, o));
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = make_double2(mx, my);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double2 m = red[0];
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w) {
+            m.x = fmax(m.x, red[w].x);
+            m.y = fmax(m.y, red[w].y);
+        }
+        ext[0] = m;
     }
 }
 
@@ -138,48 +192,172 @@ __global__ void stage_pillars_fp64(const double* __restrict__ px, const double* 
 }
 
 // ---------------------------------------------------------------------------------
-// Forward, FP32C.  Each thread owns one BLK x BLK pixel block; pillars stream through
-// LDS, scaled to turns by the plane's k/(2 pi) as they are loaded.  BLK = 1 evaluates
-// every pair at its own FP64 reference distance (no expansion).
+// Forward, FP32C.  A work-group owns kFwdTiles thread tiles (one BX x BY pixel block per
+// lane) of one plane; its 4 waves split the pillar chunk: pillars stream through LDS in
+// tiles of 256, scaled to turns by the plane's k/(2 pi) as they are loaded, and wave w
+// takes entries [64w, 64w + 64) of each.  The waves' sums are added in FP64 at the end.
+// BX = BY = 1 evaluates every pair at its own FP64 reference distance (no expansion).
 
-template <int BLK, int PO, int AO>
-__global__ __launch_bounds__(kThreads, 2) void fwd_fp32c(
+// Per-lane constants of a pixel block: its centre (turns, FP64) and the offsets of its
+// columns / rows, te = 2 e, e2 = e^2 (+ z_row'^2 - z_centre'^2 for the rows).
+template <int BX, int BY>
+struct BlockGeom {
+    double FXc, FYc, ZC2;
+    float tex[BX], ex2[BX], tey[BY], eyz[BY];
+};
+
+// One pillar against one block: th0 = frac(R0 - zeta), h = 1/(2 R0), the amplitude
+// pair c2 = 2 c1, cn = -c1 (c1 = |t| (k/2pi) / R0'), and the separable parts of
+// x1 = D h: a[c] = (2 e_c dx0 + e_c^2) h, b[r] = (2 e_r dy0 + e_r^2 + dz2_r) h.
+template <int BX, int BY>
+struct PairState {
+    float th0, h, c2, cn;
+    float a[BX], b[BY];
+};
+
+template <int BX, int BY>
+__device__ __forceinline__ void pair_state(const PillarRec& T, const BlockGeom<BX, BY>& G,
+                                           PairState<BX, BY>& S) {
+    const double dX0 = G.FXc - T.x, dY0 = G.FYc - T.y;
+    float rf;
+    const double R0 = block_distance(fma(dX0, dX0, fma(dY0, dY0, G.ZC2)), rf);
+    S.th0 = static_cast<float>(__builtin_amdgcn_fract(R0 - T.zeta));
+    S.h = 0.5f * rf;
+    const float c1 = T.amp * rf;
+    S.c2 = c1 + c1;
+    S.cn = -c1;
+    if constexpr (BX * BY > 1) {
+        const float dx0 = static_cast<float>(dX0), dy0 = static_cast<float>(dY0);
+#pragma unroll
+        for (int c = 0; c < BX; ++c) S.a[c] = fmaf(G.tex[c], dx0, G.ex2[c]) * S.h;
+#pragma unroll
+        for (int r = 0; r < BY; ++r) S.b[r] = fmaf(G.tey[r], dy0, G.eyz[r]) * S.h;
+    }
+}
+
+// The BX x BY pairs of one pillar.  With x1 = a[c] + b[r], y = x1 h = D / (4 R0^2):
+//   GEN = 0:  R - R0 = x1 w,  w = 1 - y;             R0/R = 2 w - 1 = 1 - 2 y
+//   GEN = 1:  R - R0 = x1 (1 - y + 2 y^2);           R0/R = 1 - 2 y + 6 y^2
+// and E += c1 (R0/R) (cos 2 pi th, sin 2 pi th), th = th0 + (R - R0), in turns.
+template <int GEN, int BX, int BY>
+__device__ __forceinline__ void block_pairs(const PairState<BX, BY>& S, float (&ar)[BX * BY],
+                                            float (&ai)[BX * BY]) {
+    if constexpr (BX * BY == 1) {
+        const float A = S.c2 + S.cn;
+        ar[0] = fmaf(A, __builtin_amdgcn_cosf(S.th0), ar[0]);
+        ai[0] = fmaf(A, __builtin_amdgcn_sinf(S.th0), ai[0]);
+    } else {
+#pragma unroll
+        for (int r = 0; r < BY; ++r) {
+#pragma unroll
+            for (int c = 0; c < BX; ++c) {
+                const float x1 = S.a[c] + S.b[r];
+                const float w1 = fmaf(-x1, S.h, 1.0f);
+                float w = w1, A;
+                if constexpr (GEN) {
+                    const float y = x1 * S.h;
+                    w = fmaf(y + y, y, w1);
+                    A = fmaf(3.0f * S.c2 * y, y, fmaf(S.c2, w1, S.cn));
+                } else {
+                    A = fmaf(S.c2, w1, S.cn);
+                }
+                const float th = fmaf(x1, w, S.th0);
+                const int e = r * BX + c;
+                ar[e] = fmaf(A, __builtin_amdgcn_cosf(th), ar[e]);
+                ai[e] = fmaf(A, __builtin_amdgcn_sinf(th), ai[e]);
+            }
+        }
+    }
+}
+
+// Pillars tile[k0 .. k1) (k0 < k1) against one block; the next pillar's FP64 distance
+// work is issued alongside the current pillar's pairs.
+template <int GEN, int BX, int BY>
+__device__ __forceinline__ void pillar_run(const PillarRec* tile, int k0, int k1,
+                                           const BlockGeom<BX, BY>& G, float (&ar)[BX * BY],
+                                           float (&ai)[BX * BY]) {
+    PairState<BX, BY> cur;
+    pair_state(tile[k0], G, cur);
+    for (int k = k0 + 1; k < k1; ++k) {
+        PairState<BX, BY> nxt;
+        pair_state(tile[k], G, nxt);
+        block_pairs<GEN>(cur, ar, ai);
+        cur = nxt;
+    }
+    block_pairs<GEN>(cur, ar, ai);
+}
+
+// Whether the first-order pair terms meet the error budget for every 4 x 4 block of
+// plane P against pillars within |x| <= ext.x, |y| <= ext.y (um).  With e' the block's
+// half diagonal, e_z' its half depth, z' the nearest row and sin the steepest pillar-
+// to-pixel direction: |D| / R0 <= B = 2 e' sin + 2 e_z' + (e'^2 + e_z'^2) / z', and the
+// dropped terms are 2 x1 y^2 <= B^3 / (16 z'^2) turns and 6 y^2 <= 3 B^2 / (8 z'^2).
+__device__ inline bool fp32c_first_order(const DevPlane& P, double2 ext, int bx, int by) {
+    const double xl = P.x0 + (P.nx - 1) * P.sx, yl = P.y0 + (P.ny - 1) * P.sy;
+    const double zl = P.z0 + (P.ny - 1) * P.sz;
+    const double rx = fmax(fabs(P.x0), fabs(xl)) + ext.x;
+    const double ry = fmax(fabs(P.y0), fabs(yl)) + ext.y;
+    const double zmin = fmin(fabs(P.z0), fabs(zl));
+    const double rho2 = rx * rx + ry * ry;
+    const double sn = sqrt(rho2 / (rho2 + zmin * zmin));
+    const double hx = 0.5 * (bx - 1) * fabs(P.sx) * P.kturn;
+    const double hy = 0.5 * (by - 1) * fabs(P.sy) * P.kturn;
+    const double hz = 0.5 * (by - 1) * fabs(P.sz) * P.kturn;
+    const double e = sqrt(hx * hx + hy * hy);
+    const double z = zmin * P.kturn;
+    const double Bd = 2.0 * e * sn + 2.0 * hz + (e * e + hz * hz) / z;
+    const double z2 = z * z;
+    return Bd * Bd * Bd / (16.0 * z2) <= 1.6e-7 && 3.0 * Bd * Bd / (8.0 * z2) <= 4e-6;
+}
+
+template <int BX, int BY>
+__global__ __launch_bounds__(kThreads, 4) void fwd_fp32c(
     const double2* __restrict__ pos, const float2* __restrict__ ap, int npil,
     const DevPlane* __restrict__ planes, const FwdBlock* __restrict__ blocks,
-    double2* __restrict__ partial) {
+    const double2* __restrict__ ext, double2* __restrict__ partial) {
+    constexpr int N = BX * BY;
     __shared__ PillarRec tile[kThreads];
+    __shared__ float2 run[kWaves * N * 64];
     const FwdBlock B = blocks[blockIdx.x];
     const DevPlane& P = planes[B.plane];
     const int nx = P.nx, ny = P.ny;
-    const int tid = threadIdx.x;
-    const int my_tile = B.tile0 + tid;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int my_tile = B.tile0 + lane;
     const bool active = my_tile < P.tx * P.ty;
     const int tt = active ? my_tile : 0;
     const int ti = tt % P.tx, tj = tt / P.tx;
     const double kt = P.kturn;
     const double zref = P.zref;
+    const bool gen = N > 1 && !fp32c_first_order(P, ext[0], BX, BY);
 
-    // Block centre and the offsets of its pixels (turns).
-    const double ic = ti * BLK + (BLK - 1) * 0.5, jc = tj * BLK + (BLK - 1) * 0.5;
-    const double FXc = kt * (P.x0 + ic * P.sx);
-    const double FYc = kt * (P.y0 + jc * P.sy);
-    const double zc = kt * (P.z0 + jc * P.sz);
-    const double ZC2 = zc * zc;
-    float tex[BLK], ex2[BLK], tey[BLK], eyz[BLK];
-    block_offsets<BLK>(kt * P.sx, tex, ex2);
-    block_offsets<BLK>(kt * P.sy, tey, eyz);
+    BlockGeom<BX, BY> G;
+    {
+        const double ic = ti * BX + (BX - 1) * 0.5, jc = tj * BY + (BY - 1) * 0.5;
+        G.FXc = kt * (P.x0 + ic * P.sx);
+        G.FYc = kt * (P.y0 + jc * P.sy);
+        const double zc = kt * (P.z0 + jc * P.sz);
+        G.ZC2 = zc * zc;
 #pragma unroll
-    for (int r = 0; r < BLK; ++r) {
-        const double zr = kt * (P.z0 + (tj * BLK + r) * P.sz);
-        eyz[r] += static_cast<float>(zr * zr - ZC2);
+        for (int c = 0; c < BX; ++c) {
+            const float e = static_cast<float>(kt * P.sx * (c - (BX - 1) * 0.5));
+            G.tex[c] = 2.0f * e;
+            G.ex2[c] = e * e;
+        }
+#pragma unroll
+        for (int r = 0; r < BY; ++r) {
+            const float e = static_cast<float>(kt * P.sy * (r - (BY - 1) * 0.5));
+            const double zr = kt * (P.z0 + (tj * BY + r) * P.sz);
+            G.tey[r] = 2.0f * e;
+            G.eyz[r] = e * e + static_cast<float>(zr * zr - G.ZC2);
+        }
     }
     const float amp_scale = static_cast<float>(kt);  // 1/R = (k/2pi) / R'
     const float2* __restrict__ aps = ap + static_cast<int64_t>(P.tset) * npil;
 
-    constexpr int NPIX = BLK * BLK;
-    float sr[NPIX], si[NPIX], cr[NPIX], ci[NPIX];
+    // Each lane's running sums live in LDS (float, one slot per wave, pixel and lane),
+    // fed by register sums over at most 64 pillars per LDS tile.
 #pragma unroll
-    for (int e = 0; e < NPIX; ++e) sr[e] = si[e] = cr[e] = ci[e] = 0.0f;
+    for (int e = 0; e < N; ++e) run[(wave * N + e) * 64 + lane] = make_float2(0.0f, 0.0f);
 
     for (int base = B.q0; base < B.q1; base += kThreads) {
         const int n = min(kThreads, B.q1 - base);
@@ -196,71 +374,44 @@ __global__ __launch_bounds__(kThreads, 2) void fwd_fp32c(
             tile[tid] = rec;
         }
         __syncthreads();
-        float ar[NPIX], ai[NPIX];
+        const int k0 = wave * 64, k1 = min(k0 + 64, n);
+        if (k0 < k1) {
+            float ar[N], ai[N];
 #pragma unroll
-        for (int e = 0; e < NPIX; ++e) ar[e] = ai[e] = 0.0f;
-        for (int q = 0; q < n; ++q) {
-            const PillarRec T = tile[q];
-            const double dX0 = FXc - T.x, dY0 = FYc - T.y;
-            float rf;
-            const double R0 = block_distance(fma(dX0, dX0, fma(dY0, dY0, ZC2)), rf);
-            const float th0 = static_cast<float>(__builtin_amdgcn_fract(R0 - T.zeta));
-            const float c1 = T.amp * rf;
-            if constexpr (BLK == 1) {
-                ar[0] = fmaf(c1, __builtin_amdgcn_cosf(th0), ar[0]);
-                ai[0] = fmaf(c1, __builtin_amdgcn_sinf(th0), ai[0]);
-            } else {
-                const float h = 0.5f * rf;
-                const float c1n = -2.0f * c1;
-                const float dx0 = static_cast<float>(dX0), dy0 = static_cast<float>(dY0);
-                float a[BLK], b[BLK];
+            for (int e = 0; e < N; ++e) ar[e] = ai[e] = 0.0f;
+            if (gen)
+                pillar_run<1>(tile, k0, k1, G, ar, ai);
+            else
+                pillar_run<0>(tile, k0, k1, G, ar, ai);
 #pragma unroll
-                for (int c = 0; c < BLK; ++c) a[c] = fmaf(tex[c], dx0, ex2[c]);
-#pragma unroll
-                for (int r = 0; r < BLK; ++r) b[r] = fmaf(tey[r], dy0, eyz[r]);
-#pragma unroll
-                for (int r = 0; r < BLK; ++r) {
-#pragma unroll
-                    for (int c = 0; c < BLK; ++c) {
-                        const float x1 = (a[c] + b[r]) * h;
-                        const float y = x1 * h;
-                        float dR, A;
-                        pair_terms<PO, AO>(x1, y, c1, c1n, dR, A);
-                        const float th = th0 + dR;
-                        const int e = r * BLK + c;
-                        ar[e] = fmaf(A, __builtin_amdgcn_cosf(th), ar[e]);
-                        ai[e] = fmaf(A, __builtin_amdgcn_sinf(th), ai[e]);
-                    }
-                }
+            for (int e = 0; e < N; ++e) {
+                float2& v = run[(wave * N + e) * 64 + lane];
+                v.x += ar[e];
+                v.y += ai[e];
             }
-        }
-        // Kahan: fold this pillar tile's partial into the running sum.
-#pragma unroll
-        for (int e = 0; e < NPIX; ++e) {
-            const float yr = ar[e] - cr[e];
-            const float tr = sr[e] + yr;
-            cr[e] = (tr - sr[e]) - yr;
-            sr[e] = tr;
-            const float yi = ai[e] - ci[e];
-            const float tq = si[e] + yi;
-            ci[e] = (tq - si[e]) - yi;
-            si[e] = tq;
         }
     }
 
-    if (!active) return;
+    // Wave 0 adds the 4 waves' sums in FP64, in wave order.
+    __syncthreads();
+    if (wave != 0 || !active) return;
 #pragma unroll
-    for (int r = 0; r < BLK; ++r) {
-        const int j = tj * BLK + r;
+    for (int r = 0; r < BY; ++r) {
+        const int j = tj * BY + r;
         if (j >= ny) break;
 #pragma unroll
-        for (int c = 0; c < BLK; ++c) {
-            const int i = ti * BLK + c;
+        for (int c = 0; c < BX; ++c) {
+            const int i = ti * BX + c;
             if (i >= nx) break;
-            const int e = r * BLK + c;
-            partial[B.part + static_cast<int64_t>(j) * nx + i] =
-                make_double2(static_cast<double>(sr[e]) - static_cast<double>(cr[e]),
-                             static_cast<double>(si[e]) - static_cast<double>(ci[e]));
+            const int e = r * BX + c;
+            double vr = 0.0, vi = 0.0;
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                const float2 o = run[(w * N + e) * 64 + lane];
+                vr += o.x;
+                vi += o.y;
+            }
+            partial[B.part + static_cast<int64_t>(j) * nx + i] = make_double2(vr, vi);
         }
     }
 }
@@ -657,10 +808,14 @@ hipError_t launch_stage_pillars(int mode, const double* px, const double* py, co
     const int64_t total = static_cast<int64_t>(npil) * nsets;
     const double2* tt = reinterpret_cast<const double2*>(t);
     if (mode == 0) {
-        // Layout: pos (double2 x npil) then ap (float2 x nsets*npil).
-        double2* pos = static_cast<double2*>(out);
-        float2* ap = reinterpret_cast<float2*>(pos + npil);
-        stage_pillars_fp32c<<<grid_for(total), kThreads, 0, s>>>(px, py, tt, npil, total, pos, ap);
+        const StagedLayout L = fp32c_staged_layout(npil, nsets);
+        unsigned char* base = static_cast<unsigned char*>(out);
+        double2* bmax = reinterpret_cast<double2*>(base + L.bmax);
+        stage_pillars_fp32c<<<grid_for(total), kThreads, 0, s>>>(
+            px, py, tt, npil, total, reinterpret_cast<double2*>(base + L.pos),
+            reinterpret_cast<float2*>(base + L.ap), bmax);
+        extent_reduce<<<1, kThreads, 0, s>>>(bmax, static_cast<int>(L.nbmax),
+                                             reinterpret_cast<double2*>(base + L.ext));
     } else if (mode == 2) {
         stage_pillars_fp32<<<grid_for(total), kThreads, 0, s>>>(px, py, tt, npil, total,
                                                                  static_cast<float4*>(out));
@@ -671,21 +826,22 @@ hipError_t launch_stage_pillars(int mode, const double* px, const double* py, co
     return hipGetLastError();
 }
 
-hipError_t launch_forward(int mode, int order, const void* pil, int npil, const DevPlane* planes,
-                          const FwdBlock* blocks, int nblocks, double2* partial, hipStream_t s) {
+hipError_t launch_forward(int mode, int order, const void* pil, int npil, int nsets,
+                          const DevPlane* planes, const FwdBlock* blocks, int nblocks,
+                          double2* partial, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
     if (mode == 0) {
-        const double2* pos = static_cast<const double2*>(pil);
-        const float2* ap = reinterpret_cast<const float2*>(pos + npil);
-        // order = 2 * (phase order - 1) + (amplitude order - 1) for 4 x 4 blocks;
-        // kExactOrder: 1 x 1 blocks (every pair at its own reference distance).
-        switch (order) {
-            case 0: fwd_fp32c<kBlk, 1, 1><<<nblocks, kThreads, 0, s>>>(pos, ap, npil, planes, blocks, partial); break;
-            case 1: fwd_fp32c<kBlk, 1, 2><<<nblocks, kThreads, 0, s>>>(pos, ap, npil, planes, blocks, partial); break;
-            case 2: fwd_fp32c<kBlk, 2, 1><<<nblocks, kThreads, 0, s>>>(pos, ap, npil, planes, blocks, partial); break;
-            case 3: fwd_fp32c<kBlk, 2, 2><<<nblocks, kThreads, 0, s>>>(pos, ap, npil, planes, blocks, partial); break;
-            default: fwd_fp32c<1, 1, 1><<<nblocks, kThreads, 0, s>>>(pos, ap, npil, planes, blocks, partial); break;
-        }
+        const StagedLayout L = fp32c_staged_layout(npil, nsets);
+        const unsigned char* base = static_cast<const unsigned char*>(pil);
+        const double2* pos = reinterpret_cast<const double2*>(base + L.pos);
+        const float2* ap = reinterpret_cast<const float2*>(base + L.ap);
+        const double2* ext = reinterpret_cast<const double2*>(base + L.ext);
+        // order kExactOrder: 1 x 1 blocks (every pair at its own reference distance);
+        // otherwise 4 x 4 blocks, series order chosen per work-group from ext.
+        if (order == kExactOrder)
+            fwd_fp32c<1, 1><<<nblocks, kThreads, 0, s>>>(pos, ap, npil, planes, blocks, ext, partial);
+        else
+            fwd_fp32c<kBlk, kBlk><<<nblocks, kThreads, 0, s>>>(pos, ap, npil, planes, blocks, ext, partial);
     } else if (mode == 2) {
         fwd_fp32<<<nblocks, kThreads, 0, s>>>(static_cast<const float4*>(pil), npil, planes,
                                               blocks, partial);
@@ -717,7 +873,7 @@ hipError_t launch_adjoint(int mode, int order, const void* pil, int npil, const 
                           const void* pix_or_u, double2* partial, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
     if (mode == 0) {
-        const double2* pos = static_cast<const double2*>(pil);
+        const double2* pos = static_cast<const double2*>(pil);  // staged layout: pos first
         const float4* pix = static_cast<const float4*>(pix_or_u);
         switch (order) {
             case 0: adj_fp32c<kPQ, kBlk, 1, 1><<<nblocks, kThreads, 0, s>>>(pos, npil, planes, quads, blocks, pix, partial); break;

@@ -114,7 +114,7 @@ def test_batched_uneven_planes(torch_cuda, rsctx):
         Plane.psf(0.3, -0.2, 28.5, 2 * math.pi / 0.55, 7, 1.5, tset=2, offset=13 * 13 + 5),
         Plane.zx(-2.0, 2.0, 11, 20.0, 40.0, 6, 2 * math.pi / 0.65, tset=1),
         Plane.axis(10.0, 50.0, 9, 2 * math.pi / 0.5, tset=1),
-        Plane(2 * math.pi / 0.6, -1.0, 0.13, 0.7, 0.0, 25.0, 0.0, 5, 1, 0, 0),
+        Plane(2 * math.pi / 0.6, -1.0, 0.13, 0.7, 0.0, 25.0, 0.0, 5, 1, 0, -1),
     ]
     ref = oracle.forward(px, py, t, planes)
     total = ref.size

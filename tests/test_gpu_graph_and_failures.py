@@ -129,3 +129,7 @@ def test_invalid_arguments(torch_cuda, rsctx):
         rsctx.forward(dpx, dpy, dt, bad)
     with pytest.raises(RuntimeError, match="CRS_ERR_INVALID"):
         rsctx.forward(dpx, dpy, dt, [Plane(10.0, 0, 0.1, 0, 0.1, 5, 0, 0, 3)])
+    # Two planes writing the same output elements: refused (one writer per element).
+    overlap = [Plane.psf(0, 0, 30.0, 10.0, 9, 2.0, offset=0), Plane.psf(0, 0, 31.0, 10.0, 5, 2.0, offset=80)]
+    with pytest.raises(RuntimeError, match="overlap"):
+        rsctx.forward(dpx, dpy, dt, overlap)
