@@ -56,34 +56,6 @@ __device__ inline double block_distance(double R0sq, float& rf) {
     return fma(res, static_cast<double>(0.5f * rf), R0f);
 }
 
-// Offsets of a block's B columns (or rows) from its centre, in turns:
-// e = step * (c - (B-1)/2).
-template <int B>
-__device__ inline void block_offsets(double step_turns, float (&te)[B], float (&e2)[B]) {
-#pragma unroll
-    for (int c = 0; c < B; ++c) {
-        const float e = static_cast<float>(step_turns * (c - (B - 1) * 0.5));
-        te[c] = 2.0f * e;
-        e2[c] = e * e;
-    }
-}
-
-// One pair's path excess R - R0 (turns) and amplitude A = a0 R0/R, from x1 = D h and
-// y = x1 h (y = D / (4 R0^2)); a0 is the amplitude at R0 and a0n = -2 a0.  Phase order
-// PO = 1 keeps R - R0 = x1 (1 - y), PO = 2 adds 2 x1 y^2; amplitude order AO = 1 keeps
-// R0/R = 1 - 2y, AO = 2 adds 6 y^2.
-template <int PO, int AO>
-__device__ inline void pair_terms(float x1, float y, float a0, float a0n, float& dR, float& A) {
-    if constexpr (PO == 1)
-        dR = fmaf(-y, x1, x1);
-    else
-        dR = fmaf(x1 * y, fmaf(2.0f, y, -1.0f), x1);
-    if constexpr (AO == 1)
-        A = fmaf(a0n, y, a0);
-    else
-        A = fmaf(a0 * y, fmaf(6.0f, y, -2.0f), a0);
-}
-
 // LDS record of one pillar for the FP32C forward: FP64 turn-scaled position, the
 // phase offset z'_ref - arg(t)/2pi, and |t| k/2pi.
 struct alignas(16) PillarRec {
@@ -534,8 +506,8 @@ __global__ __launch_bounds__(kThreads) void fwd_reduce(
 // Adjoint staging (FP32C).  Pixel records are laid out quad by quad (4 rows), column
 // by column, 4 records per column: record (quad u, column i, row r) sits at
 // sbase + (u*nxp + i)*4 + r, so an LDS tile of 64 columns x 4 rows is one contiguous
-// run.  Record = {|u| k/2pi, -2|u| k/2pi, arg(u)/2pi - z'_ref mod 1, 0}; padding (rows >= ny,
-// columns >= nx) has |u| = 0 and contributes nothing.
+// run.  Record = {2U, -U, arg(u)/2pi - z'_ref mod 1, 0} with U = |u| k/2pi; padding
+// (rows >= ny, columns >= nx) has U = 0 and contributes nothing.
 
 __global__ __launch_bounds__(kThreads) void stage_pixels_fp32c(
     const DevPlane* __restrict__ planes, const PixBlock* __restrict__ pblocks,
@@ -554,47 +526,155 @@ __global__ __launch_bounds__(kThreads) void stage_pixels_fp32c(
         double chi = atan2(v.y, v.x) * kInv2Pi - P.zref;
         chi -= rint(chi);
         const float U = static_cast<float>(hypot(v.x, v.y) * P.kturn);
-        rec = make_float4(U, -2.0f * U, static_cast<float>(chi), 0.0f);
+        rec = make_float4(2.0f * U, -U, static_cast<float>(chi), 0.0f);
     }
     pix[P.sbase + rix] = rec;
 }
 
 // ---------------------------------------------------------------------------------
-// Adjoint, FP32C.  Thread owns PQ pillars (FP64 positions in registers, scaled to
-// turns by each quad's plane); a quad's pixel records stream through LDS 64 columns
-// at a time and are consumed in BLK x BLK blocks, each with its own reference distance
-// (BLK = 1: every pair at its own).
-// Per pair: conj(G) u = |u| (k/2pi)/R0 * (R0/R) * exp(2 pi i (chi - th0 - (R - R0))).
+// Adjoint, FP32C.  Thread owns kPQ pillars (FP64 positions in registers); a quad's pixel
+// records stream through LDS 64 columns at a time and are consumed in BX x BY blocks
+// (4 x 4, or 1 x 1 when every pair needs its own reference distance).  Per pair
+//     conj(G) u = U / R0' (R0/R) exp(2 pi i (chi - th0 - (R - R0))),   U = |u| k/2pi,
+// with the same expansion of R - R0 and R0/R as the forward; the block's sum is scaled
+// by 1/R0' once.
 
-template <int PQ, int BLK, int PO, int AO>
-__global__ __launch_bounds__(kThreads, 2) void adj_fp32c(
+// One pillar against one block: th0 = frac(R0 - z'_ref), h = 1/(2 R0), rf = 1/R0 and
+// the separable parts a[c], b[r] of x1 = D h.
+template <int BX, int BY>
+struct AdjState {
+    float th0, h, rf;
+    float a[BX], b[BY];
+};
+
+template <int BX, int BY>
+__device__ __forceinline__ void adj_state(double dX0, double dY0, double zref,
+                                          const BlockGeom<BX, BY>& G, AdjState<BX, BY>& S) {
+    float rf;
+    const double R0 = block_distance(fma(dX0, dX0, fma(dY0, dY0, G.ZC2)), rf);
+    S.th0 = static_cast<float>(__builtin_amdgcn_fract(R0 - zref));
+    S.rf = rf;
+    S.h = 0.5f * rf;
+    if constexpr (BX * BY > 1) {
+        const float dx0 = static_cast<float>(dX0), dy0 = static_cast<float>(dY0);
+#pragma unroll
+        for (int c = 0; c < BX; ++c) S.a[c] = fmaf(G.tex[c], dx0, G.ex2[c]) * S.h;
+#pragma unroll
+        for (int r = 0; r < BY; ++r) S.b[r] = fmaf(G.tey[r], dy0, G.eyz[r]) * S.h;
+    }
+}
+
+// The pairs of one pillar with one block's records W (column-major: W[c*BY + r] =
+// {2U, -U, chi}); adds rf * sum to (gr, gi).  GEN as in block_pairs.
+template <int GEN, int BX, int BY>
+__device__ __forceinline__ void adj_pairs(const AdjState<BX, BY>& S, const float3 (&W)[BX * BY],
+                                          float& gr, float& gi) {
+    float br = 0.0f, bi = 0.0f;
+#pragma unroll
+    for (int c = 0; c < BX; ++c) {
+#pragma unroll
+        for (int r = 0; r < BY; ++r) {
+            const float3 w3 = W[c * BY + r];
+            float phi, A;
+            if constexpr (BX * BY == 1) {
+                phi = w3.z - S.th0;
+                A = w3.x + w3.y;
+            } else {
+                const float x1 = S.a[c] + S.b[r];
+                const float w1 = fmaf(-x1, S.h, 1.0f);
+                float w = w1;
+                A = fmaf(w3.x, w1, w3.y);
+                if constexpr (GEN) {
+                    const float y = x1 * S.h;
+                    w = fmaf(y + y, y, w1);
+                    A = fmaf(3.0f * w3.x * y, y, A);
+                }
+                phi = fmaf(-x1, w, w3.z - S.th0);
+            }
+            br = fmaf(A, __builtin_amdgcn_cosf(phi), br);
+            bi = fmaf(A, __builtin_amdgcn_sinf(phi), bi);
+        }
+    }
+    gr = fmaf(S.rf, br, gr);
+    gi = fmaf(S.rf, bi, gi);
+}
+
+// One 64-column LDS tile of a quad against the thread's pillars.
+template <int GEN, int BX, int BY>
+__device__ __forceinline__ void adj_tile(const float4* tile, int ncol, const DevPlane& P, int j0,
+                                         int cb, const double (&qx)[kPQ],
+                                         const double (&qy)[kPQ], float (&ar)[kPQ],
+                                         float (&ai)[kPQ]) {
+    const double kt = P.kturn;
+    BlockGeom<BX, BY> G;
+#pragma unroll
+    for (int c = 0; c < BX; ++c) {
+        const float e = static_cast<float>(kt * P.sx * (c - (BX - 1) * 0.5));
+        G.tex[c] = 2.0f * e;
+        G.ex2[c] = e * e;
+    }
+#pragma unroll
+    for (int sr0 = 0; sr0 < kQuad; sr0 += BY) {
+        const double jc = j0 + sr0 + (BY - 1) * 0.5;
+        G.FYc = kt * (P.y0 + jc * P.sy);
+        const double zc = kt * (P.z0 + jc * P.sz);
+        G.ZC2 = zc * zc;
+#pragma unroll
+        for (int r = 0; r < BY; ++r) {
+            const float e = static_cast<float>(kt * P.sy * (r - (BY - 1) * 0.5));
+            const double zr = kt * (P.z0 + (j0 + sr0 + r) * P.sz);
+            G.tey[r] = 2.0f * e;
+            G.eyz[r] = e * e + static_cast<float>(zr * zr - G.ZC2);
+        }
+        double dY0[kPQ];
+#pragma unroll
+        for (int k = 0; k < kPQ; ++k) dY0[k] = fma(-kt, qy[k], G.FYc);
+        for (int g = 0; g < ncol; g += BX) {
+            G.FXc = kt * (P.x0 + (cb + g + (BX - 1) * 0.5) * P.sx);
+            float3 W[BX * BY];
+#pragma unroll
+            for (int c = 0; c < BX; ++c)
+#pragma unroll
+                for (int r = 0; r < BY; ++r) {
+                    const float4 v = tile[(g + c) * kQuad + sr0 + r];
+                    W[c * BY + r] = make_float3(v.x, v.y, v.z);
+                }
+#pragma unroll
+            for (int k = 0; k < kPQ; ++k) {
+                AdjState<BX, BY> S;
+                adj_state(fma(-kt, qx[k], G.FXc), dY0[k], P.zref, G, S);
+                adj_pairs<GEN>(S, W, ar[k], ai[k]);
+            }
+        }
+    }
+}
+
+template <int BX, int BY>
+__global__ __launch_bounds__(kThreads, 3) void adj_fp32c(
     const double2* __restrict__ pos, int npil, const DevPlane* __restrict__ planes,
     const AdjQuad* __restrict__ quads, const AdjBlock* __restrict__ blocks,
-    const float4* __restrict__ pix, double2* __restrict__ partial) {
+    const float4* __restrict__ pix, const double2* __restrict__ ext,
+    double2* __restrict__ partial) {
     __shared__ float4 tile[kThreads];
     const AdjBlock B = blocks[blockIdx.x];
     const int tid = threadIdx.x;
-    double qx[PQ], qy[PQ];
+    const double2 E = ext[0];
+    double qx[kPQ], qy[kPQ];
 #pragma unroll
-    for (int k = 0; k < PQ; ++k) {
+    for (int k = 0; k < kPQ; ++k) {
         const double2 v = pos[min(B.q0 + tid + k * kThreads, npil - 1)];
         qx[k] = v.x;
         qy[k] = v.y;
     }
-    float sr[PQ], si[PQ], cr[PQ], ci[PQ];
+    // Kahan-compensated running sums, fed by one float sum per LDS tile.
+    float sr[kPQ], si[kPQ], cr[kPQ], ci[kPQ];
 #pragma unroll
-    for (int k = 0; k < PQ; ++k) sr[k] = si[k] = cr[k] = ci[k] = 0.0f;
+    for (int k = 0; k < kPQ; ++k) sr[k] = si[k] = cr[k] = ci[k] = 0.0f;
 
     for (int u = B.u0; u < B.u1; ++u) {
         const AdjQuad Q = quads[u];
         const DevPlane& P = planes[Q.plane];
-        const double kt = P.kturn;
-        const double zref = P.zref;
-        float tex[BLK], ex2[BLK];
-        block_offsets<BLK>(kt * P.sx, tex, ex2);
-        double X[PQ];
-#pragma unroll
-        for (int k = 0; k < PQ; ++k) X[k] = kt * qx[k];
+        const bool gen = BX * BY > 1 && !fp32c_first_order(P, E, BX, kQuad);
         const float4* __restrict__ prow =
             pix + P.sbase + static_cast<int64_t>(Q.j0 / kQuad) * P.nxp * kQuad;
         for (int cb = 0; cb < P.nxp; cb += kAdjCols) {
@@ -602,72 +682,15 @@ __global__ __launch_bounds__(kThreads, 2) void adj_fp32c(
             __syncthreads();
             if (tid < ncol * kQuad) tile[tid] = prow[cb * kQuad + tid];
             __syncthreads();
-            float ar[PQ], ai[PQ];
+            float ar[kPQ], ai[kPQ];
 #pragma unroll
-            for (int k = 0; k < PQ; ++k) ar[k] = ai[k] = 0.0f;
+            for (int k = 0; k < kPQ; ++k) ar[k] = ai[k] = 0.0f;
+            if (gen)
+                adj_tile<1, BX, BY>(tile, ncol, P, Q.j0, cb, qx, qy, ar, ai);
+            else
+                adj_tile<0, BX, BY>(tile, ncol, P, Q.j0, cb, qx, qy, ar, ai);
 #pragma unroll
-            for (int sr0 = 0; sr0 < kQuad; sr0 += BLK) {
-                // Block rows sr0 .. sr0+BLK-1 of the quad.
-                const double jc = Q.j0 + sr0 + (BLK - 1) * 0.5;
-                const double FYc = kt * (P.y0 + jc * P.sy);
-                const double zc = kt * (P.z0 + jc * P.sz);
-                const double ZC2 = zc * zc;
-                float tey[BLK], eyz[BLK];
-                block_offsets<BLK>(kt * P.sy, tey, eyz);
-#pragma unroll
-                for (int r = 0; r < BLK; ++r) {
-                    const double zr = kt * (P.z0 + (Q.j0 + sr0 + r) * P.sz);
-                    eyz[r] += static_cast<float>(zr * zr - ZC2);
-                }
-                double dY0[PQ];
-#pragma unroll
-                for (int k = 0; k < PQ; ++k) dY0[k] = FYc - kt * qy[k];
-                for (int g = 0; g < ncol; g += BLK) {
-                    const double FXc = kt * (P.x0 + (cb + g + (BLK - 1) * 0.5) * P.sx);
-#pragma unroll
-                    for (int k = 0; k < PQ; ++k) {
-                        const double dX0 = FXc - X[k];
-                        float rf;
-                        const double R0 =
-                            block_distance(fma(dX0, dX0, fma(dY0[k], dY0[k], ZC2)), rf);
-                        const float th0 = static_cast<float>(__builtin_amdgcn_fract(R0 - zref));
-                        float br = 0.0f, bi = 0.0f;
-                        if constexpr (BLK == 1) {
-                            const float4 W = tile[g * kQuad + sr0];
-                            const float th = W.z - th0;
-                            br = W.x * __builtin_amdgcn_cosf(th);
-                            bi = W.x * __builtin_amdgcn_sinf(th);
-                        } else {
-                            const float h = 0.5f * rf;
-                            const float dx0 = static_cast<float>(dX0);
-                            const float dy0 = static_cast<float>(dY0[k]);
-                            float a[BLK], b[BLK];
-#pragma unroll
-                            for (int c = 0; c < BLK; ++c) a[c] = fmaf(tex[c], dx0, ex2[c]);
-#pragma unroll
-                            for (int r = 0; r < BLK; ++r) b[r] = fmaf(tey[r], dy0, eyz[r]);
-#pragma unroll
-                            for (int c = 0; c < BLK; ++c) {
-#pragma unroll
-                                for (int r = 0; r < BLK; ++r) {
-                                    const float4 W = tile[(g + c) * kQuad + sr0 + r];
-                                    const float x1 = (a[c] + b[r]) * h;
-                                    const float y = x1 * h;
-                                    float dR, A;
-                                    pair_terms<PO, AO>(x1, y, W.x, W.y, dR, A);
-                                    const float th = (W.z - th0) - dR;
-                                    br = fmaf(A, __builtin_amdgcn_cosf(th), br);
-                                    bi = fmaf(A, __builtin_amdgcn_sinf(th), bi);
-                                }
-                            }
-                        }
-                        ar[k] = fmaf(rf, br, ar[k]);
-                        ai[k] = fmaf(rf, bi, ai[k]);
-                    }
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < PQ; ++k) {
+            for (int k = 0; k < kPQ; ++k) {
                 const float yr = ar[k] - cr[k];
                 const float tr = sr[k] + yr;
                 cr[k] = (tr - sr[k]) - yr;
@@ -680,7 +703,7 @@ __global__ __launch_bounds__(kThreads, 2) void adj_fp32c(
         }
     }
 #pragma unroll
-    for (int k = 0; k < PQ; ++k) {
+    for (int k = 0; k < kPQ; ++k) {
         const int q = B.q0 + tid + k * kThreads;
         if (q < npil)
             partial[B.part + q] =
@@ -873,15 +896,15 @@ hipError_t launch_adjoint(int mode, int order, const void* pil, int npil, const 
                           const void* pix_or_u, double2* partial, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
     if (mode == 0) {
-        const double2* pos = static_cast<const double2*>(pil);  // staged layout: pos first
+        const StagedLayout L = fp32c_staged_layout(npil, 1);
+        const unsigned char* base = static_cast<const unsigned char*>(pil);
+        const double2* pos = reinterpret_cast<const double2*>(base + L.pos);
+        const double2* ext = reinterpret_cast<const double2*>(base + L.ext);
         const float4* pix = static_cast<const float4*>(pix_or_u);
-        switch (order) {
-            case 0: adj_fp32c<kPQ, kBlk, 1, 1><<<nblocks, kThreads, 0, s>>>(pos, npil, planes, quads, blocks, pix, partial); break;
-            case 1: adj_fp32c<kPQ, kBlk, 1, 2><<<nblocks, kThreads, 0, s>>>(pos, npil, planes, quads, blocks, pix, partial); break;
-            case 2: adj_fp32c<kPQ, kBlk, 2, 1><<<nblocks, kThreads, 0, s>>>(pos, npil, planes, quads, blocks, pix, partial); break;
-            case 3: adj_fp32c<kPQ, kBlk, 2, 2><<<nblocks, kThreads, 0, s>>>(pos, npil, planes, quads, blocks, pix, partial); break;
-            default: adj_fp32c<kPQ, 1, 1, 1><<<nblocks, kThreads, 0, s>>>(pos, npil, planes, quads, blocks, pix, partial); break;
-        }
+        if (order == kExactOrder)
+            adj_fp32c<1, 1><<<nblocks, kThreads, 0, s>>>(pos, npil, planes, quads, blocks, pix, ext, partial);
+        else
+            adj_fp32c<kBlk, kQuad><<<nblocks, kThreads, 0, s>>>(pos, npil, planes, quads, blocks, pix, ext, partial);
     } else if (mode == 2) {
         adj_fp32<<<nblocks, kThreads, 0, s>>>(static_cast<const float4*>(pil), npil, planes, quads,
                                               blocks, static_cast<const double2*>(pix_or_u),
