@@ -104,6 +104,25 @@ def cpu_baseline(lens, u_np, budget_s: float) -> dict:
     }
 
 
+def profiled_traffic(launches):
+    """HBM bytes of one forward + one adjoint pair-kernel launch, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary in profiles/ (scaled by its copy-kernel calibration)
+    for exactly these kernels and work-group counts; (None, None) if not profiled."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            prof = json.load(f)
+    except (OSError, ValueError):
+        return None, None
+    total = 0.0
+    for name, groups in launches.values():
+        entry = prof.get("kernels", {}).get(f"{name}|{groups}")
+        if entry is None:
+            return None, None
+        total += entry["read_bytes"] + entry["write_bytes"]
+    return total, os.path.relpath(path, ROOT)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -114,7 +133,7 @@ def main():
         sys.exit(2)
     import torch
 
-    from celeris_amd import Context, configs, kernel_name
+    from celeris_amd import Context, configs
     from celeris_amd.autograd import encircled_energy
 
     if not torch.cuda.is_available():
@@ -174,10 +193,12 @@ def main():
             ctx.adjoint(px, py, u, [plane], nsets=1, mode=mode)
             ad.append(ctx.last_kernel_ms(True))
         ctx.set_timing(False)
-        return float(np.mean(fw)), float(np.mean(ad))
+        return float(np.median(fw)), float(np.median(ad))
 
     ms = time_steps(args.mode, args.steps, args.warmup)
-    k_fwd, k_adj = kernel_ms(args.mode, max(3, min(args.steps, 10)))
+    k_fwd, k_adj = kernel_ms(args.mode, max(10, min(args.steps, 20)))
+    launches = {"fwd": ctx.last_launch(False), "adj": ctx.last_launch(True)}
+    traffic, traffic_src = profiled_traffic(launches)
     value = pairs / (ms * 1e-3) / 1e9
     achieved = pairs * FLOP_PER_PAIR / ((k_fwd + k_adj) * 1e-3) / 1e12
     print(f"[bench] {args.mode}: {ms:.3f} ms/step, kernels fwd {k_fwd:.3f} ms adj {k_adj:.3f} ms, "
@@ -249,8 +270,11 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
-            "kernels": {kernel_name(args.mode): k_fwd, kernel_name(args.mode, True): k_adj},
+            "traffic": traffic,
+            "traffic_source": traffic_src,
+            "kernels": {launches["fwd"][0]: k_fwd, launches["adj"][0]: k_adj},
+            "workgroups": {launches["fwd"][0]: launches["fwd"][1],
+                           launches["adj"][0]: launches["adj"][1]},
             "flop_per_pair": FLOP_PER_PAIR,
         },
         "cpu_baseline": cpu,

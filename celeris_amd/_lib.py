@@ -90,6 +90,7 @@ SIGNATURES = [
     ("crs_set_timing", c_int, [c_void_p, c_int]),
     ("crs_last_kernel_ms", c_int, [c_void_p, c_int, POINTER(ctypes.c_float)]),
     ("crs_kernel_name", c_char_p, [c_int, c_int]),
+    ("crs_last_launch", c_int, [c_void_p, c_int, POINTER(c_char_p), POINTER(c_int)]),
 ]
 
 _lib = None

@@ -147,6 +147,8 @@ struct crs_context {
     bool timing = false;
     hipEvent_t ev[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [adjoint][start/stop]
     bool ev_valid[2] = {false, false};
+    const char* last_kernel[2] = {nullptr, nullptr};  // [adjoint]: last pair kernel launched
+    int last_groups[2] = {0, 0};
 };
 
 namespace {
@@ -438,6 +440,30 @@ extern "C" crs_status crs_last_kernel_ms(crs_context* ctx, int adjoint, float* m
         set_error("hipEventElapsedTime failed");
         return CRS_ERR_LAUNCH;
     }
+    return CRS_OK;
+}
+
+namespace {
+// Profiler name of the pair kernel a plan launches (launch_forward / launch_adjoint).
+const char* pair_kernel(int mode, int order, bool adjoint) {
+    if (mode == CRS_MODE_FP32C) {
+        if (order == kExactOrder) return adjoint ? "crs::adj_fp32c<1, 1>" : "crs::fwd_fp32c<1, 1>";
+        return adjoint ? "crs::adj_fp32c<4, 4>" : "crs::fwd_fp32c<4, 4>";
+    }
+    if (mode == CRS_MODE_FP64) return adjoint ? "crs::adj_fp64" : "crs::fwd_fp64";
+    return adjoint ? "crs::adj_fp32" : "crs::fwd_fp32";
+}
+}  // namespace
+
+extern "C" crs_status crs_last_launch(crs_context* ctx, int adjoint, const char** kernel,
+                                      int* workgroups) {
+    if (!ctx || !kernel || !workgroups || adjoint < 0 || adjoint > 1) {
+        set_error("bad arguments");
+        return CRS_ERR_INVALID;
+    }
+    if (!ctx->last_kernel[adjoint]) { set_error("no launch recorded"); return CRS_ERR_INVALID; }
+    *kernel = ctx->last_kernel[adjoint];
+    *workgroups = ctx->last_groups[adjoint];
     return CRS_OK;
 }
 
@@ -747,6 +773,10 @@ extern "C" crs_status crs_forward(crs_context* ctx, void* stream_, int mode, con
     st = enqueue_forward(plan, sc, px, py, t, field, intensity, stream,
                          timed ? ctx->ev[0][0] : nullptr, timed ? ctx->ev[0][1] : nullptr);
     if (st == CRS_OK && timed) ctx->ev_valid[0] = true;
+    if (st == CRS_OK) {
+        ctx->last_kernel[0] = pair_kernel(mode, plan.order, false);
+        ctx->last_groups[0] = plan.nblocks;
+    }
     return st;
 }
 
@@ -781,6 +811,10 @@ extern "C" crs_status crs_adjoint(crs_context* ctx, void* stream_, int mode, con
     st = enqueue_adjoint(plan, sc, px, py, u, g, stream, timed ? ctx->ev[1][0] : nullptr,
                          timed ? ctx->ev[1][1] : nullptr);
     if (st == CRS_OK && timed) ctx->ev_valid[1] = true;
+    if (st == CRS_OK) {
+        ctx->last_kernel[1] = pair_kernel(mode, plan.order, true);
+        ctx->last_groups[1] = plan.nblocks;
+    }
     return st;
 }
 

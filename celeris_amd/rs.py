@@ -222,6 +222,13 @@ class Context:
                    "crs_last_kernel_ms")
         return float(ms.value)
 
+    def last_launch(self, adjoint: bool = False):
+        """(kernel name as a profiler lists it, work-groups) of the last pair-sum launch."""
+        name, groups = ctypes.c_char_p(), ctypes.c_int()
+        _lib.check(_lib.load().crs_last_launch(self._h, int(bool(adjoint)), ctypes.byref(name),
+                                               ctypes.byref(groups)), "crs_last_launch")
+        return name.value.decode(), int(groups.value)
+
     def forward(self, px, py, t, planes: Sequence[Plane], mode="fp32c", field: bool = True,
                 intensity: bool = False, stream=None, out_field=None, out_intensity=None):
         """Complex field (complex128, total pixels) and/or |E|^2 (float64)."""
