@@ -553,7 +553,7 @@ crs_status plan_forward(const crs_context* ctx, int mode, int npil, int nsets,
     std::vector<PixBlock> pblocks;
     for (int p = 0; p < nplanes; ++p) {
         const int64_t npix = static_cast<int64_t>(dp[p].nx) * dp[p].ny;
-        for (int64_t b = 0; b < npix; b += kThreads) pblocks.push_back({p, static_cast<int32_t>(b)});
+        for (int64_t b = 0; b < npix; b += kReduceCols) pblocks.push_back({p, static_cast<int32_t>(b)});
     }
     plan.part_elems = part;
     plan.nblocks = static_cast<int>(blocks.size());

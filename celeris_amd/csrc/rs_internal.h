@@ -20,6 +20,9 @@ constexpr int kWaves = kThreads / 64;
 constexpr int kBlk = 4;
 constexpr int kQuad = 4;
 constexpr int kFwdTiles = 64;
+// Outputs per work-group of the partial-sum reductions (each wave adds a quarter of
+// the slots for kReduceCols outputs).
+constexpr int kReduceCols = 64;
 // Longest pillar chunk one FP32C forward work-group sums (each wave a quarter of it,
 // in float, folded every 64 pillars): bounds the float accumulation error.
 constexpr int kMaxChunk = 16384;
@@ -78,7 +81,8 @@ struct FwdBlock {
     int64_t part;
 };
 
-// A run of <= kThreads consecutive elements of one plane (staging and reductions).
+// A run of consecutive elements of one plane: kThreads (staging) or kReduceCols
+// (forward reduction).
 struct PixBlock {
     int32_t plane, pix0;
 };

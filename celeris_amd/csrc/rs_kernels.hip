@@ -243,20 +243,40 @@ __device__ __forceinline__ void block_pairs(const PairState<BX, BY>& S, float (&
 }
 
 // Pillars tile[k0 .. k1) (k0 < k1) against one block; the next pillar's FP64 distance
-// work is issued alongside the current pillar's pairs.
+// work is issued alongside the current pillar's pairs (two states, alternating, so no
+// state is copied between iterations).
 template <int GEN, int BX, int BY>
 __device__ __forceinline__ void pillar_run(const PillarRec* tile, int k0, int k1,
                                            const BlockGeom<BX, BY>& G, float (&ar)[BX * BY],
                                            float (&ai)[BX * BY]) {
-    PairState<BX, BY> cur;
-    pair_state(tile[k0], G, cur);
-    for (int k = k0 + 1; k < k1; ++k) {
-        PairState<BX, BY> nxt;
-        pair_state(tile[k], G, nxt);
-        block_pairs<GEN>(cur, ar, ai);
-        cur = nxt;
+    PairState<BX, BY> s0, s1;
+    pair_state(tile[k0], G, s0);
+    int k = k0 + 1;
+    for (; k + 1 < k1; k += 2) {
+        pair_state(tile[k], G, s1);
+        block_pairs<GEN>(s0, ar, ai);
+        pair_state(tile[k + 1], G, s0);
+        block_pairs<GEN>(s1, ar, ai);
     }
-    block_pairs<GEN>(cur, ar, ai);
+    if (k < k1) {
+        pair_state(tile[k], G, s1);
+        block_pairs<GEN>(s0, ar, ai);
+        block_pairs<GEN>(s1, ar, ai);
+    } else {
+        block_pairs<GEN>(s0, ar, ai);
+    }
+}
+
+// The same without overlap (the general-order path, which needs more registers).
+template <int GEN, int BX, int BY>
+__device__ __forceinline__ void pillar_run_plain(const PillarRec* tile, int k0, int k1,
+                                                 const BlockGeom<BX, BY>& G,
+                                                 float (&ar)[BX * BY], float (&ai)[BX * BY]) {
+    for (int k = k0; k < k1; ++k) {
+        PairState<BX, BY> st;
+        pair_state(tile[k], G, st);
+        block_pairs<GEN>(st, ar, ai);
+    }
 }
 
 // Whether the first-order pair terms meet the error budget for every 4 x 4 block of
@@ -352,7 +372,7 @@ __global__ __launch_bounds__(kThreads, 4) void fwd_fp32c(
 #pragma unroll
             for (int e = 0; e < N; ++e) ar[e] = ai[e] = 0.0f;
             if (gen)
-                pillar_run<1>(tile, k0, k1, G, ar, ai);
+                pillar_run_plain<1>(tile, k0, k1, G, ar, ai);
             else
                 pillar_run<0>(tile, k0, k1, G, ar, ai);
 #pragma unroll
@@ -472,23 +492,42 @@ __global__ __launch_bounds__(kThreads) void fwd_fp64(
 }
 
 // ---------------------------------------------------------------------------------
-// Forward reduction: add the plane's pillar-chunk partials in slot order, restore
-// the factor exp(2 pi i z'_ref) that FP32C leaves out, write field and/or |E|^2.
+// Forward reduction: add the plane's pillar-chunk partials, restore the factor
+// exp(2 pi i z'_ref) that FP32C leaves out, write field and/or |E|^2.  A work-group
+// takes kReduceCols pixels; wave w adds the w-th quarter of the slots, and wave 0 adds
+// the four quarter sums in order (a fixed order: results are bitwise repeatable).
 
 __global__ __launch_bounds__(kThreads) void fwd_reduce(
     const DevPlane* __restrict__ planes, const PixBlock* __restrict__ pblocks,
     const double2* __restrict__ partial, int restore_phase, double2* __restrict__ field,
     double* __restrict__ intensity) {
+    __shared__ double2 red[kWaves][kReduceCols];
     const PixBlock B = pblocks[blockIdx.x];
     const DevPlane& P = planes[B.plane];
     const int npix = P.nx * P.ny;
-    const int p = B.pix0 + static_cast<int>(threadIdx.x);
-    if (p >= npix) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int p = B.pix0 + lane;
+    const bool valid = p < npix;
+    const int s0 = (P.nslots * wave) / kWaves, s1 = (P.nslots * (wave + 1)) / kWaves;
     double er = 0.0, ei = 0.0;
-    for (int s = 0; s < P.nslots; ++s) {
-        const double2 v = partial[P.part + static_cast<int64_t>(s) * npix + p];
-        er += v.x;
-        ei += v.y;
+    if (valid) {
+        const double2* __restrict__ src = partial + P.part + p;
+#pragma unroll 4
+        for (int s = s0; s < s1; ++s) {
+            const double2 v = src[static_cast<int64_t>(s) * npix];
+            er += v.x;
+            ei += v.y;
+        }
+    }
+    red[wave][lane] = make_double2(er, ei);
+    __syncthreads();
+    if (wave != 0 || !valid) return;
+    er = 0.0;
+    ei = 0.0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        er += red[w][lane].x;
+        ei += red[w][lane].y;
     }
     if (restore_phase) {
         double sn, cs;
@@ -804,17 +843,34 @@ __global__ __launch_bounds__(kThreads) void adj_fp64(
     if (q < npil) partial[B.part + q] = make_double2(gr, gi);
 }
 
-// Adjoint reduction: g[idx] = sum over slots of partial[slot*n + idx], slot order.
+// Adjoint reduction: g[idx] = sum over slots of partial[slot*n + idx]; kReduceCols
+// outputs per work-group, slot quarters per wave, added in order as in fwd_reduce.
 __global__ __launch_bounds__(kThreads) void adj_reduce(const double2* __restrict__ partial,
                                                        int nslots, int64_t n,
                                                        double2* __restrict__ g) {
-    const int64_t idx = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
-    if (idx >= n) return;
+    __shared__ double2 red[kWaves][kReduceCols];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * kReduceCols + lane;
+    const bool valid = idx < n;
+    const int s0 = (nslots * wave) / kWaves, s1 = (nslots * (wave + 1)) / kWaves;
     double gr = 0.0, gi = 0.0;
-    for (int s = 0; s < nslots; ++s) {
-        const double2 v = partial[static_cast<int64_t>(s) * n + idx];
-        gr += v.x;
-        gi += v.y;
+    if (valid) {
+#pragma unroll 4
+        for (int s = s0; s < s1; ++s) {
+            const double2 v = partial[static_cast<int64_t>(s) * n + idx];
+            gr += v.x;
+            gi += v.y;
+        }
+    }
+    red[wave][lane] = make_double2(gr, gi);
+    __syncthreads();
+    if (wave != 0 || !valid) return;
+    gr = 0.0;
+    gi = 0.0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        gr += red[w][lane].x;
+        gi += red[w][lane].y;
     }
     g[idx] = make_double2(gr, gi);
 }
@@ -920,7 +976,8 @@ hipError_t launch_adjoint(int mode, int order, const void* pil, int npil, const 
 hipError_t launch_adj_reduce(const double2* partial, int nslots, int64_t n, double2* g,
                              hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    adj_reduce<<<grid_for(n), kThreads, 0, s>>>(partial, nslots, n, g);
+    adj_reduce<<<static_cast<unsigned>((n + kReduceCols - 1) / kReduceCols), kThreads, 0, s>>>(
+        partial, nslots, n, g);
     return hipGetLastError();
 }
 

@@ -2,11 +2,12 @@
 
 The forward splits its pillar sum into chunks and the adjoint its pixel sum into quad
 runs so that a launch has enough work-groups for the 256 CUs; too few leaves CUs
-idle in the last wave, too many adds partial sums and per-block setup.  The best
-count depends on the shape (pillars, pixels per plane, planes, mode), so it is
-measured: each candidate is timed with the library's own kernel events
-(crs_set_timing) and the fastest is installed on the context (crs_set_tuning) and
-remembered in a JSON cache keyed by the shape.
+idle in the last wave, too many adds partial sums (more traffic for the reduction
+kernel) and per-block setup.  The best count depends on the shape (pillars, pixels
+per plane, planes, mode), so it is measured: each candidate's whole call (staging,
+pair kernel, reduction) is timed with stream events over back-to-back calls, and the
+fastest is installed on the context (crs_set_tuning) and remembered in a JSON cache
+keyed by the shape.
 """
 from __future__ import annotations
 
@@ -63,25 +64,24 @@ def autotune(ctx: Context, px, py, t, planes: Sequence[Plane], u=None, mode: str
         ctx.set_tuning(mode, best["fwd"], best.get("adj", 0))
         return best
     fwd_ms, adj_ms = {}, {}
-    ctx.set_timing(True)
-    try:
-        for cand in candidates:
-            ctx.set_tuning(mode, cand, cand)
-            ctx.forward(px, py, t, planes, mode=mode)  # plan + warm
-            times = []
-            for _ in range(reps):
-                ctx.forward(px, py, t, planes, mode=mode)
-                times.append(ctx.last_kernel_ms(False))
-            fwd_ms[cand] = min(times)
-            if u is not None:
-                ctx.adjoint(px, py, u, planes, nsets=nsets, mode=mode)
-                times = []
-                for _ in range(reps):
-                    ctx.adjoint(px, py, u, planes, nsets=nsets, mode=mode)
-                    times.append(ctx.last_kernel_ms(True))
-                adj_ms[cand] = min(times)
-    finally:
-        ctx.set_timing(False)
+
+    def timed(fn):
+        fn()  # plan + warm
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        b.synchronize()
+        return a.elapsed_time(b) / reps
+
+    for cand in candidates:
+        ctx.set_tuning(mode, cand, cand)
+        fwd_ms[cand] = timed(lambda: ctx.forward(px, py, t, planes, mode=mode))
+        if u is not None:
+            adj_ms[cand] = timed(lambda: ctx.adjoint(px, py, u, planes, nsets=nsets, mode=mode))
     best = {"fwd": min(fwd_ms, key=fwd_ms.get), "fwd_ms": fwd_ms}
     if adj_ms:
         best["adj"] = min(adj_ms, key=adj_ms.get)
