@@ -320,7 +320,9 @@ __global__ __launch_bounds__(kThreads, 4) void fwd_fp32c(
     const int ti = tt % P.tx, tj = tt / P.tx;
     const double kt = P.kturn;
     const double zref = P.zref;
-    const bool gen = N > 1 && !fp32c_first_order(P, ext[0], BX, BY);
+    // Uniform across the work-group; readfirstlane makes the branch on it scalar.
+    const bool gen =
+        __builtin_amdgcn_readfirstlane(N > 1 && !fp32c_first_order(P, ext[0], BX, BY));
 
     BlockGeom<BX, BY> G;
     {
@@ -604,38 +606,43 @@ __device__ __forceinline__ void adj_state(double dX0, double dY0, double zref,
 }
 
 // The pairs of one pillar with one block's records W (column-major: W[c*BY + r] =
-// {2U, -U, chi}); adds rf * sum to (gr, gi).  GEN as in block_pairs.
+// {2U, -U, chi}); adds the block's sum to (gr, gi).  GEN as in block_pairs.  For 4 x 4
+// blocks the pair phases are chi - (R - R0) and the block's sum is turned by
+// exp(-2 pi i th0) and scaled by 1/R0' once; 1 x 1 blocks use chi - th0 directly.
 template <int GEN, int BX, int BY>
 __device__ __forceinline__ void adj_pairs(const AdjState<BX, BY>& S, const float3 (&W)[BX * BY],
                                           float& gr, float& gi) {
-    float br = 0.0f, bi = 0.0f;
+    if constexpr (BX * BY == 1) {
+        const float phi = W[0].z - S.th0;
+        const float A = S.rf * (W[0].x + W[0].y);
+        gr = fmaf(A, __builtin_amdgcn_cosf(phi), gr);
+        gi = fmaf(A, __builtin_amdgcn_sinf(phi), gi);
+    } else {
+        float br = 0.0f, bi = 0.0f;
 #pragma unroll
-    for (int c = 0; c < BX; ++c) {
+        for (int c = 0; c < BX; ++c) {
 #pragma unroll
-        for (int r = 0; r < BY; ++r) {
-            const float3 w3 = W[c * BY + r];
-            float phi, A;
-            if constexpr (BX * BY == 1) {
-                phi = w3.z - S.th0;
-                A = w3.x + w3.y;
-            } else {
+            for (int r = 0; r < BY; ++r) {
+                const float3 w3 = W[c * BY + r];
                 const float x1 = S.a[c] + S.b[r];
                 const float w1 = fmaf(-x1, S.h, 1.0f);
                 float w = w1;
-                A = fmaf(w3.x, w1, w3.y);
+                float A = fmaf(w3.x, w1, w3.y);
                 if constexpr (GEN) {
                     const float y = x1 * S.h;
                     w = fmaf(y + y, y, w1);
                     A = fmaf(3.0f * w3.x * y, y, A);
                 }
-                phi = fmaf(-x1, w, w3.z - S.th0);
+                const float phi = fmaf(-x1, w, w3.z);
+                br = fmaf(A, __builtin_amdgcn_cosf(phi), br);
+                bi = fmaf(A, __builtin_amdgcn_sinf(phi), bi);
             }
-            br = fmaf(A, __builtin_amdgcn_cosf(phi), br);
-            bi = fmaf(A, __builtin_amdgcn_sinf(phi), bi);
         }
+        const float cr = S.rf * __builtin_amdgcn_cosf(S.th0);
+        const float sn = S.rf * __builtin_amdgcn_sinf(S.th0);
+        gr = fmaf(br, cr, fmaf(bi, sn, gr));
+        gi = fmaf(bi, cr, fmaf(-br, sn, gi));
     }
-    gr = fmaf(S.rf, br, gr);
-    gi = fmaf(S.rf, bi, gi);
 }
 
 // One 64-column LDS tile of a quad against the thread's pillars.
@@ -713,7 +720,8 @@ __global__ __launch_bounds__(kThreads, 3) void adj_fp32c(
     for (int u = B.u0; u < B.u1; ++u) {
         const AdjQuad Q = quads[u];
         const DevPlane& P = planes[Q.plane];
-        const bool gen = BX * BY > 1 && !fp32c_first_order(P, E, BX, kQuad);
+        const bool gen =
+            __builtin_amdgcn_readfirstlane(BX * BY > 1 && !fp32c_first_order(P, E, BX, kQuad));
         const float4* __restrict__ prow =
             pix + P.sbase + static_cast<int64_t>(Q.j0 / kQuad) * P.nxp * kQuad;
         for (int cb = 0; cb < P.nxp; cb += kAdjCols) {
