@@ -87,6 +87,9 @@ SIGNATURES = [
      [_DP, _DP, _DP, c_int, c_double, c_double, c_double, c_double, c_int, c_int, _DP]),
     ("crs_analyze_field_grid", c_int,
      [_DP, _DP, _DP, c_int, c_double, c_double, c_double, c_double, c_int, c_int, c_int, _DP]),
+    ("crs_analyze_mtf", c_int,
+     [_DP, _DP, c_int, c_double, c_double, c_double, c_double, c_int, POINTER(ctypes.c_float),
+      POINTER(ctypes.c_float), POINTER(ctypes.c_float), _DP, POINTER(c_int)]),
     ("crs_set_timing", c_int, [c_void_p, c_int]),
     ("crs_last_kernel_ms", c_int, [c_void_p, c_int, POINTER(ctypes.c_float)]),
     ("crs_kernel_name", c_char_p, [c_int, c_int]),

@@ -10,6 +10,7 @@ module (src/celeris/analysis/*.hpp; bindings/python/celeris_module.cpp:415-440):
 * :func:`analyze_through_focus` throughfocus.cpp:15-132 -> :class:`ThroughFocus`
 * :func:`analyze_chromatic`     chromatic.cpp:9-58    -> list of :class:`ChromaticPoint`
 * :func:`analyze_field_grid`    field.cpp:150-233     -> :class:`FieldGrid`
+* :func:`analyze_mtf`           mtf.cpp:10-63         -> :class:`MtfCurve`
 
 Each runs its sums in one launch through the C ABI (crs_analyze_*, crs_forward_host);
 where the reference looped on the CPU (on-axis scans, the ideal peak) the sums are on
@@ -25,7 +26,7 @@ from typing import List
 import numpy as np
 
 from . import _lib
-from .design import MetalensDesign, UnitCellLibrary, aperture_pillars
+from .design import MetalensDesign, UnitCellLibrary, aperture_pillars, exit_pupil
 from .rs import _mode, crs_plane
 
 _DP = ctypes.POINTER(ctypes.c_double)
@@ -106,6 +107,17 @@ class FieldGrid:
     n: int
     max_angle_deg: float
     points: List[FieldGridPoint]
+
+
+@dataclass
+class MtfCurve:
+    """mtf.hpp: spatial frequency (cycles/mm), lens MTF and diffraction-limited MTF
+    (float32, as the reference's vectors of float) and the cutoff frequency."""
+
+    freq_cyc_mm: np.ndarray
+    mtf: np.ndarray
+    mtf_ideal: np.ndarray
+    cutoff_cyc_mm: float
 
 
 def _check(status, what):
@@ -235,3 +247,35 @@ def analyze_field_grid(lens: MetalensDesign, lib: UnitCellLibrary, focal_length_
            "crs_analyze_field_grid")
     pts = [FieldGridPoint(*map(float, out[7 * i:7 * i + 7])) for i in range(g * g)]
     return FieldGrid(g, max_angle_deg, pts)
+
+
+def analyze_mtf(lens: MetalensDesign, lib: UnitCellLibrary, focal_length_um: float,
+                wavelength_um: float, diameter_um: float) -> MtfCurve:
+    """mtf.cpp:10-63: MTF along x from the autocorrelation of the exit pupil; the
+    pupil is built on the host (exit_pupil), the O(n^3) sums run on the GPU."""
+    P, P0 = exit_pupil(lens, lib, focal_length_um, wavelength_um, diameter_um)
+    return mtf_from_pupil(P, P0, lens.period_um, focal_length_um, wavelength_um, diameter_um)
+
+
+def mtf_from_pupil(P, P0, period_um: float, focal_length_um: float, wavelength_um: float,
+                   diameter_um: float) -> MtfCurve:
+    """The MTF of a given exit pupil P (complex n x n) and aperture mask P0."""
+    P = _c(P)
+    P0 = _d(P0)
+    n = P.shape[0]
+    if P.shape != (n, n) or P0.shape != (n, n):
+        raise ValueError("pupil and mask must be n x n")
+    cap = n
+    f32 = ctypes.POINTER(ctypes.c_float)
+    freq = np.zeros(cap, np.float32)
+    mtf = np.zeros(cap, np.float32)
+    ideal = np.zeros(cap, np.float32)
+    cutoff = np.zeros(1)
+    npts = ctypes.c_int(0)
+    _check(_lib.load().crs_analyze_mtf(_p(P.view(np.float64)), _p(P0), n, period_um,
+                                       focal_length_um, wavelength_um, diameter_um, cap,
+                                       freq.ctypes.data_as(f32), mtf.ctypes.data_as(f32),
+                                       ideal.ctypes.data_as(f32), _p(cutoff), ctypes.byref(npts)),
+           "crs_analyze_mtf")
+    m = npts.value
+    return MtfCurve(freq[:m].copy(), mtf[:m].copy(), ideal[:m].copy(), float(cutoff[0]))

@@ -42,6 +42,9 @@ hipError_t launch_adjoint(int mode, int order, const void* pil, int npil, const 
                           const void* pix_or_u, double2* partial, hipStream_t s);
 hipError_t launch_adj_reduce(const double2* partial, int nslots, int64_t n, double2* g,
                              hipStream_t s);
+hipError_t launch_mtf(const double2* P, const double* P0, int n, int ns, double p, double f,
+                      double lam, double2* acc, double* acc0, float* freq, float* mtf,
+                      float* ideal, hipStream_t s);
 }  // namespace crs
 
 using namespace crs;
@@ -1100,6 +1103,79 @@ extern "C" crs_status crs_forward_host(int mode, const double* px, const double*
     if (!px || !py || !t || (!field && !intensity)) { set_error("null array argument"); return CRS_ERR_INVALID; }
     if (nplanes <= 0 || !planes) { set_error("no planes"); return CRS_ERR_INVALID; }
     return host_run(mode, px, py, t, npil, nsets, planes, nplanes, field, intensity);
+}
+
+extern "C" crs_status crs_analyze_mtf(const double* pupil, const double* pupil0, int n,
+                                      double period_um, double focal_um, double wavelength_um,
+                                      double diameter_um, int cap, float* freq_cyc_mm,
+                                      float* mtf, float* mtf_ideal, double* cutoff_cyc_mm,
+                                      int* npoints) {
+    g_last_error.clear();
+    if (!pupil || !pupil0 || !freq_cyc_mm || !mtf || !mtf_ideal || !cutoff_cyc_mm || !npoints ||
+        n <= 0 || !(period_um > 0.0)) {
+        set_error("bad MTF arguments");
+        return CRS_ERR_INVALID;
+    }
+    // Shifts s = 0, 1, ... while s <= int(D/p) + 1 and s < n  (mtf.cpp:53-54).
+    const double q = diameter_um / period_um;
+    if (std::isnan(q)) { set_error("bad MTF arguments"); return CRS_ERR_INVALID; }
+    const int64_t max_s = q >= n ? n : static_cast<int64_t>(q) + 1;  // int() truncates
+    const int ns = static_cast<int>(std::max<int64_t>(0, std::min<int64_t>(max_s, n - 1) + 1));
+    if (ns == 0) {
+        *cutoff_cyc_mm = diameter_um / (wavelength_um * focal_um) * 1000.0;
+        *npoints = 0;
+        return CRS_OK;
+    }
+    if (ns > cap) {
+        set_error("MTF needs %d points, capacity %d", ns, cap);
+        return CRS_ERR_INVALID;
+    }
+    if (!crs_available()) { set_error("no HIP device visible"); return CRS_ERR_NO_DEVICE; }
+    DropinState* st_ = acquire_state();
+    if (!st_) return CRS_ERR_NO_DEVICE;
+    const size_t nn = static_cast<size_t>(n) * n;
+    crs_status st = ensure(st_->in, nn * 3 * sizeof(double), st_->stream, "MTF pupil");
+    if (st == CRS_OK)
+        st = ensure(st_->out, static_cast<size_t>(ns) * (3 * sizeof(double) + 3 * sizeof(float)),
+                    st_->stream, "MTF sums");
+    if (st == CRS_OK) {
+        double2* d_P = static_cast<double2*>(st_->in.ptr);
+        double* d_P0 = reinterpret_cast<double*>(d_P + nn);
+        double2* d_acc = static_cast<double2*>(st_->out.ptr);
+        double* d_acc0 = reinterpret_cast<double*>(d_acc + ns);
+        float* d_f = reinterpret_cast<float*>(d_acc0 + ns);
+        std::vector<float> h(3 * static_cast<size_t>(ns));
+        if (hipMemcpyAsync(d_P, pupil, nn * sizeof(double2), hipMemcpyHostToDevice, st_->stream) != hipSuccess ||
+            hipMemcpyAsync(d_P0, pupil0, nn * sizeof(double), hipMemcpyHostToDevice, st_->stream) != hipSuccess) {
+            hipGetLastError();
+            set_error("MTF pupil: hipMemcpyAsync failed");
+            st = CRS_ERR_LAUNCH;
+        }
+        if (st == CRS_OK &&
+            (st = launch_status(launch_mtf(d_P, d_P0, n, ns, period_um, focal_um, wavelength_um,
+                                           d_acc, d_acc0, d_f, d_f + ns, d_f + 2 * ns, st_->stream),
+                                "MTF")) == CRS_OK) {
+            if (hipMemcpyAsync(h.data(), d_f, h.size() * sizeof(float), hipMemcpyDeviceToHost,
+                               st_->stream) != hipSuccess ||
+                hipStreamSynchronize(st_->stream) != hipSuccess) {
+                hipGetLastError();
+                set_error("MTF: copy or synchronize failed");
+                st = CRS_ERR_LAUNCH;
+            } else {
+                std::memcpy(freq_cyc_mm, h.data(), ns * sizeof(float));
+                std::memcpy(mtf, h.data() + ns, ns * sizeof(float));
+                std::memcpy(mtf_ideal, h.data() + 2 * ns, ns * sizeof(float));
+                *cutoff_cyc_mm = diameter_um / (wavelength_um * focal_um) * 1000.0;
+                *npoints = ns;
+            }
+        }
+    }
+    if (st != CRS_OK) {
+        hipGetLastError();
+        hipStreamSynchronize(st_->stream);
+    }
+    release_state(st_);
+    return st;
 }
 
 extern "C" int crs_set_default_mode(int mode) {

@@ -884,6 +884,66 @@ __global__ __launch_bounds__(kThreads) void adj_reduce(const double2* __restrict
 }
 
 // ---------------------------------------------------------------------------------
+// MTF (analysis/mtf.cpp:10-63): autocorrelation of the exit pupil along x.  One
+// work-group per shift s sums P[iy][ix] conj(P[iy][ix+s]) and P0[iy][ix] P0[iy][ix+s]
+// over all rows and columns ix < n - s in FP64 (each thread a column stride, then a
+// fixed-order tree), so the result is repeatable; mtf_finish normalises by shift 0 and
+// casts to float as the reference's MtfCurve does.
+
+__global__ __launch_bounds__(kThreads) void mtf_autocorr(const double2* __restrict__ P,
+                                                         const double* __restrict__ P0, int n,
+                                                         double2* __restrict__ acc,
+                                                         double* __restrict__ acc0) {
+    __shared__ double red[3][kThreads];
+    const int s = blockIdx.x, tid = threadIdx.x;
+    const int m = n - s;
+    double ar = 0.0, ai = 0.0, a0 = 0.0;
+    for (int iy = 0; iy < n; ++iy) {
+        const double2* __restrict__ row = P + static_cast<int64_t>(iy) * n;
+        const double* __restrict__ row0 = P0 + static_cast<int64_t>(iy) * n;
+        for (int ix = tid; ix < m; ix += kThreads) {
+            const double2 a = row[ix], b = row[ix + s];
+            ar += a.x * b.x + a.y * b.y;
+            ai += a.y * b.x - a.x * b.y;
+            a0 += row0[ix] * row0[ix + s];
+        }
+    }
+    red[0][tid] = ar;
+    red[1][tid] = ai;
+    red[2][tid] = a0;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if (tid < w) {
+            red[0][tid] += red[0][tid + w];
+            red[1][tid] += red[1][tid + w];
+            red[2][tid] += red[2][tid + w];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        acc[s] = make_double2(red[0][0], red[1][0]);
+        acc0[s] = red[2][0];
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void mtf_finish(const double2* __restrict__ acc,
+                                                       const double* __restrict__ acc0, int ns,
+                                                       double p, double f, double lam,
+                                                       float* __restrict__ freq,
+                                                       float* __restrict__ mtf,
+                                                       float* __restrict__ ideal) {
+    const int s = blockIdx.x * kThreads + threadIdx.x;
+    if (s >= ns) return;
+    double nP = hypot(acc[0].x, acc[0].y);
+    double n0 = fabs(acc0[0]);
+    if (nP <= 0.0) nP = 1.0;
+    if (n0 <= 0.0) n0 = 1.0;
+    freq[s] = static_cast<float>((s * p) / (lam * f) * 1000.0);
+    mtf[s] = static_cast<float>(hypot(acc[s].x, acc[s].y) / nP);
+    ideal[s] = static_cast<float>(fabs(acc0[s]) / n0);
+}
+
+// ---------------------------------------------------------------------------------
 // Launchers.
 
 namespace {
@@ -986,6 +1046,15 @@ hipError_t launch_adj_reduce(const double2* partial, int nslots, int64_t n, doub
     if (n <= 0) return hipSuccess;
     adj_reduce<<<static_cast<unsigned>((n + kReduceCols - 1) / kReduceCols), kThreads, 0, s>>>(
         partial, nslots, n, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_mtf(const double2* P, const double* P0, int n, int ns, double p, double f,
+                      double lam, double2* acc, double* acc0, float* freq, float* mtf,
+                      float* ideal, hipStream_t s) {
+    if (n <= 0 || ns <= 0) return hipSuccess;
+    mtf_autocorr<<<ns, kThreads, 0, s>>>(P, P0, n, acc, acc0);
+    mtf_finish<<<grid_for(ns), kThreads, 0, s>>>(acc, acc0, ns, p, f, lam, freq, mtf, ideal);
     return hipGetLastError();
 }
 

@@ -74,3 +74,24 @@ def aperture_pillars(lens: MetalensDesign, lib: UnitCellLibrary, diameter_um: fl
         rk = r[keep]
         out.append(-k * (np.sqrt(rk * rk + focal_length_um * focal_length_um) - focal_length_um))
     return tuple(out)
+
+
+def exit_pupil(lens: MetalensDesign, lib: UnitCellLibrary, focal_length_um: float,
+               wavelength_um: float, diameter_um: float):
+    """The exit pupil analyze_mtf correlates (mtf.cpp:20-36): on the full n_cells^2 grid,
+    P = t(fill) e^{-i phi_ideal} inside the aperture (r <= D/2) and 0 outside, and the
+    aperture mask P0.  Returns (P complex128 [n, n], P0 float64 [n, n])."""
+    n = lens.n_cells
+    p = lens.period_um
+    c = (n - 1) / 2.0
+    iy, ix = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    x = (ix - c) * p
+    y = (iy - c) * p
+    r = np.sqrt(x * x + y * y)
+    inside = ~(r > diameter_um / 2.0)
+    k = 2.0 * math.pi / wavelength_um
+    phi = -k * (np.sqrt(r * r + focal_length_um * focal_length_um) - focal_length_um)
+    t = lib.transmission_for_fill(np.asarray(lens.fill_map, np.float64).reshape(n, n))
+    rot = np.cos(-phi) + 1j * np.sin(-phi)
+    P = np.where(inside, t * rot, 0.0 + 0.0j)
+    return P.astype(np.complex128), inside.astype(np.float64)

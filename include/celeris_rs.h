@@ -194,6 +194,19 @@ crs_status crs_analyze_field_grid(const double *px, const double *py, const doub
                                   double max_angle_deg, int n_half, int psf_n, int mode,
                                   double *out7);
 
+/* MTF along x, replacing analysis/mtf.cpp:10-63 (analyze_mtf).  pupil is the exit pupil
+ * P[iy*n + ix] = t e^{-i phi_ideal} of the n x n cell grid (complex, interleaved re/im,
+ * zero outside the aperture) and pupil0 its aperture mask (1 inside, 0 outside), as
+ * mtf.cpp:20-36 builds them.  For shifts s = 0 .. min(int(D/p) + 1, n - 1) writes
+ * freq_cyc_mm[s] = s p / (lambda f) * 1000, mtf[s] = |sum P conj(P shifted by s)| /
+ * (its s = 0 value) and mtf_ideal[s] (the same for pupil0), as floats like the
+ * reference's MtfCurve, plus cutoff_cyc_mm = D / (lambda f) * 1000 and npoints.  The
+ * sums run on the GPU in FP64.  CRS_ERR_INVALID if cap < npoints. */
+crs_status crs_analyze_mtf(const double *pupil, const double *pupil0, int n, double period_um,
+                           double focal_um, double wavelength_um, double diameter_um, int cap,
+                           float *freq_cyc_mm, float *mtf, float *mtf_ideal,
+                           double *cutoff_cyc_mm, int *npoints);
+
 /* Launch-shape tuning (no reference counterpart).  target_blocks: work-groups a
  * forward/adjoint launch aims for through its split over pillar chunks (forward)
  * or pixel rows (adjoint); 0 restores the built-in default.  Applies to the

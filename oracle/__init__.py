@@ -49,6 +49,8 @@ def load():
         lib.orc_adjoint.argtypes = [dp, dp, c_int, c_int, POINTER(orc_plane), c_int, dp, dp, c_int, c_int]
         lib.orc_forward_ld.restype = c_int
         lib.orc_forward_ld.argtypes = [dp, dp, dp, c_int, POINTER(orc_plane), c_int, dp]
+        lib.orc_mtf_autocorr.restype = None
+        lib.orc_mtf_autocorr.argtypes = [dp, dp, c_int, c_int, dp, dp]
         lib.orc_num_cpus.restype = c_int
         _lib = lib
     return _lib
@@ -156,3 +158,26 @@ def ref_propagate_pillars(px, py, t, cx, cy, z, wavelength, n, half_window) -> n
 def lens_block(n_cells, period, fill_map, lib_fill, lib_phase, lib_amp):
     return np.concatenate([[n_cells, period], np.asarray(fill_map, np.float64).ravel(),
                            [len(lib_fill)], lib_fill, lib_phase, lib_amp])
+
+
+def mtf(pupil, pupil0, period_um, focal_um, wavelength_um, diameter_um):
+    """analyze_mtf's curve (mtf.cpp:10-63) from an exit pupil P (complex n x n, zero
+    outside the aperture) and its mask P0: (freq_cyc_mm, mtf, mtf_ideal) as float32
+    and cutoff_cyc_mm, normalised and cast as the reference does (:51-62)."""
+    P = np.ascontiguousarray(pupil, np.complex128)
+    P0 = np.ascontiguousarray(pupil0, np.float64)
+    n = P.shape[0]
+    p = float(period_um)
+    max_s = int(diameter_um / p) + 1
+    ns = min(max_s + 1, n)
+    acc = np.empty(2 * ns)
+    acc0 = np.empty(ns)
+    load().orc_mtf_autocorr(_dp(P.view(np.float64)), _dp(P0), n, ns, _dp(acc), _dp(acc0))
+    a = np.abs(acc[0::2] + 1j * acc[1::2])
+    a0 = np.abs(acc0)
+    normP = a[0] if a[0] > 0 else 1.0
+    normP0 = a0[0] if a0[0] > 0 else 1.0
+    s = np.arange(ns)
+    freq = ((s * p) / (wavelength_um * focal_um) * 1000.0).astype(np.float32)
+    return freq, (a / normP).astype(np.float32), (a0 / normP0).astype(np.float32), \
+        diameter_um / (wavelength_um * focal_um) * 1000.0

@@ -16,6 +16,8 @@
  *       fx = xmin + ix*xstep, z = zmin + iz*zstep, dy = y_pillar
  *   - the adjoint g_q = sum_p conj(G_pq) u_p has no reference counterpart; it is
  *     the conjugate transpose of the sum above, evaluated with the same G_pq.
+ *   - the pupil autocorrelation of analyze_mtf     src/celeris/analysis/mtf.cpp:40-49
+ *       acc(s) = sum_iy sum_{ix+s<n} P[iy][ix] conj(P[iy][ix+s]), rows then columns
  *
  * Pixel geometry is expressed as an affine "plane": pixel (i, j) sits at
  *   fx = x0 + i*sx,  fy = y0 + j*sy,  z = z0 + j*sz
@@ -281,3 +283,27 @@ int orc_forward_ld(const double *px, const double *py, const double *t, int npil
 }
 
 int orc_num_cpus(void) { return online_cpus(); }
+
+/* Pupil autocorrelation along x for shifts 0..nshift-1 (mtf.cpp:40-49), in the
+ * reference's loop order: P complex (interleaved) n x n, P0 real n x n.  Writes
+ * acc[2s], acc[2s+1] = sum P conj(P shifted by s) and acc0[s] = sum P0 P0(shifted). */
+void orc_mtf_autocorr(const double *P, const double *P0, int n, int nshift, double *acc,
+                      double *acc0) {
+    for (int s = 0; s < nshift; ++s) {
+        double ar = 0.0, ai = 0.0, a0 = 0.0;
+        for (int iy = 0; iy < n; ++iy) {
+            const double *row = P + 2 * (int64_t)iy * n;
+            const double *row0 = P0 + (int64_t)iy * n;
+            for (int ix = 0; ix + s < n; ++ix) {
+                const double px = row[2 * ix], py = row[2 * ix + 1];
+                const double qx = row[2 * (ix + s)], qy = row[2 * (ix + s) + 1];
+                ar += px * qx + py * qy;
+                ai += py * qx - px * qy;
+                a0 += row0[ix] * row0[ix + s];
+            }
+        }
+        acc[2 * s] = ar;
+        acc[2 * s + 1] = ai;
+        acc0[s] = a0;
+    }
+}

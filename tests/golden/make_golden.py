@@ -63,6 +63,19 @@ def lens_case():
                 lib_phase=lib_phase, lib_amp=lib_amp, fill_map=fill_map)
 
 
+def mtf_case():
+    """A larger lens for analyze_mtf alone: 81 x 81 cells at 0.35 um with seeded random
+    fills (a poor lens, so the MTF curve is far from the ideal one) inside a D = 27 um
+    aperture (the mask cuts the corner cells), f = 30, lambda 0.6."""
+    n_cells, period, lam, f, D = 81, 0.35, 0.6, 30.0, 27.0
+    L = lens_case()
+    fills = L["lib_fill"][np.floor(configs.uniform(configs.SEED, 12, n_cells * n_cells) *
+                                    len(L["lib_fill"])).astype(int)]
+    return dict(n_cells=n_cells, period=period, lam=lam, f=f, D=D, lib_fill=L["lib_fill"],
+                lib_phase=L["lib_phase"], lib_amp=L["lib_amp"],
+                fill_map=fills.reshape(n_cells, n_cells))
+
+
 def main():
     if not oracle.ref_available():
         sys.exit("oracle/_ref/ref_driver is not built (make -C oracle ref)")
@@ -94,6 +107,13 @@ def main():
     lens_out["psffield_args"] = np.array([5.0, 33, 3.0, on_axis_peak])
     lens_out["psffield"] = oracle.ref_run("psffield", np.concatenate([base, [5.0, 33, 3.0, on_axis_peak]]))
     np.savez_compressed(os.path.join(HERE, "lens_analyses.npz"), **lens_out)
+
+    M = mtf_case()
+    blk = oracle.lens_block(M["n_cells"], M["period"], M["fill_map"], M["lib_fill"], M["lib_phase"],
+                            M["lib_amp"])
+    mtf_out = {k: np.asarray(v) for k, v in M.items()}
+    mtf_out["mtf"] = oracle.ref_run("mtf", np.concatenate([blk, [M["f"], M["lam"], M["D"]]]))
+    np.savez_compressed(os.path.join(HERE, "mtf_large.npz"), **mtf_out)
     print("wrote", sorted(os.listdir(HERE)))
 
 

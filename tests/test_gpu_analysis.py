@@ -122,3 +122,39 @@ def test_compute_psf_field(lens, mode, tol):
     assert p.rel_strehl == pytest.approx(ref[2], rel=tol)
     I_ref = ref[3:].reshape(int(n), int(n))
     assert np.linalg.norm(p.psf.intensity - I_ref) / np.linalg.norm(I_ref) <= tol
+
+
+@pytest.mark.parametrize("name", ["lens_analyses.npz", "mtf_large.npz"])
+def test_analyze_mtf_golden(torch_cuda, name):
+    """analyze_mtf on the GPU (FP64 sums) against the reference's own float curve."""
+    d = np.load(os.path.join(os.path.dirname(GOLDEN), name))
+    L = MetalensDesign(int(d["n_cells"]), float(d["period"]), d["fill_map"])
+    lib = UnitCellLibrary(float(d["period"]), d["lib_fill"], d["lib_phase"], d["lib_amp"])
+    ref = d["mtf"]
+    k = int(ref[0])
+    print(f"[analysis] analyze_mtf {name}", flush=True)
+    c = analysis.analyze_mtf(L, lib, float(d["f"]), float(d["lam"]), float(d["D"]))
+    assert c.freq_cyc_mm.size == k
+    assert np.array_equal(c.freq_cyc_mm, ref[1:1 + k].astype(np.float32))
+    # FP64 sums in another order than the reference's loop: equal up to the float cast
+    # (at most one float ulp where a value sits on a rounding boundary).
+    for got, want in ((c.mtf, ref[1 + k:1 + 2 * k]), (c.mtf_ideal, ref[1 + 2 * k:1 + 3 * k])):
+        assert np.all(np.abs(got.astype(np.float64) - want) <= 1.2e-7 * np.abs(want) + 1e-30)
+    assert c.cutoff_cyc_mm == ref[1 + 3 * k]
+
+
+def test_analyze_mtf_vs_oracle_large(torch_cuda):
+    """A 201 x 201-cell pupil with random phases: GPU curve vs the oracle's."""
+    import oracle
+
+    n, p, f, lam, D = 201, 0.3, 40.0, 0.55, 58.0
+    rng = np.random.default_rng(11)
+    c = (n - 1) / 2
+    iy, ix = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    P0 = (~(np.hypot((ix - c) * p, (iy - c) * p) > D / 2)).astype(np.float64)
+    P = P0 * rng.uniform(0.6, 1.0, (n, n)) * np.exp(1j * rng.uniform(-np.pi, np.pi, (n, n)))
+    cv = analysis.mtf_from_pupil(P, P0, p, f, lam, D)
+    fr, m, mi, cut = oracle.mtf(P, P0, p, f, lam, D)
+    assert np.array_equal(cv.freq_cyc_mm, fr) and cv.cutoff_cyc_mm == cut
+    assert np.all(np.abs(cv.mtf.astype(np.float64) - m) <= 1.2e-7 * np.abs(m) + 1e-30)
+    assert np.all(np.abs(cv.mtf_ideal.astype(np.float64) - mi) <= 1.2e-7 * np.abs(mi) + 1e-30)

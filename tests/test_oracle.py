@@ -188,3 +188,53 @@ def test_row_stride_sample_matches_full():
     rows[::4] = True
     assert np.array_equal(part[rows.ravel()], full[rows.ravel()])
     assert np.all(part[~rows.ravel()] == 0)
+
+
+def _mtf_golden(name):
+    d = np.load(os.path.join(GOLDEN, name))
+    lens = MetalensDesign(int(d["n_cells"]), float(d["period"]), d["fill_map"])
+    lib = UnitCellLibrary(float(d["period"]), d["lib_fill"], d["lib_phase"], d["lib_amp"])
+    ref = d["mtf"]
+    k = int(ref[0])
+    return lens, lib, float(d["f"]), float(d["lam"]), float(d["D"]), \
+        (ref[1:1 + k], ref[1 + k:1 + 2 * k], ref[1 + 2 * k:1 + 3 * k], ref[1 + 3 * k])
+
+
+@pytest.mark.parametrize("name", ["lens_analyses.npz", "mtf_large.npz"])
+def test_oracle_mtf_matches_reference(name):
+    """analyze_mtf (mtf.cpp:10-63): the reference's float curve, reproduced bit for bit
+    from exit_pupil + the oracle's autocorrelation."""
+    from celeris_amd.design import exit_pupil
+
+    lens, lib, f, lam, D, (rf, rm, ri, rc) = _mtf_golden(name)
+    P, P0 = exit_pupil(lens, lib, f, lam, D)
+    fr, m, mi, cut = oracle.mtf(P, P0, lens.period_um, f, lam, D)
+    assert fr.size == rf.size
+    assert np.array_equal(fr, rf.astype(np.float32))
+    assert np.array_equal(m, rm.astype(np.float32))
+    assert np.array_equal(mi, ri.astype(np.float32))
+    assert cut == rc
+
+
+def test_oracle_mtf_invariants():
+    """A perfect pupil's MTF is the mask's; a phase ramp along x, e^{i a x}, leaves the
+    MTF along x unchanged (|acc(s)| gains only e^{-i a s}); a uniform disc's MTF falls
+    from 1 and reaches 0 past the aperture width (no overlap)."""
+    n, p, f, lam, D = 41, 0.5, 20.0, 0.5, 10.0
+    c = (n - 1) / 2
+    iy, ix = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    r = np.hypot((ix - c) * p, (iy - c) * p)
+    P0 = (~(r > D / 2)).astype(np.float64)
+    fr, m, mi, cut = oracle.mtf(P0.astype(np.complex128), P0, p, f, lam, D)
+    assert np.array_equal(m, mi) and m[0] == 1.0 and np.all(np.diff(mi) <= 0)
+    assert mi[-1] == 0.0 and cut == D / (lam * f) * 1000.0
+    rng = np.random.default_rng(3)
+    P = P0 * np.exp(1j * rng.uniform(-np.pi, np.pi, (n, n)))
+    _, m1, _, _ = oracle.mtf(P, P0, p, f, lam, D)
+    _, m2, _, _ = oracle.mtf(P * np.exp(1j * 0.7 * ix), P0, p, f, lam, D)
+    assert np.allclose(m1, m2, rtol=1e-6, atol=1e-7)
+    # ... but not under a conjugate-free correlation (catches a dropped conj)
+    acc = [np.sum(P[:, :n - s] * P[:, s:]) for s in range(5)]
+    acc2 = [np.sum((P * np.exp(1j * 0.7 * ix))[:, :n - s] * (P * np.exp(1j * 0.7 * ix))[:, s:])
+            for s in range(5)]
+    assert not np.allclose(np.abs(acc), np.abs(acc2), rtol=1e-3)
