@@ -87,6 +87,11 @@ SIGNATURES = [
      [_DP, _DP, _DP, c_int, c_double, c_double, c_double, c_double, c_int, c_int, _DP]),
     ("crs_analyze_field_grid", c_int,
      [_DP, _DP, _DP, c_int, c_double, c_double, c_double, c_double, c_int, c_int, c_int, _DP]),
+    ("crs_analyze_field_of_view", c_int,
+     [_DP, _DP, _DP, c_int, c_double, c_double, c_double, _DP, c_int, c_int, _DP]),
+    ("crs_analyze_wide_fov", c_int,
+     [_DP, _DP, _DP, c_int, c_double, c_double, c_double, c_double, c_double, _DP, c_int, c_int,
+      _DP]),
     ("crs_analyze_mtf", c_int,
      [_DP, _DP, c_int, c_double, c_double, c_double, c_double, c_int, POINTER(ctypes.c_float),
       POINTER(ctypes.c_float), POINTER(ctypes.c_float), _DP, POINTER(c_int)]),

@@ -10,6 +10,8 @@ module (src/celeris/analysis/*.hpp; bindings/python/celeris_module.cpp:415-440):
 * :func:`analyze_through_focus` throughfocus.cpp:15-132 -> :class:`ThroughFocus`
 * :func:`analyze_chromatic`     chromatic.cpp:9-58    -> list of :class:`ChromaticPoint`
 * :func:`analyze_field_grid`    field.cpp:150-233     -> :class:`FieldGrid`
+* :func:`analyze_field_of_view` field.cpp:12-76       -> list of :class:`FieldPoint`
+* :func:`analyze_wide_fov`      field.cpp:78-148      -> list of :class:`FieldPoint`
 * :func:`analyze_mtf`           mtf.cpp:10-63         -> :class:`MtfCurve`
 
 Each runs its sums in one launch through the C ABI (crs_analyze_*, crs_forward_host);
@@ -107,6 +109,15 @@ class FieldGrid:
     n: int
     max_angle_deg: float
     points: List[FieldGridPoint]
+
+
+@dataclass
+class FieldPoint:
+    """field.hpp:14-18."""
+
+    angle_deg: float
+    rel_strehl: float
+    spot_shift_um: float
 
 
 @dataclass
@@ -279,3 +290,34 @@ def mtf_from_pupil(P, P0, period_um: float, focal_length_um: float, wavelength_u
            "crs_analyze_mtf")
     m = npts.value
     return MtfCurve(freq[:m].copy(), mtf[:m].copy(), ideal[:m].copy(), float(cutoff[0]))
+
+
+def analyze_field_of_view(lens: MetalensDesign, lib: UnitCellLibrary, focal_length_um: float,
+                          wavelength_um: float, diameter_um: float, angles_deg,
+                          mode="fp32c") -> List[FieldPoint]:
+    """field.cpp:12-76: relative peak and spot position per field angle, one launch."""
+    px, py, t = aperture_pillars(lens, lib, diameter_um)
+    px, py, t = _d(px), _d(py), _c(t)
+    ang = _d(np.atleast_1d(angles_deg))
+    out = np.zeros(3 * max(ang.size, 1))
+    _check(_lib.load().crs_analyze_field_of_view(_p(px), _p(py), _p(t), px.size, focal_length_um,
+                                                 wavelength_um, diameter_um, _p(ang), ang.size,
+                                                 _mode(mode), _p(out)),
+           "crs_analyze_field_of_view")
+    return [FieldPoint(*map(float, out[3 * i:3 * i + 3])) for i in range(ang.size)]
+
+
+def analyze_wide_fov(lens: MetalensDesign, lib: UnitCellLibrary, focal_length_um: float,
+                     wavelength_um: float, lens_diameter_um: float, stop_diameter_um: float,
+                     stop_distance_um: float, angles_deg, mode="fp32c") -> List[FieldPoint]:
+    """field.cpp:78-148: field angles seen through a decentred stop patch, one launch."""
+    px, py, t = aperture_pillars(lens, lib, lens_diameter_um)
+    px, py, t = _d(px), _d(py), _c(t)
+    ang = _d(np.atleast_1d(angles_deg))
+    out = np.zeros(3 * max(ang.size, 1))
+    _check(_lib.load().crs_analyze_wide_fov(_p(px), _p(py), _p(t), px.size, focal_length_um,
+                                            wavelength_um, lens_diameter_um, stop_diameter_um,
+                                            stop_distance_um, _p(ang), ang.size, _mode(mode),
+                                            _p(out)),
+           "crs_analyze_wide_fov")
+    return [FieldPoint(*map(float, out[3 * i:3 * i + 3])) for i in range(ang.size)]

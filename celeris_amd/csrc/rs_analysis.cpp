@@ -5,6 +5,8 @@
 //   crs_analyze_through_focus  analysis/throughfocus.cpp:15-132
 //   crs_analyze_chromatic      analysis/chromatic.cpp:9-58
 //   crs_analyze_field_grid     analysis/field.cpp:150-233
+//   crs_analyze_field_of_view  analysis/field.cpp:12-76
+//   crs_analyze_wide_fov       analysis/field.cpp:78-148
 //
 // Inputs are the aperture pillar list the reference builds from a MetalensDesign and
 // a UnitCellLibrary (focal.cpp:42-57): positions, complex transmissions and, for the
@@ -300,6 +302,143 @@ extern "C" crs_status crs_analyze_field_grid(const double* px, const double* py,
         o[4] = fwy[p];
         o[5] = cxs[p];
         o[6] = cys[p];
+    }
+    return CRS_OK;
+}
+
+namespace {
+
+// The tilted-illumination spot search shared by analyze_field_of_view and
+// analyze_wide_fov (field.cpp:34-62, :105-135): for each field angle an 81 x 81 window
+// at z = f, centred at (xc, 0) with half-width W, of the cells under a stop patch
+// (radius^2 r2 around (x_patch, 0); r2 < 0: every cell) lit by exp(i k sin(theta) x);
+// returns each window's peak |E|^2 and the x of the first pixel reaching it (rows outer,
+// columns inner, strict >).  All windows run as one launch, one transmission set each.
+struct TiltSpot {
+    double theta, xc, W, x_patch;
+};
+
+crs_status tilt_spots(const double* px, const double* py, const double* t, int npil,
+                      double focal_um, double k, double r2, const std::vector<TiltSpot>& spots,
+                      int mode, std::vector<double>& peak, std::vector<double>& peak_x) {
+    const int NG = 81;
+    const int nsp = static_cast<int>(spots.size());
+    const int64_t npix = static_cast<int64_t>(NG) * NG;
+    std::vector<double> tt(2 * static_cast<size_t>(npil) * nsp);
+    std::vector<crs_plane> planes(nsp);
+    for (int p = 0; p < nsp; ++p) {
+        const TiltSpot& sp = spots[p];
+        const double sn = std::sin(sp.theta);
+        double* ts = tt.data() + 2 * static_cast<size_t>(npil) * p;
+        for (int q = 0; q < npil; ++q) {
+            const double dx = px[q] - sp.x_patch, dy = py[q];
+            if (r2 >= 0.0 && dx * dx + dy * dy > r2) {  // outside the stop patch
+                ts[2 * q] = ts[2 * q + 1] = 0.0;
+                continue;
+            }
+            const std::complex<double> v =
+                std::complex<double>(t[2 * q], t[2 * q + 1]) * std::polar(1.0, k * sn * px[q]);
+            ts[2 * q] = v.real();
+            ts[2 * q + 1] = v.imag();
+        }
+        crs_plane& pl = planes[p];
+        pl = crs_plane{};
+        pl.k = k;
+        pl.x0 = sp.xc - sp.W;
+        pl.sx = 2.0 * sp.W / (NG - 1);
+        pl.y0 = -sp.W;
+        pl.sy = pl.sx;
+        pl.z0 = focal_um;
+        pl.nx = NG;
+        pl.ny = NG;
+        pl.tset = p;
+        pl.offset = npix * p;
+    }
+    std::vector<double> I(npix * nsp);
+    const crs_status st =
+        crs_forward_host(mode, px, py, tt.data(), npil, nsp, planes.data(), nsp, nullptr, I.data());
+    if (st != CRS_OK) return st;
+    peak.assign(nsp, 0.0);
+    peak_x.assign(nsp, 0.0);
+    for (int p = 0; p < nsp; ++p) {
+        const TiltSpot& sp = spots[p];
+        const double* m = I.data() + npix * p;
+        double best = 0.0, best_x = sp.xc;
+        for (int j = 0; j < NG; ++j)
+            for (int i = 0; i < NG; ++i) {
+                const double v = m[static_cast<int64_t>(j) * NG + i];
+                if (v > best) {
+                    best = v;
+                    best_x = sp.xc - sp.W + 2.0 * sp.W * i / (NG - 1);
+                }
+            }
+        peak[p] = best;
+        peak_x[p] = best_x;
+    }
+    return CRS_OK;
+}
+
+}  // namespace
+
+extern "C" crs_status crs_analyze_field_of_view(const double* px, const double* py,
+                                                const double* t, int npil, double focal_um,
+                                                double wavelength_um, double diameter_um,
+                                                const double* angles_deg, int nangles, int mode,
+                                                double* out3) {
+    if (!px || !py || !t || npil <= 0 || nangles < 0 || (nangles > 0 && (!angles_deg || !out3)))
+        return CRS_ERR_INVALID;
+    const double k = 2.0 * kPi / wavelength_um;
+    const double dl = wavelength_um * focal_um / diameter_um;  // field.cpp:40
+    auto spot = [&](double theta, double xc) {
+        const double x_sin = focal_um * std::sin(theta), x_tan = focal_um * std::tan(theta);
+        return TiltSpot{theta, xc, std::max(3.0 * dl, 2.0) + std::abs(x_tan - x_sin), 0.0};
+    };
+    std::vector<TiltSpot> spots{spot(0.0, 0.0)};  // the on-axis reference (field.cpp:64)
+    for (int a = 0; a < nangles; ++a) {
+        const double theta = angles_deg[a] * kPi / 180.0;
+        spots.push_back(spot(theta, focal_um * std::tan(theta)));
+    }
+    std::vector<double> pk, pkx;
+    const crs_status st = tilt_spots(px, py, t, npil, focal_um, k, -1.0, spots, mode, pk, pkx);
+    if (st != CRS_OK) return st;
+    for (int a = 0; a < nangles; ++a) {
+        out3[3 * a] = angles_deg[a];
+        out3[3 * a + 1] = pk[0] > 0 ? pk[a + 1] / pk[0] : 0.0;
+        out3[3 * a + 2] = pkx[a + 1];
+    }
+    return CRS_OK;
+}
+
+extern "C" crs_status crs_analyze_wide_fov(const double* px, const double* py, const double* t,
+                                           int npil, double focal_um, double wavelength_um,
+                                           double lens_diameter_um, double stop_diameter_um,
+                                           double stop_distance_um, const double* angles_deg,
+                                           int nangles, int mode, double* out3) {
+    (void)lens_diameter_um;  // the caller's pillar list is already cut to the lens aperture
+    if (!px || !py || !t || npil <= 0 || nangles < 0 || (nangles > 0 && (!angles_deg || !out3)))
+        return CRS_ERR_INVALID;
+    const double k = 2.0 * kPi / wavelength_um;
+    const double dl_stop = wavelength_um * focal_um / stop_diameter_um;  // field.cpp:91
+    const double R_stop = stop_diameter_um / 2.0;
+    auto spot = [&](double theta, double x_patch) {
+        const double x_sin = focal_um * std::sin(theta), x_tan = focal_um * std::tan(theta);
+        const double xc = 0.5 * (x_sin + x_tan);
+        return TiltSpot{theta, xc, std::max(3.0 * dl_stop, 2.0) + 0.5 * std::abs(x_tan - x_sin),
+                        x_patch};
+    };
+    std::vector<TiltSpot> spots{spot(0.0, 0.0)};  // field.cpp:137
+    for (int a = 0; a < nangles; ++a) {
+        const double theta = angles_deg[a] * kPi / 180.0;
+        spots.push_back(spot(theta, stop_distance_um * std::tan(theta)));
+    }
+    std::vector<double> pk, pkx;
+    const crs_status st =
+        tilt_spots(px, py, t, npil, focal_um, k, R_stop * R_stop, spots, mode, pk, pkx);
+    if (st != CRS_OK) return st;
+    for (int a = 0; a < nangles; ++a) {
+        out3[3 * a] = angles_deg[a];
+        out3[3 * a + 1] = pk[0] > 0 ? pk[a + 1] / pk[0] : 0.0;
+        out3[3 * a + 2] = pkx[a + 1];
     }
     return CRS_OK;
 }

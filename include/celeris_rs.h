@@ -194,6 +194,26 @@ crs_status crs_analyze_field_grid(const double *px, const double *py, const doub
                                   double max_angle_deg, int n_half, int psf_n, int mode,
                                   double *out7);
 
+/* Field of view, replacing analysis/field.cpp:12-76 (analyze_field_of_view): for each
+ * field angle, the peak |E|^2 of an 81 x 81 window at z = f around the chief ray
+ * (x = f tan theta) under illumination exp(i k sin(theta) x), relative to the on-axis
+ * peak, and the x of that peak.  All windows (and the on-axis reference) in one launch.
+ * out3 = per angle {angle_deg, rel_strehl, spot_shift_um}. */
+crs_status crs_analyze_field_of_view(const double *px, const double *py, const double *t,
+                                     int npil, double focal_um, double wavelength_um,
+                                     double diameter_um, const double *angles_deg, int nangles,
+                                     int mode, double *out3);
+
+/* Wide field of view behind a remote stop, replacing analysis/field.cpp:78-148
+ * (analyze_wide_fov): as crs_analyze_field_of_view, but only the pillars inside the stop
+ * patch of diameter stop_diameter_um centred at x = stop_distance_um tan(theta) are lit,
+ * and the window sits between f sin(theta) and f tan(theta).  The pillar list is the
+ * lens aperture's (lens_diameter_um). */
+crs_status crs_analyze_wide_fov(const double *px, const double *py, const double *t, int npil,
+                                double focal_um, double wavelength_um, double lens_diameter_um,
+                                double stop_diameter_um, double stop_distance_um,
+                                const double *angles_deg, int nangles, int mode, double *out3);
+
 /* MTF along x, replacing analysis/mtf.cpp:10-63 (analyze_mtf).  pupil is the exit pupil
  * P[iy*n + ix] = t e^{-i phi_ideal} of the n x n cell grid (complex, interleaved re/im,
  * zero outside the aperture) and pupil0 its aperture mask (1 inside, 0 outside), as

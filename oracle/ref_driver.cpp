@@ -29,6 +29,10 @@
 //                                             (analyze_field_grid, field.cpp:150)
 //   psffield     : lens, f, lambda, D, angle, n, W, on_axis_peak -> angle, cx, rel_strehl,
 //                  intensity[n*n]             (compute_psf_field, focal.cpp:240)
+//   fov          : lens, f, lambda, D, m, angles[m] -> m x (angle, rel_strehl, shift)
+//                                             (analyze_field_of_view, field.cpp:12)
+//   widefov      : lens, f, lambda, D_lens, D_stop, stop_dist, m, angles[m]
+//                  -> m x (angle, rel_strehl, shift)   (analyze_wide_fov, field.cpp:78)
 #include "celeris/analysis/chromatic.hpp"
 #include "celeris/analysis/field.hpp"
 #include "celeris/analysis/focal.hpp"
@@ -169,6 +173,19 @@ int main(int argc, char** argv) {
             FieldPsf p = compute_psf_field(lens, lib, f, lam, D, angle, n, W, on_axis_peak);
             out = {p.angle_deg, p.cx_um, p.rel_strehl};
             out.insert(out.end(), p.psf.intensity.begin(), p.psf.intensity.end());
+        } else if (cmd == "fov" || cmd == "widefov") {
+            double Dstop = 0.0, stop_dist = 0.0;
+            if (cmd == "widefov") {
+                Dstop = r.next();
+                stop_dist = r.next();
+            }
+            const int m = r.next_int();
+            std::vector<double> angles(m);
+            for (double& a : angles) a = r.next();
+            std::vector<FieldPoint> pts =
+                cmd == "fov" ? analyze_field_of_view(lens, lib, f, lam, D, angles)
+                             : analyze_wide_fov(lens, lib, f, lam, D, Dstop, stop_dist, angles);
+            for (const auto& p : pts) out.insert(out.end(), {p.angle_deg, p.rel_strehl, p.spot_shift_um});
         } else {
             std::fprintf(stderr, "ref_driver: unknown command %s\n", cmd.c_str());
             return 2;

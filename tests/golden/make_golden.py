@@ -108,6 +108,16 @@ def main():
     lens_out["psffield"] = oracle.ref_run("psffield", np.concatenate([base, [5.0, 33, 3.0, on_axis_peak]]))
     np.savez_compressed(os.path.join(HERE, "lens_analyses.npz"), **lens_out)
 
+    fov = {k: np.asarray(v) for k, v in L.items()}
+    fov["fov_angles"] = np.array([0.0, 3.0, 8.0, 15.0])
+    fov["fov"] = oracle.ref_run("fov", np.concatenate([base, [fov["fov_angles"].size],
+                                                       fov["fov_angles"]]))
+    fov["widefov_args"] = np.array([4.0, 10.0])  # stop diameter, stop distance
+    fov["widefov_angles"] = np.array([0.0, 5.0, 12.0, 20.0])
+    fov["widefov"] = oracle.ref_run("widefov", np.concatenate(
+        [base, fov["widefov_args"], [fov["widefov_angles"].size], fov["widefov_angles"]]))
+    np.savez_compressed(os.path.join(HERE, "field_of_view.npz"), **fov)
+
     M = mtf_case()
     blk = oracle.lens_block(M["n_cells"], M["period"], M["fill_map"], M["lib_fill"], M["lib_phase"],
                             M["lib_amp"])

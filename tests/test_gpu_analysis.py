@@ -6,6 +6,7 @@ In FP64 mode every sum matches the reference's FP64 CPU loops to roundoff, so th
 derived metrics match to 1e-9 relative; discrete outputs (FWHM in whole pixel steps,
 argmax positions, z grids) must match exactly.  FP32C metrics match to 1e-5.
 """
+import math
 import os
 
 import numpy as np
@@ -158,3 +159,38 @@ def test_analyze_mtf_vs_oracle_large(torch_cuda):
     assert np.array_equal(cv.freq_cyc_mm, fr) and cv.cutoff_cyc_mm == cut
     assert np.all(np.abs(cv.mtf.astype(np.float64) - m) <= 1.2e-7 * np.abs(m) + 1e-30)
     assert np.all(np.abs(cv.mtf_ideal.astype(np.float64) - mi) <= 1.2e-7 * np.abs(mi) + 1e-30)
+
+
+@pytest.mark.parametrize("mode,tol", [("fp64", 1e-9), ("fp32c", 1e-5)])
+def test_analyze_field_of_view_and_wide_fov(torch_cuda, mode, tol):
+    """analyze_field_of_view / analyze_wide_fov (field.cpp:12-148) against the
+    reference's own outputs (tests/golden/field_of_view.npz)."""
+    d = np.load(os.path.join(os.path.dirname(GOLDEN), "field_of_view.npz"))
+    L = MetalensDesign(int(d["n_cells"]), float(d["period"]), d["fill_map"])
+    lib = UnitCellLibrary(float(d["period"]), d["lib_fill"], d["lib_phase"], d["lib_amp"])
+    f, lam, D = float(d["f"]), float(d["lam"]), float(d["D"])
+    print(f"[analysis] field of view {mode}", flush=True)
+    pts = analysis.analyze_field_of_view(L, lib, f, lam, D, d["fov_angles"], mode=mode)
+    ref = d["fov"].reshape(-1, 3)
+    got = np.array([[p.angle_deg, p.rel_strehl, p.spot_shift_um] for p in pts])
+    print(mode, got, ref)
+    assert np.array_equal(got[:, 0], ref[:, 0])
+    assert np.allclose(got[:, 1], ref[:, 1], rtol=tol, atol=0)
+    # the spot position is a pixel of the 81 x 81 window: equal, or one pixel step away
+    # where two pixels tie to within the mode's roundoff
+    W = np.array([max(3 * lam * f / D, 2.0) + abs(f * math.tan(math.radians(a)) - f * math.sin(math.radians(a)))
+                  for a in ref[:, 0]])
+    slack = 1e-9 if mode == "fp64" else 2 * W / 80 * 1.0001
+    assert np.all(np.abs(got[:, 2] - ref[:, 2]) <= slack)
+    Dstop, sdist = d["widefov_args"]
+    pts = analysis.analyze_wide_fov(L, lib, f, lam, D, Dstop, sdist, d["widefov_angles"], mode=mode)
+    ref = d["widefov"].reshape(-1, 3)
+    got = np.array([[p.angle_deg, p.rel_strehl, p.spot_shift_um] for p in pts])
+    print(mode, got, ref)
+    assert np.array_equal(got[:, 0], ref[:, 0])
+    assert np.allclose(got[:, 1], ref[:, 1], rtol=tol, atol=0)
+    dls = lam * f / Dstop
+    W = np.array([max(3 * dls, 2.0) + 0.5 * abs(f * math.tan(math.radians(a)) - f * math.sin(math.radians(a)))
+                  for a in ref[:, 0]])
+    slack = 1e-9 if mode == "fp64" else 2 * W / 80 * 1.0001
+    assert np.all(np.abs(got[:, 2] - ref[:, 2]) <= slack)

@@ -238,3 +238,51 @@ def test_oracle_mtf_invariants():
     acc2 = [np.sum((P * np.exp(1j * 0.7 * ix))[:, :n - s] * (P * np.exp(1j * 0.7 * ix))[:, s:])
             for s in range(5)]
     assert not np.allclose(np.abs(acc), np.abs(acc2), rtol=1e-3)
+
+
+def _fov_windows(px, py, t, f, lam, spots, r2):
+    """The oracle's version of field.cpp's tilted-spot search: per spot (theta, xc, W,
+    x_patch) an 81 x 81 window at z = f of the lit cells under exp(i k sin(theta) x)."""
+    k = 2 * math.pi / lam
+    NG = 81
+    out = []
+    for theta, xc, W, xp in spots:
+        lit = np.ones(px.size, bool) if r2 < 0 else ~((px - xp) ** 2 + py ** 2 > r2)
+        ts = np.where(lit, t * np.exp(1j * k * math.sin(theta) * px), 0)
+        pl = Plane(k, xc - W, 2 * W / (NG - 1), -W, 2 * W / (NG - 1), f, 0.0, NG, NG)
+        I = np.abs(oracle.forward(px, py, ts[None], [pl])) ** 2
+        j, i = np.unravel_index(np.argmax(I), (NG, NG))  # first maximum, rows outer
+        out.append((I.max(), xc - W + 2.0 * W * i / (NG - 1)))
+    return out
+
+
+def test_oracle_field_of_view_matches_reference():
+    """analyze_field_of_view / analyze_wide_fov (field.cpp:12-148) restated as tilted
+    transmission sets on 81 x 81 windows: the reference's rel_strehl and spot shifts."""
+    d = np.load(os.path.join(GOLDEN, "field_of_view.npz"))
+    lens = MetalensDesign(int(d["n_cells"]), float(d["period"]), d["fill_map"])
+    lib = UnitCellLibrary(float(d["period"]), d["lib_fill"], d["lib_phase"], d["lib_amp"])
+    f, lam, D = float(d["f"]), float(d["lam"]), float(d["D"])
+    px, py, t = aperture_pillars(lens, lib, D)
+    dl = lam * f / D
+    sp = [(0.0, 0.0, max(3 * dl, 2.0), 0.0)]
+    for a in d["fov_angles"]:
+        th = math.radians(a)
+        sp.append((th, f * math.tan(th), max(3 * dl, 2.0) + abs(f * math.tan(th) - f * math.sin(th)), 0.0))
+    res = _fov_windows(px, py, t, f, lam, sp, -1.0)
+    ref = d["fov"].reshape(-1, 3)
+    for a, (pk, x), r in zip(d["fov_angles"], res[1:], ref):
+        assert r[0] == a and pk / res[0][0] == pytest.approx(r[1], rel=1e-12)
+        assert x == pytest.approx(r[2], abs=1e-12)
+    Dstop, sdist = d["widefov_args"]
+    dls = lam * f / Dstop
+    sp = []
+    for a in np.concatenate([[0.0], d["widefov_angles"]]):
+        th = math.radians(a)
+        xs, xt = f * math.sin(th), f * math.tan(th)
+        sp.append((th, 0.5 * (xs + xt), max(3 * dls, 2.0) + 0.5 * abs(xt - xs), sdist * math.tan(th)))
+    res = _fov_windows(px, py, t, f, lam, sp, (Dstop / 2) ** 2)
+    ref = d["widefov"].reshape(-1, 3)
+    for a, (pk, x), r in zip(d["widefov_angles"], res[1:], ref):
+        assert r[0] == a and pk / res[0][0] == pytest.approx(r[1], rel=1e-12)
+        assert x == pytest.approx(r[2], abs=1e-12)
