@@ -181,3 +181,16 @@ def mtf(pupil, pupil0, period_um, focal_um, wavelength_um, diameter_um):
     freq = ((s * p) / (wavelength_um * focal_um) * 1000.0).astype(np.float32)
     return freq, (a / normP).astype(np.float32), (a0 / normP0).astype(np.float32), \
         diameter_um / (wavelength_um * focal_um) * 1000.0
+
+
+def ref_pgm(nx: int, ny: int, values, gamma: float) -> bytes:
+    """The reference's write_pgm (io/image.cpp:10-37) output for a map, as bytes."""
+    if not ref_available():
+        raise FileNotFoundError(REF_DRIVER)
+    vals = np.concatenate([[nx, ny, gamma], np.asarray(values, np.float64).ravel()])
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.f64"), os.path.join(d, "out.f64")
+        vals.tofile(fin)
+        subprocess.run([REF_DRIVER, "pgm", fin, fout], check=True)
+        with open(fout + ".pgm", "rb") as f:
+            return f.read()

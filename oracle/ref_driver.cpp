@@ -33,12 +33,16 @@
 //                                             (analyze_field_of_view, field.cpp:12)
 //   widefov      : lens, f, lambda, D_lens, D_stop, stop_dist, m, angles[m]
 //                  -> m x (angle, rel_strehl, shift)   (analyze_wide_fov, field.cpp:78)
+//   pgm          : nx, ny, gamma, values[nx*ny] -> ok; the image itself is written to
+//                  <out.f64>.pgm              (write_pgm, io/image.cpp:10)
 #include "celeris/analysis/chromatic.hpp"
 #include "celeris/analysis/field.hpp"
 #include "celeris/analysis/focal.hpp"
 #include "celeris/analysis/mtf.hpp"
 #include "celeris/analysis/throughfocus.hpp"
+#include "celeris/io/image.hpp"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -118,6 +122,12 @@ int main(int argc, char** argv) {
         const double W = r.next();
         PsfMap m = propagate_pillars(px, py, t, cx, cy, z, lam, n, W);
         out = m.intensity;
+    } else if (cmd == "pgm") {
+        const int nx = r.next_int(), ny = r.next_int();
+        const double gamma = r.next();
+        std::vector<double> v(static_cast<std::size_t>(std::max(nx, 0)) * std::max(ny, 0));
+        for (double& x : v) x = r.next();
+        out = {write_pgm(std::string(argv[3]) + ".pgm", nx, ny, v, gamma) ? 1.0 : 0.0};
     } else {
         MetalensDesign lens;
         UnitCellLibrary lib;

@@ -118,6 +118,22 @@ def main():
         [base, fov["widefov_args"], [fov["widefov_angles"].size], fov["widefov_angles"]]))
     np.savez_compressed(os.path.join(HERE, "field_of_view.npz"), **fov)
 
+    # PGM images (io/image.cpp) of the golden PSF, a caustic-like map, signed and
+    # all-zero maps, with the CLI's gammas.
+    rng = np.random.default_rng(5)
+    caustic = np.asarray(oracle.ref_run("throughfocus", base))
+    Nz = int(caustic[0])
+    cnx, cnz = int(caustic[3 + 2 * Nz]), int(caustic[4 + 2 * Nz])
+    maps = [(33, 33, lens_out["psf"], 2.2), (cnx, cnz, caustic[5 + 2 * Nz:5 + 2 * Nz + cnx * cnz], 1.6),
+            (7, 5, rng.normal(size=35), 1.0), (4, 3, np.zeros(12), 2.2),
+            (16, 9, rng.uniform(0, 1, 144) ** 3, 0.5)]
+    pgm = {}
+    for i, (nx, ny, v, g) in enumerate(maps):
+        pgm[f"case{i}__args"] = np.array([nx, ny, g])
+        pgm[f"case{i}__values"] = np.asarray(v, np.float64)
+        pgm[f"case{i}__pgm"] = np.frombuffer(oracle.ref_pgm(nx, ny, v, g), dtype=np.uint8)
+    np.savez_compressed(os.path.join(HERE, "pgm.npz"), **pgm)
+
     M = mtf_case()
     blk = oracle.lens_block(M["n_cells"], M["period"], M["fill_map"], M["lib_fill"], M["lib_phase"],
                             M["lib_amp"])
