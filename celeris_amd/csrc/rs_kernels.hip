@@ -368,7 +368,7 @@ __global__ __launch_bounds__(kThreads, 4) void fwd_fp32c(
             tile[tid] = rec;
         }
         __syncthreads();
-        const int k0 = wave * 64, k1 = min(k0 + 64, n);
+        const int k0 = (wave * n) / kWaves, k1 = ((wave + 1) * n) / kWaves;
         if (k0 < k1) {
             float ar[N], ai[N];
 #pragma unroll
