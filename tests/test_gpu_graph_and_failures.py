@@ -133,3 +133,42 @@ def test_invalid_arguments(torch_cuda, rsctx):
     overlap = [Plane.psf(0, 0, 30.0, 10.0, 9, 2.0, offset=0), Plane.psf(0, 0, 31.0, 10.0, 5, 2.0, offset=80)]
     with pytest.raises(RuntimeError, match="overlap"):
         rsctx.forward(dpx, dpy, dt, overlap)
+
+
+def test_tuning_timing_and_launch_info(torch_cuda, tmp_path, monkeypatch):
+    """crs_set_tuning changes the split (work-group count) but not the result beyond
+    roundoff; crs_last_launch names the pair kernel; the autotuner installs one of its
+    candidates and caches it by shape."""
+    torch = torch_cuda
+    _, _, _, planes = batch()
+    px, py = configs.square_grid(64, 0.35)  # 4,096 pillars: up to 16 pillar chunks
+    rng = np.random.default_rng(4)
+    t = rng.uniform(0.5, 1, (2, px.size)) * np.exp(1j * rng.uniform(-np.pi, np.pi, (2, px.size)))
+    ctx = Context(0)
+    dpx, dpy, dt = dev(torch, px), dev(torch, py), dev(torch, t)
+    with pytest.raises(RuntimeError):
+        ctx.last_launch(False)  # nothing launched yet
+    outs, groups = [], []
+    for target in (3, 4096):
+        ctx.set_tuning("fp32c", target, target)
+        f, _ = ctx.forward(dpx, dpy, dt, planes, mode="fp32c")
+        outs.append(f.cpu().numpy())
+        name, g = ctx.last_launch(False)
+        assert name.startswith("crs::fwd_fp32c<") and g > 0  # 1 x 1 blocks here (coarse zx plane)
+        groups.append(g)
+    assert groups[1] > groups[0]
+    assert np.linalg.norm(outs[0] - outs[1]) <= 1e-6 * np.linalg.norm(outs[1])
+    ctx.set_timing(True)
+    ctx.forward(dpx, dpy, dt, planes, mode="fp64")
+    assert ctx.last_kernel_ms(False) > 0 and ctx.last_launch(False)[0] == "crs::fwd_fp64"
+    ctx.set_timing(False)
+
+    from celeris_amd import tune
+
+    monkeypatch.setattr(tune, "CACHE", str(tmp_path / "tune.json"))
+    u = dev(torch, np.ones(sum(p.npix for p in planes), np.complex128))
+    best = tune.autotune(ctx, dpx, dpy, dt, planes, u, mode="fp32c", candidates=(512, 2048), reps=1)
+    assert best["fwd"] in (512, 2048) and best["adj"] in (512, 2048)
+    again = tune.autotune(ctx, dpx, dpy, dt, planes, u, mode="fp32c", candidates=(512, 2048), reps=1)
+    assert again["fwd"] == best["fwd"] and again["adj"] == best["adj"]  # from the cache
+    ctx.close()
