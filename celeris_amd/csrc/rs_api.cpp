@@ -692,8 +692,9 @@ crs_status plan_adjoint(const crs_context* ctx, int mode, int npil, int nsets,
 
 crs_status size_adjoint(const AdjPlan& plan, const Scratch& sc, hipStream_t stream) {
     crs_status st;
-    // Positions only: the adjoint pair loop reads x and y.
-    if ((st = ensure(*sc.pil, staged_pillar_bytes(plan.mode, plan.npil, 1), stream,
+    // Positions only (FP32 / FP64; FP32C reads the caller's px, py directly).
+    if (plan.mode != CRS_MODE_FP32C &&
+        (st = ensure(*sc.pil, staged_pillar_bytes(plan.mode, plan.npil, 1), stream,
                      "staged pillars")) != CRS_OK)
         return st;
     if (plan.mode == CRS_MODE_FP32C &&
@@ -710,11 +711,16 @@ crs_status enqueue_adjoint(const AdjPlan& plan, const Scratch& sc, const double*
     crs_status st;
     const int mode = plan.mode;
     const DevPlane* d_planes = reinterpret_cast<const DevPlane*>(sc.tables + plan.o_planes);
-    // t = NULL stages positions with zero transmissions.
-    if ((st = launch_status(launch_stage_pillars(mode, px, py, nullptr, plan.npil, 1, sc.pil->ptr,
-                                                 stream),
-                            "stage positions")) != CRS_OK)
-        return st;
+    // FP32 / FP64: t = NULL stages positions with zero transmissions.
+    const double* const xy[2] = {px, py};
+    const void* pil = xy;
+    if (mode != CRS_MODE_FP32C) {
+        if ((st = launch_status(launch_stage_pillars(mode, px, py, nullptr, plan.npil, 1,
+                                                     sc.pil->ptr, stream),
+                                "stage positions")) != CRS_OK)
+            return st;
+        pil = sc.pil->ptr;
+    }
     const void* pix_or_u = u;
     if (mode == CRS_MODE_FP32C) {
         if ((st = launch_status(
@@ -727,7 +733,7 @@ crs_status enqueue_adjoint(const AdjPlan& plan, const Scratch& sc, const double*
         pix_or_u = sc.pix->ptr;
     }
     if (ev0) hipEventRecord(ev0, stream);
-    if ((st = launch_status(launch_adjoint(mode, plan.order, sc.pil->ptr, plan.npil, d_planes,
+    if ((st = launch_status(launch_adjoint(mode, plan.order, pil, plan.npil, d_planes,
                                            reinterpret_cast<const AdjQuad*>(sc.tables + plan.o_quads),
                                            reinterpret_cast<const AdjBlock*>(sc.tables + plan.o_blocks),
                                            plan.nblocks, pix_or_u,

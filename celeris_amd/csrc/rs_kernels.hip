@@ -697,20 +697,37 @@ __device__ __forceinline__ void adj_tile(const float4* tile, int ncol, const Dev
 
 template <int BX, int BY>
 __global__ __launch_bounds__(kThreads, 3) void adj_fp32c(
-    const double2* __restrict__ pos, int npil, const DevPlane* __restrict__ planes,
-    const AdjQuad* __restrict__ quads, const AdjBlock* __restrict__ blocks,
-    const float4* __restrict__ pix, const double2* __restrict__ ext,
+    const double* __restrict__ px, const double* __restrict__ py, int npil,
+    const DevPlane* __restrict__ planes, const AdjQuad* __restrict__ quads,
+    const AdjBlock* __restrict__ blocks, const float4* __restrict__ pix,
     double2* __restrict__ partial) {
     __shared__ float4 tile[kThreads];
+    __shared__ double2 red[kWaves];
     const AdjBlock B = blocks[blockIdx.x];
     const int tid = threadIdx.x;
-    const double2 E = ext[0];
     double qx[kPQ], qy[kPQ];
+    double mx = 0.0, my = 0.0;
 #pragma unroll
     for (int k = 0; k < kPQ; ++k) {
-        const double2 v = pos[min(B.q0 + tid + k * kThreads, npil - 1)];
-        qx[k] = v.x;
-        qy[k] = v.y;
+        const int q = min(B.q0 + tid + k * kThreads, npil - 1);
+        qx[k] = px[q];
+        qy[k] = py[q];
+        mx = fmax(mx, fabs(qx[k]));
+        my = fmax(my, fabs(qy[k]));
+    }
+    // The work-group's own pillar extent decides the series order (no global pass).
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        mx = fmax(mx, __shfl_xor(mx, o));
+        my = fmax(my, __shfl_xor(my, o));
+    }
+    if ((tid & 63) == 0) red[tid >> 6] = make_double2(mx, my);
+    __syncthreads();
+    double2 E = red[0];
+#pragma unroll
+    for (int w = 1; w < kWaves; ++w) {
+        E.x = fmax(E.x, red[w].x);
+        E.y = fmax(E.y, red[w].y);
     }
     // Kahan-compensated running sums, fed by one float sum per LDS tile.
     float sr[kPQ], si[kPQ], cr[kPQ], ci[kPQ];
@@ -1020,15 +1037,13 @@ hipError_t launch_adjoint(int mode, int order, const void* pil, int npil, const 
                           const void* pix_or_u, double2* partial, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
     if (mode == 0) {
-        const StagedLayout L = fp32c_staged_layout(npil, 1);
-        const unsigned char* base = static_cast<const unsigned char*>(pil);
-        const double2* pos = reinterpret_cast<const double2*>(base + L.pos);
-        const double2* ext = reinterpret_cast<const double2*>(base + L.ext);
+        // FP32C reads the caller's positions directly: pil = {px, py}.
+        const double* const* xy = static_cast<const double* const*>(pil);
         const float4* pix = static_cast<const float4*>(pix_or_u);
         if (order == kExactOrder)
-            adj_fp32c<1, 1><<<nblocks, kThreads, 0, s>>>(pos, npil, planes, quads, blocks, pix, ext, partial);
+            adj_fp32c<1, 1><<<nblocks, kThreads, 0, s>>>(xy[0], xy[1], npil, planes, quads, blocks, pix, partial);
         else
-            adj_fp32c<kBlk, kQuad><<<nblocks, kThreads, 0, s>>>(pos, npil, planes, quads, blocks, pix, ext, partial);
+            adj_fp32c<kBlk, kQuad><<<nblocks, kThreads, 0, s>>>(xy[0], xy[1], npil, planes, quads, blocks, pix, partial);
     } else if (mode == 2) {
         adj_fp32<<<nblocks, kThreads, 0, s>>>(static_cast<const float4*>(pil), npil, planes, quads,
                                               blocks, static_cast<const double2*>(pix_or_u),
