@@ -303,14 +303,17 @@ DevPlane make_dev_plane(const crs_plane& P, int px_per_thread, int py_per_thread
     return d;
 }
 
-// FP32C series orders for a set of planes.  A 4 x 4 pixel block spans at most e turns
-// around its centre; at distance >= z' the expansion parameter is r = e / z' (y <= r/2).
-// Dropped terms, at worst (a pillar seen at grazing angle): phase order 1 leaves
-// 2 x1 y^2 <= e r^2 / 2 turns, order 2 leaves ~5 x1 y^3 <= 0.625 e r^3; amplitude order
-// 1 leaves 6 y^2 <= 1.5 r^2, order 2 leaves ~20 y^3 <= 2.5 r^3.  Each order is the
-// lowest that keeps its term below 1.6e-7 turns (1e-6 rad) / 4e-6 of amplitude.
-// Returns 2 * (phase order - 1) + (amplitude order - 1), or kExactOrder when even
-// order 2 would not (then every pair is evaluated at its own reference distance).
+// Whether 4 x 4 blocks can be used for a set of planes, with no knowledge of where the
+// pillars are (the kernels refine the series order per work-group from the pillars'
+// extent).  A block spans at most e turns around its centre; at distance >= z' the
+// expansion parameter is r = e / z' (y <= r/2).  Dropped terms, at worst (a pillar seen
+// at grazing angle): phase order 1 leaves 2 x1 y^2 <= e r^2 / 2 turns, order 2 leaves
+// ~5 x1 y^3 <= 0.625 e r^3; amplitude order 1 leaves 6 y^2 <= 1.5 r^2, order 2 leaves
+// ~20 y^3 <= 2.5 r^3, against budgets of 1.6e-7 turns (1e-6 rad) / 4e-6 of amplitude.
+// Returns kExactOrder when even order 2 would exceed them (then every pair is evaluated
+// at its own reference distance, 1 x 1 blocks), else 2 * (phase order - 1) + (amplitude
+// order - 1) for grazing incidence (informational; any code below kExactOrder selects the
+// 4 x 4 kernels).
 int fp32c_order(const std::vector<DevPlane>& dp) {
     int po = 1, ao = 1;
     for (const DevPlane& d : dp) {

@@ -26,16 +26,16 @@ constexpr int kReduceCols = 64;
 // Longest pillar chunk one FP32C forward work-group sums (each wave a quarter of it,
 // in float, folded every 64 pillars): bounds the float accumulation error.
 constexpr int kMaxChunk = 16384;
-// Launch order code of the 1 x 1 (unexpanded) FP32C kernels; 0..3 are the 4 x 4
-// kernels' series orders, 2 * (phase order - 1) + (amplitude order - 1).
+// Plan order code of the 1 x 1 (unexpanded) FP32C kernels (fp32c_order); codes 0..3
+// select the 4 x 4 kernels, which choose their series order per work-group.
 constexpr int kExactOrder = 4;
 // FP32C adjoint: pillars per thread (q0 + tid + k*kThreads, k < kPQ).
 constexpr int kPQ = 4;
 // FP32C adjoint: columns per LDS tile of a quad (kAdjCols x 4 pixel records).
 constexpr int kAdjCols = kThreads / kQuad;
 
-// FP32C staged pillar buffer: positions (double2, npil), {|t|, arg t / 2 pi} (float2,
-// nsets x npil), the staging blocks' maxima of |x|, |y| (double2 per block of
+// FP32C staged pillar buffer (forward): positions (double2, npil), {|t|, arg t / 2 pi}
+// (float2, nsets x npil), the staging blocks' maxima of |x|, |y| (double2 per block of
 // kThreads staged elements) and their overall maximum (one double2, um).
 struct StagedLayout {
     size_t pos, ap, bmax, ext, bytes;

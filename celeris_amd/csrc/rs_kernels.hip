@@ -23,16 +23,18 @@
 //     R^2 = R0^2 + D,   D = e_x (2 dx0 + e_x) + e_y (2 dy0 + e_y) + (z_p'^2 - z_c'^2)
 // where D is tiny next to R0^2 and is formed in float without cancellation (each
 // term is one fma against per-block constants).  Then, with h = 1/(2 R0),
-// x1 = D h and y = x1 h = D / (4 R0^2):
-//     R - R0 = x1 (1 - y + 2 y^2 - ...)          (phase order 1 keeps 1 - y)
-//     R0 / R = 1 - 2 y + 6 y^2 - ...              (amplitude order 1 keeps 1 - 2 y)
+// x1 = D h = a[c] + b[r] (separable per column / row) and y = x1 h = D / (4 R0^2):
+//     R - R0 = x1 (1 - y + 2 y^2 - ...)          (first order keeps x1 w, w = 1 - y)
+//     R0 / R = 1 - 2 y + 6 y^2 - ...              (first order keeps 2 w - 1)
 // and the pair is
-//     E += |t| (k/2 pi) / R0 * (R0/R) * (cos 2 pi th, sin 2 pi th),  th = frac(...) + (R - R0)
-// with v_cos_f32 / v_sin_f32 (which take turns).  No float ever holds kR (~10^3 rad)
-// or R^2 (~10^4 turns^2): the phase is carried to ~1e-7 turns.  The host picks both
-// orders per launch from the block size relative to the distance.  The factor exp(2 pi i
-// z'_ref) is restored in FP64 by the reduction.  Each pillar tile's float partial is
-// folded into a Kahan-compensated running sum; split partials are added in FP64.
+//     E += |t| (k/2 pi) / R0 * (R0/R) * (cos 2 pi th, sin 2 pi th),  th = th0 + (R - R0)
+// with v_cos_f32 / v_sin_f32 (which take turns): 6 VALU ops and 2 transcendentals per
+// pair.  No float ever holds kR (~10^3 rad) or R^2 (~10^4 turns^2): the phase is carried
+// to ~1e-7 turns.  Each work-group checks, from its plane and the pillars' extent,
+// that the first-order terms meet the error budget and otherwise takes the second
+// order (fp32c_first_order); the host falls back to 1 x 1 blocks (every pair at its own
+// FP64 R0) where even that would not.  The factor exp(2 pi i z'_ref) is restored in
+// FP64 by the reduction; split partials are added in FP64.
 #include "rs_internal.h"
 
 #include <hip/hip_runtime.h>
