@@ -90,6 +90,14 @@ def cpu_baseline(lens, u_np, budget_s: float) -> dict:
                               return_pairs=True)
     t_adj = time.perf_counter() - t0
     tau = t_fwd / p_fwd + t_adj / p_adj  # seconds per pair, forward + adjoint
+    # One thread, for reference: a few rows forward and a pillar sample adjoint.
+    t0 = time.perf_counter()
+    p1 = oracle.forward_pairs(lens.px, lens.py, lens.t, planes, 1, row_stride=max(1, n // 4))
+    t1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, p1a = oracle.adjoint(lens.px, lens.py, u_np, planes, 1, 1, pillar_stride=2000,
+                            return_pairs=True)
+    t1a = time.perf_counter() - t0
     return {
         "value": 1.0 / tau / 1e9,
         "unit": "G pairs/s",
@@ -101,6 +109,7 @@ def cpu_baseline(lens, u_np, budget_s: float) -> dict:
         "cpu_model": cpu_model(),
         "fwd_s_per_pair": t_fwd / p_fwd,
         "adj_s_per_pair": t_adj / p_adj,
+        "single_thread_G_pairs_s": 1.0 / (t1 / p1 + t1a / p1a) / 1e9,
     }
 
 
