@@ -169,3 +169,30 @@ def test_config4_gradient_fp32c_vs_fp64(torch_cuda):
         print(q, gphi[q], fd)
         assert abs(gphi[q] - fd) <= 1e-6 * np.max(np.abs(gphi)) + 1e-9
     ctx.close()
+
+
+@pytest.mark.parametrize("mode,tol", [("fp64", 1e-12), ("fp32c", 1e-5)])
+def test_adjoint_general_and_first_order_planes(torch_cuda, rsctx, mode, tol):
+    """One launch whose work-groups take different series orders: the config-1 window
+    (81 x 81 over +-5 um at f = 50: 4 x 4 blocks need the second-order terms) and a
+    finer 41 x 41 window over +-1 um (first order suffices), both against the oracle."""
+    L = configs.config1()
+    k = L.planes[0].k
+    planes = [L.planes[0], Plane.psf(0.0, 0.0, 50.0, k, 41, 1.0)]
+    total = sum(p.npix for p in planes)
+    rng = np.random.default_rng(17)
+    u = rng.standard_normal(total) + 1j * rng.standard_normal(total)
+    ref = oracle.adjoint(L.px, L.py, u, planes)
+    g = gpu_adjoint(torch_cuda, rsctx, L.px, L.py, u, planes, 1, mode)
+    e = rel_l2(g, ref)
+    print(f"adjoint mixed orders {mode} relL2 {e:.3e}")
+    assert e <= tol
+    # each plane alone, so that an error confined to one order's path cannot hide
+    off = 0
+    for p in planes:
+        up = u[off:off + p.npix]
+        off += p.npix
+        gp = gpu_adjoint(torch_cuda, rsctx, L.px, L.py, up, [Plane(**{**p.__dict__, "offset": -1})], 1, mode)
+        ep = rel_l2(gp, oracle.adjoint(L.px, L.py, up, [p]))
+        print(f"  plane {p.nx}x{p.ny} relL2 {ep:.3e}")
+        assert ep <= tol

@@ -235,3 +235,19 @@ def test_default_mode_switch(torch_cuda):
         crs.set_default_mode(prev)
     ref = np.abs(oracle.forward(L.px, L.py, L.t, L.planes)) ** 2
     assert rel_l2(out.ravel(), ref) <= 1e-12
+
+
+@pytest.mark.parametrize("mode,tol", [("fp64", 1e-12), ("fp32c", 1e-5)])
+def test_forward_general_and_first_order_planes(torch_cuda, rsctx, mode, tol):
+    """One launch whose work-groups take different series orders (see the adjoint test
+    of the same name): each plane against the oracle."""
+    L = configs.config1()
+    k = L.planes[0].k
+    planes = [L.planes[0], Plane.psf(0.0, 0.0, 50.0, k, 41, 1.0)]
+    ref = oracle.forward(L.px, L.py, L.t, planes)
+    f, _ = gpu_forward(torch_cuda, rsctx, L.px, L.py, L.t, planes, mode)
+    for p in crs.resolve_offsets(planes):
+        sl = slice(p.offset, p.offset + p.npix)
+        e = rel_l2(f[sl], ref[sl])
+        print(f"forward mixed orders {mode} plane {p.nx}x{p.ny} relL2 {e:.3e}")
+        assert e <= tol
