@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -157,10 +158,13 @@ struct crs_context {
 namespace {
 
 // Built-in launch targets: work-groups per forward/adjoint launch.  256 CUs x a few
-// resident work-groups, several waves deep so the tail of uneven splits is short.
+// resident work-groups, several waves deep so the tail of uneven splits is short.  On
+// the 91809-pillar, 161 x 161 case both FP32C pair kernels measure within 2% of their
+// best between 3072 and 6144.
 int default_target(int mode, bool adjoint) {
+    (void)mode;
     (void)adjoint;
-    return mode == CRS_MODE_FP32C ? 2048 : 4096;
+    return 4096;
 }
 
 crs_status ensure(DevBuffer& b, size_t bytes, hipStream_t stream, const char* what) {
@@ -453,8 +457,8 @@ namespace {
 // Profiler name of the pair kernel a plan launches (launch_forward / launch_adjoint).
 const char* pair_kernel(int mode, int order, bool adjoint) {
     if (mode == CRS_MODE_FP32C) {
-        if (order == kExactOrder) return adjoint ? "crs::adj_fp32c<1, 1>" : "crs::fwd_fp32c<1, 1>";
-        return adjoint ? "crs::adj_fp32c<4, 4>" : "crs::fwd_fp32c<4, 4>";
+        if (order == kExactOrder) return adjoint ? "crs::adj_fp32c<1, 1>" : "crs::fwd_fp32c<1>";
+        return adjoint ? "crs::adj_fp32c<4, 4>" : "crs::fwd_fp32c<4>";
     }
     if (mode == CRS_MODE_FP64) return adjoint ? "crs::adj_fp64" : "crs::fwd_fp64";
     return adjoint ? "crs::adj_fp32" : "crs::fwd_fp32";
@@ -522,11 +526,27 @@ crs_status plan_forward(const crs_context* ctx, int mode, int npil, int nsets,
     // Split over pillar chunks: enough work-groups to fill the chip, chunks of whole
     // LDS tiles (FP32C: no longer than kMaxChunk).  FP32C work-groups take kFwdTiles
     // thread tiles of a plane, FP32 / FP64 ones kThreads pixels.
+    // FP32C tiles come in four classes (kernel: tile_count / tile_origin): 4 x 4 tiles
+    // over the largest multiple-of-4 part of the plane, then 1 x 4, 4 x 1 and 1 x 1
+    // tiles for the leftover columns, rows and corner, so no lane computes a pixel
+    // outside the plane.  With 1 x 1 blocks (tx = ty = 0) the last class is the plane.
     const int per_wg = mode == CRS_MODE_FP32C ? kFwdTiles : kThreads;
+    const int nclass = mode == CRS_MODE_FP32C ? 4 : 1;
+    std::vector<std::array<int64_t, 4>> counts(nplanes);
     int64_t base_blocks = 0;
     for (int p = 0; p < nplanes; ++p) {
-        dp[p] = make_dev_plane(planes[p], blk, blk);
-        base_blocks += ceil_div(static_cast<int64_t>(dp[p].tx) * dp[p].ty, per_wg);
+        dp[p] = make_dev_plane(planes[p], 1, 1);
+        std::array<int64_t, 4>& c = counts[p];
+        if (mode == CRS_MODE_FP32C) {
+            dp[p].tx = blk > 1 ? dp[p].nx / blk : 0;
+            dp[p].ty = blk > 1 ? dp[p].ny / blk : 0;
+            const int64_t tx = dp[p].tx, ty = dp[p].ty;
+            const int64_t rx = dp[p].nx - 4 * tx, ry = dp[p].ny - 4 * ty;
+            c = {tx * ty, rx * ty, tx * ry, rx * ry};
+        } else {
+            c = {static_cast<int64_t>(dp[p].nx) * dp[p].ny, 0, 0, 0};
+        }
+        for (int k = 0; k < nclass; ++k) base_blocks += ceil_div(c[k], per_wg);
     }
     const int target = ctx->fwd_target[mode] > 0 ? ctx->fwd_target[mode] : default_target(mode, false);
     int64_t nsplit = std::max<int64_t>(1, ceil_div(target, base_blocks));
@@ -550,11 +570,28 @@ crs_status plan_forward(const crs_context* ctx, int mode, int npil, int nsets,
         const int32_t q0 = static_cast<int32_t>(s * chunk);
         const int32_t q1 = static_cast<int32_t>(std::min<int64_t>(npil, (s + 1) * chunk));
         for (int p = 0; p < nplanes; ++p) {
-            const int64_t tiles = static_cast<int64_t>(dp[p].tx) * dp[p].ty;
             const int64_t npix = static_cast<int64_t>(dp[p].nx) * dp[p].ny;
-            for (int64_t b = 0; b < tiles; b += per_wg)
-                blocks.push_back({p, static_cast<int32_t>(b), q0, q1, dp[p].part + s * npix});
+            for (int k = 0; k < nclass; ++k)
+                for (int64_t b = 0; b < counts[p][k]; b += per_wg)
+                    blocks.push_back({p, static_cast<int32_t>(b), q0, q1, dp[p].part + s * npix,
+                                      k, 0});
         }
+    }
+    // Heaviest work-groups first.  The dispatcher hands work-groups to the 8 XCDs round
+    // robin in blockIdx order, so a run of equal heavy groups spreads evenly over the
+    // XCDs whatever the groups per split, and the light edge and short-chunk groups
+    // fill the tail instead of landing on the same XCDs split after split.  Each block
+    // names its own output slot, so the order does not change any result.
+    {
+        const int64_t pix_per_tile[4] = {int64_t(blk) * blk, blk, blk, 1};
+        auto cost = [&](const FwdBlock& B) {
+            const int64_t tiles = std::min<int64_t>(per_wg, counts[B.plane][B.shape] - B.tile0);
+            const int64_t pix = mode == CRS_MODE_FP32C ? pix_per_tile[B.shape] : 1;
+            return static_cast<int64_t>(B.q1 - B.q0) * tiles * (pix + 2);
+        };
+        std::stable_sort(blocks.begin(), blocks.end(), [&](const FwdBlock& a, const FwdBlock& b) {
+            return cost(a) > cost(b);
+        });
     }
     std::vector<PixBlock> pblocks;
     for (int p = 0; p < nplanes; ++p) {
@@ -655,7 +692,9 @@ crs_status plan_adjoint(const crs_context* ctx, int mode, int npil, int nsets,
 
     // Cut each set's quads into nsplit runs of about equal work.
     std::vector<AdjBlock> blocks;
+    std::vector<int64_t> block_cost;
     blocks.reserve(static_cast<size_t>(base_blocks * nsplit));
+    block_cost.reserve(static_cast<size_t>(base_blocks * nsplit));
     const int64_t n_out = static_cast<int64_t>(nsets) * npil;
     for (int64_t c = 0; c < nsplit; ++c) {
         for (int s = 0; s < nsets; ++s) {
@@ -668,12 +707,27 @@ crs_status plan_adjoint(const crs_context* ctx, int mode, int npil, int nsets,
                 return static_cast<int64_t>(std::lower_bound(pre.begin(), pre.end(), want) - pre.begin());
             };
             const int64_t u0 = std::min(cut(c), nq), u1 = std::min(cut(c + 1), nq);
-            for (int64_t gq = 0; gq < groups; ++gq)
+            for (int64_t gq = 0; gq < groups; ++gq) {
                 blocks.push_back({s, static_cast<int32_t>(gq * kThreads * PQ),
                                   static_cast<int32_t>(set_q0[s] + u0),
                                   static_cast<int32_t>(set_q0[s] + std::max(u0, u1)),
                                   c * n_out + static_cast<int64_t>(s) * npil});
+                const int64_t pil = std::min<int64_t>(int64_t(kThreads) * PQ, npil - gq * kThreads * PQ);
+                block_cost.push_back(pil * (pre[std::max(u0, u1)] - pre[u0]));
+            }
         }
+    }
+    // Heaviest work-groups first (quad runs hold whole quads, so runs differ by up to
+    // one quad): the XCDs take work-groups round robin in blockIdx order, and the
+    // light ones then fill the tail.  Each block names its own output slot.
+    {
+        std::vector<int32_t> order(blocks.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = static_cast<int32_t>(i);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int32_t a, int32_t b) { return block_cost[a] > block_cost[b]; });
+        std::vector<AdjBlock> sorted(blocks.size());
+        for (size_t i = 0; i < order.size(); ++i) sorted[i] = blocks[order[i]];
+        blocks.swap(sorted);
     }
     // Staging blocks walk each plane's staged records.
     std::vector<PixBlock> pblocks;

@@ -62,8 +62,8 @@ struct DevPlane {
     double z0, sz;  // z  = z0 + j*sz
     double zref;    // kturn * z0: FP32C phase reference, restored as exp(2 pi i zref)
     int32_t nx, ny;
-    int32_t tx;     // forward thread tiles across a row: ceil(nx / px_per_thread)
-    int32_t ty;     // forward thread tiles down a column: ceil(ny / py_per_thread)
+    int32_t tx;     // forward thread tiles across a row (FP32C: floor(nx / 4) 4 x 4 tiles)
+    int32_t ty;     // forward thread tiles down a column (FP32C: floor(ny / 4))
     int32_t tset;
     int32_t nslots; // pillar chunks the forward sum is split into
     int64_t offset; // first output element of the plane
@@ -73,12 +73,14 @@ struct DevPlane {
     int32_t nquads; // ceil(ny / 4)
 };
 
-// Forward work-group: thread tiles [tile0, tile0 + kFwdTiles) (FP32C) or pixels
-// [tile0, tile0 + kThreads) (FP32, FP64) of `plane`, summed over pillars [q0, q1);
-// results go to partial[part + pixel].
+// Forward work-group: thread tiles [tile0, tile0 + kFwdTiles) of class `shape`
+// (FP32C) or pixels [tile0, tile0 + kThreads) (FP32, FP64) of `plane`, summed over
+// pillars [q0, q1); results go to partial[part + pixel].
 struct FwdBlock {
     int32_t plane, tile0, q0, q1;
     int64_t part;
+    int32_t shape;  // FP32C tile class: 0 = 4 x 4, 1 = 1 x 4, 2 = 4 x 1, 3 = 1 x 1
+    int32_t pad;
 };
 
 // A run of consecutive elements of one plane: kThreads (staging) or kReduceCols

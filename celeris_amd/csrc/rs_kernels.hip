@@ -304,22 +304,44 @@ __device__ inline bool fp32c_first_order(const DevPlane& P, double2 ext, int bx,
     return Bd * Bd * Bd / (16.0 * z2) <= 1.6e-7 && 3.0 * Bd * Bd / (8.0 * z2) <= 4e-6;
 }
 
+// Pixel origin of thread tile t of a plane's tile class (shape): 0 = the 4 x 4 tiles
+// of the largest multiple-of-4 corner of the plane (tx x ty of them), 1 = 1 x 4 tiles
+// of the rx = nx - 4 tx leftover columns, 2 = 4 x 1 tiles of the ry = ny - 4 ty
+// leftover rows, 3 = 1 x 1 tiles of the leftover corner.  With tx = ty = 0 (no
+// blocks) class 3 covers the whole plane.  No tile reaches outside the plane.
+__device__ inline int tile_count(const DevPlane& P, int shape) {
+    const int rx = P.nx - 4 * P.tx, ry = P.ny - 4 * P.ty;
+    switch (shape) {
+        case 0: return P.tx * P.ty;
+        case 1: return rx * P.ty;
+        case 2: return P.tx * ry;
+        default: return rx * ry;
+    }
+}
+
+__device__ inline void tile_origin(const DevPlane& P, int shape, int t, int& ox, int& oy) {
+    const int rx = P.nx - 4 * P.tx;
+    switch (shape) {
+        case 0: ox = 4 * (t % P.tx); oy = 4 * (t / P.tx); break;
+        case 1: ox = 4 * P.tx + t % rx; oy = 4 * (t / rx); break;
+        case 2: ox = 4 * (t % P.tx); oy = 4 * P.ty + t / P.tx; break;
+        default: ox = 4 * P.tx + t % rx; oy = 4 * P.ty + t / rx; break;
+    }
+}
+
+// One work-group's pixel tiles of one class, all BX x BY, against its pillar chunk.
 template <int BX, int BY>
-__global__ __launch_bounds__(kThreads, 4) void fwd_fp32c(
+__device__ __forceinline__ void fwd_tiles(
     const double2* __restrict__ pos, const float2* __restrict__ ap, int npil,
-    const DevPlane* __restrict__ planes, const FwdBlock* __restrict__ blocks,
-    const double2* __restrict__ ext, double2* __restrict__ partial) {
+    const DevPlane& P, const FwdBlock& B, const double2* __restrict__ ext,
+    PillarRec* tile, float2* run, double2* __restrict__ partial) {
     constexpr int N = BX * BY;
-    __shared__ PillarRec tile[kThreads];
-    __shared__ float2 run[kWaves * N * 64];
-    const FwdBlock B = blocks[blockIdx.x];
-    const DevPlane& P = planes[B.plane];
-    const int nx = P.nx, ny = P.ny;
+    const int nx = P.nx;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int my_tile = B.tile0 + lane;
-    const bool active = my_tile < P.tx * P.ty;
-    const int tt = active ? my_tile : 0;
-    const int ti = tt % P.tx, tj = tt / P.tx;
+    const bool active = my_tile < tile_count(P, B.shape);
+    int ox, oy;
+    tile_origin(P, B.shape, active ? my_tile : B.tile0, ox, oy);
     const double kt = P.kturn;
     const double zref = P.zref;
     // Uniform across the work-group; readfirstlane makes the branch on it scalar.
@@ -328,7 +350,7 @@ __global__ __launch_bounds__(kThreads, 4) void fwd_fp32c(
 
     BlockGeom<BX, BY> G;
     {
-        const double ic = ti * BX + (BX - 1) * 0.5, jc = tj * BY + (BY - 1) * 0.5;
+        const double ic = ox + (BX - 1) * 0.5, jc = oy + (BY - 1) * 0.5;
         G.FXc = kt * (P.x0 + ic * P.sx);
         G.FYc = kt * (P.y0 + jc * P.sy);
         const double zc = kt * (P.z0 + jc * P.sz);
@@ -342,7 +364,7 @@ __global__ __launch_bounds__(kThreads, 4) void fwd_fp32c(
 #pragma unroll
         for (int r = 0; r < BY; ++r) {
             const float e = static_cast<float>(kt * P.sy * (r - (BY - 1) * 0.5));
-            const double zr = kt * (P.z0 + (tj * BY + r) * P.sz);
+            const double zr = kt * (P.z0 + (oy + r) * P.sz);
             G.tey[r] = 2.0f * e;
             G.eyz[r] = e * e + static_cast<float>(zr * zr - G.ZC2);
         }
@@ -393,12 +415,8 @@ __global__ __launch_bounds__(kThreads, 4) void fwd_fp32c(
     if (wave != 0 || !active) return;
 #pragma unroll
     for (int r = 0; r < BY; ++r) {
-        const int j = tj * BY + r;
-        if (j >= ny) break;
 #pragma unroll
         for (int c = 0; c < BX; ++c) {
-            const int i = ti * BX + c;
-            if (i >= nx) break;
             const int e = r * BX + c;
             double vr = 0.0, vi = 0.0;
 #pragma unroll
@@ -407,8 +425,33 @@ __global__ __launch_bounds__(kThreads, 4) void fwd_fp32c(
                 vr += o.x;
                 vi += o.y;
             }
-            partial[B.part + static_cast<int64_t>(j) * nx + i] = make_double2(vr, vi);
+            partial[B.part + static_cast<int64_t>(oy + r) * nx + (ox + c)] = make_double2(vr, vi);
         }
+    }
+}
+
+// BLK = 4: 4 x 4 tiles plus the edge classes (1 x 4, 4 x 1, 1 x 1), chosen per
+// work-group from its descriptor; BLK = 1: 1 x 1 tiles only (every pair at its own
+// FP64 reference distance).
+template <int BLK>
+__global__ __launch_bounds__(kThreads, 4) void fwd_fp32c(
+    const double2* __restrict__ pos, const float2* __restrict__ ap, int npil,
+    const DevPlane* __restrict__ planes, const FwdBlock* __restrict__ blocks,
+    const double2* __restrict__ ext, double2* __restrict__ partial) {
+    constexpr int NMAX = BLK * BLK;
+    __shared__ PillarRec tile[kThreads];
+    __shared__ float2 run[kWaves * NMAX * 64];
+    const FwdBlock B = blocks[blockIdx.x];
+    const DevPlane& P = planes[B.plane];
+    if constexpr (BLK == kBlk) {
+        switch (B.shape) {
+            case 0: fwd_tiles<kBlk, kBlk>(pos, ap, npil, P, B, ext, tile, run, partial); break;
+            case 1: fwd_tiles<1, kBlk>(pos, ap, npil, P, B, ext, tile, run, partial); break;
+            case 2: fwd_tiles<kBlk, 1>(pos, ap, npil, P, B, ext, tile, run, partial); break;
+            default: fwd_tiles<1, 1>(pos, ap, npil, P, B, ext, tile, run, partial); break;
+        }
+    } else {
+        fwd_tiles<1, 1>(pos, ap, npil, P, B, ext, tile, run, partial);
     }
 }
 
@@ -1005,9 +1048,9 @@ hipError_t launch_forward(int mode, int order, const void* pil, int npil, int ns
         // order kExactOrder: 1 x 1 blocks (every pair at its own reference distance);
         // otherwise 4 x 4 blocks, series order chosen per work-group from ext.
         if (order == kExactOrder)
-            fwd_fp32c<1, 1><<<nblocks, kThreads, 0, s>>>(pos, ap, npil, planes, blocks, ext, partial);
+            fwd_fp32c<1><<<nblocks, kThreads, 0, s>>>(pos, ap, npil, planes, blocks, ext, partial);
         else
-            fwd_fp32c<kBlk, kBlk><<<nblocks, kThreads, 0, s>>>(pos, ap, npil, planes, blocks, ext, partial);
+            fwd_fp32c<kBlk><<<nblocks, kThreads, 0, s>>>(pos, ap, npil, planes, blocks, ext, partial);
     } else if (mode == 2) {
         fwd_fp32<<<nblocks, kThreads, 0, s>>>(static_cast<const float4*>(pil), npil, planes,
                                               blocks, partial);

@@ -47,7 +47,7 @@ def _store_cache(cache: Dict[str, dict]) -> None:
 
 
 def autotune(ctx: Context, px, py, t, planes: Sequence[Plane], u=None, mode: str = "fp32c",
-             candidates: Sequence[int] = CANDIDATES, reps: int = 3,
+             candidates: Sequence[int] = CANDIDATES, reps: int = 10,
              use_cache: bool = True) -> dict:
     """Time the forward (and, if a cotangent u is given, the adjoint) of this shape for
     each candidate work-group target, install the fastest on ctx, and return
@@ -66,7 +66,7 @@ def autotune(ctx: Context, px, py, t, planes: Sequence[Plane], u=None, mode: str
     fwd_ms, adj_ms = {}, {}
 
     def timed(fn):
-        fn()  # plan + warm
+        fn()  # plan
         torch.cuda.synchronize()
         a = torch.cuda.Event(enable_timing=True)
         b = torch.cuda.Event(enable_timing=True)
@@ -77,6 +77,12 @@ def autotune(ctx: Context, px, py, t, planes: Sequence[Plane], u=None, mode: str
         b.synchronize()
         return a.elapsed_time(b) / reps
 
+    # The clocks ramp up over the first few milliseconds of work; without a warm-up the
+    # first candidate reads several percent slow.
+    ctx.set_tuning(mode, candidates[0], candidates[0])
+    for _ in range(20):
+        ctx.forward(px, py, t, planes, mode=mode)
+    torch.cuda.synchronize()
     for cand in candidates:
         ctx.set_tuning(mode, cand, cand)
         fwd_ms[cand] = timed(lambda: ctx.forward(px, py, t, planes, mode=mode))

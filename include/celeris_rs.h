@@ -262,7 +262,7 @@ const char *crs_kernel_name(int mode, int adjoint);
 
 /* The pair-sum kernel the last successful crs_forward (adjoint = 0) or crs_adjoint
  * (adjoint = 1) on ctx launched: its name as a profiler lists it (demangled, without
- * the argument list, e.g. "crs::fwd_fp32c<4, 4>") and its number of work-groups.
+ * the argument list, e.g. "crs::fwd_fp32c<4>") and its number of work-groups.
  * CRS_ERR_INVALID before the first such call. */
 crs_status crs_last_launch(crs_context *ctx, int adjoint, const char **kernel,
                            int *workgroups);

@@ -251,3 +251,26 @@ def test_forward_general_and_first_order_planes(torch_cuda, rsctx, mode, tol):
         e = rel_l2(f[sl], ref[sl])
         print(f"forward mixed orders {mode} plane {p.nx}x{p.ny} relL2 {e:.3e}")
         assert e <= tol
+
+
+@pytest.mark.parametrize("mode,tol", [("fp64", 1e-12), ("fp32c", 1e-5)])
+def test_forward_ragged_edges_blocked(torch_cuda, rsctx, mode, tol):
+    """4 x 4 blocks with every kind of leftover: nx, ny = 4k + 1, 2, 3 (the 1 x 4, 4 x 1
+    and 1 x 1 edge tiles), rectangular planes, a plane narrower than one block."""
+    L = configs.config1()
+    k = L.planes[0].k
+    planes = [
+        Plane.psf(0.0, 0.0, 50.0, k, 42, 1.0),                           # 42 = 4*10 + 2
+        Plane(k, -1.1, 0.05, -0.9, 0.05, 50.0, 0.0, 43, 41),             # 43 = 4*10 + 3, 41 = 4*10 + 1
+        Plane(k, 0.3, 0.05, -0.5, 0.05, 49.0, 0.0, 3, 22),               # narrower than a block
+        Plane(k, -0.4, 0.04, 0.2, 0.04, 51.0, 0.0, 17, 2),               # shorter than a block
+    ]
+    ref = oracle.forward(L.px, L.py, L.t, planes)
+    f, _ = gpu_forward(torch_cuda, rsctx, L.px, L.py, L.t, planes, mode)
+    for p in crs.resolve_offsets(planes):
+        sl = slice(p.offset, p.offset + p.npix)
+        e = rel_l2(f[sl], ref[sl])
+        print(f"ragged {mode} {p.nx}x{p.ny} relL2 {e:.3e}")
+        assert e <= tol
+    if mode == "fp32c":
+        assert rsctx.last_launch(False)[0] == "crs::fwd_fp32c<4>"
