@@ -48,6 +48,22 @@ class crs_plane(ctypes.Structure):
 
 assert ctypes.sizeof(crs_plane) == 7 * 8 + 4 * 4 + 8
 
+
+class crs_axis_scan_desc(ctypes.Structure):
+    """Mirror of ``crs_axis_scan_desc`` (include/celeris_rs.h)."""
+
+    _fields_ = [
+        ("tset", c_int32),
+        ("nz", c_int32),
+        ("wavelength_um", c_double),
+        ("zlo", c_double),
+        ("zspan", c_double),
+        ("z_default", c_double),
+    ]
+
+
+assert ctypes.sizeof(crs_axis_scan_desc) == 2 * 4 + 4 * 8
+
 _DP = POINTER(c_double)
 
 # (name, restype, argtypes) for every entry point of include/celeris_rs.h.
@@ -95,6 +111,12 @@ SIGNATURES = [
     ("crs_analyze_mtf", c_int,
      [_DP, _DP, c_int, c_double, c_double, c_double, c_double, c_int, POINTER(ctypes.c_float),
       POINTER(ctypes.c_float), POINTER(ctypes.c_float), _DP, POINTER(c_int)]),
+    ("crs_axis_scan", c_int,
+     [_DP, _DP, _DP, c_int, c_int, POINTER(crs_axis_scan_desc), c_int, c_int, _DP, _DP, _DP]),
+    ("crs_verify_achromatic_focus", c_int,
+     [_DP, _DP, _DP, c_int, c_int, _DP, c_int, c_double, c_double, c_int, _DP]),
+    ("crs_polar_focus_check", c_int,
+     [_DP, _DP, _DP, _DP, c_int, c_double, c_double, c_double, c_int, c_int, _DP]),
     ("crs_set_timing", c_int, [c_void_p, c_int]),
     ("crs_last_kernel_ms", c_int, [c_void_p, c_int, POINTER(ctypes.c_float)]),
     ("crs_kernel_name", c_char_p, [c_int, c_int]),

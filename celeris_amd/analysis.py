@@ -13,6 +13,13 @@ module (src/celeris/analysis/*.hpp; bindings/python/celeris_module.cpp:415-440):
 * :func:`analyze_field_of_view` field.cpp:12-76       -> list of :class:`FieldPoint`
 * :func:`analyze_wide_fov`      field.cpp:78-148      -> list of :class:`FieldPoint`
 * :func:`analyze_mtf`           mtf.cpp:10-63         -> :class:`MtfCurve`
+* :func:`verify_achromatic_focus`    design/achromatic.cpp:288-338    -> list of
+  :class:`AchromaticFocalPoint`; :func:`verify_pb_achromatic_focus`
+  design/pb_achromatic.cpp:255-301 (same)
+* :func:`polar_focus_check`     cli/cmd_polar.cpp:85-126, gui/workers.cpp:181-197
+  -> :class:`PolarFocus`
+* :func:`profile_focus_z`       cli/cli_common.cpp:112-133 (focusing profiles)
+* :func:`axis_scan`             the batched on-axis scan under the last three
 
 Each runs its sums in one launch through the C ABI (crs_analyze_*, crs_forward_host);
 where the reference looped on the CPU (on-axis scans, the ideal peak) the sums are on
@@ -28,7 +35,9 @@ from typing import List
 import numpy as np
 
 from . import _lib
-from .design import MetalensDesign, UnitCellLibrary, aperture_pillars, exit_pupil
+from .design import (AchromaticDesign, DispersiveLibrary, DispersivePbLibrary, MetalensDesign,
+                     PbAchromaticDesign, UnitCellLibrary, achromatic_band_transmissions,
+                     aperture_pillars, exit_pupil, pb_achromatic_band_transmissions)
 from .rs import _mode, crs_plane
 
 _DP = ctypes.POINTER(ctypes.c_double)
@@ -321,3 +330,104 @@ def analyze_wide_fov(lens: MetalensDesign, lib: UnitCellLibrary, focal_length_um
                                             _p(out)),
            "crs_analyze_wide_fov")
     return [FieldPoint(*map(float, out[3 * i:3 * i + 3])) for i in range(ang.size)]
+
+
+# ---- on-axis focus scans ----------------------------------------------------------------
+
+@dataclass
+class AchromaticFocalPoint:
+    """achromatic.hpp:150-154: best on-axis focus per band, peak relative to the design
+    point (the centre wavelength at the design focus)."""
+
+    wavelength_um: float
+    focal_length_um: float
+    rel_peak: float
+
+
+@dataclass
+class PolarFocus:
+    """The polarisation-multiplexed lens check (cmd_polar.cpp:85-126): each channel's
+    on-axis focus and its focal isolation (dB) over the other channel there."""
+
+    z_x_um: float
+    z_y_um: float
+    isolation_x_db: float
+    isolation_y_db: float
+
+
+def axis_scan(px, py, t_sets, scans, mode="fp32c"):
+    """Batched on-axis scans (crs_axis_scan).  t_sets: [nsets, npil] complex;
+    scans: (tset, nz, wavelength_um, zlo, zspan, z_default) each, samples
+    z_j = zlo + zspan j / (nz - 1).  Returns (best_z, best_I, curves) with the callers'
+    first-maximum rule."""
+    px, py = _d(px), _d(py)
+    t = _c(np.atleast_2d(t_sets))
+    descs = (_lib.crs_axis_scan_desc * len(scans))()
+    total = 0
+    for i, (tset, nz, lam, zlo, zspan, zdef) in enumerate(scans):
+        descs[i] = _lib.crs_axis_scan_desc(int(tset), int(nz), float(lam), float(zlo),
+                                           float(zspan), float(zdef))
+        total += int(nz)
+    bz, bi, curves = np.zeros(len(scans)), np.zeros(len(scans)), np.zeros(total)
+    _check(_lib.load().crs_axis_scan(_p(px), _p(py), _p(t), px.size, t.shape[0], descs,
+                                     len(scans), _mode(mode), _p(bz), _p(bi), _p(curves)),
+           "crs_axis_scan")
+    return bz, bi, curves
+
+
+def verify_achromatic_focus_bands(px, py, t_bands, wavelengths_um, center_index: int,
+                                  center_wavelength_um: float, focal_length_um: float,
+                                  mode="fp32c") -> List[AchromaticFocalPoint]:
+    """The focus check shared by verify_achromatic_focus / verify_pb_achromatic_focus on
+    the aperture cells' per-band transmissions t_bands [nb, npil]."""
+    px, py = _d(px), _d(py)
+    t = _c(np.atleast_2d(t_bands))
+    lam = _d(wavelengths_um)
+    nb = lam.size
+    if t.shape != (nb, px.size):
+        raise ValueError("t_bands must be [len(wavelengths), npil]")
+    out = np.zeros(3 * nb)
+    _check(_lib.load().crs_verify_achromatic_focus(_p(px), _p(py), _p(t), px.size, nb, _p(lam),
+                                                   int(center_index), float(center_wavelength_um),
+                                                   float(focal_length_um), _mode(mode), _p(out)),
+           "crs_verify_achromatic_focus")
+    return [AchromaticFocalPoint(*map(float, out[3 * j:3 * j + 3])) for j in range(nb)]
+
+
+def verify_achromatic_focus(lib: DispersiveLibrary, design: AchromaticDesign,
+                            mode="fp32c") -> List[AchromaticFocalPoint]:
+    """achromatic.cpp:288-338: per band, the on-axis focus over 200 samples in
+    [0.4 f, 1.8 f] and its peak relative to the design point."""
+    px, py, t = achromatic_band_transmissions(lib, design)
+    return verify_achromatic_focus_bands(px, py, t, lib.wavelengths_um, lib.center_index,
+                                         lib.center_wavelength_um, design.focal_length_um, mode)
+
+
+def verify_pb_achromatic_focus(lib: DispersivePbLibrary, design: PbAchromaticDesign,
+                               mode="fp32c") -> List[AchromaticFocalPoint]:
+    """pb_achromatic.cpp:255-301: as verify_achromatic_focus with the cross-circular
+    transmission a_cross(atom) exp(i phi_geo) of each cell."""
+    px, py, t = pb_achromatic_band_transmissions(lib, design)
+    return verify_achromatic_focus_bands(px, py, t, lib.wavelengths_um, lib.center_index,
+                                         lib.center_wavelength_um, design.focal_length_um, mode)
+
+
+def polar_focus_check(px, py, tx, ty, wavelength_um: float, focal_x_um: float,
+                      focal_y_um: float, nz: int = 240, mode="fp32c") -> PolarFocus:
+    """cmd_polar.cpp:85-126 (nz = 240; gui/workers.cpp:181-197 uses nz = 200)."""
+    px, py, tx, ty = _d(px), _d(py), _c(tx), _c(ty)
+    out = np.zeros(4)
+    _check(_lib.load().crs_polar_focus_check(_p(px), _p(py), _p(tx), _p(ty), px.size,
+                                             float(wavelength_um), float(focal_x_um),
+                                             float(focal_y_um), int(nz), _mode(mode), _p(out)),
+           "crs_polar_focus_check")
+    return PolarFocus(*map(float, out))
+
+
+def profile_focus_z(px, py, t, wavelength_um: float, focal_um: float, nz: int = 240,
+                    mode="fp32c") -> float:
+    """The focusing-profile proof of cli_common.cpp:112-133: the on-axis focus over
+    z = 0.5 f + f j / (nz - 1), j < nz."""
+    bz, _, _ = axis_scan(px, py, np.atleast_2d(_c(t)),
+                         [(0, nz, wavelength_um, 0.5 * focal_um, focal_um, 0.5 * focal_um)], mode)
+    return float(bz[0])

@@ -7,6 +7,9 @@
 //   crs_analyze_field_grid     analysis/field.cpp:150-233
 //   crs_analyze_field_of_view  analysis/field.cpp:12-76
 //   crs_analyze_wide_fov       analysis/field.cpp:78-148
+//   crs_axis_scan, crs_verify_achromatic_focus, crs_polar_focus_check: the on-axis
+//                              focus scans of design/achromatic.cpp, design/pb_achromatic.cpp,
+//                              cli/cmd_polar.cpp, cli/cli_common.cpp, gui/workers.cpp
 //
 // Inputs are the aperture pillar list the reference builds from a MetalensDesign and
 // a UnitCellLibrary (focal.cpp:42-57): positions, complex transmissions and, for the
@@ -439,6 +442,127 @@ extern "C" crs_status crs_analyze_wide_fov(const double* px, const double* py, c
         out3[3 * a] = angles_deg[a];
         out3[3 * a + 1] = pk[0] > 0 ? pk[a + 1] / pk[0] : 0.0;
         out3[3 * a + 2] = pkx[a + 1];
+    }
+    return CRS_OK;
+}
+
+// ---- on-axis focus scans ---------------------------------------------------------------
+//   crs_axis_scan                the scan loop of the callers below, batched
+//   crs_verify_achromatic_focus  design/achromatic.cpp:288-338, design/pb_achromatic.cpp:255-301
+//   crs_polar_focus_check        cli/cmd_polar.cpp:85-126, gui/workers.cpp:181-197
+
+namespace {
+
+// Every scan as a one-column plane of one launch; I receives the samples scan after scan.
+crs_status run_axis_scans(const double* px, const double* py, const double* t, int npil, int nsets,
+                          const crs_axis_scan_desc* scans, int nscans, int mode,
+                          std::vector<double>& I) {
+    std::vector<crs_plane> planes(nscans);
+    int64_t total = 0;
+    for (int s = 0; s < nscans; ++s) {
+        const crs_axis_scan_desc& d = scans[s];
+        if (d.nz < 1 || d.tset < 0 || d.tset >= nsets || !(d.wavelength_um > 0.0))
+            return CRS_ERR_INVALID;
+        planes[s] = axis_plane(d.zlo, d.zlo + d.zspan, d.nz, 2.0 * kPi / d.wavelength_um, d.tset,
+                               total);
+        if (d.nz > 1) planes[s].sz = d.zspan / (d.nz - 1);
+        total += d.nz;
+    }
+    if (total > (int64_t(1) << 31)) return CRS_ERR_INVALID;
+    I.assign(static_cast<size_t>(total), 0.0);
+    return crs_forward_host(mode, px, py, t, npil, nsets, planes.data(), nscans, nullptr, I.data());
+}
+
+// The callers' argmax: the first sample beating every earlier one and -1.
+void scan_argmax(const crs_axis_scan_desc& d, const double* I, double* best_z, double* best_I,
+                 int* best_j) {
+    double bz = d.z_default, bi = -1.0;
+    int bj = -1;
+    for (int j = 0; j < d.nz; ++j) {
+        const double z = d.nz > 1 ? d.zlo + d.zspan * j / (d.nz - 1) : d.zlo;
+        if (I[j] > bi) { bi = I[j]; bz = z; bj = j; }
+    }
+    *best_z = bz;
+    *best_I = bi;
+    if (best_j) *best_j = bj;
+}
+
+}  // namespace
+
+extern "C" crs_status crs_axis_scan(const double* px, const double* py, const double* t, int npil,
+                                    int nsets, const crs_axis_scan_desc* scans, int nscans,
+                                    int mode, double* best_z, double* best_I, double* curves) {
+    if (!px || !py || !t || !scans || !best_z || !best_I || npil <= 0 || nsets <= 0 || nscans <= 0)
+        return CRS_ERR_INVALID;
+    std::vector<double> I;
+    crs_status st = run_axis_scans(px, py, t, npil, nsets, scans, nscans, mode, I);
+    if (st != CRS_OK) return st;
+    int64_t off = 0;
+    for (int s = 0; s < nscans; ++s) {
+        scan_argmax(scans[s], I.data() + off, best_z + s, best_I + s, nullptr);
+        off += scans[s].nz;
+    }
+    if (curves) std::copy(I.begin(), I.end(), curves);
+    return CRS_OK;
+}
+
+extern "C" crs_status crs_verify_achromatic_focus(const double* px, const double* py,
+                                                  const double* t_bands, int npil, int nb,
+                                                  const double* wavelengths_um, int center_index,
+                                                  double center_wavelength_um, double focal_um,
+                                                  int mode, double* out3nb) {
+    if (!px || !py || !t_bands || !wavelengths_um || !out3nb || npil <= 0 || nb <= 0 ||
+        center_index < 0 || center_index >= nb)
+        return CRS_ERR_INVALID;
+    const int NZ = 200;  // achromatic.cpp:322, pb_achromatic.cpp:289
+    const double zlo = 0.4 * focal_um, zhi = 1.8 * focal_um;
+    // Scan 0: the design peak, I(f) of the centre band at the centre wavelength
+    // (achromatic.cpp:311-313); then one 200-sample scan per band.
+    std::vector<crs_axis_scan_desc> scans(1 + static_cast<size_t>(nb));
+    scans[0] = {center_index, 1, center_wavelength_um, focal_um, 0.0, focal_um};
+    for (int j = 0; j < nb; ++j)
+        scans[1 + j] = {j, NZ, wavelengths_um[j], zlo, zhi - zlo, focal_um};
+    std::vector<double> I;
+    crs_status st = run_axis_scans(px, py, t_bands, npil, nb, scans.data(), 1 + nb, mode, I);
+    if (st != CRS_OK) return st;
+    const double design_peak = I[0];
+    for (int j = 0; j < nb; ++j) {
+        double bz, bi;
+        scan_argmax(scans[1 + j], I.data() + 1 + static_cast<size_t>(j) * NZ, &bz, &bi, nullptr);
+        out3nb[3 * j] = wavelengths_um[j];
+        out3nb[3 * j + 1] = bz;
+        out3nb[3 * j + 2] = design_peak > 0 ? bi / design_peak : 0.0;
+    }
+    return CRS_OK;
+}
+
+extern "C" crs_status crs_polar_focus_check(const double* px, const double* py, const double* tx,
+                                            const double* ty, int npil, double wavelength_um,
+                                            double focal_x_um, double focal_y_um, int nz, int mode,
+                                            double* out4) {
+    if (!px || !py || !tx || !ty || !out4 || npil <= 0 || nz < 1) return CRS_ERR_INVALID;
+    const double zlo = 0.5 * std::min(focal_x_um, focal_y_um);
+    const double zhi = 1.5 * std::max(focal_x_um, focal_y_um);
+    std::vector<double> t(4 * static_cast<size_t>(npil));
+    std::copy(tx, tx + 2 * static_cast<size_t>(npil), t.begin());
+    std::copy(ty, ty + 2 * static_cast<size_t>(npil), t.begin() + 2 * static_cast<size_t>(npil));
+    const crs_axis_scan_desc scans[2] = {{0, nz, wavelength_um, zlo, zhi - zlo, zlo},
+                                         {1, nz, wavelength_um, zlo, zhi - zlo, zlo}};
+    std::vector<double> I;
+    crs_status st = run_axis_scans(px, py, t.data(), npil, 2, scans, 2, mode, I);
+    if (st != CRS_OK) return st;
+    const double* Ix = I.data();
+    const double* Iy = I.data() + nz;
+    double bi;
+    int jx, jy;
+    scan_argmax(scans[0], Ix, &out4[0], &bi, &jx);
+    scan_argmax(scans[1], Iy, &out4[1], &bi, &jy);
+    out4[2] = out4[3] = 0.0;
+    if (std::abs(focal_x_um - focal_y_um) > 1e-6 && jx >= 0 && jy >= 0) {
+        // Both channels are sampled on the same z grid, so the other channel's intensity
+        // at a channel's focus is its sample there (cmd_polar.cpp:113-123 recompute it).
+        out4[2] = 10.0 * std::log10(Ix[jx] / std::max(Iy[jx], 1e-30));
+        out4[3] = 10.0 * std::log10(Iy[jy] / std::max(Ix[jy], 1e-30));
     }
     return CRS_OK;
 }

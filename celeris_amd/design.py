@@ -95,3 +95,103 @@ def exit_pupil(lens: MetalensDesign, lib: UnitCellLibrary, focal_length_um: floa
     rot = np.cos(-phi) + 1j * np.sin(-phi)
     P = np.where(inside, t * rot, 0.0 + 0.0j)
     return P.astype(np.complex128), inside.astype(np.float64)
+
+
+def aperture_cells(n_cells: int, period_um: float, diameter_um: float):
+    """Cells of an n_cells^2 lattice inside the circular aperture, in row-major order:
+    (x, y, site index) with x = (ix - c) p, y = (iy - c) p, c = (n_cells - 1)/2, kept
+    unless sqrt(x^2 + y^2) > D/2 (achromatic.cpp:294-303, pb_achromatic.cpp:263-272)."""
+    c = (n_cells - 1) / 2.0
+    iy, ix = np.meshgrid(np.arange(n_cells), np.arange(n_cells), indexing="ij")
+    x = ((ix - c) * period_um).ravel()
+    y = ((iy - c) * period_um).ravel()
+    keep = ~(np.sqrt(x * x + y * y) > diameter_um / 2.0)
+    return x[keep].copy(), y[keep].copy(), np.flatnonzero(keep)
+
+
+# ---- dispersive (achromatic) designs: the fields their focus checks read -----------------
+
+@dataclass
+class DispersiveAtom:
+    """One meta-atom across the band (achromatic.hpp:38-49): |t| and phase per band
+    sample.  The RCWA sweep that measures them stays outside this package."""
+
+    amplitude: np.ndarray
+    phase_rad: np.ndarray
+
+
+@dataclass
+class DispersiveLibrary:
+    """achromatic.hpp:65-73: band samples (ascending), the centre sample, the atoms."""
+
+    wavelengths_um: np.ndarray
+    center_wavelength_um: float
+    center_index: int
+    atoms: list
+    period_um: float = 0.0
+
+
+@dataclass
+class AchromaticDesign:
+    """achromatic.hpp:112-130: per-site atom choice on an n_cells^2 lattice (row-major)."""
+
+    n_cells: int
+    period_um: float
+    atom_index: np.ndarray
+    focal_length_um: float
+    diameter_um: float
+    center_wavelength_um: float = 0.0
+
+
+@dataclass
+class DispersivePbAtom:
+    """pb_achromatic.hpp:43-51: the spin-flip (cross-circular) transmission a_cross per
+    band sample, complex."""
+
+    cross: np.ndarray
+
+
+@dataclass
+class DispersivePbLibrary:
+    """pb_achromatic.hpp:53-62."""
+
+    wavelengths_um: np.ndarray
+    center_wavelength_um: float
+    center_index: int
+    atoms: list
+    period_um: float = 0.0
+
+
+@dataclass
+class PbAchromaticDesign:
+    """pb_achromatic.hpp:72-92: per-site atom and geometric-phase rotation theta; the
+    realised transmission is a_cross(atom) exp(i phi_geo), phi_geo = -2 handedness theta."""
+
+    n_cells: int
+    period_um: float
+    atom_index: np.ndarray
+    rotation_rad: np.ndarray
+    focal_length_um: float
+    diameter_um: float
+    handedness: int = 1
+    center_wavelength_um: float = 0.0
+
+
+def achromatic_band_transmissions(lib: DispersiveLibrary, design: AchromaticDesign):
+    """Aperture cells and t[j, cell] = polar(amplitude[j], phase[j]) of the cell's atom
+    (achromatic.cpp:294-309).  Returns px, py, t [nb, ncells]."""
+    x, y, site = aperture_cells(design.n_cells, design.period_um, design.diameter_um)
+    atom = np.asarray(design.atom_index).ravel()[site]
+    amp = np.stack([np.asarray(a.amplitude, np.float64) for a in lib.atoms])[atom].T
+    ph = np.stack([np.asarray(a.phase_rad, np.float64) for a in lib.atoms])[atom].T
+    return x, y, amp * np.cos(ph) + 1j * (amp * np.sin(ph))
+
+
+def pb_achromatic_band_transmissions(lib: DispersivePbLibrary, design: PbAchromaticDesign):
+    """Aperture cells and t[j, cell] = a_cross[j](atom) polar(1, phi_geo),
+    phi_geo = -2 handedness theta (pb_achromatic.cpp:263-282).  Returns px, py, t."""
+    x, y, site = aperture_cells(design.n_cells, design.period_um, design.diameter_um)
+    atom = np.asarray(design.atom_index).ravel()[site]
+    cross = np.stack([np.asarray(a.cross, np.complex128) for a in lib.atoms])[atom].T
+    phi_geo = -2.0 * design.handedness * np.asarray(design.rotation_rad, np.float64).ravel()[site]
+    return x, y, cross * (np.cos(phi_geo) + 1j * np.sin(phi_geo))

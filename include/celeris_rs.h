@@ -227,6 +227,62 @@ crs_status crs_analyze_mtf(const double *pupil, const double *pupil0, int n, dou
                            float *freq_cyc_mm, float *mtf, float *mtf_ideal,
                            double *cutoff_cyc_mm, int *npoints);
 
+/* ---- on-axis focus scans -------------------------------------------------------------
+ * The reference's design checks find a lens's focus by scanning the on-axis intensity
+ * I(z) = |sum_q t_q exp(i k R_q) / R_q|^2, R_q = sqrt(x_q^2 + y_q^2 + z^2), over z and
+ * taking the first maximum, each in an FP64 CPU loop: verify_achromatic_focus
+ * (design/achromatic.cpp:288-338), verify_pb_achromatic_focus
+ * (design/pb_achromatic.cpp:255-301), the polarisation check (cli/cmd_polar.cpp:85-126,
+ * gui/workers.cpp:181-197) and the profile proof (cli/cli_common.cpp:112-133).  Here
+ * every scan of a call is one plane of one batched launch. */
+
+/* One scan: transmission set `tset` (0 .. nsets-1) at wavelength_um (k = 2 pi /
+ * wavelength_um), nz >= 1 samples z_j = zlo + zspan * j / (nz - 1) (z_0 = zlo when
+ * nz == 1), the callers' own expression: zspan = zhi - zlo for the verify and polarisation
+ * scans, zspan = f for the profile proof.  best_z is the first sample whose I beats every
+ * earlier one and -1 (the reference's `I > best_I` from best_I = -1); z_default when
+ * none does (every I is NaN). */
+typedef struct crs_axis_scan_desc {
+    int tset;
+    int nz;
+    double wavelength_um;
+    double zlo, zspan;
+    double z_default;
+} crs_axis_scan_desc;
+
+/* Batched on-axis scans (no single reference counterpart: the loop the callers above
+ * share).  t: nsets x npil complex transmissions (interleaved re/im, set-major).
+ * best_z[s], best_I[s] per scan; curves (optional) receives every I, scan after scan
+ * (sum of nz).  Synchronous. */
+crs_status crs_axis_scan(const double *px, const double *py, const double *t, int npil,
+                         int nsets, const crs_axis_scan_desc *scans, int nscans, int mode,
+                         double *best_z, double *best_I, double *curves);
+
+/* verify_achromatic_focus (design/achromatic.cpp:288-338) and
+ * verify_pb_achromatic_focus (design/pb_achromatic.cpp:255-301), which differ only in
+ * how each band's transmissions are built (the caller builds them: polar(amplitude,
+ * phase) of the site's atom, or its cross transmission times exp(i phi_geo)).
+ * t_bands: nb x npil complex (the aperture cells, band-major).  For each band j: 200
+ * on-axis samples over [0.4 f, 1.8 f] at wavelengths_um[j]; out3nb = nb x {lambda, best
+ * z, peak / design peak}, the design peak being I(f) of band center_index at
+ * center_wavelength_um (rel = 0 when that peak is not positive). */
+crs_status crs_verify_achromatic_focus(const double *px, const double *py,
+                                       const double *t_bands, int npil, int nb,
+                                       const double *wavelengths_um, int center_index,
+                                       double center_wavelength_um, double focal_um, int mode,
+                                       double *out3nb);
+
+/* Polarisation-multiplexed focus check (cli/cmd_polar.cpp:85-126 with nz = 240,
+ * gui/workers.cpp:181-197 with nz = 200): tx, ty the X and Y channel transmissions of
+ * the aperture cells; each channel's on-axis focus over [0.5 min(fx, fy), 1.5 max(fx,
+ * fy)], then the focal isolation 10 log10(I_own(z) / max(I_other(z), 1e-30)) at each
+ * channel's focus.  out4 = {z_x, z_y, iso_x_db, iso_y_db}; the isolations are 0 when
+ * |fx - fy| <= 1e-6 (the reference prints none then). */
+crs_status crs_polar_focus_check(const double *px, const double *py, const double *tx,
+                                 const double *ty, int npil, double wavelength_um,
+                                 double focal_x_um, double focal_y_um, int nz, int mode,
+                                 double *out4);
+
 /* Launch-shape tuning (no reference counterpart).  target_blocks: work-groups a
  * forward/adjoint launch aims for through its split over pillar chunks (forward)
  * or pixel rows (adjoint); 0 restores the built-in default.  Applies to the

@@ -13,6 +13,7 @@ never imports it.
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 import subprocess
 import tempfile
@@ -127,6 +128,80 @@ def forward_ld(px, py, t, planes):
 
 def num_cpus() -> int:
     return int(load().orc_num_cpus())
+
+
+class _AxisPoint:
+    """One on-axis sample as a 1 x 1 plane at exactly z (the scans' own z values)."""
+
+    def __init__(self, k, z, tset):
+        self.k, self.x0, self.sx, self.y0, self.sy = k, 0.0, 0.0, 0.0, 0.0
+        self.z0, self.sz, self.nx, self.ny, self.tset, self.offset = z, 0.0, 1, 1, tset, -1
+
+
+def axis_intensity(px, py, t_sets, points):
+    """I(z) = |sum_q t_q polar(1/R, kR)|^2 at (0, 0, z) for points = [(k, z, tset)], the
+    on-axis sums of achromatic.cpp:306-315, pb_achromatic.cpp:275-284,
+    cmd_polar.cpp:88-100 / :113-119 and cli_common.cpp:115-123 (R = sqrt(x^2 + y^2 + z^2),
+    pillars in list order)."""
+    planes = [_AxisPoint(k, z, s) for k, z, s in points]
+    f = forward(px, py, t_sets, planes)
+    return f.real * f.real + f.imag * f.imag
+
+
+def axis_scan(px, py, t_sets, scans):
+    """scans = [(tset, nz, wavelength, zlo, zspan, z_default)]: samples
+    z_j = zlo + zspan * j / (nz - 1) (zlo if nz == 1) and the callers' argmax, the first
+    sample with I > every earlier one and > -1 (achromatic.cpp:323-328); returns
+    (best_z, best_I, curves) like crs_axis_scan."""
+    pts, zs = [], []
+    for tset, nz, lam, zlo, zspan, _ in scans:
+        k = 2.0 * math.pi / lam
+        z = [zlo + zspan * j / (nz - 1) if nz > 1 else zlo for j in range(nz)]
+        zs.append(z)
+        pts += [(k, zz, tset) for zz in z]
+    I = axis_intensity(px, py, t_sets, pts)
+    best_z, best_I, off = [], [], 0
+    for (tset, nz, lam, zlo, zspan, zdef), z in zip(scans, zs):
+        bz, bi = zdef, -1.0
+        for j in range(nz):
+            if I[off + j] > bi:
+                bi, bz = I[off + j], z[j]
+        best_z.append(bz)
+        best_I.append(bi)
+        off += nz
+    return np.array(best_z), np.array(best_I), I
+
+
+def verify_achromatic_focus(px, py, t_bands, wavelengths, center_index, center_wavelength, f):
+    """verify_achromatic_focus (achromatic.cpp:288-338) / verify_pb_achromatic_focus
+    (pb_achromatic.cpp:255-301) on the aperture cells' per-band transmissions t_bands
+    [nb, npil]: nb x (lambda, best z over 200 samples in [0.4 f, 1.8 f], peak / I(f) of
+    the centre band at the centre wavelength)."""
+    design_peak = axis_intensity(px, py, t_bands, [(2.0 * math.pi / center_wavelength, f,
+                                                     center_index)])[0]
+    zlo, zhi = 0.4 * f, 1.8 * f
+    scans = [(j, 200, lam, zlo, zhi - zlo, f) for j, lam in enumerate(wavelengths)]
+    bz, bi, _ = axis_scan(px, py, t_bands, scans)
+    return np.array([[lam, z, i / design_peak if design_peak > 0 else 0.0]
+                     for lam, z, i in zip(wavelengths, bz, bi)])
+
+
+def polar_focus_check(px, py, tx, ty, wavelength, fx, fy, nz=240):
+    """The polarisation check of cmd_polar.cpp:85-126 (nz = 240; gui/workers.cpp:181-197
+    uses 200): each channel's on-axis focus over [0.5 min(fx, fy), 1.5 max(fx, fy)] and
+    the focal isolations 10 log10(I_own(z) / max(I_other(z), 1e-30)) there (0 when
+    |fx - fy| <= 1e-6).  Returns (z_x, z_y, iso_x, iso_y)."""
+    zlo, zhi = 0.5 * min(fx, fy), 1.5 * max(fx, fy)
+    t2 = np.stack([np.asarray(tx, np.complex128), np.asarray(ty, np.complex128)])
+    (zx, zy), _, _ = axis_scan(px, py, t2, [(0, nz, wavelength, zlo, zhi - zlo, zlo),
+                                            (1, nz, wavelength, zlo, zhi - zlo, zlo)])
+    iso_x = iso_y = 0.0
+    if abs(fx - fy) > 1e-6:
+        k = 2.0 * math.pi / wavelength
+        a = axis_intensity(px, py, t2, [(k, zx, 0), (k, zx, 1), (k, zy, 1), (k, zy, 0)])
+        iso_x = 10.0 * math.log10(a[0] / max(a[1], 1e-30))
+        iso_y = 10.0 * math.log10(a[2] / max(a[3], 1e-30))
+    return zx, zy, iso_x, iso_y
 
 
 # ----------------------------------------------------------------------------------

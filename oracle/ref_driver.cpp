@@ -35,11 +35,20 @@
 //                  -> m x (angle, rel_strehl, shift)   (analyze_wide_fov, field.cpp:78)
 //   pgm          : nx, ny, gamma, values[nx*ny] -> ok; the image itself is written to
 //                  <out.f64>.pgm              (write_pgm, io/image.cpp:10)
+//   achro        : n_cells, period, D, f, nb, lambda[nb], center_index, center_lambda,
+//                  natoms, per atom (amplitude[nb], phase[nb]), atom_index[n_cells^2]
+//                  -> nb x (lambda, f, rel)   (verify_achromatic_focus, achromatic.cpp:288)
+//   pbachro      : n_cells, period, D, f, handedness, nb, lambda[nb], center_index,
+//                  center_lambda, natoms, per atom (re, im)[nb], atom_index[n_cells^2],
+//                  rotation[n_cells^2] -> nb x (lambda, f, rel)
+//                                             (verify_pb_achromatic_focus, pb_achromatic.cpp:255)
 #include "celeris/analysis/chromatic.hpp"
 #include "celeris/analysis/field.hpp"
 #include "celeris/analysis/focal.hpp"
 #include "celeris/analysis/mtf.hpp"
 #include "celeris/analysis/throughfocus.hpp"
+#include "celeris/design/achromatic.hpp"
+#include "celeris/design/pb_achromatic.hpp"
 #include "celeris/io/image.hpp"
 
 #include <algorithm>
@@ -128,6 +137,65 @@ int main(int argc, char** argv) {
         std::vector<double> v(static_cast<std::size_t>(std::max(nx, 0)) * std::max(ny, 0));
         for (double& x : v) x = r.next();
         out = {write_pgm(std::string(argv[3]) + ".pgm", nx, ny, v, gamma) ? 1.0 : 0.0};
+    } else if (cmd == "achro" || cmd == "pbachro") {
+        const int n_cells = r.next_int();
+        const double period = r.next(), D = r.next(), f = r.next();
+        const int hand = cmd == "pbachro" ? r.next_int() : 1;
+        const int nb = r.next_int();
+        std::vector<double> lam(nb);
+        for (double& x : lam) x = r.next();
+        const int center_index = r.next_int();
+        const double center_lambda = r.next();
+        const int natoms = r.next_int();
+        const std::size_t nsite = static_cast<std::size_t>(n_cells) * n_cells;
+        std::vector<AchromaticFocalPoint> pts;
+        if (cmd == "achro") {
+            DispersiveLibrary lib;
+            lib.wavelengths_um = lam;
+            lib.center_index = center_index;
+            lib.center_wavelength_um = center_lambda;
+            lib.period_um = period;
+            lib.atoms.resize(natoms);
+            for (DispersiveAtom& a : lib.atoms) {
+                a.amplitude.resize(nb);
+                a.phase_rad.resize(nb);
+                for (double& x : a.amplitude) x = r.next();
+                for (double& x : a.phase_rad) x = r.next();
+            }
+            AchromaticDesign d;
+            d.n_cells = n_cells;
+            d.period_um = period;
+            d.diameter_um = D;
+            d.focal_length_um = f;
+            d.center_wavelength_um = center_lambda;
+            d.atom_index.resize(nsite);
+            for (int& x : d.atom_index) x = r.next_int();
+            pts = verify_achromatic_focus(lib, d);
+        } else {
+            DispersivePbLibrary lib;
+            lib.wavelengths_um = lam;
+            lib.center_index = center_index;
+            lib.center_wavelength_um = center_lambda;
+            lib.period_um = period;
+            lib.atoms.resize(natoms);
+            for (DispersivePbAtom& a : lib.atoms) {
+                a.cross.resize(nb);
+                for (cdouble& c : a.cross) { const double re = r.next(); const double im = r.next(); c = {re, im}; }
+            }
+            PbAchromaticDesign d;
+            d.n_cells = n_cells;
+            d.period_um = period;
+            d.diameter_um = D;
+            d.focal_length_um = f;
+            d.handedness = hand;
+            d.center_wavelength_um = center_lambda;
+            d.atom_index.resize(nsite);
+            for (int& x : d.atom_index) x = r.next_int();
+            d.rotation_rad.resize(nsite);
+            for (double& x : d.rotation_rad) x = r.next();
+            pts = verify_pb_achromatic_focus(lib, d);
+        }
+        for (const auto& p : pts) out.insert(out.end(), {p.wavelength_um, p.focal_length_um, p.rel_peak});
     } else {
         MetalensDesign lens;
         UnitCellLibrary lib;

@@ -42,3 +42,28 @@ def rsctx(torch_cuda):
     ctx = Context(0)
     yield ctx
     ctx.close()
+
+
+def achromatic_golden():
+    """(name, kind, px, py, t_bands, lam, center_index, center_lambda, f, reference out,
+    library, design) for every case of tests/golden/achromatic.npz (written by the reference's
+    verify_achromatic_focus and verify_pb_achromatic_focus)."""
+    from celeris_amd import design as dz
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                             "achromatic.npz"))
+    out = []
+    for name in sorted({k.split("__")[0] for k in d.files}):
+        g = lambda k: d[f"{name}__{k}"]  # noqa: E731
+        lam, ci, lc, f = g("lam"), int(g("ci")), float(g("lc")), float(g("f"))
+        lib = dz.DispersiveLibrary(lam, lc, ci, [dz.DispersiveAtom(a, ph) for a, ph in
+                                                 zip(g("amp"), g("phase"))])
+        des = dz.AchromaticDesign(int(g("n_cells")), float(g("period")), g("atom"), f,
+                                  float(g("D")))
+        px, py, t = dz.achromatic_band_transmissions(lib, des)
+        out.append((name, "achro", px, py, t, lam, ci, lc, f, g("achro"), lib, des))
+        plib = dz.DispersivePbLibrary(lam, lc, ci, [dz.DispersivePbAtom(c) for c in g("cross")])
+        pdes = dz.PbAchromaticDesign(int(g("n_cells")), float(g("period")), g("pb_atom"),
+                                     g("rotation"), f, float(g("D")), int(g("hand")))
+        px, py, t = dz.pb_achromatic_band_transmissions(plib, pdes)
+        out.append((name, "pbachro", px, py, t, lam, ci, lc, f, g("pbachro"), plib, pdes))
+    return out

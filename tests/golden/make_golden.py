@@ -3,11 +3,12 @@
 
 Run where the reference sources are present (they are compiled by oracle/ref.mk into
 oracle/_ref/ref_driver, from src/celeris/analysis/{focal,throughfocus,chromatic,mtf,field}.cpp
-and src/celeris/design/{metalens,phase_profile}.cpp).  The fixtures are data only:
+and src/celeris/design/{metalens,phase_profile,achromatic,pb_achromatic}.cpp).  The fixtures are data only:
 seeded inputs and the reference's outputs, used by tests/test_oracle.py to pin the
 FP64 restatement in oracle/rs_oracle.c where the reference build is absent.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # every fixture
+    python tests/golden/make_golden.py achromatic # only achromatic.npz
 """
 import math
 import os
@@ -76,9 +77,69 @@ def mtf_case():
                 fill_map=fills.reshape(n_cells, n_cells))
 
 
+def achromatic_cases():
+    """Dispersive libraries and designs for verify_achromatic_focus (achromatic.cpp:288)
+    and verify_pb_achromatic_focus (pb_achromatic.cpp:255).  Atoms carry a seeded phase
+    offset, a dispersion slope and |t| per band; each site takes the atom whose centre-band
+    phase is nearest the hyperbolic target (PB: a seeded atom, and the rotation that puts
+    the centre-band cross phase on the target), so the lens focuses near f at the centre
+    wavelength and elsewhere at the other bands.  Sites outside the aperture get seeded
+    atoms too; the checks must skip them."""
+    rng = np.random.default_rng(20261203)
+    cases = []
+    for name, n_cells, period, D, f, lam, ci, natoms, hand in (
+            ("band7", 45, 0.35, 15.0, 20.0, np.linspace(0.46, 0.64, 7), 3, 32, -1),
+            ("band1", 17, 0.4, 6.0, 9.0, np.array([0.532]), 0, 8, 1)):
+        nb = lam.size
+        lc = float(lam[ci])
+        phase0 = rng.uniform(-math.pi, math.pi, natoms)
+        tau = rng.uniform(0.0, 1.5, natoms)
+        phase = phase0[:, None] + 2 * math.pi * tau[:, None] * (lc / lam[None, :] - 1.0)
+        amp = rng.uniform(0.6, 1.0, (natoms, nb))
+        c = (n_cells - 1) / 2.0
+        iy, ix = np.meshgrid(np.arange(n_cells), np.arange(n_cells), indexing="ij")
+        x, y = (ix - c) * period, (iy - c) * period
+        target = -2 * math.pi / lc * (np.sqrt(x * x + y * y + f * f) - f)
+        d = np.angle(np.exp(1j * (target[..., None] - phase0[None, None, :])))
+        atom = np.argmin(np.abs(d), axis=-1).ravel()
+        outside = np.sqrt(x * x + y * y).ravel() > D / 2
+        atom[outside] = rng.integers(0, natoms, int(outside.sum()))
+        cross = amp * np.exp(1j * phase)
+        pb_atom = rng.integers(0, natoms, n_cells * n_cells)
+        geo = target.ravel() - np.angle(cross[pb_atom, ci])
+        rotation = -geo / (2.0 * hand)
+        cases.append(dict(name=name, n_cells=n_cells, period=period, D=D, f=f, lam=lam, ci=ci,
+                          lc=lc, hand=hand, amp=amp, phase=phase, atom=atom, cross=cross,
+                          pb_atom=pb_atom, rotation=rotation))
+    return cases
+
+
+def write_achromatic():
+    out = {}
+    for C in achromatic_cases():
+        n, nb = C["n_cells"], C["lam"].size
+        head = [n, C["period"], C["D"], C["f"]]
+        band = [nb, *C["lam"], C["ci"], C["lc"], C["amp"].shape[0]]
+        achro_in = np.concatenate([head, band, np.concatenate([C["amp"], C["phase"]], 1).ravel(),
+                                   C["atom"]])
+        pb_in = np.concatenate([head, [C["hand"]], band,
+                                np.stack([C["cross"].real, C["cross"].imag], -1).ravel(),
+                                C["pb_atom"], C["rotation"]])
+        for k in ("n_cells", "period", "D", "f", "lam", "ci", "lc", "hand", "amp", "phase", "atom",
+                  "cross", "pb_atom", "rotation"):
+            out[f"{C['name']}__{k}"] = np.asarray(C[k])
+        out[f"{C['name']}__achro"] = oracle.ref_run("achro", achro_in).reshape(-1, 3)
+        out[f"{C['name']}__pbachro"] = oracle.ref_run("pbachro", pb_in).reshape(-1, 3)
+    np.savez_compressed(os.path.join(HERE, "achromatic.npz"), **out)
+
+
 def main():
     if not oracle.ref_available():
         sys.exit("oracle/_ref/ref_driver is not built (make -C oracle ref)")
+    if sys.argv[1:] == ["achromatic"]:
+        write_achromatic()
+        return
+    write_achromatic()
     out = {}
     for name, px, py, t, cx, cy, z, lam, n, W in pillars_cases():
         I = oracle.ref_propagate_pillars(px, py, t, cx, cy, z, lam, n, W)

@@ -7,7 +7,10 @@ transposed index, a wrong pixel step, a missing conjugate) fails at least one:
   * closed forms: one pillar, the in-phase focus of a hyperbolic lens;
   * long-double brute force on tiny inputs;
   * linearity and mirror symmetry;
-  * the adjoint: dot-product identity and column-by-column construction.
+  * the adjoint: dot-product identity and column-by-column construction;
+  * the on-axis focus scans: the reference's verify_achromatic_focus /
+    verify_pb_achromatic_focus outputs, and closed forms for the scan rule and the
+    polarisation isolation.
 """
 import math
 import os
@@ -18,7 +21,7 @@ import pytest
 import oracle
 from celeris_amd import Plane, configs
 from celeris_amd.design import MetalensDesign, UnitCellLibrary, aperture_pillars
-from conftest import rel_l2
+from conftest import achromatic_golden, rel_l2
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -286,3 +289,59 @@ def test_oracle_field_of_view_matches_reference():
     for a, (pk, x), r in zip(d["widefov_angles"], res[1:], ref):
         assert r[0] == a and pk / res[0][0] == pytest.approx(r[1], rel=1e-12)
         assert x == pytest.approx(r[2], abs=1e-12)
+
+
+@pytest.mark.parametrize("case", achromatic_golden(), ids=lambda c: f"{c[0]}-{c[1]}")
+def test_oracle_achromatic_focus_matches_reference(case):
+    """verify_achromatic_focus (achromatic.cpp:288-338) and verify_pb_achromatic_focus
+    (pb_achromatic.cpp:255-301): the restated scan picks the reference's z sample in
+    every band and its relative peak to 1e-12 (the transmissions are rebuilt with
+    numpy's sin/cos, the reference's with glibc's)."""
+    name, kind, px, py, t, lam, ci, lc, f, ref = case[:10]
+    got = oracle.verify_achromatic_focus(px, py, t, lam, ci, lc, f)
+    assert np.array_equal(got[:, 0], ref[:, 0])
+    assert np.array_equal(got[:, 1], ref[:, 1])
+    assert np.allclose(got[:, 2], ref[:, 2], rtol=1e-12, atol=0)
+
+
+def test_oracle_axis_scan_closed_forms():
+    """A single pillar at the origin: I(z) = a^2 / z^2 exactly decreasing, so every scan
+    stops at its first sample; a pillar at (0, 0) with t = 0 gives I = 0 everywhere, and
+    the callers' rule (I > best_I from -1) still takes the first sample; nz = 1 is z = zlo;
+    the sample grid is zlo + zspan j / (nz - 1)."""
+    px, py = np.zeros(1), np.zeros(1)
+    t = np.array([[0.8 + 0.6j], [0.0j]])
+    scans = [(0, 7, 0.5, 3.0, 9.0, -5.0), (1, 4, 0.6, 2.0, 1.5, -5.0), (0, 1, 0.5, 4.0, 0.0, -5.0)]
+    bz, bi, I = oracle.axis_scan(px, py, t, scans)
+    assert list(bz) == [3.0, 2.0, 4.0]
+    assert bi[0] == pytest.approx(1.0 / 9.0, rel=1e-15) and bi[1] == 0.0
+    z = 3.0 + 9.0 * np.arange(7) / 6
+    assert np.allclose(I[:7], 1.0 / z ** 2, rtol=1e-15)
+
+
+def test_oracle_polar_focus_closed_form():
+    """cmd_polar.cpp:85-126 on one pillar with t_x / t_y = 2: both channels peak at the
+    first sample (I = |t|^2 / z^2 decreases) and each isolation is 10 log10(|t_x / t_y|^2)
+    = +-6.0206 dB; equal focal lengths give no isolation; swapping the channels swaps
+    the answers."""
+    px, py = np.zeros(1), np.zeros(1)
+    zx, zy, ix, iy = oracle.polar_focus_check(px, py, [1.0 + 0j], [0.5j], 0.532, 20.0, 30.0)
+    assert zx == zy == 10.0
+    assert ix == pytest.approx(10 * math.log10(4.0), rel=1e-14)
+    assert iy == pytest.approx(-10 * math.log10(4.0), rel=1e-14)
+    assert oracle.polar_focus_check(px, py, [1.0 + 0j], [0.5j], 0.532, 20.0, 20.0)[2:] == (0.0, 0.0)
+
+
+def test_oracle_polar_focus_bifocal_lens():
+    """A bifocal lens (X channel hyperbolic to f_x, Y to f_y): each channel focuses near
+    its own plane, before the other channel's, and is brighter there; swapping the
+    channels swaps the foci and the isolations."""
+    px, py = configs.square_grid(41, 0.4)
+    lam, fx, fy = 0.532, 30.0, 45.0
+    tx = np.exp(1j * configs.hyperbolic_phase(px, py, lam, fx))
+    ty = np.exp(1j * configs.hyperbolic_phase(px, py, lam, fy))
+    zx, zy, ix, iy = oracle.polar_focus_check(px, py, tx, ty, lam, fx, fy, nz=120)
+    assert 0.8 * fx < zx < 1.05 * fx and 0.8 * fy < zy < 1.05 * fy
+    assert ix > 3.0 and iy > 3.0
+    zy2, zx2, iy2, ix2 = oracle.polar_focus_check(px, py, ty, tx, lam, fx, fy, nz=120)
+    assert (zx2, zy2) == (zx, zy) and ix2 == pytest.approx(ix, rel=1e-12)
