@@ -111,6 +111,9 @@ SIGNATURES = [
     ("crs_analyze_mtf", c_int,
      [_DP, _DP, c_int, c_double, c_double, c_double, c_double, c_int, POINTER(ctypes.c_float),
       POINTER(ctypes.c_float), POINTER(ctypes.c_float), _DP, POINTER(c_int)]),
+    ("crs_analyze_tolerance", c_int,
+     [_DP, c_int, c_double, _DP, _DP, _DP, c_int, c_double, c_double, c_double, _DP, c_int,
+      c_int, ctypes.c_uint, c_int, _DP, _DP]),
     ("crs_axis_scan", c_int,
      [_DP, _DP, _DP, c_int, c_int, POINTER(crs_axis_scan_desc), c_int, c_int, _DP, _DP, _DP]),
     ("crs_verify_achromatic_focus", c_int,

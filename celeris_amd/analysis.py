@@ -13,6 +13,7 @@ module (src/celeris/analysis/*.hpp; bindings/python/celeris_module.cpp:415-440):
 * :func:`analyze_field_of_view` field.cpp:12-76       -> list of :class:`FieldPoint`
 * :func:`analyze_wide_fov`      field.cpp:78-148      -> list of :class:`FieldPoint`
 * :func:`analyze_mtf`           mtf.cpp:10-63         -> :class:`MtfCurve`
+* :func:`analyze_tolerance`     tolerance.cpp:12-48   -> list of :class:`ToleranceResult`
 * :func:`verify_achromatic_focus`    design/achromatic.cpp:288-338    -> list of
   :class:`AchromaticFocalPoint`; :func:`verify_pb_achromatic_focus`
   design/pb_achromatic.cpp:255-301 (same)
@@ -330,6 +331,43 @@ def analyze_wide_fov(lens: MetalensDesign, lib: UnitCellLibrary, focal_length_um
                                             _p(out)),
            "crs_analyze_wide_fov")
     return [FieldPoint(*map(float, out[3 * i:3 * i + 3])) for i in range(ang.size)]
+
+
+# ---- fabrication tolerance ----------------------------------------------------------------
+
+@dataclass
+class ToleranceResult:
+    """tolerance.hpp: Strehl statistics over the Monte-Carlo trials of one CD error."""
+
+    sigma_nm: float
+    mean_strehl: float
+    std_strehl: float
+    worst_strehl: float
+
+
+def analyze_tolerance(lens: MetalensDesign, lib: UnitCellLibrary, focal_length_um: float,
+                      wavelength_um: float, diameter_um: float, sigmas_nm, trials: int,
+                      seed: int, mode="fp32c", return_trials: bool = False):
+    """tolerance.cpp:12-48: per sigma, max(1, trials) trials of jittered fills (the
+    reference's std::mt19937 / std::normal_distribution draws), each trial's Strehl from
+    analyze_focus's grid; all trials in batched launches.  With return_trials, also the
+    per-trial Strehl ratios [len(sigmas), max(1, trials)]."""
+    fm = _d(np.asarray(lens.fill_map, np.float64).ravel())
+    if fm.size != lens.n_cells * lens.n_cells:
+        raise ValueError("fill_map must hold n_cells^2 fills")
+    lf, lp, la = _d(lib.fill), _d(lib.phase), _d(lib.amplitude)
+    sg = _d(np.atleast_1d(sigmas_nm))
+    n = max(1, int(trials))
+    out = np.zeros(4 * sg.size)
+    per = np.zeros(sg.size * n)
+    _check(_lib.load().crs_analyze_tolerance(_p(fm), int(lens.n_cells), float(lens.period_um),
+                                             _p(lf), _p(lp), _p(la), lf.size, float(focal_length_um),
+                                             float(wavelength_um), float(diameter_um), _p(sg),
+                                             sg.size, int(trials), int(seed) & 0xFFFFFFFF,
+                                             _mode(mode), _p(out), _p(per)),
+           "crs_analyze_tolerance")
+    res = [ToleranceResult(*map(float, out[4 * i:4 * i + 4])) for i in range(sg.size)]
+    return (res, per.reshape(sg.size, n)) if return_trials else res
 
 
 # ---- on-axis focus scans ----------------------------------------------------------------

@@ -31,7 +31,8 @@ HIP_FLAGS = [
     "-Wno-unused-result",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
-SOURCES = ["rs_kernels.hip", "rs_api.cpp", "rs_analysis.cpp", "propagate_shim.cpp"]
+SOURCES = ["rs_kernels.hip", "rs_api.cpp", "rs_analysis.cpp", "rs_tolerance.cpp",
+           "propagate_shim.cpp"]
 
 
 def _run(cmd, cwd=None):

@@ -227,6 +227,24 @@ crs_status crs_analyze_mtf(const double *pupil, const double *pupil0, int n, dou
                            float *freq_cyc_mm, float *mtf, float *mtf_ideal,
                            double *cutoff_cyc_mm, int *npoints);
 
+/* Fabrication tolerance, replacing analysis/tolerance.cpp:12-48 (analyze_tolerance).
+ * For each sigma (nm of CD error) and each of max(1, trials) trials: every fill of the
+ * n_cells^2 map (row-major) gets N(0, (sigma/1000)/period) added from a std::mt19937
+ * seeded seed + trial * 7919 + unsigned(sigma * 13) (one std::normal_distribution draw
+ * per cell, in map order) and is clamped to [0.05, 0.95]; the cells inside the aperture
+ * take the nearest library fill's transmission (metalens.cpp:112-120), and the trial's
+ * Strehl ratio is the peak of analyze_focus's 121 x 121 grid at z = f over the ideal
+ * lens's peak (focal.cpp:32-146).  All trials run as transmission sets of batched
+ * launches.  out4 = per sigma {sigma_nm, mean, standard deviation (population), worst};
+ * strehl (optional) = every trial's ratio, sigma-major.  lib_* : the unit-cell library's
+ * nlib sampled fills, phases (rad) and amplitudes.  Synchronous. */
+crs_status crs_analyze_tolerance(const double *fill_map, int n_cells, double period_um,
+                                 const double *lib_fill, const double *lib_phase,
+                                 const double *lib_amp, int nlib, double focal_um,
+                                 double wavelength_um, double diameter_um,
+                                 const double *sigmas_nm, int nsigmas, int trials,
+                                 unsigned seed, int mode, double *out4, double *strehl);
+
 /* ---- on-axis focus scans -------------------------------------------------------------
  * The reference's design checks find a lens's focus by scanning the on-axis intensity
  * I(z) = |sum_q t_q exp(i k R_q) / R_q|^2, R_q = sqrt(x_q^2 + y_q^2 + z^2), over z and

@@ -53,6 +53,10 @@ def load():
         lib.orc_mtf_autocorr.restype = None
         lib.orc_mtf_autocorr.argtypes = [dp, dp, c_int, c_int, dp, dp]
         lib.orc_num_cpus.restype = c_int
+        lib.orc_mt19937_nth.restype = ctypes.c_uint32
+        lib.orc_mt19937_nth.argtypes = [ctypes.c_uint32, c_int64]
+        lib.orc_tolerance_fills.restype = None
+        lib.orc_tolerance_fills.argtypes = [ctypes.c_uint32, c_int, c_double, c_double, dp, c_int64, dp]
         _lib = lib
     return _lib
 
@@ -269,3 +273,77 @@ def ref_pgm(nx: int, ny: int, values, gamma: float) -> bytes:
         subprocess.run([REF_DRIVER, "pgm", fin, fout], check=True)
         with open(fout + ".pgm", "rb") as f:
             return f.read()
+
+
+
+def mt19937_nth(seed: int, nth: int) -> int:
+    """The nth output of the oracle's MT19937 restatement seeded with seed."""
+    return int(load().orc_mt19937_nth(seed & 0xFFFFFFFF, nth))
+
+
+def tolerance_fills(seed: int, trial: int, sigma_nm: float, period_um: float, fill_map):
+    """One trial's jittered fill map of analyze_tolerance (tolerance.cpp:20-31)."""
+    f = np.ascontiguousarray(np.asarray(fill_map, np.float64).ravel())
+    out = np.empty_like(f)
+    load().orc_tolerance_fills(seed & 0xFFFFFFFF, int(trial), float(sigma_nm), float(period_um),
+                               _dp(f), f.size, _dp(out))
+    return out
+
+
+def tolerance(n_cells, period, fill_map, lib_fill, lib_phase, lib_amp, f, lam, D, sigmas_nm,
+              trials, seed, return_trials=False):
+    """analyze_tolerance (tolerance.cpp:12-48) on the oracle: per sigma and trial the
+    jittered map, the aperture cells' nearest-fill transmissions (metalens.cpp:112-120,
+    first on ties), the 121 x 121 grid at z = f (focal.cpp:61-64, :90-99) and the ideal
+    peak (:102), Strehl = peak / peak_ideal (:140); then mean, population std and worst
+    in trial order (:37-46)."""
+    c = (n_cells - 1) / 2.0
+    iy, ix = np.meshgrid(np.arange(n_cells), np.arange(n_cells), indexing="ij")
+    x, y = ((ix - c) * period).ravel(), ((iy - c) * period).ravel()
+    r = np.sqrt(x * x + y * y)
+    keep = ~(r > D / 2.0)
+    px, py, rk = x[keep], y[keep], r[keep]
+    k = 2.0 * math.pi / lam
+    phi = -k * (np.sqrt(rk * rk + f * f) - f)
+    dl = lam * f / D
+    W = max(6.0 * dl, 4.0)
+    NG = 121
+
+    class P:
+        pass
+
+    grid = P()
+    grid.k, grid.x0, grid.sx, grid.y0, grid.sy = k, -W, 2.0 * W / (NG - 1), -W, 2.0 * W / (NG - 1)
+    grid.z0, grid.sz, grid.nx, grid.ny, grid.tset, grid.offset = f, 0.0, NG, NG, 0, -1
+    ideal = axis_intensity(px, py, np.cos(phi) + 1j * np.sin(phi), [(k, f, 0)])[0]
+    lf = np.asarray(lib_fill, np.float64)
+    n = max(1, int(trials))
+    out, per = [], []
+    for s in sigmas_nm:
+        st = []
+        for t in range(n):
+            jf = tolerance_fills(seed, t, s, period, fill_map)[keep]
+            e = np.array([_nearest(lf, v) for v in jf])
+            tt = np.asarray(lib_amp)[e] * np.cos(np.asarray(lib_phase)[e]) + \
+                1j * (np.asarray(lib_amp)[e] * np.sin(np.asarray(lib_phase)[e]))
+            _, I = forward(px, py, tt, [grid], intensity=True)
+            st.append(I.max() / ideal if ideal > 0 else 0.0)
+        s1 = s2 = 0.0
+        worst = 1e300
+        for v in st:
+            s1 += v
+            s2 += v * v
+            worst = min(worst, v)
+        mean = s1 / n
+        out.append((s, mean, math.sqrt(max(0.0, s2 / n - mean * mean)), worst))
+        per.append(st)
+    return (np.array(out), np.array(per)) if return_trials else np.array(out)
+
+
+def _nearest(lib_fill, v):
+    best, best_d = 0, abs(lib_fill[0] - v)
+    for i in range(1, lib_fill.size):
+        d = abs(lib_fill[i] - v)
+        if d < best_d:
+            best, best_d = i, d
+    return best

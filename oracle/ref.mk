@@ -3,7 +3,7 @@
 # Test infrastructure only; outputs go to oracle/_ref/ and nowhere else.
 #
 # Only the translation units on the propagation path are compiled (analysis/{focal,
-# throughfocus,chromatic,mtf,field}.cpp, design/{metalens,phase_profile,achromatic,
+# throughfocus,chromatic,mtf,field,tolerance}.cpp, design/{metalens,phase_profile,achromatic,
 # pb_achromatic}.cpp, and the PGM writer io/image.cpp).  They need
 # no Eigen, RCWA or CUDA code for the functions the driver calls; the linker's
 # --gc-sections drops the unreferenced functions of those units (for example
@@ -17,6 +17,7 @@ CXX     ?= g++
 FLAGS   := -std=c++20 -O3 -mavx2 -mfma -ffunction-sections -fdata-sections -I$(REF)/src
 OUT     := _ref
 UNITS   := analysis/focal analysis/throughfocus analysis/chromatic analysis/mtf analysis/field \
+           analysis/tolerance \
            design/metalens design/phase_profile design/achromatic design/pb_achromatic io/image
 OBJS    := $(addprefix $(OUT)/,$(addsuffix .o,$(notdir $(UNITS))))
 
@@ -46,6 +47,9 @@ $(OUT)/metalens.o: $(SRC)/design/metalens.cpp
 	@mkdir -p $(OUT)
 	$(CXX) $(FLAGS) -c -o $@ $<
 $(OUT)/phase_profile.o: $(SRC)/design/phase_profile.cpp
+	@mkdir -p $(OUT)
+	$(CXX) $(FLAGS) -c -o $@ $<
+$(OUT)/tolerance.o: $(SRC)/analysis/tolerance.cpp
 	@mkdir -p $(OUT)
 	$(CXX) $(FLAGS) -c -o $@ $<
 $(OUT)/achromatic.o: $(SRC)/design/achromatic.cpp

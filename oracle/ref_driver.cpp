@@ -33,6 +33,8 @@
 //                                             (analyze_field_of_view, field.cpp:12)
 //   widefov      : lens, f, lambda, D_lens, D_stop, stop_dist, m, angles[m]
 //                  -> m x (angle, rel_strehl, shift)   (analyze_wide_fov, field.cpp:78)
+//   tolerance    : lens, f, lambda, D, nsig, sigmas[nsig], trials, seed
+//                  -> nsig x (sigma, mean, std, worst)   (analyze_tolerance, tolerance.cpp:12)
 //   pgm          : nx, ny, gamma, values[nx*ny] -> ok; the image itself is written to
 //                  <out.f64>.pgm              (write_pgm, io/image.cpp:10)
 //   achro        : n_cells, period, D, f, nb, lambda[nb], center_index, center_lambda,
@@ -47,6 +49,7 @@
 #include "celeris/analysis/focal.hpp"
 #include "celeris/analysis/mtf.hpp"
 #include "celeris/analysis/throughfocus.hpp"
+#include "celeris/analysis/tolerance.hpp"
 #include "celeris/design/achromatic.hpp"
 #include "celeris/design/pb_achromatic.hpp"
 #include "celeris/io/image.hpp"
@@ -264,6 +267,14 @@ int main(int argc, char** argv) {
                 cmd == "fov" ? analyze_field_of_view(lens, lib, f, lam, D, angles)
                              : analyze_wide_fov(lens, lib, f, lam, D, Dstop, stop_dist, angles);
             for (const auto& p : pts) out.insert(out.end(), {p.angle_deg, p.rel_strehl, p.spot_shift_um});
+        } else if (cmd == "tolerance") {
+            const int nsig = r.next_int();
+            std::vector<double> sig(nsig);
+            for (double& x : sig) x = r.next();
+            const int trials = r.next_int();
+            const unsigned seed = static_cast<unsigned>(r.next());
+            for (const ToleranceResult& t : analyze_tolerance(lens, lib, f, lam, D, sig, trials, seed))
+                out.insert(out.end(), {t.sigma_nm, t.mean_strehl, t.std_strehl, t.worst_strehl});
         } else {
             std::fprintf(stderr, "ref_driver: unknown command %s\n", cmd.c_str());
             return 2;

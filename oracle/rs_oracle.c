@@ -18,6 +18,10 @@
  *     the conjugate transpose of the sum above, evaluated with the same G_pq.
  *   - the pupil autocorrelation of analyze_mtf     src/celeris/analysis/mtf.cpp:40-49
  *       acc(s) = sum_iy sum_{ix+s<n} P[iy][ix] conj(P[iy][ix+s]), rows then columns
+ *   - the fill jitter of analyze_tolerance         src/celeris/analysis/tolerance.cpp:20-31
+ *       fill = clamp(fill + N(0, sigma_fill), 0.05, 0.95) per map cell, from a
+ *       std::mt19937 seeded seed + trial*7919 + unsigned(sigma*13) and
+ *       std::normal_distribution<double>, restated from the published algorithms below
  *
  * Pixel geometry is expressed as an affine "plane": pixel (i, j) sits at
  *   fx = x0 + i*sx,  fy = y0 + j*sy,  z = z0 + j*sz
@@ -305,5 +309,104 @@ void orc_mtf_autocorr(const double *P, const double *P0, int n, int nshift, doub
         acc[2 * s] = ar;
         acc[2 * s + 1] = ai;
         acc0[s] = a0;
+    }
+}
+
+
+/* ---- fill jitter of analyze_tolerance (tolerance.cpp:20-31) ----------------------------
+ * The reference draws with the C++ standard library's std::mt19937 and
+ * std::normal_distribution<double> (libstdc++).  Restated from their published
+ * definitions, not from any library source:
+ *   MT19937      Matsumoto & Nishimura, "Mersenne Twister", ACM TOMACS 8(1), 1998: state
+ *                of 624 words seeded by x_i = 1812433253 (x_{i-1} ^ (x_{i-1} >> 30)) + i,
+ *                twist with matrix 0x9908b0df at offset 397, tempering (11, 7/0x9d2c5680,
+ *                15/0xefc60000, 18).  C++ [rand.predef] fixes its 10000th output from the
+ *                default seed 5489 at 4123659995 (checked in tests/test_oracle.py).
+ *   canonical    C++ [rand.util.canonical] for 53 bits from a 32-bit engine: two draws
+ *                a, b and (a + b 2^32) / 2^64, a result of 1 replaced by the largest
+ *                double below 1.
+ *   normal       Marsaglia & Bray's polar method (SIAM Review 6(3), 1964): draw
+ *                x, y = 2u - 1 until 0 < r2 = x^2 + y^2 <= 1, m = sqrt(-2 ln(r2) / r2);
+ *                the pair gives y m now and x m on the next call; value * sigma + mean. */
+typedef struct {
+    uint32_t mt[624];
+    int idx;
+    int have_saved;
+    double saved;
+} orc_rng;
+
+static void mt_seed(orc_rng *g, uint32_t s) {
+    g->mt[0] = s;
+    for (int i = 1; i < 624; ++i)
+        g->mt[i] = 1812433253u * (g->mt[i - 1] ^ (g->mt[i - 1] >> 30)) + (uint32_t)i;
+    g->idx = 624;
+    g->have_saved = 0;
+}
+
+static uint32_t mt_next(orc_rng *g) {
+    if (g->idx >= 624) {
+        for (int i = 0; i < 624; ++i) {
+            const uint32_t y = (g->mt[i] & 0x80000000u) | (g->mt[(i + 1) % 624] & 0x7fffffffu);
+            g->mt[i] = g->mt[(i + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+        }
+        g->idx = 0;
+    }
+    uint32_t y = g->mt[g->idx++];
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+}
+
+static double canonical53(orc_rng *g) {
+    const double a = (double)mt_next(g);
+    const double b = (double)mt_next(g);
+    const double r = 4294967296.0;
+    double v = (a + b * r) / (r * r);
+    if (v >= 1.0) v = nextafter(1.0, 0.0);
+    return v;
+}
+
+static double normal_draw(orc_rng *g, double mean, double sigma) {
+    double v;
+    if (g->have_saved) {
+        g->have_saved = 0;
+        v = g->saved;
+    } else {
+        double x, y, r2;
+        do {
+            x = 2.0 * canonical53(g) - 1.0;
+            y = 2.0 * canonical53(g) - 1.0;
+            r2 = x * x + y * y;
+        } while (r2 > 1.0 || r2 == 0.0);
+        const double m = sqrt(-2.0 * log(r2) / r2);
+        g->saved = x * m;
+        g->have_saved = 1;
+        v = y * m;
+    }
+    return v * sigma + mean;
+}
+
+/* The n-th (1-based) output of an MT19937 seeded with s (for the known-answer test). */
+uint32_t orc_mt19937_nth(uint32_t s, int64_t nth) {
+    orc_rng g;
+    mt_seed(&g, s);
+    uint32_t v = 0;
+    for (int64_t i = 0; i < nth; ++i) v = mt_next(&g);
+    return v;
+}
+
+/* One trial's jittered fill map: fill_out[c] = clamp(fill_in[c] + N(0, sigma_fill),
+ * 0.05, 0.95) in map order, sigma_fill = (sigma_nm / 1000) / period_um, engine seeded
+ * seed + trial * 7919 + unsigned(sigma_nm * 13) (32-bit unsigned arithmetic). */
+void orc_tolerance_fills(uint32_t seed, int trial, double sigma_nm, double period_um,
+                         const double *fill_in, int64_t n, double *fill_out) {
+    orc_rng g;
+    mt_seed(&g, seed + (uint32_t)trial * 7919u + (uint32_t)(sigma_nm * 13.0));
+    const double sigma_fill = (sigma_nm / 1000.0) / period_um;
+    for (int64_t c = 0; c < n; ++c) {
+        const double v = fill_in[c] + normal_draw(&g, 0.0, sigma_fill);
+        fill_out[c] = v < 0.05 ? 0.05 : (0.95 < v ? 0.95 : v);
     }
 }

@@ -8,7 +8,7 @@ seeded inputs and the reference's outputs, used by tests/test_oracle.py to pin t
 FP64 restatement in oracle/rs_oracle.c where the reference build is absent.
 
     python tests/golden/make_golden.py            # every fixture
-    python tests/golden/make_golden.py achromatic # only achromatic.npz
+    python tests/golden/make_golden.py achromatic # only achromatic.npz (or: tolerance)
 """
 import math
 import os
@@ -133,13 +133,33 @@ def write_achromatic():
     np.savez_compressed(os.path.join(HERE, "achromatic.npz"), **out)
 
 
+def write_tolerance():
+    """analyze_tolerance (tolerance.cpp:12-48) on the golden lens: the CLI's sweep
+    (sigmas 0, 5, 10, 20 nm, 12 trials, seed 12345; cmd_design.cpp:478-480), and a
+    single-trial sweep (trials = 0 means one) with another seed."""
+    L = lens_case()
+    blk = oracle.lens_block(L["n_cells"], L["period"], L["fill_map"], L["lib_fill"], L["lib_phase"],
+                            L["lib_amp"])
+    base = np.concatenate([blk, [L["f"], L["lam"], L["D"]]])
+    out = {k: np.asarray(v) for k, v in L.items()}
+    for name, sig, trials, seed in (("cli", [0.0, 5.0, 10.0, 20.0], 12, 12345),
+                                    ("single", [3.0, 40.0], 0, 4000000000)):
+        out[f"{name}__args"] = np.array([trials, seed], np.float64)
+        out[f"{name}__sigmas"] = np.array(sig)
+        out[f"{name}__out"] = oracle.ref_run("tolerance", np.concatenate(
+            [base, [len(sig)], sig, [trials, seed]])).reshape(-1, 4)
+    np.savez_compressed(os.path.join(HERE, "tolerance.npz"), **out)
+
+
 def main():
     if not oracle.ref_available():
         sys.exit("oracle/_ref/ref_driver is not built (make -C oracle ref)")
-    if sys.argv[1:] == ["achromatic"]:
-        write_achromatic()
+    only = {"achromatic": write_achromatic, "tolerance": write_tolerance}
+    if sys.argv[1:] and sys.argv[1] in only:
+        only[sys.argv[1]]()
         return
     write_achromatic()
+    write_tolerance()
     out = {}
     for name, px, py, t, cx, cy, z, lam, n, W in pillars_cases():
         I = oracle.ref_propagate_pillars(px, py, t, cx, cy, z, lam, n, W)

@@ -194,3 +194,72 @@ def test_analyze_field_of_view_and_wide_fov(torch_cuda, mode, tol):
                   for a in ref[:, 0]])
     slack = 1e-9 if mode == "fp64" else 2 * W / 80 * 1.0001
     assert np.all(np.abs(got[:, 2] - ref[:, 2]) <= slack)
+
+
+TOL_GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "tolerance.npz")
+
+
+@pytest.mark.parametrize("mode,rtol,stdtol", [("fp64", 1e-9, 1e-8), ("fp32c", 1e-5, 2e-6)])
+@pytest.mark.parametrize("name", ["cli", "single"])
+def test_analyze_tolerance_vs_reference(torch_cuda, name, mode, rtol, stdtol):
+    """analyze_tolerance (tolerance.cpp:12-48) against the reference's own sweep: the
+    same jitter draws (std::mt19937 / std::normal_distribution seeded per trial), every
+    trial's Strehl from the batched grids."""
+    import oracle
+
+    d = np.load(TOL_GOLDEN)
+    L = MetalensDesign(int(d["n_cells"]), float(d["period"]), d["fill_map"])
+    lib = UnitCellLibrary(float(d["period"]), d["lib_fill"], d["lib_phase"], d["lib_amp"])
+    trials, seed = map(int, d[f"{name}__args"])
+    print(f"[analysis] tolerance {name} {mode}", flush=True)
+    res, per = analysis.analyze_tolerance(L, lib, float(d["f"]), float(d["lam"]), float(d["D"]),
+                                          d[f"{name}__sigmas"], trials, seed, mode=mode,
+                                          return_trials=True)
+    got = np.array([[r.sigma_nm, r.mean_strehl, r.std_strehl, r.worst_strehl] for r in res])
+    ref = d[f"{name}__out"]
+    print(mode, name, got, ref)
+    assert np.array_equal(got[:, 0], ref[:, 0])
+    assert np.allclose(got[:, [1, 3]], ref[:, [1, 3]], rtol=rtol, atol=0)
+    assert np.allclose(got[:, 2], ref[:, 2], rtol=0, atol=stdtol)
+    _, oper = oracle.tolerance(int(d["n_cells"]), float(d["period"]), d["fill_map"], d["lib_fill"],
+                               d["lib_phase"], d["lib_amp"], float(d["f"]), float(d["lam"]),
+                               float(d["D"]), d[f"{name}__sigmas"], trials, seed,
+                               return_trials=True)
+    assert np.allclose(per, oper, rtol=rtol, atol=0)
+
+
+def test_analyze_tolerance_many_trials_batches(torch_cuda):
+    """300 trials of one sigma run in two launches (256 + 44 trials); trials on both
+    sides of the boundary match the oracle's, and the statistics are the per-trial
+    ratios' own."""
+    import oracle
+
+    d = np.load(TOL_GOLDEN)
+    L = MetalensDesign(int(d["n_cells"]), float(d["period"]), d["fill_map"])
+    lib = UnitCellLibrary(float(d["period"]), d["lib_fill"], d["lib_phase"], d["lib_amp"])
+    f, lam, D = float(d["f"]), float(d["lam"]), float(d["D"])
+    res, per = analysis.analyze_tolerance(L, lib, f, lam, D, [15.0], 300, 777, mode="fp64",
+                                          return_trials=True)
+    assert per.shape == (1, 300)
+    s = per[0]
+    assert res[0].mean_strehl == pytest.approx(s.mean(), rel=1e-12)
+    assert res[0].worst_strehl == s.min()
+    assert res[0].std_strehl == pytest.approx(s.std(), rel=1e-6)
+    # oracle Strehl of single trials
+    n = int(d["n_cells"])
+    c = (n - 1) / 2.0
+    iy, ix = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    x, y = ((ix - c) * L.period_um).ravel(), ((iy - c) * L.period_um).ravel()
+    r = np.sqrt(x * x + y * y)
+    keep = ~(r > D / 2)
+    k = 2 * math.pi / lam
+    phi = -k * (np.sqrt(r[keep] ** 2 + f * f) - f)
+    ideal = oracle.axis_intensity(x[keep], y[keep], np.exp(1j * phi), [(k, f, 0)])[0]
+    W = max(6 * lam * f / D, 4.0)
+    from celeris_amd import Plane
+    grid = Plane.psf(0.0, 0.0, f, k, 121, W)
+    for trial in (0, 1, 255, 256, 299):
+        jf = oracle.tolerance_fills(777, trial, 15.0, L.period_um, d["fill_map"])[keep]
+        t = lib.transmission_for_fill(jf)
+        _, I = oracle.forward(x[keep], y[keep], t, [grid], intensity=True)
+        assert s[trial] == pytest.approx(I.max() / ideal, rel=1e-9), trial

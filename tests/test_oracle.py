@@ -345,3 +345,51 @@ def test_oracle_polar_focus_bifocal_lens():
     assert ix > 3.0 and iy > 3.0
     zy2, zx2, iy2, ix2 = oracle.polar_focus_check(px, py, ty, tx, lam, fx, fy, nz=120)
     assert (zx2, zy2) == (zx, zy) and ix2 == pytest.approx(ix, rel=1e-12)
+
+
+def test_oracle_mt19937_known_answer():
+    """C++ [rand.predef]: the 10000th output of std::mt19937 from the default seed 5489 is
+    4123659995 (the restated engine under the tolerance jitter)."""
+    assert oracle.mt19937_nth(5489, 10000) == 4123659995
+
+
+def tolerance_golden():
+    d = np.load(os.path.join(GOLDEN, "tolerance.npz"))
+    return d, sorted({k.split("__")[0] for k in d.files if k.endswith("__out")})
+
+
+@pytest.mark.parametrize("name", tolerance_golden()[1])
+def test_oracle_tolerance_matches_reference(name):
+    """analyze_tolerance (tolerance.cpp:12-48) from the reference build: the restated
+    mt19937 / normal_distribution draws give the reference's Strehl mean and worst to
+    1e-12 and its spread to 1e-8 absolute (the reference computes the variance
+    sum2/n - mean^2 with a fused multiply-add, so a single trial's spread is ~1e-9, not 0)."""
+    d, _ = tolerance_golden()
+    trials, seed = d[f"{name}__args"]
+    got = oracle.tolerance(int(d["n_cells"]), float(d["period"]), d["fill_map"], d["lib_fill"],
+                           d["lib_phase"], d["lib_amp"], float(d["f"]), float(d["lam"]),
+                           float(d["D"]), d[f"{name}__sigmas"], int(trials), int(seed))
+    ref = d[f"{name}__out"]
+    assert np.array_equal(got[:, 0], ref[:, 0])
+    assert np.allclose(got[:, [1, 3]], ref[:, [1, 3]], rtol=1e-12, atol=0)
+    assert np.allclose(got[:, 2], ref[:, 2], rtol=1e-9, atol=1e-8)
+    if name == "cli":
+        assert np.all(ref[1:, 2] > 1e-4)  # the jitter really spreads the trials
+
+
+def test_oracle_tolerance_jitter_properties():
+    """The jitter at sigma = 0 leaves every fill in [0.05, 0.95] unchanged and clamps the
+    rest; a different trial or sigma gives a different map; draws are N(0, sigma_fill):
+    over 40,000 cells the sample mean and spread match within 4 standard errors."""
+    n = 200
+    fm = np.linspace(0.0, 1.0, n * n)
+    same = oracle.tolerance_fills(12345, 0, 0.0, 0.4, fm)
+    assert np.array_equal(same, np.clip(fm, 0.05, 0.95))
+    mid = np.full(n * n, 0.5)
+    a = oracle.tolerance_fills(12345, 0, 10.0, 0.4, mid) - 0.5
+    b = oracle.tolerance_fills(12345, 1, 10.0, 0.4, mid) - 0.5
+    c = oracle.tolerance_fills(12345, 0, 11.0, 0.4, mid) - 0.5
+    assert not np.array_equal(a, b) and not np.array_equal(a, c)
+    sf = 0.010 / 0.4
+    assert abs(a.mean()) < 4 * sf / math.sqrt(a.size)
+    assert abs(a.std() / sf - 1) < 4 / math.sqrt(2 * a.size)
