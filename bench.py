@@ -48,6 +48,10 @@ def parse():
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-split autotuner")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="also run a plain device copy of BYTES (reference for HBM counters)")
+    ap.add_argument("--psfbench", action="store_true",
+                    help="run the reference's psfbench instead (cli/bench.cpp:96-170)")
+    ap.add_argument("--diameter", type=float, default=120.0, help="psfbench lens diameter (um)")
+    ap.add_argument("--focal", type=float, default=50.0, help="psfbench focal length (um)")
     return ap.parse_args()
 
 
@@ -132,8 +136,59 @@ def profiled_traffic(launches):
     return total, os.path.relpath(path, ROOT)
 
 
+def psfbench_cpu_baseline(L) -> tuple:
+    """psfbench's CPU leg, measured for real: the oracle's FP64 restatement of
+    compute_psf's CPU loop (focal.cpp:205-238, rows over every host thread), whole map."""
+    import oracle
+    from celeris_amd import Plane
+
+    threads = oracle.num_cpus()
+    plane = Plane.psf(0.0, 0.0, L.focal_um, 2 * math.pi / L.wavelength_um, L.grid, L.half_window_um)
+    t0 = time.perf_counter()
+    _, inten = oracle.forward(L.px, L.py, L.t, [plane], threads=threads, intensity=True)
+    return (time.perf_counter() - t0) * 1e3, threads, inten.reshape(L.grid, L.grid)
+
+
+def run_psfbench(args) -> None:
+    """The reference's psfbench (cli/bench.cpp:96-170) on this build: same lens
+    construction, grid and window; the CPU leg timed on the CPU and compared against
+    (the reference timed its GPU path there and compared GPU with GPU)."""
+    from celeris_amd import psfbench as pb
+
+    print("PSF propagation benchmark (building lens...)")
+    L = pb.psfbench_lens(args.diameter, args.focal, args.grid)
+    print(f"  {L.lens.n_cells} x {L.lens.n_cells} cells, {L.px.size} pillars in aperture, "
+          f"{L.grid}x{L.grid} focal grid")
+    pairs = L.px.size * L.grid * L.grid
+    gpu = pb.run_gpu(L)
+    cpu_ms, threads, cpu_I = psfbench_cpu_baseline(L)
+    print("")
+    print(f"  CPU (double, parallel) : {cpu_ms:8.1f} ms   ({threads} threads, "
+          f"{pairs / cpu_ms / 1e6:.2f} G pairs/s)")
+    rec = {"workload": "psfbench", "pillars": int(L.px.size), "grid": L.grid,
+           "pairs": int(pairs), "diameter_um": L.diameter_um, "focal_um": L.focal_um,
+           "wavelength_um": L.wavelength_um, "half_window_um": L.half_window_um,
+           "library": "stand-in (no RCWA): 16 fills, phase ramp over 2 pi, |t| 0.86-0.95",
+           "cpu_baseline": {"ms": cpu_ms, "threads": threads, "kind": "port",
+                            "cpu_model": cpu_model()},
+           "gpu": {}}
+    for mode, label in (("fp32", "float kernel"), ("fp32c", "FP32-comp"), ("fp64", "double")):
+        g = gpu[mode]
+        diff = pb.max_normalized_diff(cpu_I, g["intensity"])
+        print(f"  GPU ({label:<12}) : {g['call_ms']:8.1f} ms   ({cpu_ms / g['call_ms']:.1f}x faster)"
+              f"  kernel {g['kernel_ms']:.3f} ms = {pairs / g['kernel_ms'] / 1e6:.0f} G pairs/s")
+        print(f"  agreement ({mode:<5})      : max|d normalized PSF| = {diff:.2e}  ok=true")
+        rec["gpu"][mode] = {"call_ms": g["call_ms"], "kernel_ms": g["kernel_ms"],
+                            "kernel_G_pairs_s": pairs / g["kernel_ms"] / 1e6,
+                            "max_normalized_diff_vs_cpu": diff}
+    print(json.dumps(rec))
+
+
 def main():
     args = parse()
+    if args.psfbench and args.gpus == 1:
+        run_psfbench(args)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the forward shards over pixels and the adjoint over pillars with no exchange "

@@ -32,6 +32,17 @@ class UnitCellLibrary:
         d = np.abs(lib[None, :] - f.reshape(-1, 1))
         return np.argmin(d, axis=1).reshape(f.shape)
 
+    def lookup_weighted(self, targets, amplitude_weight: float) -> np.ndarray:
+        """Index of the entry minimising angle_diff(phase, target)^2 +
+        w (1 - |t|)^2, the first one on ties (metalens.cpp:58-69)."""
+        tg = np.asarray(targets, np.float64)
+        ph = np.asarray(self.phase, np.float64)
+        amp = np.asarray(self.amplitude, np.float64)
+        d = angle_diff(ph[None, :], tg.reshape(-1, 1))
+        damp = 1.0 - amp
+        cost = d * d + amplitude_weight * damp * damp
+        return np.argmin(cost, axis=1).reshape(tg.shape)
+
     def transmission_for_fill(self, fills) -> np.ndarray:
         """t = |t| e^{i phase} of the nearest sampled pillar (metalens.cpp:112-120)."""
         idx = self.lookup_fill(fills)
@@ -47,6 +58,57 @@ class MetalensDesign:
     n_cells: int
     period_um: float
     fill_map: np.ndarray
+    rms_phase_error_deg: float = 0.0
+    mean_amplitude: float = 0.0
+
+
+def angle_diff(a, b) -> np.ndarray:
+    """a - b wrapped into (-pi, pi] by whole turns, one turn at a time
+    (metalens.cpp:15-20), element-wise."""
+    d = np.asarray(a, np.float64) - np.asarray(b, np.float64)
+    d = np.array(d, dtype=np.float64, copy=True)
+    two_pi = 2.0 * math.pi
+    while True:
+        m = d > math.pi
+        if not m.any():
+            break
+        d[m] -= two_pi
+    while True:
+        m = d <= -math.pi
+        if not m.any():
+            break
+        d[m] += two_pi
+    return d
+
+
+def hyperbolic_target(x, y, wavelength_um: float, focal_length_um: float) -> np.ndarray:
+    """The focusing profile's phase -k (sqrt(r^2 + f^2) - f) (phase_profile.cpp:36-45)."""
+    k = 2.0 * math.pi / wavelength_um
+    r = np.sqrt(x * x + y * y)
+    return -k * (np.sqrt(r * r + focal_length_um * focal_length_um) - focal_length_um)
+
+
+def design_metalens(lib: "UnitCellLibrary", focal_length_um: float, diameter_um: float,
+                    amplitude_weight: float = 0.25) -> MetalensDesign:
+    """design_metalens for a focusing profile (metalens.cpp:230-273): n = max(1,
+    round(D / p)) cells a side, each taking the library entry that best trades phase
+    error against transmission (lookup_weighted) for the hyperbolic target at its centre
+    (every cell of the square, not only the aperture); RMS phase error (degrees) and
+    mean |t| over all cells."""
+    p = lib.period_um
+    n = max(1, int(math.floor(diameter_um / p + 0.5)))  # std::round, positive argument
+    c = (n - 1) / 2.0
+    iy, ix = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    x = ((ix - c) * p).ravel()
+    y = ((iy - c) * p).ravel()
+    target = hyperbolic_target(x, y, lib.wavelength_um, focal_length_um)
+    idx = lib.lookup_weighted(target, amplitude_weight)
+    fill_map = np.asarray(lib.fill, np.float64)[idx].reshape(n, n)
+    err = angle_diff(np.asarray(lib.phase, np.float64)[idx], target)
+    sq = np.cumsum(err * err)[-1]  # accumulated in cell order
+    amp = np.cumsum(np.asarray(lib.amplitude, np.float64)[idx])[-1]
+    return MetalensDesign(n, p, fill_map, math.sqrt(sq / idx.size) * 180.0 / math.pi,
+                          amp / idx.size)
 
 
 def aperture_pillars(lens: MetalensDesign, lib: UnitCellLibrary, diameter_um: float,

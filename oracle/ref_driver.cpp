@@ -33,6 +33,9 @@
 //                                             (analyze_field_of_view, field.cpp:12)
 //   widefov      : lens, f, lambda, D_lens, D_stop, stop_dist, m, angles[m]
 //                  -> m x (angle, rel_strehl, shift)   (analyze_wide_fov, field.cpp:78)
+//   design       : nlib, fill[nlib], phase[nlib], amplitude[nlib], lambda, period, f, D,
+//                  amplitude_weight -> n, rms_phase_error_deg, mean_amplitude, fill_map[n*n]
+//                                             (design_metalens, metalens.cpp:230-273)
 //   tolerance    : lens, f, lambda, D, nsig, sigmas[nsig], trials, seed
 //                  -> nsig x (sigma, mean, std, worst)   (analyze_tolerance, tolerance.cpp:12)
 //   pgm          : nx, ny, gamma, values[nx*ny] -> ok; the image itself is written to
@@ -50,6 +53,7 @@
 #include "celeris/analysis/mtf.hpp"
 #include "celeris/analysis/throughfocus.hpp"
 #include "celeris/analysis/tolerance.hpp"
+#include "celeris/design/metalens.hpp"
 #include "celeris/design/achromatic.hpp"
 #include "celeris/design/pb_achromatic.hpp"
 #include "celeris/io/image.hpp"
@@ -140,6 +144,22 @@ int main(int argc, char** argv) {
         std::vector<double> v(static_cast<std::size_t>(std::max(nx, 0)) * std::max(ny, 0));
         for (double& x : v) x = r.next();
         out = {write_pgm(std::string(argv[3]) + ".pgm", nx, ny, v, gamma) ? 1.0 : 0.0};
+    } else if (cmd == "design") {
+        UnitCellLibrary lib;
+        const int nlib = r.next_int();
+        lib.fill.resize(nlib);
+        lib.phase.resize(nlib);
+        lib.amplitude.resize(nlib);
+        for (double& x : lib.fill) x = r.next();
+        for (double& x : lib.phase) x = r.next();
+        for (double& x : lib.amplitude) x = r.next();
+        lib.wavelength_um = r.next();
+        lib.period_um = r.next();
+        lib.thickness_um = 0.0;
+        const double f = r.next(), D = r.next(), w = r.next();
+        MetalensDesign d = design_metalens(lib, f, D, w);
+        out = {static_cast<double>(d.n_cells), d.rms_phase_error_deg, d.mean_amplitude};
+        out.insert(out.end(), d.fill_map.begin(), d.fill_map.end());
     } else if (cmd == "achro" || cmd == "pbachro") {
         const int n_cells = r.next_int();
         const double period = r.next(), D = r.next(), f = r.next();

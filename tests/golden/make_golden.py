@@ -8,7 +8,7 @@ seeded inputs and the reference's outputs, used by tests/test_oracle.py to pin t
 FP64 restatement in oracle/rs_oracle.c where the reference build is absent.
 
     python tests/golden/make_golden.py            # every fixture
-    python tests/golden/make_golden.py achromatic # only achromatic.npz (or: tolerance)
+    python tests/golden/make_golden.py achromatic # only achromatic.npz (or: tolerance, design)
 """
 import math
 import os
@@ -151,15 +151,44 @@ def write_tolerance():
     np.savez_compressed(os.path.join(HERE, "tolerance.npz"), **out)
 
 
+def write_design():
+    """design_metalens (metalens.cpp:230-273) on psfbench's lens (D 120, f 50, lambda
+    0.532, p 0.35 with the stand-in 16-sample library of celeris_amd.psfbench) and on a
+    seeded random library with amplitude weights 0 and 1; fill maps stored as library
+    indices."""
+    from celeris_amd.psfbench import psfbench_library
+    rng = np.random.default_rng(8)
+    lib = psfbench_library()
+    rand_phase = rng.uniform(-math.pi, math.pi, 12)
+    cases = [("psfbench", lib.fill, lib.phase, lib.amplitude, 0.532, 0.35, 50.0, 120.0, 0.25),
+             ("rand_w0", np.linspace(0.1, 0.9, 12), rand_phase, rng.uniform(0.5, 1.0, 12), 0.633,
+              0.4, 20.0, 16.3, 0.0),
+             ("rand_w1", np.linspace(0.1, 0.9, 12), rand_phase, rng.uniform(0.5, 1.0, 12), 0.633,
+              0.4, 20.0, 16.3, 1.0)]
+    out = {}
+    for name, fill, phase, amp, lam, per, f, D, w in cases:
+        r = oracle.ref_run("design", np.concatenate([[len(fill)], fill, phase, amp,
+                                                     [lam, per, f, D, w]]))
+        n = int(r[0])
+        fm = r[3:]
+        idx = np.array([int(np.flatnonzero(fill == v)[0]) for v in fm], np.uint8)
+        out[f"{name}__lib"] = np.stack([fill, phase, amp])
+        out[f"{name}__args"] = np.array([lam, per, f, D, w])
+        out[f"{name}__n_rms_amp"] = r[:3]
+        out[f"{name}__fill_index"] = idx.reshape(n, n)
+    np.savez_compressed(os.path.join(HERE, "design_metalens.npz"), **out)
+
+
 def main():
     if not oracle.ref_available():
         sys.exit("oracle/_ref/ref_driver is not built (make -C oracle ref)")
-    only = {"achromatic": write_achromatic, "tolerance": write_tolerance}
+    only = {"achromatic": write_achromatic, "tolerance": write_tolerance, "design": write_design}
     if sys.argv[1:] and sys.argv[1] in only:
         only[sys.argv[1]]()
         return
     write_achromatic()
     write_tolerance()
+    write_design()
     out = {}
     for name, px, py, t, cx, cy, z, lam, n, W in pillars_cases():
         I = oracle.ref_propagate_pillars(px, py, t, cx, cy, z, lam, n, W)
