@@ -43,9 +43,11 @@ typedef enum crs_status {
 
 /* Arithmetic modes. */
 typedef enum crs_mode {
-    /* FP32 pair arithmetic; the phase argument k*(R - z) is formed and reduced mod
-     * 2*pi in FP64; pillar-tile partial sums are Kahan-compensated; partials over
-     * pillar chunks are combined in FP64.  Default and fastest. */
+    /* FP32 pair arithmetic; each pixel block's phase reference k*(R0 - z) is formed
+     * and reduced mod 2*pi in FP64.  Forward: float sums over at most 64 pillars,
+     * folded into float running sums over at most 4,096 pillars per wave, then FP64
+     * across waves and pillar chunks.  Adjoint: float sums per pixel block, Kahan-
+     * compensated across pixel tiles, then FP64 across runs.  Default and fastest. */
     CRS_MODE_FP32C = 0,
     /* The reference algorithm in FP64 throughout (sqrt, sincos, 1/R, complex MAC). */
     CRS_MODE_FP64 = 1,
