@@ -284,9 +284,14 @@ __device__ __forceinline__ void pillar_run_plain(const PillarRec* tile, int k0, 
 
 // Whether the first-order pair terms meet the error budget for every 4 x 4 block of
 // plane P against pillars within |x| <= ext.x, |y| <= ext.y (um).  With e' the block's
-// half diagonal, e_z' its half depth, z' the nearest row and sin the steepest pillar-
-// to-pixel direction: |D| / R0 <= B = 2 e' sin + 2 e_z' + (e'^2 + e_z'^2) / z', and the
-// dropped terms are 2 x1 y^2 <= B^3 / (16 z'^2) turns and 6 y^2 <= 3 B^2 / (8 z'^2).
+// half diagonal, e_z' its half depth, z' the nearest row and s = rho / R0 a pair's
+// obliquity (at most sn, the steepest pillar-to-pixel direction): |D| / R0 <= B(s) =
+// a s + c, a = 2 e', c = 2 e_z' + (e'^2 + e_z'^2) / z', and the dropped terms are
+// 2 x1 y^2 <= B^3 / (16 R0^2) turns and 6 y^2 <= 3 B^2 / (8 R0^2).  An oblique pair is
+// also a far one, R0^2 >= z'^2 / (1 - s^2), so the bounds are (a s + c)^k (1 - s^2) /
+// z'^2 (k = 3, 2), which rise up to the root s_k of their derivative,
+// s_3 = (sqrt(c^2 + 15 a^2) - c) / (5 a) and s_2 = (sqrt(c^2 + 8 a^2) - c) / (4 a), and
+// fall after it: each is checked at min(s_k, sn).
 __device__ inline bool fp32c_first_order(const DevPlane& P, double2 ext, int bx, int by) {
     const double xl = P.x0 + (P.nx - 1) * P.sx, yl = P.y0 + (P.ny - 1) * P.sy;
     const double zl = P.z0 + (P.ny - 1) * P.sz;
@@ -300,9 +305,13 @@ __device__ inline bool fp32c_first_order(const DevPlane& P, double2 ext, int bx,
     const double hz = 0.5 * (by - 1) * fabs(P.sz) * P.kturn;
     const double e = sqrt(hx * hx + hy * hy);
     const double z = zmin * P.kturn;
-    const double Bd = 2.0 * e * sn + 2.0 * hz + (e * e + hz * hz) / z;
+    const double a = 2.0 * e, c = 2.0 * hz + (e * e + hz * hz) / z;
+    const double s3 = a > 0.0 ? fmin(sn, (sqrt(c * c + 15.0 * a * a) - c) / (5.0 * a)) : 0.0;
+    const double s2 = a > 0.0 ? fmin(sn, (sqrt(c * c + 8.0 * a * a) - c) / (4.0 * a)) : 0.0;
+    const double B3 = a * s3 + c, B2 = a * s2 + c;
     const double z2 = z * z;
-    return Bd * Bd * Bd / (16.0 * z2) <= 1.6e-7 && 3.0 * Bd * Bd / (8.0 * z2) <= 4e-6;
+    return B3 * B3 * B3 * (1.0 - s3 * s3) / (16.0 * z2) <= 1.6e-7 &&
+           3.0 * B2 * B2 * (1.0 - s2 * s2) / (8.0 * z2) <= 4e-6;
 }
 
 // Pixel origin of thread tile t of a plane's tile class (shape): 0 = the 4 x 4 tiles
