@@ -32,16 +32,16 @@ hipError_t launch_stage_pillars(int mode, const double* px, const double* py, co
                                 int npil, int nsets, void* out, hipStream_t s);
 hipError_t launch_forward(int mode, int order, const void* pil, int npil, int nsets,
                           const DevPlane* planes, const FwdBlock* blocks, int nblocks,
-                          double2* partial, hipStream_t s);
-hipError_t launch_fwd_reduce(int restore_phase, const DevPlane* planes, const PixBlock* pblocks,
-                             int nblocks, const double2* partial, double2* field,
+                          void* partial, hipStream_t s);
+hipError_t launch_fwd_reduce(int mode, const DevPlane* planes, const PixBlock* pblocks,
+                             int nblocks, const void* partial, double2* field,
                              double* intensity, hipStream_t s);
 hipError_t launch_stage_pixels(const DevPlane* planes, const PixBlock* pblocks, int nblocks,
                                const double2* u, float4* pix, hipStream_t s);
 hipError_t launch_adjoint(int mode, int order, const void* pil, int npil, const DevPlane* planes,
                           const AdjQuad* quads, const AdjBlock* blocks, int nblocks,
-                          const void* pix_or_u, double2* partial, hipStream_t s);
-hipError_t launch_adj_reduce(const double2* partial, int nslots, int64_t n, double2* g,
+                          const void* pix_or_u, void* partial, hipStream_t s);
+hipError_t launch_adj_reduce(int mode, const void* partial, int nslots, int64_t n, double2* g,
                              hipStream_t s);
 hipError_t launch_mtf(const double2* P, const double* P0, int n, int ns, double p, double f,
                       double lam, double2* acc, double* acc0, float* freq, float* mtf,
@@ -607,12 +607,18 @@ crs_status plan_forward(const crs_context* ctx, int mode, int npil, int nsets,
     return CRS_OK;
 }
 
+// Bytes of one split-sum partial: FP32C keeps each chunk's (or quad run's) sum as a
+// float pair, rounded once from its FP64 wave combine; FP64 and FP32 keep doubles.
+size_t partial_bytes(int mode) {
+    return mode == CRS_MODE_FP32C ? sizeof(float2) : sizeof(double2);
+}
+
 crs_status size_forward(const FwdPlan& plan, const Scratch& sc, hipStream_t stream) {
     crs_status st = ensure(*sc.pil, staged_pillar_bytes(plan.mode, plan.npil, plan.nsets), stream,
                            "staged pillars");
     if (st != CRS_OK) return st;
-    return ensure(*sc.partial, static_cast<size_t>(plan.part_elems) * sizeof(double2), stream,
-                  "forward partials");
+    return ensure(*sc.partial, static_cast<size_t>(plan.part_elems) * partial_bytes(plan.mode),
+                  stream, "forward partials");
 }
 
 crs_status enqueue_forward(const FwdPlan& plan, const Scratch& sc, const double* px,
@@ -628,15 +634,13 @@ crs_status enqueue_forward(const FwdPlan& plan, const Scratch& sc, const double*
     if (ev0) hipEventRecord(ev0, stream);
     if ((st = launch_status(launch_forward(mode, plan.order, sc.pil->ptr, plan.npil, plan.nsets, d_planes,
                                            reinterpret_cast<const FwdBlock*>(sc.tables + plan.o_blocks),
-                                           plan.nblocks, static_cast<double2*>(sc.partial->ptr),
-                                           stream),
+                                           plan.nblocks, sc.partial->ptr, stream),
                             "forward")) != CRS_OK)
         return st;
     if (ev1) hipEventRecord(ev1, stream);
-    return launch_status(launch_fwd_reduce(mode == CRS_MODE_FP32C ? 1 : 0, d_planes,
+    return launch_status(launch_fwd_reduce(mode, d_planes,
                                            reinterpret_cast<const PixBlock*>(sc.tables + plan.o_pblocks),
-                                           plan.npblocks,
-                                           static_cast<const double2*>(sc.partial->ptr),
+                                           plan.npblocks, sc.partial->ptr,
                                            reinterpret_cast<double2*>(field), intensity, stream),
                          "forward reduce");
 }
@@ -758,8 +762,9 @@ crs_status size_adjoint(const AdjPlan& plan, const Scratch& sc, hipStream_t stre
         (st = ensure(*sc.pix, static_cast<size_t>(plan.staged_records) * sizeof(float4), stream,
                      "staged pixels")) != CRS_OK)
         return st;
-    return ensure(*sc.partial, static_cast<size_t>(plan.nsplit * plan.n_out) * sizeof(double2),
-                  stream, "adjoint partials");
+    return ensure(*sc.partial,
+                  static_cast<size_t>(plan.nsplit * plan.n_out) * partial_bytes(plan.mode), stream,
+                  "adjoint partials");
 }
 
 crs_status enqueue_adjoint(const AdjPlan& plan, const Scratch& sc, const double* px,
@@ -793,12 +798,11 @@ crs_status enqueue_adjoint(const AdjPlan& plan, const Scratch& sc, const double*
     if ((st = launch_status(launch_adjoint(mode, plan.order, pil, plan.npil, d_planes,
                                            reinterpret_cast<const AdjQuad*>(sc.tables + plan.o_quads),
                                            reinterpret_cast<const AdjBlock*>(sc.tables + plan.o_blocks),
-                                           plan.nblocks, pix_or_u,
-                                           static_cast<double2*>(sc.partial->ptr), stream),
+                                           plan.nblocks, pix_or_u, sc.partial->ptr, stream),
                             "adjoint")) != CRS_OK)
         return st;
     if (ev1) hipEventRecord(ev1, stream);
-    return launch_status(launch_adj_reduce(static_cast<const double2*>(sc.partial->ptr),
+    return launch_status(launch_adj_reduce(mode, sc.partial->ptr,
                                            static_cast<int>(plan.nsplit), plan.n_out,
                                            reinterpret_cast<double2*>(g), stream),
                          "adjoint reduce");

@@ -11,8 +11,9 @@
 // other side through LDS in tiles that every lane reads by broadcast.  Sums are split
 // over pillar chunks (forward) or pixel quads (adjoint) so that small focal grids
 // still launch thousands of work-groups across the 256 CUs; each split writes a
-// float64 partial and a second kernel adds the partials in a fixed order, so results
-// are bitwise repeatable run to run.
+// partial (float64, or for FP32C a float pair rounded once from its FP64 combine) and a
+// second kernel adds the partials in FP64 in a fixed order, so results are bitwise
+// repeatable run to run.
 //
 // FP32C ("FP32-compensated").  Lengths are measured in turns (x' = k x / 2 pi), so a
 // distance is directly a phase.  Pixels are taken in 4 x 4 blocks.  For each pillar q
@@ -34,7 +35,9 @@
 // that the first-order terms meet the error budget and otherwise takes the second
 // order (fp32c_first_order); the host falls back to 1 x 1 blocks (every pair at its own
 // FP64 R0) where even that would not.  The factor exp(2 pi i z'_ref) is restored in
-// FP64 by the reduction; split partials are added in FP64.
+// FP64 by the reduction; split partials are added in FP64.  Storing the FP32C partials
+// as floats halves the reductions' HBM reads (they are bandwidth-bound) for one more
+// float rounding of each chunk's sum, ~6e-8 of a partial.
 #include "rs_internal.h"
 
 #include <hip/hip_runtime.h>
@@ -344,7 +347,7 @@ template <int BX, int BY>
 __device__ __forceinline__ void fwd_tiles(
     const double2* __restrict__ pos, const float2* __restrict__ ap, int npil,
     const DevPlane& P, const FwdBlock& B, const double2* __restrict__ ext,
-    PillarRec* tile, float2* run, double2* __restrict__ partial) {
+    PillarRec* tile, float2* run, float2* __restrict__ partial) {
     constexpr int N = BX * BY;
     const int nx = P.nx;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -420,7 +423,7 @@ __device__ __forceinline__ void fwd_tiles(
         }
     }
 
-    // Wave 0 adds the 4 waves' sums in FP64, in wave order.
+    // Wave 0 adds the 4 waves' sums in FP64, in wave order, and stores the float pair.
     __syncthreads();
     if (wave != 0 || !active) return;
 #pragma unroll
@@ -435,7 +438,8 @@ __device__ __forceinline__ void fwd_tiles(
                 vr += o.x;
                 vi += o.y;
             }
-            partial[B.part + static_cast<int64_t>(oy + r) * nx + (ox + c)] = make_double2(vr, vi);
+            partial[B.part + static_cast<int64_t>(oy + r) * nx + (ox + c)] =
+                make_float2(static_cast<float>(vr), static_cast<float>(vi));
         }
     }
 }
@@ -447,7 +451,7 @@ template <int BLK>
 __global__ __launch_bounds__(kThreads, 4) void fwd_fp32c(
     const double2* __restrict__ pos, const float2* __restrict__ ap, int npil,
     const DevPlane* __restrict__ planes, const FwdBlock* __restrict__ blocks,
-    const double2* __restrict__ ext, double2* __restrict__ partial) {
+    const double2* __restrict__ ext, float2* __restrict__ partial) {
     constexpr int NMAX = BLK * BLK;
     __shared__ PillarRec tile[kThreads];
     __shared__ float2 run[kWaves * NMAX * 64];
@@ -554,9 +558,11 @@ __global__ __launch_bounds__(kThreads) void fwd_fp64(
 // takes kReduceCols pixels; wave w adds the w-th quarter of the slots, and wave 0 adds
 // the four quarter sums in order (a fixed order: results are bitwise repeatable).
 
+// T: float2 (FP32C partials, each one chunk's sum rounded once) or double2.
+template <typename T>
 __global__ __launch_bounds__(kThreads) void fwd_reduce(
     const DevPlane* __restrict__ planes, const PixBlock* __restrict__ pblocks,
-    const double2* __restrict__ partial, int restore_phase, double2* __restrict__ field,
+    const T* __restrict__ partial, int restore_phase, double2* __restrict__ field,
     double* __restrict__ intensity) {
     __shared__ double2 red[kWaves][kReduceCols];
     const PixBlock B = pblocks[blockIdx.x];
@@ -568,10 +574,10 @@ __global__ __launch_bounds__(kThreads) void fwd_reduce(
     const int s0 = (P.nslots * wave) / kWaves, s1 = (P.nslots * (wave + 1)) / kWaves;
     double er = 0.0, ei = 0.0;
     if (valid) {
-        const double2* __restrict__ src = partial + P.part + p;
+        const T* __restrict__ src = partial + P.part + p;
 #pragma unroll 4
         for (int s = s0; s < s1; ++s) {
-            const double2 v = src[static_cast<int64_t>(s) * npix];
+            const T v = src[static_cast<int64_t>(s) * npix];
             er += v.x;
             ei += v.y;
         }
@@ -756,7 +762,7 @@ __global__ __launch_bounds__(kThreads, 3) void adj_fp32c(
     const double* __restrict__ px, const double* __restrict__ py, int npil,
     const DevPlane* __restrict__ planes, const AdjQuad* __restrict__ quads,
     const AdjBlock* __restrict__ blocks, const float4* __restrict__ pix,
-    double2* __restrict__ partial) {
+    float2* __restrict__ partial) {
     __shared__ float4 tile[kThreads];
     __shared__ double2 red[kWaves];
     const AdjBlock B = blocks[blockIdx.x];
@@ -827,8 +833,8 @@ __global__ __launch_bounds__(kThreads, 3) void adj_fp32c(
         const int q = B.q0 + tid + k * kThreads;
         if (q < npil)
             partial[B.part + q] =
-                make_double2(static_cast<double>(sr[k]) - static_cast<double>(cr[k]),
-                             static_cast<double>(si[k]) - static_cast<double>(ci[k]));
+                make_float2(static_cast<float>(static_cast<double>(sr[k]) - static_cast<double>(cr[k])),
+                            static_cast<float>(static_cast<double>(si[k]) - static_cast<double>(ci[k])));
     }
 }
 
@@ -926,7 +932,8 @@ __global__ __launch_bounds__(kThreads) void adj_fp64(
 
 // Adjoint reduction: g[idx] = sum over slots of partial[slot*n + idx]; kReduceCols
 // outputs per work-group, slot quarters per wave, added in order as in fwd_reduce.
-__global__ __launch_bounds__(kThreads) void adj_reduce(const double2* __restrict__ partial,
+template <typename T>
+__global__ __launch_bounds__(kThreads) void adj_reduce(const T* __restrict__ partial,
                                                        int nslots, int64_t n,
                                                        double2* __restrict__ g) {
     __shared__ double2 red[kWaves][kReduceCols];
@@ -938,7 +945,7 @@ __global__ __launch_bounds__(kThreads) void adj_reduce(const double2* __restrict
     if (valid) {
 #pragma unroll 4
         for (int s = s0; s < s1; ++s) {
-            const double2 v = partial[static_cast<int64_t>(s) * n + idx];
+            const T v = partial[static_cast<int64_t>(s) * n + idx];
             gr += v.x;
             gi += v.y;
         }
@@ -1048,9 +1055,10 @@ hipError_t launch_stage_pillars(int mode, const double* px, const double* py, co
 
 hipError_t launch_forward(int mode, int order, const void* pil, int npil, int nsets,
                           const DevPlane* planes, const FwdBlock* blocks, int nblocks,
-                          double2* partial, hipStream_t s) {
+                          void* partial_v, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
     if (mode == 0) {
+        float2* partial = static_cast<float2*>(partial_v);
         const StagedLayout L = fp32c_staged_layout(npil, nsets);
         const unsigned char* base = static_cast<const unsigned char*>(pil);
         const double2* pos = reinterpret_cast<const double2*>(base + L.pos);
@@ -1064,20 +1072,24 @@ hipError_t launch_forward(int mode, int order, const void* pil, int npil, int ns
             fwd_fp32c<kBlk><<<nblocks, kThreads, 0, s>>>(pos, ap, npil, planes, blocks, ext, partial);
     } else if (mode == 2) {
         fwd_fp32<<<nblocks, kThreads, 0, s>>>(static_cast<const float4*>(pil), npil, planes,
-                                              blocks, partial);
+                                              blocks, static_cast<double2*>(partial_v));
     } else {
         fwd_fp64<<<nblocks, kThreads, 0, s>>>(static_cast<const double4*>(pil), npil, planes,
-                                              blocks, partial);
+                                              blocks, static_cast<double2*>(partial_v));
     }
     return hipGetLastError();
 }
 
-hipError_t launch_fwd_reduce(int restore_phase, const DevPlane* planes, const PixBlock* pblocks,
-                             int nblocks, const double2* partial, double2* field,
+hipError_t launch_fwd_reduce(int mode, const DevPlane* planes, const PixBlock* pblocks,
+                             int nblocks, const void* partial, double2* field,
                              double* intensity, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
-    fwd_reduce<<<nblocks, kThreads, 0, s>>>(planes, pblocks, partial, restore_phase, field,
-                                            intensity);
+    if (mode == 0)  // FP32C: float partials; the reduction restores exp(2 pi i z'_ref)
+        fwd_reduce<float2><<<nblocks, kThreads, 0, s>>>(
+            planes, pblocks, static_cast<const float2*>(partial), 1, field, intensity);
+    else
+        fwd_reduce<double2><<<nblocks, kThreads, 0, s>>>(
+            planes, pblocks, static_cast<const double2*>(partial), 0, field, intensity);
     return hipGetLastError();
 }
 
@@ -1090,9 +1102,10 @@ hipError_t launch_stage_pixels(const DevPlane* planes, const PixBlock* pblocks, 
 
 hipError_t launch_adjoint(int mode, int order, const void* pil, int npil, const DevPlane* planes,
                           const AdjQuad* quads, const AdjBlock* blocks, int nblocks,
-                          const void* pix_or_u, double2* partial, hipStream_t s) {
+                          const void* pix_or_u, void* partial_v, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
     if (mode == 0) {
+        float2* partial = static_cast<float2*>(partial_v);
         // FP32C reads the caller's positions directly: pil = {px, py}.
         const double* const* xy = static_cast<const double* const*>(pil);
         const float4* pix = static_cast<const float4*>(pix_or_u);
@@ -1103,20 +1116,25 @@ hipError_t launch_adjoint(int mode, int order, const void* pil, int npil, const 
     } else if (mode == 2) {
         adj_fp32<<<nblocks, kThreads, 0, s>>>(static_cast<const float4*>(pil), npil, planes, quads,
                                               blocks, static_cast<const double2*>(pix_or_u),
-                                              partial);
+                                              static_cast<double2*>(partial_v));
     } else {
         adj_fp64<<<nblocks, kThreads, 0, s>>>(static_cast<const double4*>(pil), npil, planes, quads,
                                               blocks, static_cast<const double2*>(pix_or_u),
-                                              partial);
+                                              static_cast<double2*>(partial_v));
     }
     return hipGetLastError();
 }
 
-hipError_t launch_adj_reduce(const double2* partial, int nslots, int64_t n, double2* g,
+hipError_t launch_adj_reduce(int mode, const void* partial, int nslots, int64_t n, double2* g,
                              hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    adj_reduce<<<static_cast<unsigned>((n + kReduceCols - 1) / kReduceCols), kThreads, 0, s>>>(
-        partial, nslots, n, g);
+    const unsigned grid = static_cast<unsigned>((n + kReduceCols - 1) / kReduceCols);
+    if (mode == 0)
+        adj_reduce<float2><<<grid, kThreads, 0, s>>>(static_cast<const float2*>(partial), nslots,
+                                                     n, g);
+    else
+        adj_reduce<double2><<<grid, kThreads, 0, s>>>(static_cast<const double2*>(partial),
+                                                      nslots, n, g);
     return hipGetLastError();
 }
 
