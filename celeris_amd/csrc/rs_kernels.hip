@@ -183,16 +183,19 @@ struct BlockGeom {
     float tex[BX], ex2[BX], tey[BY], eyz[BY];
 };
 
-// One pillar against one block: th0 = frac(R0 - zeta), h = 1/(2 R0), the amplitude
-// pair c2 = 2 c1, cn = -c1 (c1 = |t| (k/2pi) / R0'), and the separable parts of
-// x1 = D h: a[c] = (2 e_c dx0 + e_c^2) h, b[r] = (2 e_r dy0 + e_r^2 + dz2_r) h.
+// One pillar against one block: th0 = frac(R0 - zeta), h = 1/(2 R0), c1 = |t| (k/2pi)
+// / R0', and the separable parts of D (first order) or of x1 = D h (second order):
+// a[c] = 2 e_c dx0 + e_c^2, b[r] = 2 e_r dy0 + e_r^2 + dz2_r, times h for GEN = 1.
+//   GEN = 0:  p = h^3, q = -2 c1 h^2, c = c1   (block_pairs folds h into them)
+//   GEN = 1:  p = 2 c1, q = -c1
+//   1 x 1 blocks: c = c1 only (R = R0).
 template <int BX, int BY>
 struct PairState {
-    float th0, h, c2, cn;
+    float th0, h, p, q, c;
     float a[BX], b[BY];
 };
 
-template <int BX, int BY>
+template <int GEN, int BX, int BY>
 __device__ __forceinline__ void pair_state(const PillarRec& T, const BlockGeom<BX, BY>& G,
                                            PairState<BX, BY>& S) {
     const double dX0 = G.FXc - T.x, dY0 = G.FYc - T.y;
@@ -201,27 +204,44 @@ __device__ __forceinline__ void pair_state(const PillarRec& T, const BlockGeom<B
     S.th0 = static_cast<float>(__builtin_amdgcn_fract(R0 - T.zeta));
     S.h = 0.5f * rf;
     const float c1 = T.amp * rf;
-    S.c2 = c1 + c1;
-    S.cn = -c1;
+    if constexpr (BX * BY == 1) {
+        S.c = c1;
+    } else if constexpr (GEN) {
+        S.p = c1 + c1;
+        S.q = -c1;
+    } else {
+        const float h2 = S.h * S.h;
+        S.p = h2 * S.h;
+        S.q = -2.0f * c1 * h2;
+        S.c = c1;
+    }
     if constexpr (BX * BY > 1) {
         const float dx0 = static_cast<float>(dX0), dy0 = static_cast<float>(dY0);
 #pragma unroll
-        for (int c = 0; c < BX; ++c) S.a[c] = fmaf(G.tex[c], dx0, G.ex2[c]) * S.h;
+        for (int c = 0; c < BX; ++c) {
+            S.a[c] = fmaf(G.tex[c], dx0, G.ex2[c]);
+            if constexpr (GEN) S.a[c] *= S.h;
+        }
 #pragma unroll
-        for (int r = 0; r < BY; ++r) S.b[r] = fmaf(G.tey[r], dy0, G.eyz[r]) * S.h;
+        for (int r = 0; r < BY; ++r) {
+            S.b[r] = fmaf(G.tey[r], dy0, G.eyz[r]);
+            if constexpr (GEN) S.b[r] *= S.h;
+        }
     }
 }
 
-// The BX x BY pairs of one pillar.  With x1 = a[c] + b[r], y = x1 h = D / (4 R0^2):
+// The BX x BY pairs of one pillar.  With x1 = D h, y = x1 h = D / (4 R0^2):
 //   GEN = 0:  R - R0 = x1 w,  w = 1 - y;             R0/R = 2 w - 1 = 1 - 2 y
 //   GEN = 1:  R - R0 = x1 w,  w = 1 - y (1 - 2 y);   R0/R = 1 + y (6 y - 2)
 // and E += c1 (R0/R) (cos 2 pi th, sin 2 pi th), th = th0 + (R - R0), in turns.
-// Both are Horner forms: 6 and 9 FP32 ops per pair besides the two transcendentals.
+// GEN = 0 works on D itself: th = th0 + D (h - D h^3), A = c1 + (-2 c1 h^2) D (h folded
+// into per-pillar constants); GEN = 1 on x1 in Horner form.  6 and 9 FP32 ops per pair
+// besides the two transcendentals.
 template <int GEN, int BX, int BY>
 __device__ __forceinline__ void block_pairs(const PairState<BX, BY>& S, float (&ar)[BX * BY],
                                             float (&ai)[BX * BY]) {
     if constexpr (BX * BY == 1) {
-        const float A = S.c2 + S.cn;
+        const float A = S.c;
         ar[0] = fmaf(A, __builtin_amdgcn_cosf(S.th0), ar[0]);
         ai[0] = fmaf(A, __builtin_amdgcn_sinf(S.th0), ai[0]);
     } else {
@@ -229,17 +249,17 @@ __device__ __forceinline__ void block_pairs(const PairState<BX, BY>& S, float (&
         for (int r = 0; r < BY; ++r) {
 #pragma unroll
             for (int c = 0; c < BX; ++c) {
-                const float x1 = S.a[c] + S.b[r];
-                float w, A;
+                const float d = S.a[c] + S.b[r];  // D (GEN = 0) or x1 (GEN = 1)
+                float th, A;
                 if constexpr (GEN) {
-                    const float y = x1 * S.h;
-                    w = fmaf(-y, fmaf(-2.0f, y, 1.0f), 1.0f);
-                    A = fmaf(fmaf(3.0f * S.c2, y, -S.c2), y, -S.cn);
+                    const float y = d * S.h;
+                    const float w = fmaf(-y, fmaf(-2.0f, y, 1.0f), 1.0f);
+                    A = fmaf(fmaf(3.0f * S.p, y, -S.p), y, -S.q);
+                    th = fmaf(d, w, S.th0);
                 } else {
-                    w = fmaf(-x1, S.h, 1.0f);
-                    A = fmaf(S.c2, w, S.cn);
+                    th = fmaf(d, fmaf(-d, S.p, S.h), S.th0);
+                    A = fmaf(S.q, d, S.c);
                 }
-                const float th = fmaf(x1, w, S.th0);
                 const int e = r * BX + c;
                 ar[e] = fmaf(A, __builtin_amdgcn_cosf(th), ar[e]);
                 ai[e] = fmaf(A, __builtin_amdgcn_sinf(th), ai[e]);
@@ -256,16 +276,16 @@ __device__ __forceinline__ void pillar_run(const PillarRec* tile, int k0, int k1
                                            const BlockGeom<BX, BY>& G, float (&ar)[BX * BY],
                                            float (&ai)[BX * BY]) {
     PairState<BX, BY> s0, s1;
-    pair_state(tile[k0], G, s0);
+    pair_state<GEN>(tile[k0], G, s0);
     int k = k0 + 1;
     for (; k + 1 < k1; k += 2) {
-        pair_state(tile[k], G, s1);
+        pair_state<GEN>(tile[k], G, s1);
         block_pairs<GEN>(s0, ar, ai);
-        pair_state(tile[k + 1], G, s0);
+        pair_state<GEN>(tile[k + 1], G, s0);
         block_pairs<GEN>(s1, ar, ai);
     }
     if (k < k1) {
-        pair_state(tile[k], G, s1);
+        pair_state<GEN>(tile[k], G, s1);
         block_pairs<GEN>(s0, ar, ai);
         block_pairs<GEN>(s1, ar, ai);
     } else {
@@ -280,7 +300,7 @@ __device__ __forceinline__ void pillar_run_plain(const PillarRec* tile, int k0, 
                                                  float (&ar)[BX * BY], float (&ai)[BX * BY]) {
     for (int k = k0; k < k1; ++k) {
         PairState<BX, BY> st;
-        pair_state(tile[k], G, st);
+        pair_state<GEN>(tile[k], G, st);
         block_pairs<GEN>(st, ar, ai);
     }
 }
@@ -446,9 +466,12 @@ __device__ __forceinline__ void fwd_tiles(
 
 // BLK = 4: 4 x 4 tiles plus the edge classes (1 x 4, 4 x 1, 1 x 1), chosen per
 // work-group from its descriptor; BLK = 1: 1 x 1 tiles only (every pair at its own
-// FP64 reference distance).
+// FP64 reference distance).  Bounded for 3 work-groups per CU: the first-order loop
+// compiles to 143 VGPRs and runs 6% faster than the same loop squeezed into 128
+// (measured at 91,809 pillars x 161^2); a 4th work-group per CU does not help, since the
+// kernel is issue-bound either way.
 template <int BLK>
-__global__ __launch_bounds__(kThreads, 4) void fwd_fp32c(
+__global__ __launch_bounds__(kThreads, 3) void fwd_fp32c(
     const double2* __restrict__ pos, const float2* __restrict__ ap, int npil,
     const DevPlane* __restrict__ planes, const FwdBlock* __restrict__ blocks,
     const double2* __restrict__ ext, float2* __restrict__ partial) {
