@@ -196,3 +196,28 @@ def test_adjoint_general_and_first_order_planes(torch_cuda, rsctx, mode, tol):
         ep = rel_l2(gp, oracle.adjoint(L.px, L.py, up, [p]))
         print(f"  plane {p.nx}x{p.ny} relL2 {ep:.3e}")
         assert ep <= tol
+
+
+def test_inverse_design_loop_raises_encircled_energy(torch_cuda):
+    """The design loop the adjoint feeds: Adam on the phases of a 24 x 24 lens from
+    seeded random phases.  Encircled energy within 1.22 lambda f / D of the centre of a
+    41 x 41 window at f must rise several-fold, and every step's gradient comes from the
+    adjoint kernel (FP32C)."""
+    from celeris_amd import configs
+    from celeris_amd.inverse_design import optimize_phase
+
+    px, py = configs.square_grid(24, 0.4)
+    lam, f = 0.532, 15.0
+    k = 2 * math.pi / lam
+    D = 24 * 0.4
+    W = 3.0
+    plane = Plane.psf(0.0, 0.0, f, k, 41, W)
+    rng = np.random.default_rng(5)
+    phi0 = rng.uniform(-math.pi, math.pi, px.size)
+    amp = np.full(px.size, 0.9)
+    print("[adjoint] inverse design loop", flush=True)
+    res = optimize_phase(px, py, amp, phi0, plane, radius=1.22 * lam * f / D, steps=40, lr=0.15)
+    ee = res.encircled_energy
+    print(f"inverse design EE {ee[0]:.4f} -> {ee[-1]:.4f}")
+    assert len(ee) == 41 and res.phase.shape == phi0.shape
+    assert ee[-1] > 3.0 * ee[0] and ee[-1] > 0.3
