@@ -140,9 +140,11 @@ static void fwd_row(fwd_job *J, int64_t r) {
 
 static void *fwd_worker(void *arg) {
     fwd_job *J = (fwd_job *)arg;
-    /* parallel_rows: worker w takes rows w, w+W, ... (focal.cpp:26) */
-    for (int64_t r = J->w; r < J->rt->total_rows; r += J->workers)
-        if (r % J->row_stride == 0) fwd_row(J, r);
+    /* parallel_rows: worker w takes rows w, w+W, ... (focal.cpp:26).  A row sample
+     * (row_stride > 1) is striped the same way over the sampled rows, so every worker
+     * stays busy whatever the stride and the worker count have in common. */
+    for (int64_t s = J->w; s * J->row_stride < J->rt->total_rows; s += J->workers)
+        fwd_row(J, s * J->row_stride);
     return NULL;
 }
 
@@ -195,8 +197,9 @@ typedef struct {
 
 static void *adj_worker(void *arg) {
     adj_job *J = (adj_job *)arg;
-    for (int q = J->w; q < J->npil; q += J->workers) {
-        if (q % J->pillar_stride) continue;
+    /* sampled pillars q = i * pillar_stride, striped over workers by i */
+    for (int64_t i = J->w; i * J->pillar_stride < J->npil; i += J->workers) {
+        const int q = (int)(i * J->pillar_stride);
         const double qx = J->px[q], qy = J->py[q];
         for (int s = 0; s < J->nsets; ++s) {
             double gr = 0.0, gi = 0.0;
