@@ -325,8 +325,7 @@ int fp32c_order(const std::vector<DevPlane>& dp) {
         const double ey = 1.5 * std::fabs(d.sy) * d.kturn;
         const double ez = 1.5 * std::fabs(d.sz) * d.kturn;
         const double e = std::sqrt(ex * ex + ey * ey) + ez;
-        const double zlast = d.z0 + (d.ny - 1) * d.sz;
-        const double zmin = d.kturn * std::min(std::fabs(d.z0), std::fabs(zlast));
+        const double zmin = d.kturn * plane_min_abs_z(d.z0, d.sz, d.ny);
         if (!(std::isfinite(e) && zmin > 0.0)) {
             if (std::isfinite(e)) return kExactOrder;  // a z = 0 row: no distance bound
             continue;  // NaN/inf geometry (n == 1 windows): NaN out, as the reference

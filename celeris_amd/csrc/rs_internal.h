@@ -34,6 +34,17 @@ constexpr int kPQ = 4;
 // FP32C adjoint: columns per LDS tile of a quad (kAdjCols x 4 pixel records).
 constexpr int kAdjCols = kThreads / kQuad;
 
+// Lower bound on |z| of every row z0 + j sz (0 <= j < ny) of a plane and of every
+// pixel block's centre: the nearer end when all rows lie on one side of the lens
+// plane, else 0 (a block of rows straddling z = 0 can have its centre there).  The
+// series-order bounds need the nearest distance a pixel block can have to the pillars'
+// plane; 0 sends the plane to the exact (1 x 1) path.
+__host__ __device__ inline double plane_min_abs_z(double z0, double sz, int ny) {
+    const double zl = z0 + (ny - 1) * sz;
+    if ((z0 > 0.0 && zl > 0.0) || (z0 < 0.0 && zl < 0.0)) return fmin(fabs(z0), fabs(zl));
+    return 0.0;
+}
+
 // FP32C staged pillar buffer (forward): positions (double2, npil), {|t|, arg t / 2 pi}
 // (float2, nsets x npil), the staging blocks' maxima of |x|, |y| (double2 per block of
 // kThreads staged elements) and their overall maximum (one double2, um).

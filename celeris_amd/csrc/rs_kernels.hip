@@ -317,10 +317,9 @@ __device__ __forceinline__ void pillar_run_plain(const PillarRec* tile, int k0, 
 // fall after it: each is checked at min(s_k, sn).
 __device__ inline bool fp32c_first_order(const DevPlane& P, double2 ext, int bx, int by) {
     const double xl = P.x0 + (P.nx - 1) * P.sx, yl = P.y0 + (P.ny - 1) * P.sy;
-    const double zl = P.z0 + (P.ny - 1) * P.sz;
     const double rx = fmax(fabs(P.x0), fabs(xl)) + ext.x;
     const double ry = fmax(fabs(P.y0), fabs(yl)) + ext.y;
-    const double zmin = fmin(fabs(P.z0), fabs(zl));
+    const double zmin = plane_min_abs_z(P.z0, P.sz, P.ny);
     const double rho2 = rx * rx + ry * ry;
     const double sn = sqrt(rho2 / (rho2 + zmin * zmin));
     const double hx = 0.5 * (bx - 1) * fabs(P.sx) * P.kturn;
