@@ -48,6 +48,10 @@ def parse():
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-split autotuner")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="also run a plain device copy of BYTES (reference for HBM counters)")
+    ap.add_argument("--lens", default="headline", choices=["headline", "config1"],
+                    help="headline: 91,809 pillars (303 x 303 at 0.35 um, lambda 0.633, f 100) "
+                         "on a --grid focal grid; config1: the small case, 4,096 pillars (64 x 64 "
+                         "at 0.4 um, lambda 0.532, f 50) on 81 x 81 over +-5 um")
     ap.add_argument("--psfbench", action="store_true",
                     help="run the reference's psfbench instead (cli/bench.cpp:96-170)")
     ap.add_argument("--diameter", type=float, default=120.0, help="psfbench lens diameter (um)")
@@ -212,7 +216,7 @@ def main():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
     dev = torch.device("cuda:0")
-    lens = configs.config2(n=args.grid)
+    lens = configs.config2(n=args.grid) if args.lens == "headline" else configs.config1()
     plane = lens.planes[0]
     npix = plane.npix
     pairs = lens.npil * npix
@@ -326,8 +330,10 @@ def main():
         "dtype": "f32" if args.mode != "fp64" else "f64",
         "data": "synthetic",
         "config": {
-            "workload": (f"RS far field fwd+adjoint, {lens.npil} pillars (303x303 @0.35um, "
-                         f"lambda 0.633, f 100) x {plane.nx}x{plane.ny} focal grid, mode {args.mode}"),
+            "workload": (f"RS far field fwd+adjoint, {lens.npil} pillars "
+                         + ("(303x303 @0.35um, lambda 0.633, f 100)" if args.lens == "headline"
+                            else "(64x64 @0.4um, lambda 0.532, f 50)")
+                         + f" x {plane.nx}x{plane.ny} focal grid, mode {args.mode}"),
             "pillars": lens.npil,
             "grid": plane.nx,
             "pairs_per_step": pairs,

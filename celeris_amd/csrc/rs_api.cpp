@@ -38,9 +38,9 @@ hipError_t launch_fwd_reduce(int mode, const DevPlane* planes, const PixBlock* p
                              double* intensity, hipStream_t s);
 hipError_t launch_stage_pixels(const DevPlane* planes, const PixBlock* pblocks, int nblocks,
                                const double2* u, float4* pix, hipStream_t s);
-hipError_t launch_adjoint(int mode, int order, const void* pil, int npil, const DevPlane* planes,
-                          const AdjQuad* quads, const AdjBlock* blocks, int nblocks,
-                          const void* pix_or_u, void* partial, hipStream_t s);
+hipError_t launch_adjoint(int mode, int order, const void* pil, int npil,
+                          const DevPlane* planes, const AdjQuad* quads, const AdjBlock* blocks,
+                          int nblocks, const void* pix_or_u, void* partial, hipStream_t s);
 hipError_t launch_adj_reduce(int mode, const void* partial, int nslots, int64_t n, double2* g,
                              hipStream_t s);
 hipError_t launch_mtf(const double2* P, const double* P0, int n, int ns, double p, double f,
@@ -550,8 +550,11 @@ crs_status plan_forward(const crs_context* ctx, int mode, int npil, int nsets,
     const int target = ctx->fwd_target[mode] > 0 ? ctx->fwd_target[mode] : default_target(mode, false);
     int64_t nsplit = std::max<int64_t>(1, ceil_div(target, base_blocks));
     if (mode == CRS_MODE_FP32C) nsplit = std::max<int64_t>(nsplit, ceil_div(npil, kMaxChunk));
-    nsplit = std::min<int64_t>(nsplit, std::max<int64_t>(1, npil / kThreads));
-    const int64_t chunk = ceil_div(ceil_div(npil, nsplit), kThreads) * kThreads;
+    // Chunks are whole LDS tiles of kThreads pillars, or, when a problem is too small to
+    // fill the chip that way, multiples of kMinChunk (each wave then takes kMinChunk / 4).
+    nsplit = std::min<int64_t>(nsplit, std::max<int64_t>(1, npil / kMinChunk));
+    int64_t chunk = ceil_div(npil, nsplit);
+    chunk = chunk >= kThreads ? ceil_div(chunk, kThreads) * kThreads : ceil_div(chunk, kMinChunk) * kMinChunk;
     nsplit = ceil_div(npil, chunk);
     if (base_blocks * nsplit > (int64_t(1) << 31) - 1) {
         set_error("launch too large (%lld work-groups)", (long long)(base_blocks * nsplit));
@@ -647,7 +650,6 @@ crs_status enqueue_forward(const FwdPlan& plan, const Scratch& sc, const double*
 
 crs_status plan_adjoint(const crs_context* ctx, int mode, int npil, int nsets,
                         const crs_plane* planes, int nplanes, AdjPlan& plan) {
-    const int PQ = mode == CRS_MODE_FP32C ? kPQ : 1;
     plan.mode = mode;
     plan.npil = npil;
     plan.nsets = nsets;
@@ -684,11 +686,12 @@ crs_status plan_adjoint(const crs_context* ctx, int mode, int npil, int nsets,
     set_q0[nsets] = static_cast<int64_t>(quads.size());
     if (quads.size() > static_cast<size_t>(INT32_MAX)) { set_error("too many rows"); return CRS_ERR_INVALID; }
 
-    const int64_t groups = ceil_div(npil, static_cast<int64_t>(kThreads) * PQ);
-    const int64_t base_blocks = groups * nsets;
     int64_t max_quads = 1;
     for (int s = 0; s < nsets; ++s) max_quads = std::max<int64_t>(max_quads, set_q0[s + 1] - set_q0[s]);
     const int target = ctx->adj_target[mode] > 0 ? ctx->adj_target[mode] : default_target(mode, true);
+    const int PQ = mode == CRS_MODE_FP32C ? kPQ : 1;
+    const int64_t groups = ceil_div(npil, static_cast<int64_t>(kThreads) * PQ);
+    const int64_t base_blocks = groups * nsets;
     int64_t nsplit = std::max<int64_t>(1, ceil_div(target, base_blocks));
     nsplit = std::min<int64_t>(nsplit, max_quads);
     if (base_blocks * nsplit > (int64_t(1) << 31) - 1) { set_error("launch too large"); return CRS_ERR_INVALID; }

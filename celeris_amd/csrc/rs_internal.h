@@ -26,11 +26,15 @@ constexpr int kReduceCols = 64;
 // Longest pillar chunk one FP32C forward work-group sums (each wave a quarter of it,
 // in float, folded every 64 pillars): bounds the float accumulation error.
 constexpr int kMaxChunk = 16384;
+// Shortest pillar chunk of a forward split (small problems only; 16 pillars per wave).
+constexpr int kMinChunk = 64;
 // Plan order code of the 1 x 1 (unexpanded) FP32C kernels (fp32c_order); codes 0..3
 // select the 4 x 4 kernels, which choose their series order per work-group.
 constexpr int kExactOrder = 4;
-// FP32C adjoint: pillars per thread (q0 + tid + k*kThreads, k < kPQ).
-constexpr int kPQ = 4;
+// FP32C adjoint: pillars per thread (q0 + tid + k*kThreads, k < kPQ).  One: four per
+// thread (sharing each block's broadcast pixel records) measured 5-6% slower at 91,809
+// pillars x 161^2 and 321^2 and at 1,002,001 x 1024^2.
+constexpr int kPQ = 1;
 // FP32C adjoint: columns per LDS tile of a quad (kAdjCols x 4 pixel records).
 constexpr int kAdjCols = kThreads / kQuad;
 

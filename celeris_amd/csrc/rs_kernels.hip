@@ -656,7 +656,7 @@ __global__ __launch_bounds__(kThreads) void stage_pixels_fp32c(
 }
 
 // ---------------------------------------------------------------------------------
-// Adjoint, FP32C.  Thread owns kPQ pillars (FP64 positions in registers); a quad's pixel
+// Adjoint, FP32C.  Thread owns kPQ = 1 pillar (FP64 position in registers); a quad's pixel
 // records stream through LDS 64 columns at a time and are consumed in BX x BY blocks
 // (4 x 4, or 1 x 1 when every pair needs its own reference distance).  Per pair
 //     conj(G) u = U / R0' (R0/R) exp(2 pi i (chi - th0 - (R - R0))),   U = |u| k/2pi,
@@ -1122,9 +1122,9 @@ hipError_t launch_stage_pixels(const DevPlane* planes, const PixBlock* pblocks, 
     return hipGetLastError();
 }
 
-hipError_t launch_adjoint(int mode, int order, const void* pil, int npil, const DevPlane* planes,
-                          const AdjQuad* quads, const AdjBlock* blocks, int nblocks,
-                          const void* pix_or_u, void* partial_v, hipStream_t s) {
+hipError_t launch_adjoint(int mode, int order, const void* pil, int npil,
+                          const DevPlane* planes, const AdjQuad* quads, const AdjBlock* blocks,
+                          int nblocks, const void* pix_or_u, void* partial_v, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
     if (mode == 0) {
         float2* partial = static_cast<float2*>(partial_v);

@@ -221,3 +221,34 @@ def test_inverse_design_loop_raises_encircled_energy(torch_cuda):
     print(f"inverse design EE {ee[0]:.4f} -> {ee[-1]:.4f}")
     assert len(ee) == 41 and res.phase.shape == phi0.shape
     assert ee[-1] > 3.0 * ee[0] and ee[-1] > 0.3
+
+
+def test_small_problem_work_splits(torch_cuda, rsctx):
+    """Small problems: at a high work-group target the forward splits the pillar sum into
+    64-pillar chunks (whole 256-pillar tiles at a low one) and the adjoint into more quad
+    runs.  Both splits against the oracle, and against each other."""
+    L = configs.config1()
+    planes = L.planes
+    rng = np.random.default_rng(11)
+    npix = planes[0].npix
+    u = rng.standard_normal(npix) + 1j * rng.standard_normal(npix)
+    ref_g = oracle.adjoint(L.px, L.py, u, planes)
+    ref_f = oracle.forward(L.px, L.py, L.t, planes)
+    dpx, dpy = torch_cuda.from_numpy(L.px).cuda(), torch_cuda.from_numpy(L.py).cuda()
+    dt, du = torch_cuda.from_numpy(L.t).cuda(), torch_cuda.from_numpy(u).cuda()
+    out = {}
+    for target, pq in ((4096, 1), (64, 4)):  # pq: key of the result only
+        rsctx.set_tuning("fp32c", target, target)
+        print(f"[adjoint] small problem, work-group target {target}", flush=True)
+        f, _ = rsctx.forward(dpx, dpy, dt, planes, mode="fp32c")
+        g = rsctx.adjoint(dpx, dpy, du, planes, nsets=1, mode="fp32c")
+        torch_cuda.cuda.synchronize()
+        name, groups = rsctx.last_launch(True)
+        fname, fgroups = rsctx.last_launch(False)
+        print(f"  {fname} x {fgroups}, {name} x {groups}")
+        assert name == "crs::adj_fp32c<4, 4>" and fname == "crs::fwd_fp32c<4>"
+        out[pq] = (f.cpu().numpy(), g.cpu().numpy(), fgroups)
+        assert rel_l2(out[pq][0], ref_f) <= 1e-5 and rel_l2(out[pq][1], ref_g) <= 1e-5
+    rsctx.set_tuning("fp32c", 0, 0)
+    assert out[1][2] > out[4][2]  # finer forward chunks at the high target
+    assert rel_l2(out[1][1], out[4][1]) <= 1e-6

@@ -199,12 +199,15 @@ def test_analyze_field_of_view_and_wide_fov(torch_cuda, mode, tol):
 TOL_GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "tolerance.npz")
 
 
-@pytest.mark.parametrize("mode,rtol,stdtol", [("fp64", 1e-9, 1e-8), ("fp32c", 1e-5, 2e-6)])
+@pytest.mark.parametrize("mode,rtol,stdtol", [("fp64", 1e-9, 3e-8), ("fp32c", 1e-5, 2e-6)])
 @pytest.mark.parametrize("name", ["cli", "single"])
 def test_analyze_tolerance_vs_reference(torch_cuda, name, mode, rtol, stdtol):
     """analyze_tolerance (tolerance.cpp:12-48) against the reference's own sweep: the
     same jitter draws (std::mt19937 / std::normal_distribution seeded per trial), every
-    trial's Strehl from the batched grids."""
+    trial's Strehl from the batched grids.  The spread is the reference's
+    sqrt(max(0, sum2/n - mean^2)), which cancels to a few ulp of mean^2 (~0.5): a zero
+    spread (sigma 0, or one trial) reads anywhere up to sqrt(4 * 2^-53 * 0.5) ~ 1.5e-8,
+    hence 3e-8 absolute in FP64."""
     import oracle
 
     d = np.load(TOL_GOLDEN)
