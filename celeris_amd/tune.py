@@ -17,7 +17,7 @@ from typing import Dict, Optional, Sequence
 
 from .rs import Context, Plane
 
-CANDIDATES = (1024, 2048, 3072, 4096, 6144, 8192, 12288)
+CANDIDATES = (1024, 2048, 3072, 4096, 6144, 8192, 12288, 16384, 24576)
 CACHE = os.environ.get("CELERIS_RS_TUNE_CACHE",
                        os.path.join(os.path.expanduser("~"), ".cache", "celeris_rs_tune.json"))
 
