@@ -268,34 +268,11 @@ __device__ __forceinline__ void block_pairs(const PairState<BX, BY>& S, float (&
     }
 }
 
-// Pillars tile[k0 .. k1) (k0 < k1) against one block; the next pillar's FP64 distance
-// work is issued alongside the current pillar's pairs (two states, alternating, so no
-// state is copied between iterations).
+// Pillars tile[k0 .. k1) against one block, one pillar state at a time.  (Issuing the
+// next pillar's FP64 distance work alongside the current pillar's pairs, with two
+// states alternating, took 143 VGPRs and 3 work-groups per CU and ran 3% slower.)
 template <int GEN, int BX, int BY>
 __device__ __forceinline__ void pillar_run(const PillarRec* tile, int k0, int k1,
-                                           const BlockGeom<BX, BY>& G, float (&ar)[BX * BY],
-                                           float (&ai)[BX * BY]) {
-    PairState<BX, BY> s0, s1;
-    pair_state<GEN>(tile[k0], G, s0);
-    int k = k0 + 1;
-    for (; k + 1 < k1; k += 2) {
-        pair_state<GEN>(tile[k], G, s1);
-        block_pairs<GEN>(s0, ar, ai);
-        pair_state<GEN>(tile[k + 1], G, s0);
-        block_pairs<GEN>(s1, ar, ai);
-    }
-    if (k < k1) {
-        pair_state<GEN>(tile[k], G, s1);
-        block_pairs<GEN>(s0, ar, ai);
-        block_pairs<GEN>(s1, ar, ai);
-    } else {
-        block_pairs<GEN>(s0, ar, ai);
-    }
-}
-
-// The same without overlap (the general-order path, which needs more registers).
-template <int GEN, int BX, int BY>
-__device__ __forceinline__ void pillar_run_plain(const PillarRec* tile, int k0, int k1,
                                                  const BlockGeom<BX, BY>& G,
                                                  float (&ar)[BX * BY], float (&ai)[BX * BY]) {
     for (int k = k0; k < k1; ++k) {
@@ -430,7 +407,7 @@ __device__ __forceinline__ void fwd_tiles(
 #pragma unroll
             for (int e = 0; e < N; ++e) ar[e] = ai[e] = 0.0f;
             if (gen)
-                pillar_run_plain<1>(tile, k0, k1, G, ar, ai);
+                pillar_run<1>(tile, k0, k1, G, ar, ai);
             else
                 pillar_run<0>(tile, k0, k1, G, ar, ai);
 #pragma unroll
@@ -465,12 +442,9 @@ __device__ __forceinline__ void fwd_tiles(
 
 // BLK = 4: 4 x 4 tiles plus the edge classes (1 x 4, 4 x 1, 1 x 1), chosen per
 // work-group from its descriptor; BLK = 1: 1 x 1 tiles only (every pair at its own
-// FP64 reference distance).  Bounded for 3 work-groups per CU: the first-order loop
-// compiles to 143 VGPRs and runs 6% faster than the same loop squeezed into 128
-// (measured at 91,809 pillars x 161^2); a 4th work-group per CU does not help, since the
-// kernel is issue-bound either way.
+// FP64 reference distance).  123 VGPRs and 40 KB of LDS: 4 work-groups per CU.
 template <int BLK>
-__global__ __launch_bounds__(kThreads, 3) void fwd_fp32c(
+__global__ __launch_bounds__(kThreads, 4) void fwd_fp32c(
     const double2* __restrict__ pos, const float2* __restrict__ ap, int npil,
     const DevPlane* __restrict__ planes, const FwdBlock* __restrict__ blocks,
     const double2* __restrict__ ext, float2* __restrict__ partial) {
