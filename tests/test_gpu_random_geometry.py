@@ -1,0 +1,145 @@
+"""FP32C against FP64 over random geometries: forward and adjoint, per output element.
+
+The FP32-compensated kernels choose, per work-group, between first-order and
+second-order expansions of R about a 4 x 4 block centre, and the host falls back to
+every pair at its own FP64 distance where neither bound holds.  The fixed workloads
+exercise a few of these choices; this sweep draws wavelength, pillar pitch and count,
+transmission (a focusing lens or random complex values), plane kind (psf windows
+anywhere in front of the lens, zx slices from near the lens plane outwards), grid
+sizes (ragged against the 4 x 4 blocks), window sizes and offsets from a seeded
+generator, so the choices and their boundaries are met in many combinations.
+
+Bound, per output element: every pair's error is held to 1e-6 rad of phase and 4e-6 of
+amplitude by the expansion checks, and float accumulation over at most 64 terms per
+register sum adds at most ~64 * 2^-24 of the summed magnitudes.  So
+    |E_fp32c - E_fp64|_p  <=  1e-5 * sum_q |t_q| / R_pq
+    |g_fp32c - g_fp64|_q  <=  1e-5 * sum_p |u_p| / R_pq
+with the right-hand sides computed here in FP64 from the geometry.  The FP64 kernels
+are themselves held to 1e-12 of the oracle by the other parity tests.  (Relative L2 is
+printed, not asserted: a window in a dark region, where the terms cancel, can have any
+ratio of error to field.)
+"""
+import math
+
+import numpy as np
+import pytest
+
+from celeris_amd import Plane
+from celeris_amd import rs as crs
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+SEEDS = list(range(40))
+BOUND = 1e-5
+
+
+def geometry(seed):
+    rng = np.random.default_rng(1000 + seed)
+    lam = rng.uniform(0.4, 1.0)
+    k = 2.0 * math.pi / lam
+    p = rng.uniform(0.2, 0.6)
+    n_side = int(rng.integers(6, 72))
+    c = (np.arange(n_side) - (n_side - 1) / 2.0) * p
+    X, Y = np.meshgrid(c, c)
+    px, py = X.ravel(), Y.ravel()
+    if rng.random() < 0.5:  # circular aperture
+        keep = np.hypot(px, py) <= (n_side * p) / 2.0
+        px, py = px[keep], py[keep]
+    half = n_side * p / 2.0
+    kind = "focus" if rng.random() < 0.5 else "random"
+    if kind == "focus":
+        f = float(np.exp(rng.uniform(np.log(2.0), np.log(300.0))))
+        r = np.hypot(px, py)
+        t = rng.uniform(0.5, 1.0, px.size) * np.exp(-1j * k * (np.sqrt(r * r + f * f) - f))
+    else:
+        f = None
+        t = rng.standard_normal(px.size) + 1j * rng.standard_normal(px.size)
+    # Pixel pitches from 0.02 to 1 wavelength (focal sampling is usually well below one),
+    # so that first-order blocks, second-order blocks and exact pairs all occur.
+    logu = lambda lo, hi: float(np.exp(rng.uniform(np.log(lo), np.log(hi))))  # noqa: E731
+    planes = []
+    for _ in range(int(rng.integers(1, 4))):
+        if rng.random() < 0.7:
+            z = f if (f is not None and rng.random() < 0.5) else logu(0.5, 300.0)
+            n = int(rng.integers(2, 98))
+            W = 0.5 * (n - 1) * lam * logu(0.02, 1.0)
+            cx, cy = rng.uniform(-1.5 * half, 1.5 * half, 2)
+            planes.append(Plane.psf(float(cx), float(cy), z, k, n, W))
+        else:
+            nx, nz = int(rng.integers(2, 98)), int(rng.integers(2, 98))
+            z1 = logu(0.3, 150.0)
+            z2 = z1 + (nz - 1) * lam * logu(0.02, 1.0)
+            xw = 0.5 * (nx - 1) * lam * logu(0.02, 1.0)
+            x0 = float(rng.uniform(-half, half))
+            planes.append(Plane.zx(x0 - xw, x0 + xw, nx, z1, z2, nz, k))
+    return dict(lam=lam, p=p, px=px, py=py, t=t, kind=kind, f=f, planes=planes)
+
+
+def pixel_coords(planes):
+    xs, ys, zs = [], [], []
+    for P in planes:
+        i = np.arange(P.nx, dtype=np.float64)
+        j = np.arange(P.ny, dtype=np.float64)
+        J, I = np.meshgrid(j, i, indexing="ij")
+        xs.append((P.x0 + I * P.sx).ravel())
+        ys.append((P.y0 + J * P.sy).ravel())
+        zs.append((P.z0 + J * P.sz).ravel())
+    return np.concatenate(xs), np.concatenate(ys), np.concatenate(zs)
+
+
+def magnitude_sums(torch, px, py, at, fx, fy, fz, au):
+    """S_p = sum_q at_q / R_pq and T_q = sum_p au_p / R_pq, in FP64 on the GPU."""
+    dev = torch.device("cuda:0")
+    qx = torch.from_numpy(px).to(dev)
+    qy = torch.from_numpy(py).to(dev)
+    ta = torch.from_numpy(at).to(dev)
+    S = np.empty(fx.size)
+    T = torch.zeros(px.size, dtype=torch.float64, device=dev)
+    step = max(1, (1 << 24) // max(1, px.size))
+    for s in range(0, fx.size, step):
+        e = min(fx.size, s + step)
+        dx = torch.from_numpy(fx[s:e]).to(dev)[:, None] - qx[None, :]
+        dy = torch.from_numpy(fy[s:e]).to(dev)[:, None] - qy[None, :]
+        z = torch.from_numpy(fz[s:e]).to(dev)[:, None]
+        inv = torch.rsqrt(dx * dx + dy * dy + z * z)
+        S[s:e] = (inv @ ta).cpu().numpy()
+        T += torch.from_numpy(au[s:e]).to(dev) @ inv
+    return S, T.cpu().numpy()
+
+
+@pytest.mark.parametrize("seed", SEEDS)
+def test_fp32c_vs_fp64_random_geometry(torch_cuda, rsctx, seed):
+    torch = torch_cuda
+    g = geometry(seed)
+    px, py, t, planes = g["px"], g["py"], g["t"], g["planes"]
+    desc = ", ".join(f"{'zx' if P.sz else 'psf'} {P.nx}x{P.ny} z0={P.z0:.3g}"
+                     for P in planes)
+    print(f"[random geometry {seed}] lambda={g['lam']:.3f} p={g['p']:.3f} npil={px.size} "
+          f"{g['kind']} planes: {desc}", flush=True)
+    dev = torch.device("cuda:0")
+    dpx = torch.from_numpy(px).to(dev)
+    dpy = torch.from_numpy(py).to(dev)
+    dt = torch.from_numpy(np.ascontiguousarray(t[None, :])).to(dev)
+
+    E64, _ = rsctx.forward(dpx, dpy, dt, planes, mode="fp64", field=True)
+    E32, _ = rsctx.forward(dpx, dpy, dt, planes, mode="fp32c", field=True)
+    kernel = rsctx.last_launch(False)[0]
+    E64 = E64.cpu().numpy()
+    E32 = E32.cpu().numpy()
+    rng = np.random.default_rng(7 + seed)
+    u = rng.standard_normal(E64.size) + 1j * rng.standard_normal(E64.size)
+    du = torch.from_numpy(u).to(dev)
+    g64 = rsctx.adjoint(dpx, dpy, du, planes, nsets=1, mode="fp64").cpu().numpy()
+    g32 = rsctx.adjoint(dpx, dpy, du, planes, nsets=1, mode="fp32c").cpu().numpy()
+
+    fx, fy, fz = pixel_coords(crs.resolve_offsets(planes))
+    S, T = magnitude_sums(torch, px, py, np.abs(t), fx, fy, fz, np.abs(u))
+    fe = float(np.max(np.abs(E32 - E64) / S))
+    ge = float(np.max(np.abs(g32 - g64) / T))
+    print(f"[random geometry {seed}] {kernel}: forward max |dE|/sum|t|/R {fe:.2e}, "
+          f"adjoint max |dg|/sum|u|/R {ge:.2e}", flush=True)
+    assert fe <= BOUND
+    assert ge <= BOUND
+    print(f"[random geometry {seed}] field relL2 {rel_l2(E32, E64):.2e}, gradient relL2 "
+          f"{rel_l2(g32, g64):.2e}")
