@@ -115,10 +115,13 @@ struct AdjPlan {
     int mode = 0;
     int order = 1;
     int npil = 0, nsets = 0;
-    int64_t nsplit = 1, n_out = 0, staged_records = 0;
+    int64_t nsplit = 1, n_out = 0, staged_records = 0;  // nsplit: partial slots, all launches
     int nblocks = 0, npblocks = 0;
+    // FP32C calls mixing planes flagged exact with blocked ones: a second launch, of the
+    // 1 x 1 kernel over the flagged planes' quads, into partial slots after the first's.
+    int nblocks_exact = 0;
     TableBuilder tb;
-    size_t o_planes = 0, o_quads = 0, o_blocks = 0, o_pblocks = 0;
+    size_t o_planes = 0, o_quads = 0, o_blocks = 0, o_blocks_exact = 0, o_pblocks = 0;
 };
 
 // A context remembers the last plan of each direction; a call with the same shape
@@ -307,40 +310,53 @@ DevPlane make_dev_plane(const crs_plane& P, int px_per_thread, int py_per_thread
     return d;
 }
 
-// Whether 4 x 4 blocks can be used for a set of planes, with no knowledge of where the
-// pillars are (the kernels refine the series order per work-group from the pillars'
-// extent).  A block spans at most e turns around its centre; at distance >= z' the
-// expansion parameter is r = e / z' (y <= r/2).  Dropped terms, at worst (a pillar seen
-// at grazing angle): phase order 1 leaves 2 x1 y^2 <= e r^2 / 2 turns, order 2 leaves
-// ~5 x1 y^3 <= 0.625 e r^3; amplitude order 1 leaves 6 y^2 <= 1.5 r^2, order 2 leaves
-// ~20 y^3 <= 2.5 r^3, against budgets of 1.6e-7 turns (1e-6 rad) / 4e-6 of amplitude.
-// Returns kExactOrder when even order 2 would exceed them (then every pair is evaluated
-// at its own reference distance, 1 x 1 blocks), else 2 * (phase order - 1) + (amplitude
-// order - 1) for grazing incidence (informational; any code below kExactOrder selects the
-// 4 x 4 kernels).
-int fp32c_order(const std::vector<DevPlane>& dp) {
+// Series order 4 x 4 blocks need on one plane, with no knowledge of where the pillars
+// are (the kernels refine it per work-group from the pillars' extent).  A block spans at
+// most e turns around its centre; at distance >= z' the expansion parameter is
+// r = e / z' (y <= r/2).  Dropped terms, at worst (a pillar seen at grazing angle): phase
+// order 1 leaves 2 x1 y^2 <= e r^2 / 2 turns, order 2 leaves ~5 x1 y^3 <= 0.625 e r^3;
+// amplitude order 1 leaves 6 y^2 <= 1.5 r^2, order 2 leaves ~20 y^3 <= 2.5 r^3, against
+// budgets of 1.6e-7 turns (1e-6 rad) / 4e-6 of amplitude.  Returns kExactOrder when even
+// order 2 would exceed them (the plane's pairs are then evaluated one by one at their
+// own reference distances), else 2 * (phase order - 1) + (amplitude order - 1).  Planes
+// with non-finite geometry (n == 1 windows) give NaN either way and return 0.
+int fp32c_plane_order(const DevPlane& d) {
+    const double ex = 1.5 * std::fabs(d.sx) * d.kturn;
+    const double ey = 1.5 * std::fabs(d.sy) * d.kturn;
+    const double ez = 1.5 * std::fabs(d.sz) * d.kturn;
+    const double e = std::sqrt(ex * ex + ey * ey) + ez;
+    const double zmin = d.kturn * plane_min_abs_z(d.z0, d.sz, d.ny);
+    if (!std::isfinite(e)) return 0;
+    if (!(zmin > 0.0)) return kExactOrder;  // a z = 0 row: no distance bound
     int po = 1, ao = 1;
-    for (const DevPlane& d : dp) {
-        const double ex = 1.5 * std::fabs(d.sx) * d.kturn;
-        const double ey = 1.5 * std::fabs(d.sy) * d.kturn;
-        const double ez = 1.5 * std::fabs(d.sz) * d.kturn;
-        const double e = std::sqrt(ex * ex + ey * ey) + ez;
-        const double zmin = d.kturn * plane_min_abs_z(d.z0, d.sz, d.ny);
-        if (!(std::isfinite(e) && zmin > 0.0)) {
-            if (std::isfinite(e)) return kExactOrder;  // a z = 0 row: no distance bound
-            continue;  // NaN/inf geometry (n == 1 windows): NaN out, as the reference
-        }
-        const double r = e / zmin;
-        if (0.5 * e * r * r > 1.6e-7) {
-            if (0.625 * e * r * r * r > 1.6e-7) return kExactOrder;
-            po = 2;
-        }
-        if (1.5 * r * r > 4e-6) {
-            if (2.5 * r * r * r > 4e-6) return kExactOrder;
-            ao = 2;
-        }
+    const double r = e / zmin;
+    if (0.5 * e * r * r > 1.6e-7) {
+        if (0.625 * e * r * r * r > 1.6e-7) return kExactOrder;
+        po = 2;
+    }
+    if (1.5 * r * r > 4e-6) {
+        if (2.5 * r * r * r > 4e-6) return kExactOrder;
+        ao = 2;
     }
     return 2 * (po - 1) + (ao - 1);
+}
+
+// Flags (exact[p]) each plane that needs pair-by-pair evaluation and returns the call's
+// order code: kExactOrder when every plane does (the 1 x 1 kernels), else the highest
+// order of the others (informational; any code below kExactOrder selects the 4 x 4
+// kernels, and the flagged planes of such a call are taken pair by pair).
+int fp32c_order(const std::vector<DevPlane>& dp, std::vector<char>& exact) {
+    int order = 0;
+    bool all_exact = !dp.empty();
+    exact.assign(dp.size(), 0);
+    for (size_t p = 0; p < dp.size(); ++p) {
+        const int o = fp32c_plane_order(dp[p]);
+        exact[p] = o == kExactOrder;
+        if (exact[p]) continue;
+        all_exact = false;
+        order = std::max(order, o);
+    }
+    return all_exact ? kExactOrder : order;
 }
 
 // Bytes of staged pillar data: FP32C keeps FP64 positions once plus {|t|, arg t} per
@@ -518,8 +534,9 @@ crs_status plan_forward(const crs_context* ctx, int mode, int npil, int nsets,
     std::vector<DevPlane> dp(nplanes);
     for (int p = 0; p < nplanes; ++p) dp[p] = make_dev_plane(planes[p], 1, 1);
     int blk = 1;
+    std::vector<char> exact(nplanes, 0);
     if (mode == CRS_MODE_FP32C) {
-        plan.order = fp32c_order(dp);
+        plan.order = fp32c_order(dp, exact);
         blk = plan.order == kExactOrder ? 1 : kBlk;
     }
     // Split over pillar chunks: enough work-groups to fill the chip, chunks of whole
@@ -528,17 +545,18 @@ crs_status plan_forward(const crs_context* ctx, int mode, int npil, int nsets,
     // FP32C tiles come in four classes (kernel: tile_count / tile_origin): 4 x 4 tiles
     // over the largest multiple-of-4 part of the plane, then 1 x 4, 4 x 1 and 1 x 1
     // tiles for the leftover columns, rows and corner, so no lane computes a pixel
-    // outside the plane.  With 1 x 1 blocks (tx = ty = 0) the last class is the plane.
+    // outside the plane.  With 1 x 1 blocks (tx = ty = 0: the 1 x 1 kernels, or a plane
+    // flagged exact in a call of the 4 x 4 kernels) the last class is the plane.
     const int per_wg = mode == CRS_MODE_FP32C ? kFwdTiles : kThreads;
     const int nclass = mode == CRS_MODE_FP32C ? 4 : 1;
     std::vector<std::array<int64_t, 4>> counts(nplanes);
     int64_t base_blocks = 0;
     for (int p = 0; p < nplanes; ++p) {
-        dp[p] = make_dev_plane(planes[p], 1, 1);
         std::array<int64_t, 4>& c = counts[p];
         if (mode == CRS_MODE_FP32C) {
-            dp[p].tx = blk > 1 ? dp[p].nx / blk : 0;
-            dp[p].ty = blk > 1 ? dp[p].ny / blk : 0;
+            const bool blocked = blk > 1 && !exact[p];
+            dp[p].tx = blocked ? dp[p].nx / blk : 0;
+            dp[p].ty = blocked ? dp[p].ny / blk : 0;
             const int64_t tx = dp[p].tx, ty = dp[p].ty;
             const int64_t rx = dp[p].nx - 4 * tx, ry = dp[p].ny - 4 * ty;
             c = {tx * ty, rx * ty, tx * ry, rx * ry};
@@ -661,79 +679,93 @@ crs_status plan_adjoint(const crs_context* ctx, int mode, int npil, int nsets,
         srec += static_cast<int64_t>(dp[p].nquads) * dp[p].nxp * kQuad;
     }
     if (srec > (int64_t(1) << 31)) { set_error("too many pixels"); return CRS_ERR_INVALID; }
-    if (mode == CRS_MODE_FP32C) plan.order = fp32c_order(dp);
+    std::vector<char> exact(nplanes, 0);
+    if (mode == CRS_MODE_FP32C) plan.order = fp32c_order(dp, exact);
+    // A mixed FP32C call (some planes flagged exact, the order below kExactOrder) gets
+    // two launches: the 4 x 4 kernel over the other planes, then the 1 x 1 kernel over
+    // the flagged ones, so the 4 x 4 kernel carries no pair-by-pair path.
+    bool mixed = false;
+    if (mode == CRS_MODE_FP32C && plan.order != kExactOrder)
+        for (char e : exact) mixed = mixed || e;
 
-    // Per-set quad lists (planes in order, quads in order) and their work, in pixel
-    // records (FP32C computes the padded 4 x nxp records of a quad).
-    std::vector<AdjQuad> quads;
-    std::vector<int64_t> set_q0(nsets + 1, 0);
-    std::vector<std::vector<int64_t>> work_prefix(nsets);
-    for (int s = 0; s < nsets; ++s) {
-        set_q0[s] = static_cast<int64_t>(quads.size());
-        std::vector<int64_t>& pre = work_prefix[s];
-        pre.push_back(0);
-        for (int p = 0; p < nplanes; ++p) {
-            if (dp[p].tset != s) continue;
-            for (int j = 0; j < dp[p].ny; j += kQuad) {
-                quads.push_back({p, j});
-                const int64_t w = mode == CRS_MODE_FP32C
-                                      ? static_cast<int64_t>(kQuad) * dp[p].nxp
-                                      : static_cast<int64_t>(std::min(kQuad, dp[p].ny - j)) * dp[p].nx;
-                pre.push_back(pre.back() + w);
-            }
-        }
-    }
-    set_q0[nsets] = static_cast<int64_t>(quads.size());
-    if (quads.size() > static_cast<size_t>(INT32_MAX)) { set_error("too many rows"); return CRS_ERR_INVALID; }
-
-    int64_t max_quads = 1;
-    for (int s = 0; s < nsets; ++s) max_quads = std::max<int64_t>(max_quads, set_q0[s + 1] - set_q0[s]);
     const int target = ctx->adj_target[mode] > 0 ? ctx->adj_target[mode] : default_target(mode, true);
     const int PQ = mode == CRS_MODE_FP32C ? kPQ : 1;
     const int64_t groups = ceil_div(npil, static_cast<int64_t>(kThreads) * PQ);
     const int64_t base_blocks = groups * nsets;
-    int64_t nsplit = std::max<int64_t>(1, ceil_div(target, base_blocks));
-    nsplit = std::min<int64_t>(nsplit, max_quads);
-    if (base_blocks * nsplit > (int64_t(1) << 31) - 1) { set_error("launch too large"); return CRS_ERR_INVALID; }
-
-    // Cut each set's quads into nsplit runs of about equal work.
-    std::vector<AdjBlock> blocks;
-    std::vector<int64_t> block_cost;
-    blocks.reserve(static_cast<size_t>(base_blocks * nsplit));
-    block_cost.reserve(static_cast<size_t>(base_blocks * nsplit));
     const int64_t n_out = static_cast<int64_t>(nsets) * npil;
-    for (int64_t c = 0; c < nsplit; ++c) {
+    std::vector<AdjQuad> quads;
+
+    // One launch's work-groups over the planes `take` selects: per-set quad lists (planes
+    // in order, quads in order, appended to `quads`), each set's quads cut into nsplit
+    // runs of about equal work (in pixel records: FP32C computes the padded 4 x nxp
+    // records of a quad), results to partial slots slot0, slot0 + 1, ...  Returns nsplit.
+    auto add_launch = [&](auto take, int64_t slot0, std::vector<AdjBlock>& blocks) -> int64_t {
+        std::vector<int64_t> set_q0(nsets + 1, 0);
+        std::vector<std::vector<int64_t>> work_prefix(nsets);
         for (int s = 0; s < nsets; ++s) {
-            const std::vector<int64_t>& pre = work_prefix[s];
-            const int64_t nq = static_cast<int64_t>(pre.size()) - 1;
-            const int64_t total = pre.back();
-            auto cut = [&](int64_t k) -> int64_t {  // first quad whose prefix >= k*total/nsplit
-                if (k >= nsplit) return nq;
-                const int64_t want = (total * k) / nsplit;
-                return static_cast<int64_t>(std::lower_bound(pre.begin(), pre.end(), want) - pre.begin());
-            };
-            const int64_t u0 = std::min(cut(c), nq), u1 = std::min(cut(c + 1), nq);
-            for (int64_t gq = 0; gq < groups; ++gq) {
-                blocks.push_back({s, static_cast<int32_t>(gq * kThreads * PQ),
-                                  static_cast<int32_t>(set_q0[s] + u0),
-                                  static_cast<int32_t>(set_q0[s] + std::max(u0, u1)),
-                                  c * n_out + static_cast<int64_t>(s) * npil});
-                const int64_t pil = std::min<int64_t>(int64_t(kThreads) * PQ, npil - gq * kThreads * PQ);
-                block_cost.push_back(pil * (pre[std::max(u0, u1)] - pre[u0]));
+            set_q0[s] = static_cast<int64_t>(quads.size());
+            std::vector<int64_t>& pre = work_prefix[s];
+            pre.push_back(0);
+            for (int p = 0; p < nplanes; ++p) {
+                if (dp[p].tset != s || !take(p)) continue;
+                for (int j = 0; j < dp[p].ny; j += kQuad) {
+                    quads.push_back({p, j});
+                    const int64_t w = mode == CRS_MODE_FP32C
+                                          ? static_cast<int64_t>(kQuad) * dp[p].nxp
+                                          : static_cast<int64_t>(std::min(kQuad, dp[p].ny - j)) * dp[p].nx;
+                    pre.push_back(pre.back() + w);
+                }
             }
         }
-    }
-    // Heaviest work-groups first (quad runs hold whole quads, so runs differ by up to
-    // one quad): the XCDs take work-groups round robin in blockIdx order, and the
-    // light ones then fill the tail.  Each block names its own output slot.
-    {
-        std::vector<int32_t> order(blocks.size());
+        set_q0[nsets] = static_cast<int64_t>(quads.size());
+        int64_t max_quads = 1;
+        for (int s = 0; s < nsets; ++s) max_quads = std::max<int64_t>(max_quads, set_q0[s + 1] - set_q0[s]);
+        int64_t nsplit = std::max<int64_t>(1, ceil_div(target, base_blocks));
+        nsplit = std::min<int64_t>(nsplit, max_quads);
+        std::vector<int64_t> block_cost;
+        const size_t first = blocks.size();
+        for (int64_t c = 0; c < nsplit; ++c) {
+            for (int s = 0; s < nsets; ++s) {
+                const std::vector<int64_t>& pre = work_prefix[s];
+                const int64_t nq = static_cast<int64_t>(pre.size()) - 1;
+                const int64_t total = pre.back();
+                auto cut = [&](int64_t k) -> int64_t {  // first quad whose prefix >= k*total/nsplit
+                    if (k >= nsplit) return nq;
+                    const int64_t want = (total * k) / nsplit;
+                    return static_cast<int64_t>(std::lower_bound(pre.begin(), pre.end(), want) - pre.begin());
+                };
+                const int64_t u0 = std::min(cut(c), nq), u1 = std::min(cut(c + 1), nq);
+                for (int64_t gq = 0; gq < groups; ++gq) {
+                    blocks.push_back({s, static_cast<int32_t>(gq * kThreads * PQ),
+                                      static_cast<int32_t>(set_q0[s] + u0),
+                                      static_cast<int32_t>(set_q0[s] + std::max(u0, u1)),
+                                      (slot0 + c) * n_out + static_cast<int64_t>(s) * npil});
+                    const int64_t pil = std::min<int64_t>(int64_t(kThreads) * PQ, npil - gq * kThreads * PQ);
+                    block_cost.push_back(pil * (pre[std::max(u0, u1)] - pre[u0]));
+                }
+            }
+        }
+        // Heaviest work-groups first (quad runs hold whole quads, so runs differ by up to
+        // one quad): the XCDs take work-groups round robin in blockIdx order, and the
+        // light ones then fill the tail.  Each block names its own output slot.
+        std::vector<int32_t> order(block_cost.size());
         for (size_t i = 0; i < order.size(); ++i) order[i] = static_cast<int32_t>(i);
         std::stable_sort(order.begin(), order.end(),
                          [&](int32_t a, int32_t b) { return block_cost[a] > block_cost[b]; });
-        std::vector<AdjBlock> sorted(blocks.size());
-        for (size_t i = 0; i < order.size(); ++i) sorted[i] = blocks[order[i]];
-        blocks.swap(sorted);
+        std::vector<AdjBlock> sorted(order.size());
+        for (size_t i = 0; i < order.size(); ++i) sorted[i] = blocks[first + order[i]];
+        std::copy(sorted.begin(), sorted.end(), blocks.begin() + static_cast<std::ptrdiff_t>(first));
+        return nsplit;
+    };
+
+    std::vector<AdjBlock> blocks, blocks_exact;
+    int64_t nsplit = mixed ? add_launch([&](int p) { return !exact[p]; }, 0, blocks)
+                           : add_launch([](int) { return true; }, 0, blocks);
+    if (mixed) nsplit += add_launch([&](int p) { return exact[p] != 0; }, nsplit, blocks_exact);
+    if (quads.size() > static_cast<size_t>(INT32_MAX) ||
+        blocks.size() + blocks_exact.size() > static_cast<size_t>(INT32_MAX)) {
+        set_error("launch too large");
+        return CRS_ERR_INVALID;
     }
     // Staging blocks walk each plane's staged records.
     std::vector<PixBlock> pblocks;
@@ -745,10 +777,12 @@ crs_status plan_adjoint(const crs_context* ctx, int mode, int npil, int nsets,
     plan.n_out = n_out;
     plan.staged_records = srec;
     plan.nblocks = static_cast<int>(blocks.size());
+    plan.nblocks_exact = static_cast<int>(blocks_exact.size());
     plan.npblocks = static_cast<int>(pblocks.size());
     plan.o_planes = plan.tb.add(dp.data(), dp.size() * sizeof(DevPlane));
     plan.o_quads = plan.tb.add(quads.data(), quads.size() * sizeof(AdjQuad));
     plan.o_blocks = plan.tb.add(blocks.data(), blocks.size() * sizeof(AdjBlock));
+    plan.o_blocks_exact = plan.tb.add(blocks_exact.data(), blocks_exact.size() * sizeof(AdjBlock));
     plan.o_pblocks = plan.tb.add(pblocks.data(), pblocks.size() * sizeof(PixBlock));
     return CRS_OK;
 }
@@ -797,11 +831,17 @@ crs_status enqueue_adjoint(const AdjPlan& plan, const Scratch& sc, const double*
         pix_or_u = sc.pix->ptr;
     }
     if (ev0) hipEventRecord(ev0, stream);
-    if ((st = launch_status(launch_adjoint(mode, plan.order, pil, plan.npil, d_planes,
-                                           reinterpret_cast<const AdjQuad*>(sc.tables + plan.o_quads),
+    const AdjQuad* d_quads = reinterpret_cast<const AdjQuad*>(sc.tables + plan.o_quads);
+    if ((st = launch_status(launch_adjoint(mode, plan.order, pil, plan.npil, d_planes, d_quads,
                                            reinterpret_cast<const AdjBlock*>(sc.tables + plan.o_blocks),
                                            plan.nblocks, pix_or_u, sc.partial->ptr, stream),
                             "adjoint")) != CRS_OK)
+        return st;
+    if (plan.nblocks_exact > 0 &&
+        (st = launch_status(launch_adjoint(mode, kExactOrder, pil, plan.npil, d_planes, d_quads,
+                                           reinterpret_cast<const AdjBlock*>(sc.tables + plan.o_blocks_exact),
+                                           plan.nblocks_exact, pix_or_u, sc.partial->ptr, stream),
+                            "adjoint (planes evaluated pair by pair)")) != CRS_OK)
         return st;
     if (ev1) hipEventRecord(ev1, stream);
     return launch_status(launch_adj_reduce(mode, sc.partial->ptr,

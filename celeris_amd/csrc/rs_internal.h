@@ -28,8 +28,11 @@ constexpr int kReduceCols = 64;
 constexpr int kMaxChunk = 16384;
 // Shortest pillar chunk of a forward split (small problems only; 16 pillars per wave).
 constexpr int kMinChunk = 64;
-// Plan order code of the 1 x 1 (unexpanded) FP32C kernels (fp32c_order); codes 0..3
-// select the 4 x 4 kernels, which choose their series order per work-group.
+// Plan order code of the 1 x 1 (unexpanded) FP32C kernels, used when every plane of a
+// call needs each pair at its own reference distance (fp32c_order); codes 0..3 select
+// the 4 x 4 kernels, which choose their series order per work-group; planes flagged
+// exact in such a call are evaluated pair by pair (forward: 1 x 1 tiles; adjoint: a
+// second launch of the 1 x 1 kernel).
 constexpr int kExactOrder = 4;
 // FP32C adjoint: pillars per thread (q0 + tid + k*kThreads, k < kPQ).  One: four per
 // thread (sharing each block's broadcast pixel records) measured 5-6% slower at 91,809

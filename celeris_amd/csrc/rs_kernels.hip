@@ -33,11 +33,12 @@
 // pair.  No float ever holds kR (~10^3 rad) or R^2 (~10^4 turns^2): the phase is carried
 // to ~1e-7 turns.  Each work-group checks, from its plane and the pillars' extent,
 // that the first-order terms meet the error budget and otherwise takes the second
-// order (fp32c_first_order); the host falls back to 1 x 1 blocks (every pair at its own
-// FP64 R0) where even that would not.  The factor exp(2 pi i z'_ref) is restored in
-// FP64 by the reduction; split partials are added in FP64.  Storing the FP32C partials
-// as floats halves the reductions' HBM reads (they are bandwidth-bound) for one more
-// float rounding of each chunk's sum, ~6e-8 of a partial.
+// order (fp32c_first_order); the host flags each plane where even that would not, and
+// such a plane is taken in 1 x 1 blocks (every pair at its own FP64 R0).  The factor
+// exp(2 pi i z'_ref) is restored in FP64 by the reduction; split partials are added in
+// FP64.  Storing the FP32C partials as floats halves the reductions' HBM reads (they
+// are bandwidth-bound) for one more float rounding of each chunk's sum, ~6e-8 of a
+// partial.
 #include "rs_internal.h"
 
 #include <hip/hip_runtime.h>
@@ -441,7 +442,8 @@ __device__ __forceinline__ void fwd_tiles(
 }
 
 // BLK = 4: 4 x 4 tiles plus the edge classes (1 x 4, 4 x 1, 1 x 1), chosen per
-// work-group from its descriptor; BLK = 1: 1 x 1 tiles only (every pair at its own
+// work-group from its descriptor (a plane flagged exact has 1 x 1 tiles only); BLK = 1,
+// for calls whose planes are all flagged: 1 x 1 tiles only (every pair at its own
 // FP64 reference distance).  123 VGPRs and 40 KB of LDS: 4 work-groups per CU.
 template <int BLK>
 __global__ __launch_bounds__(kThreads, 4) void fwd_fp32c(
@@ -632,7 +634,8 @@ __global__ __launch_bounds__(kThreads) void stage_pixels_fp32c(
 // ---------------------------------------------------------------------------------
 // Adjoint, FP32C.  Thread owns kPQ = 1 pillar (FP64 position in registers); a quad's pixel
 // records stream through LDS 64 columns at a time and are consumed in BX x BY blocks
-// (4 x 4, or 1 x 1 when every pair needs its own reference distance).  Per pair
+// (4 x 4; 1 x 1 for planes flagged exact, where every pair needs its own reference
+// distance, which the host sends to the <1, 1> instance).  Per pair
 //     conj(G) u = U / R0' (R0/R) exp(2 pi i (chi - th0 - (R - R0))),   U = |u| k/2pi,
 // with the same expansion of R - R0 and R0/R as the forward; the block's sum is scaled
 // by 1/R0' once.

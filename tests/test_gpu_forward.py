@@ -253,6 +253,37 @@ def test_forward_general_and_first_order_planes(torch_cuda, rsctx, mode, tol):
         assert e <= tol
 
 
+def test_mixed_exact_and_blocked_planes(torch_cuda, rsctx):
+    """A call mixing a plane the block expansion cannot serve (0.5 um from the lens,
+    1 um pixels: every pair at its own distance) with planes it can: the 4 x 4 kernels
+    run, take the flagged plane pair by pair, and every plane, forward and adjoint,
+    matches the oracle to 1e-5."""
+    L = configs.config1()
+    k = L.planes[0].k
+    planes = [Plane.psf(0.0, 0.0, 50.0, k, 41, 1.0), Plane.psf(0.3, -0.2, 0.5, k, 9, 4.0),
+              L.planes[0]]
+    ref = oracle.forward(L.px, L.py, L.t, planes)
+    f, _ = gpu_forward(torch_cuda, rsctx, L.px, L.py, L.t, planes, "fp32c")
+    assert rsctx.last_launch(False)[0] == "crs::fwd_fp32c<4>"
+    for p in crs.resolve_offsets(planes):
+        sl = slice(p.offset, p.offset + p.npix)
+        e = rel_l2(f[sl], ref[sl])
+        print(f"mixed exact/blocked forward plane z={p.z0} {p.nx}x{p.ny} relL2 {e:.3e}")
+        assert e <= 1e-5
+    rng = np.random.default_rng(11)
+    u = rng.standard_normal(ref.size) + 1j * rng.standard_normal(ref.size)
+    gref = oracle.adjoint(L.px, L.py, u, planes)
+    g = rsctx.adjoint(dev(torch_cuda, L.px), dev(torch_cuda, L.py), dev(torch_cuda, u), planes,
+                      nsets=1, mode="fp32c")
+    assert rsctx.last_launch(True)[0] == "crs::adj_fp32c<4, 4>"
+    eg = rel_l2(g.cpu().numpy(), gref)
+    print(f"mixed exact/blocked adjoint relL2 {eg:.3e}")
+    assert eg <= 1e-5
+    # The near plane alone still takes the 1 x 1 kernels.
+    gpu_forward(torch_cuda, rsctx, L.px, L.py, L.t, planes[1:2], "fp32c")
+    assert rsctx.last_launch(False)[0] == "crs::fwd_fp32c<1>"
+
+
 @pytest.mark.parametrize("mode,tol", [("fp64", 1e-12), ("fp32c", 1e-5)])
 def test_forward_ragged_edges_blocked(torch_cuda, rsctx, mode, tol):
     """4 x 4 blocks with every kind of leftover: nx, ny = 4k + 1, 2, 3 (the 1 x 4, 4 x 1
