@@ -199,7 +199,9 @@ class Context:
         s = stream if stream is not None else torch.cuda.current_stream(self.device)
         return ctypes.c_void_p(s.cuda_stream)
 
-    def _check_dev(self, name, x, dtype):
+    def _check_dev(self, name, x, dtype, min_numel: int = 0):
+        """x must be a contiguous `dtype` tensor on this context's GPU with at least
+        `min_numel` elements (the kernels read and write through its raw pointer)."""
         torch = _torch()
         if not isinstance(x, torch.Tensor) or x.device.type != "cuda" or x.device.index != self.device:
             raise ValueError(f"{name} must be a CUDA tensor on device {self.device}")
@@ -207,6 +209,8 @@ class Context:
             raise ValueError(f"{name} must be {dtype}, got {x.dtype}")
         if not x.is_contiguous():
             raise ValueError(f"{name} must be contiguous")
+        if x.numel() < min_numel:
+            raise ValueError(f"{name} has {x.numel()} elements, {min_numel} needed")
 
     def set_tuning(self, mode, fwd_target_blocks: int = 0, adj_target_blocks: int = 0) -> None:
         _lib.check(_lib.load().crs_set_tuning(self._h, _mode(mode), int(fwd_target_blocks),
@@ -243,10 +247,14 @@ class Context:
         arr, total = pack_planes(planes)
         dev = px.device
         f = out_field
-        if field and f is None:
+        if f is not None:
+            self._check_dev("out_field", f, torch.complex128, total)
+        elif field:
             f = torch.empty(total, dtype=torch.complex128, device=dev)
         it = out_intensity
-        if intensity and it is None:
+        if it is not None:
+            self._check_dev("out_intensity", it, torch.float64, total)
+        elif intensity:
             it = torch.empty(total, dtype=torch.float64, device=dev)
         st = _lib.load().crs_forward(
             self._h, self._stream(stream), _mode(mode), ctypes.c_void_p(px.data_ptr()),
@@ -264,9 +272,13 @@ class Context:
         self._check_dev("py", py, torch.float64)
         self._check_dev("u", u, torch.complex128)
         npil = px.numel()
+        if py.numel() != npil:
+            raise ValueError("px, py sizes disagree")
         arr, total = pack_planes(planes)
         if u.numel() < total:
             raise ValueError(f"u has {u.numel()} pixels, planes need {total}")
+        if out is not None:
+            self._check_dev("out", out, torch.complex128, int(nsets) * npil)
         g = out if out is not None else torch.empty((nsets, npil), dtype=torch.complex128,
                                                     device=px.device)
         st = _lib.load().crs_adjoint(
@@ -305,8 +317,21 @@ class Graph:
 def graph_forward(ctx: "Context", px, py, t, planes: Sequence[Plane], field, intensity=None,
                   mode="fp32c") -> Graph:
     """Capture ctx.forward(px, py, t, planes) writing into the given output tensors."""
-    arr, _ = pack_planes(planes)
-    nsets = t.numel() // max(px.numel(), 1)
+    torch = _torch()
+    arr, total = pack_planes(planes)
+    npil = px.numel()
+    ctx._check_dev("px", px, torch.float64)
+    ctx._check_dev("py", py, torch.float64, npil)
+    ctx._check_dev("t", t, torch.complex128, npil)
+    if t.numel() % max(npil, 1):
+        raise ValueError("t must hold whole transmission sets of px.numel() pillars")
+    if field is None and intensity is None:
+        raise ValueError("graph_forward needs field and/or intensity")
+    if field is not None:
+        ctx._check_dev("field", field, torch.complex128, total)
+    if intensity is not None:
+        ctx._check_dev("intensity", intensity, torch.float64, total)
+    nsets = t.numel() // max(npil, 1)
     h = ctypes.c_void_p()
     _lib.check(_lib.load().crs_graph_forward(
         ctx._h, _mode(mode), ctypes.c_void_p(px.data_ptr()), ctypes.c_void_p(py.data_ptr()),
@@ -320,7 +345,13 @@ def graph_forward(ctx: "Context", px, py, t, planes: Sequence[Plane], field, int
 def graph_adjoint(ctx: "Context", px, py, u, planes: Sequence[Plane], g, nsets: int = 1,
                   mode="fp32c") -> Graph:
     """Capture ctx.adjoint(px, py, u, planes) writing into g (complex128 [nsets, npil])."""
-    arr, _ = pack_planes(planes)
+    torch = _torch()
+    arr, total = pack_planes(planes)
+    npil = px.numel()
+    ctx._check_dev("px", px, torch.float64)
+    ctx._check_dev("py", py, torch.float64, npil)
+    ctx._check_dev("u", u, torch.complex128, total)
+    ctx._check_dev("g", g, torch.complex128, int(nsets) * npil)
     h = ctypes.c_void_p()
     _lib.check(_lib.load().crs_graph_adjoint(
         ctx._h, _mode(mode), ctypes.c_void_p(px.data_ptr()), ctypes.c_void_p(py.data_ptr()),

@@ -133,6 +133,24 @@ def test_invalid_arguments(torch_cuda, rsctx):
     overlap = [Plane.psf(0, 0, 30.0, 10.0, 9, 2.0, offset=0), Plane.psf(0, 0, 31.0, 10.0, 5, 2.0, offset=80)]
     with pytest.raises(RuntimeError, match="overlap"):
         rsctx.forward(dpx, dpy, dt, overlap)
+    # Caller-provided buffers are checked before any launch writes through them.
+    total = sum(p.npix for p in planes)
+    with pytest.raises(ValueError, match="elements"):
+        rsctx.forward(dpx, dpy, dt, planes,
+                      out_field=torch.empty(total - 1, dtype=torch.complex128, device="cuda:0"))
+    with pytest.raises(ValueError, match="float64"):
+        rsctx.forward(dpx, dpy, dt, planes, intensity=True,
+                      out_intensity=torch.empty(total, dtype=torch.float32, device="cuda:0"))
+    u = torch.zeros(total, dtype=torch.complex128, device="cuda:0")
+    with pytest.raises(ValueError, match="elements"):
+        rsctx.adjoint(dpx, dpy, u, planes, nsets=2,
+                      out=torch.empty((1, px.size), dtype=torch.complex128, device="cuda:0"))
+    with pytest.raises(ValueError, match="complex128"):
+        crs.graph_forward(rsctx, dpx, dpy, dt.to(torch.complex64), planes,
+                          torch.empty(total, dtype=torch.complex128, device="cuda:0"))
+    with pytest.raises(ValueError, match="elements"):
+        crs.graph_adjoint(rsctx, dpx, dpy, u, planes,
+                          torch.empty((1, px.size - 1), dtype=torch.complex128, device="cuda:0"))
 
 
 def test_tuning_timing_and_launch_info(torch_cuda, tmp_path, monkeypatch):
