@@ -56,6 +56,20 @@ def _d(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+def _pillars(px, py, *per_pillar):
+    """px, py as contiguous float64 and each per-pillar array as contiguous complex128
+    (transmissions) or float64 (phases), all 1-D of one length: the C entry points read
+    npil elements of each."""
+    px, py = _d(px), _d(py)
+    out = [px, py]
+    for a in per_pillar:
+        a = np.asarray(a)
+        out.append(_c(a) if np.iscomplexobj(a) else _d(a))
+    if any(a.ndim != 1 or a.size != px.size for a in out):
+        raise ValueError("px, py and the per-pillar arrays must be 1-D arrays of one length")
+    return out
+
+
 @dataclass
 class FocalAnalysis:
     strehl: float
@@ -153,7 +167,7 @@ def analyze_focus(lens: MetalensDesign, lib: UnitCellLibrary, focal_length_um: f
 
 def analyze_focus_pillars(px, py, t, phi_ideal, focal_length_um, wavelength_um, diameter_um,
                           mode="fp32c") -> FocalAnalysis:
-    px, py, t, phi = _d(px), _d(py), _c(t), _d(phi_ideal)
+    px, py, t, phi = _pillars(px, py, _c(t), _d(phi_ideal))
     out = np.zeros(5)
     grid = np.zeros((121, 121))
     _check(_lib.load().crs_analyze_focus(_p(px), _p(py), _p(t), _p(phi), px.size,
@@ -165,7 +179,7 @@ def analyze_focus_pillars(px, py, t, phi_ideal, focal_length_um, wavelength_um, 
 def propagate_pillars(px, py, t, cx, cy, z, wavelength_um, n, half_window_um,
                       mode="fp32c") -> PsfMap:
     """focal.cpp:205-238: |E|^2 on an n x n window; an empty list or n <= 0 gives zeros."""
-    px, py, t = _d(px), _d(py), _c(t)
+    px, py, t = _pillars(px, py, _c(t))
     n = int(n)
     m = PsfMap(n, half_window_um, np.zeros((max(n, 0), max(n, 0))))
     if px.size == 0 or n <= 0:  # focal.cpp:216
@@ -220,7 +234,7 @@ def analyze_through_focus(lens: MetalensDesign, lib: UnitCellLibrary, focal_leng
 
 def analyze_through_focus_pillars(px, py, t, focal_length_um, wavelength_um, diameter_um,
                                   caustic_n=201, mode="fp32c") -> ThroughFocus:
-    px, py, t = _d(px), _d(py), _c(t)
+    px, py, t = _pillars(px, py, _c(t))
     z = np.zeros(161)
     inten = np.zeros(161)
     sc = np.zeros(6)
@@ -245,7 +259,7 @@ def analyze_chromatic(lens: MetalensDesign, lib: UnitCellLibrary, design_focal: 
 
 def analyze_chromatic_pillars(px, py, t, design_focal, design_wavelength, lambda_min,
                               lambda_max, n, mode="fp32c") -> List[ChromaticPoint]:
-    px, py, t = _d(px), _d(py), _c(t)
+    px, py, t = _pillars(px, py, _c(t))
     out = np.zeros(3 * n)
     _check(_lib.load().crs_analyze_chromatic(_p(px), _p(py), _p(t), px.size, design_focal,
                                              design_wavelength, lambda_min, lambda_max, int(n),
@@ -259,7 +273,7 @@ def analyze_field_grid(lens: MetalensDesign, lib: UnitCellLibrary, focal_length_
     """field.cpp:150-233: all (2 n_half + 1)^2 field points plus the on-axis reference
     in one launch."""
     px, py, t = aperture_pillars(lens, lib, diameter_um)
-    px, py, t = _d(px), _d(py), _c(t)
+    px, py, t = _pillars(px, py, _c(t))
     g = 2 * n_half + 1
     out = np.zeros(7 * g * g)
     _check(_lib.load().crs_analyze_field_grid(_p(px), _p(py), _p(t), px.size, focal_length_um,
@@ -307,7 +321,7 @@ def analyze_field_of_view(lens: MetalensDesign, lib: UnitCellLibrary, focal_leng
                           mode="fp32c") -> List[FieldPoint]:
     """field.cpp:12-76: relative peak and spot position per field angle, one launch."""
     px, py, t = aperture_pillars(lens, lib, diameter_um)
-    px, py, t = _d(px), _d(py), _c(t)
+    px, py, t = _pillars(px, py, _c(t))
     ang = _d(np.atleast_1d(angles_deg))
     out = np.zeros(3 * max(ang.size, 1))
     _check(_lib.load().crs_analyze_field_of_view(_p(px), _p(py), _p(t), px.size, focal_length_um,
@@ -322,7 +336,7 @@ def analyze_wide_fov(lens: MetalensDesign, lib: UnitCellLibrary, focal_length_um
                      stop_distance_um: float, angles_deg, mode="fp32c") -> List[FieldPoint]:
     """field.cpp:78-148: field angles seen through a decentred stop patch, one launch."""
     px, py, t = aperture_pillars(lens, lib, lens_diameter_um)
-    px, py, t = _d(px), _d(py), _c(t)
+    px, py, t = _pillars(px, py, _c(t))
     ang = _d(np.atleast_1d(angles_deg))
     out = np.zeros(3 * max(ang.size, 1))
     _check(_lib.load().crs_analyze_wide_fov(_p(px), _p(py), _p(t), px.size, focal_length_um,
@@ -356,6 +370,8 @@ def analyze_tolerance(lens: MetalensDesign, lib: UnitCellLibrary, focal_length_u
     if fm.size != lens.n_cells * lens.n_cells:
         raise ValueError("fill_map must hold n_cells^2 fills")
     lf, lp, la = _d(lib.fill), _d(lib.phase), _d(lib.amplitude)
+    if not (lf.ndim == lp.ndim == la.ndim == 1 and lf.size == lp.size == la.size > 0):
+        raise ValueError("library fill, phase and amplitude must be 1-D arrays of one length")
     sg = _d(np.atleast_1d(sigmas_nm))
     n = max(1, int(trials))
     out = np.zeros(4 * sg.size)
@@ -398,8 +414,10 @@ def axis_scan(px, py, t_sets, scans, mode="fp32c"):
     scans: (tset, nz, wavelength_um, zlo, zspan, z_default) each, samples
     z_j = zlo + zspan j / (nz - 1).  Returns (best_z, best_I, curves) with the callers'
     first-maximum rule."""
-    px, py = _d(px), _d(py)
+    px, py = _pillars(px, py)
     t = _c(np.atleast_2d(t_sets))
+    if t.ndim != 2 or t.shape[1] != px.size:
+        raise ValueError("t_sets must be [nsets, npil]")
     descs = (_lib.crs_axis_scan_desc * len(scans))()
     total = 0
     for i, (tset, nz, lam, zlo, zspan, zdef) in enumerate(scans):
@@ -418,7 +436,7 @@ def verify_achromatic_focus_bands(px, py, t_bands, wavelengths_um, center_index:
                                   mode="fp32c") -> List[AchromaticFocalPoint]:
     """The focus check shared by verify_achromatic_focus / verify_pb_achromatic_focus on
     the aperture cells' per-band transmissions t_bands [nb, npil]."""
-    px, py = _d(px), _d(py)
+    px, py = _pillars(px, py)
     t = _c(np.atleast_2d(t_bands))
     lam = _d(wavelengths_um)
     nb = lam.size
@@ -453,7 +471,7 @@ def verify_pb_achromatic_focus(lib: DispersivePbLibrary, design: PbAchromaticDes
 def polar_focus_check(px, py, tx, ty, wavelength_um: float, focal_x_um: float,
                       focal_y_um: float, nz: int = 240, mode="fp32c") -> PolarFocus:
     """cmd_polar.cpp:85-126 (nz = 240; gui/workers.cpp:181-197 uses nz = 200)."""
-    px, py, tx, ty = _d(px), _d(py), _c(tx), _c(ty)
+    px, py, tx, ty = _pillars(px, py, _c(tx), _c(ty))
     out = np.zeros(4)
     _check(_lib.load().crs_polar_focus_check(_p(px), _p(py), _p(tx), _p(ty), px.size,
                                              float(wavelength_um), float(focal_x_um),

@@ -105,3 +105,22 @@ def test_set_default_mode_roundtrip():
     prev = crs.set_default_mode("fp64")
     assert crs.set_default_mode(prev) == "fp64"
     assert _lib.load().crs_set_default_mode(7) == -1
+
+
+def test_host_arrays_of_unequal_length_are_refused():
+    """The C entry points read npil elements of every per-pillar array; the Python
+    layer refuses mismatched lengths before calling them."""
+    from celeris_amd import analysis
+
+    x = np.zeros(4)
+    with pytest.raises(ValueError, match="one length"):
+        analysis.propagate_pillars(x, x, np.ones(3, complex), 0, 0, 50.0, 0.5, 9, 2.0)
+    with pytest.raises(ValueError, match="one length"):
+        analysis.analyze_focus_pillars(x, x, np.ones(4, complex), np.zeros(5), 50.0, 0.5, 20.0)
+    with pytest.raises(ValueError, match="one length"):
+        analysis.polar_focus_check(x, np.zeros(3), np.ones(4, complex), np.ones(4, complex),
+                                   0.5, 50.0, 60.0)
+    with pytest.raises(ValueError, match="npil"):
+        analysis.axis_scan(x, x, np.ones((2, 3), complex), [(0, 10, 0.5, 10.0, 20.0, 10.0)])
+    with pytest.raises(ValueError, match="one length"):
+        crs.propagate_psf(x, x, np.ones(3, complex), 0, 0, 50.0, 12.0, 9, 2.0)
