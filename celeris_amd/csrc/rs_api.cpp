@@ -252,9 +252,8 @@ std::vector<unsigned char> plan_key(int mode, int npil, int nsets, const crs_pla
 }
 
 
-crs_status check_common(crs_context* ctx, int mode, int npil, int nsets, const crs_plane* planes,
-                        int nplanes) {
-    if (!ctx) { set_error("null context"); return CRS_ERR_INVALID; }
+// Arguments every batched entry point shares, checked before any device work.
+crs_status check_args(int mode, int npil, int nsets, const crs_plane* planes, int nplanes) {
     if (mode < 0 || mode > 2) { set_error("unknown mode %d", mode); return CRS_ERR_INVALID; }
     if (npil <= 0) { set_error("npil must be positive (got %d)", npil); return CRS_ERR_INVALID; }
     if (nsets <= 0) { set_error("nsets must be positive (got %d)", nsets); return CRS_ERR_INVALID; }
@@ -281,6 +280,14 @@ crs_status check_common(crs_context* ctx, int mode, int npil, int nsets, const c
             return CRS_ERR_INVALID;
         }
     }
+    return CRS_OK;
+}
+
+crs_status check_common(crs_context* ctx, int mode, int npil, int nsets, const crs_plane* planes,
+                        int nplanes) {
+    if (!ctx) { set_error("null context"); return CRS_ERR_INVALID; }
+    const crs_status st = check_args(mode, npil, nsets, planes, nplanes);
+    if (st != CRS_OK) return st;
     if (hipSetDevice(ctx->device) != hipSuccess) {
         hipGetLastError();
         set_error("hipSetDevice(%d) failed", ctx->device);
@@ -1208,9 +1215,12 @@ extern "C" crs_status crs_forward_host(int mode, const double* px, const double*
                                        const crs_plane* planes, int nplanes, double* field,
                                        double* intensity) {
     g_last_error.clear();
-    if (!crs_available()) { set_error("no HIP device visible"); return CRS_ERR_NO_DEVICE; }
     if (!px || !py || !t || (!field && !intensity)) { set_error("null array argument"); return CRS_ERR_INVALID; }
     if (nplanes <= 0 || !planes) { set_error("no planes"); return CRS_ERR_INVALID; }
+    // Sizes and planes are checked here, before host_run sizes buffers from them.
+    const crs_status st = check_args(mode, npil, nsets, planes, nplanes);
+    if (st != CRS_OK) return st;
+    if (!crs_available()) { set_error("no HIP device visible"); return CRS_ERR_NO_DEVICE; }
     return host_run(mode, px, py, t, npil, nsets, planes, nplanes, field, intensity);
 }
 

@@ -124,3 +124,27 @@ def test_host_arrays_of_unequal_length_are_refused():
         analysis.axis_scan(x, x, np.ones((2, 3), complex), [(0, 10, 0.5, 10.0, 20.0, 10.0)])
     with pytest.raises(ValueError, match="one length"):
         crs.propagate_psf(x, x, np.ones(3, complex), 0, 0, 50.0, 12.0, 9, 2.0)
+
+
+def test_forward_host_checks_arguments_before_device_work():
+    """crs_forward_host refuses bad sizes and planes with CRS_ERR_INVALID (2) before it
+    sizes any buffer from them, with or without a device; valid arguments without a
+    device give CRS_ERR_NO_DEVICE (1)."""
+    lib = _lib.load()
+    px = np.zeros(3)
+    t = np.ones(3, complex)
+    out = np.zeros(25)
+    dp = ctypes.POINTER(ctypes.c_double)
+    good = crs.pack_planes([Plane.psf(0, 0, 10.0, 10.0, 5, 1.0)])[0]
+    bad_tset = crs.pack_planes([Plane.psf(0, 0, 10.0, 10.0, 5, 1.0, tset=1)])[0]
+
+    def call(npil, nsets, planes):
+        return lib.crs_forward_host(0, px.ctypes.data_as(dp), px.ctypes.data_as(dp),
+                                    t.ctypes.data_as(dp), npil, nsets, planes, 1, None,
+                                    out.ctypes.data_as(dp))
+
+    assert call(-5, 1, good) == 2
+    assert call(3, -1, good) == 2
+    assert call(3, 1, bad_tset) == 2
+    assert "tset" in _lib.last_error()
+    assert call(3, 1, good) == (0 if crs.available() else 1)
