@@ -18,6 +18,7 @@
 // by line; only the sums move to the GPU.  In the reference the on-axis scans of
 // through-focus and chromatic, and analyze_focus's ideal peak, were FP64 CPU loops.
 #include "../../include/celeris_rs.h"
+#include "rs_host.h"
 
 #include <algorithm>
 #include <cmath>
@@ -76,7 +77,11 @@ extern "C" crs_status crs_analyze_focus(const double* px, const double* py, cons
                                         const double* phi_ideal, int npil, double focal_um,
                                         double wavelength_um, double diameter_um, int mode,
                                         double* out5, double* intensity) {
-    if (!px || !py || !t || !phi_ideal || !out5 || npil <= 0) return CRS_ERR_INVALID;
+    crs::clear_error();
+    if (!px || !py || !t || !phi_ideal || !out5 || npil <= 0) {
+        crs::set_error("crs_analyze_focus: null array or npil <= 0");
+        return CRS_ERR_INVALID;
+    }
     const double f = focal_um;
     const double k = 2.0 * kPi / wavelength_um;
     const double dl = wavelength_um * focal_um / diameter_um;
@@ -137,8 +142,11 @@ extern "C" crs_status crs_analyze_through_focus(const double* px, const double* 
                                                 int caustic_n, int mode, double* z161,
                                                 double* I161, double* scalars6,
                                                 double* caustic) {
-    if (!px || !py || !t || !z161 || !I161 || !scalars6 || npil <= 0 || caustic_n < 2)
+    crs::clear_error();
+    if (!px || !py || !t || !z161 || !I161 || !scalars6 || npil <= 0 || caustic_n < 2) {
+        crs::set_error("crs_analyze_through_focus: null array, npil <= 0 or caustic_n < 2");
         return CRS_ERR_INVALID;
+    }
     const double f = focal_um;
     const double k = 2.0 * kPi / wavelength_um;
     const double R = diameter_um / 2.0;
@@ -200,7 +208,11 @@ extern "C" crs_status crs_analyze_chromatic(const double* px, const double* py, 
                                             int npil, double design_focal, double design_wavelength,
                                             double lambda_min, double lambda_max, int n, int mode,
                                             double* out3n) {
-    if (!px || !py || !t || !out3n || npil <= 0 || n < 2) return CRS_ERR_INVALID;
+    crs::clear_error();
+    if (!px || !py || !t || !out3n || npil <= 0 || n < 2) {
+        crs::set_error("crs_analyze_chromatic: null array, npil <= 0 or n < 2");
+        return CRS_ERR_INVALID;
+    }
     const int NZ = 120;  // chromatic.cpp:48
     std::vector<crs_plane> planes;
     planes.reserve(n + 1);
@@ -239,7 +251,11 @@ extern "C" crs_status crs_analyze_field_grid(const double* px, const double* py,
                                              int npil, double focal_um, double wavelength_um,
                                              double diameter_um, double max_angle_deg, int n_half,
                                              int psf_n, int mode, double* out7) {
-    if (!px || !py || !t || !out7 || npil <= 0 || n_half < 0 || psf_n < 2) return CRS_ERR_INVALID;
+    crs::clear_error();
+    if (!px || !py || !t || !out7 || npil <= 0 || n_half < 0 || psf_n < 2) {
+        crs::set_error("crs_analyze_field_grid: null array, npil <= 0, n_half < 0 or psf_n < 2");
+        return CRS_ERR_INVALID;
+    }
     const double k = 2.0 * kPi / wavelength_um;
     const double dl = wavelength_um * focal_um / diameter_um;  // field.cpp:176-178
     const double W = std::max(6.0 * dl, 4.0);
@@ -388,8 +404,11 @@ extern "C" crs_status crs_analyze_field_of_view(const double* px, const double* 
                                                 double wavelength_um, double diameter_um,
                                                 const double* angles_deg, int nangles, int mode,
                                                 double* out3) {
-    if (!px || !py || !t || npil <= 0 || nangles < 0 || (nangles > 0 && (!angles_deg || !out3)))
+    crs::clear_error();
+    if (!px || !py || !t || npil <= 0 || nangles < 0 || (nangles > 0 && (!angles_deg || !out3))) {
+        crs::set_error("crs_analyze_field_of_view: null array, npil <= 0 or nangles < 0");
         return CRS_ERR_INVALID;
+    }
     const double k = 2.0 * kPi / wavelength_um;
     const double dl = wavelength_um * focal_um / diameter_um;  // field.cpp:40
     auto spot = [&](double theta, double xc) {
@@ -418,8 +437,11 @@ extern "C" crs_status crs_analyze_wide_fov(const double* px, const double* py, c
                                            double stop_distance_um, const double* angles_deg,
                                            int nangles, int mode, double* out3) {
     (void)lens_diameter_um;  // the caller's pillar list is already cut to the lens aperture
-    if (!px || !py || !t || npil <= 0 || nangles < 0 || (nangles > 0 && (!angles_deg || !out3)))
+    crs::clear_error();
+    if (!px || !py || !t || npil <= 0 || nangles < 0 || (nangles > 0 && (!angles_deg || !out3))) {
+        crs::set_error("crs_analyze_wide_fov: null array, npil <= 0 or nangles < 0");
         return CRS_ERR_INVALID;
+    }
     const double k = 2.0 * kPi / wavelength_um;
     const double dl_stop = wavelength_um * focal_um / stop_diameter_um;  // field.cpp:91
     const double R_stop = stop_diameter_um / 2.0;
@@ -461,14 +483,20 @@ crs_status run_axis_scans(const double* px, const double* py, const double* t, i
     int64_t total = 0;
     for (int s = 0; s < nscans; ++s) {
         const crs_axis_scan_desc& d = scans[s];
-        if (d.nz < 1 || d.tset < 0 || d.tset >= nsets || !(d.wavelength_um > 0.0))
+        if (d.nz < 1 || d.tset < 0 || d.tset >= nsets || !(d.wavelength_um > 0.0)) {
+            crs::set_error("axis scan %d: nz < 1, tset outside [0, %d) or wavelength <= 0", s,
+                           nsets);
             return CRS_ERR_INVALID;
+        }
         planes[s] = axis_plane(d.zlo, d.zlo + d.zspan, d.nz, 2.0 * kPi / d.wavelength_um, d.tset,
                                total);
         if (d.nz > 1) planes[s].sz = d.zspan / (d.nz - 1);
         total += d.nz;
     }
-    if (total > (int64_t(1) << 31)) return CRS_ERR_INVALID;
+    if (total > (int64_t(1) << 31)) {
+        crs::set_error("axis scans: more than 2^31 samples");
+        return CRS_ERR_INVALID;
+    }
     I.assign(static_cast<size_t>(total), 0.0);
     return crs_forward_host(mode, px, py, t, npil, nsets, planes.data(), nscans, nullptr, I.data());
 }
@@ -492,8 +520,11 @@ void scan_argmax(const crs_axis_scan_desc& d, const double* I, double* best_z, d
 extern "C" crs_status crs_axis_scan(const double* px, const double* py, const double* t, int npil,
                                     int nsets, const crs_axis_scan_desc* scans, int nscans,
                                     int mode, double* best_z, double* best_I, double* curves) {
-    if (!px || !py || !t || !scans || !best_z || !best_I || npil <= 0 || nsets <= 0 || nscans <= 0)
+    crs::clear_error();
+    if (!px || !py || !t || !scans || !best_z || !best_I || npil <= 0 || nsets <= 0 || nscans <= 0) {
+        crs::set_error("crs_axis_scan: null array or a count <= 0");
         return CRS_ERR_INVALID;
+    }
     std::vector<double> I;
     crs_status st = run_axis_scans(px, py, t, npil, nsets, scans, nscans, mode, I);
     if (st != CRS_OK) return st;
@@ -511,9 +542,13 @@ extern "C" crs_status crs_verify_achromatic_focus(const double* px, const double
                                                   const double* wavelengths_um, int center_index,
                                                   double center_wavelength_um, double focal_um,
                                                   int mode, double* out3nb) {
+    crs::clear_error();
     if (!px || !py || !t_bands || !wavelengths_um || !out3nb || npil <= 0 || nb <= 0 ||
-        center_index < 0 || center_index >= nb)
+        center_index < 0 || center_index >= nb) {
+        crs::set_error("crs_verify_achromatic_focus: null array, a count <= 0 or center_index "
+                       "outside [0, nb)");
         return CRS_ERR_INVALID;
+    }
     const int NZ = 200;  // achromatic.cpp:322, pb_achromatic.cpp:289
     const double zlo = 0.4 * focal_um, zhi = 1.8 * focal_um;
     // Scan 0: the design peak, I(f) of the centre band at the centre wavelength
@@ -540,7 +575,11 @@ extern "C" crs_status crs_polar_focus_check(const double* px, const double* py, 
                                             const double* ty, int npil, double wavelength_um,
                                             double focal_x_um, double focal_y_um, int nz, int mode,
                                             double* out4) {
-    if (!px || !py || !tx || !ty || !out4 || npil <= 0 || nz < 1) return CRS_ERR_INVALID;
+    crs::clear_error();
+    if (!px || !py || !tx || !ty || !out4 || npil <= 0 || nz < 1) {
+        crs::set_error("crs_polar_focus_check: null array, npil <= 0 or nz < 1");
+        return CRS_ERR_INVALID;
+    }
     const double zlo = 0.5 * std::min(focal_x_um, focal_y_um);
     const double zhi = 1.5 * std::max(focal_x_um, focal_y_um);
     std::vector<double> t(4 * static_cast<size_t>(npil));

@@ -11,6 +11,7 @@
 //   - synchronous; callable from several host threads at once (the reference's
 //     callers do so from std::async trials and GUI threads)
 #include "../../include/celeris_rs.h"
+#include "rs_host.h"
 #include "rs_internal.h"
 
 #include <hip/hip_runtime.h>
@@ -51,10 +52,10 @@ hipError_t launch_mtf(const double2* P, const double* P0, int n, int ns, double 
 using namespace crs;
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
 
-void set_error(const char* fmt, ...) {
+void crs::set_error(const char* fmt, ...) {
     char buf[512];
     va_list ap;
     va_start(ap, fmt);
@@ -62,6 +63,10 @@ void set_error(const char* fmt, ...) {
     va_end(ap);
     g_last_error = buf;
 }
+
+void crs::clear_error() { g_last_error.clear(); }
+
+namespace {
 
 // Fault injection for tests of the failure paths: CELERIS_RS_FAIL=alloc makes every
 // scratch allocation fail, CELERIS_RS_FAIL=launch makes every launch fail; in both

@@ -10,6 +10,7 @@
 // with the ideal lens's peak.
 
 #include "../../include/celeris_rs.h"
+#include "rs_host.h"
 
 #include <algorithm>
 #include <cmath>
@@ -42,11 +43,18 @@ extern "C" crs_status crs_analyze_tolerance(const double* fill_map, int n_cells,
                                             const double* sigmas_nm, int nsigmas, int trials,
                                             unsigned seed, int mode, double* out4,
                                             double* strehl) {
+    crs::clear_error();
     if (!fill_map || !lib_fill || !lib_phase || !lib_amp || !sigmas_nm || !out4 || n_cells <= 0 ||
-        nlib <= 0 || nsigmas <= 0 || !(period_um > 0.0) || !(wavelength_um > 0.0))
+        nlib <= 0 || nsigmas <= 0 || !(period_um > 0.0) || !(wavelength_um > 0.0)) {
+        crs::set_error("crs_analyze_tolerance: null array, a count <= 0, or period or "
+                       "wavelength not positive");
         return CRS_ERR_INVALID;
+    }
     for (int s = 0; s < nsigmas; ++s)
-        if (!(sigmas_nm[s] >= 0.0)) return CRS_ERR_INVALID;
+        if (!(sigmas_nm[s] >= 0.0)) {
+            crs::set_error("crs_analyze_tolerance: sigma %d is negative or NaN", s);
+            return CRS_ERR_INVALID;
+        }
 
     // Aperture cells (focal.cpp:42-57): site, position, ideal focusing phase.
     const double p = period_um;
@@ -68,7 +76,10 @@ extern "C" crs_status crs_analyze_tolerance(const double* fill_map, int n_cells,
             phi.push_back(-k * (std::sqrt(r * r + f * f) - f));
         }
     const int ncell = static_cast<int>(site.size());
-    if (ncell == 0) return CRS_ERR_INVALID;
+    if (ncell == 0) {
+        crs::set_error("crs_analyze_tolerance: no cell inside the aperture");
+        return CRS_ERR_INVALID;
+    }
 
     const double dl = wavelength_um * focal_um / diameter_um;
     const double W = std::max(6.0 * dl, 4.0);  // focal.cpp:61-64

@@ -148,3 +148,22 @@ def test_forward_host_checks_arguments_before_device_work():
     assert call(3, 1, bad_tset) == 2
     assert "tset" in _lib.last_error()
     assert call(3, 1, good) == (0 if crs.available() else 1)
+
+
+def test_analysis_entry_points_report_invalid_arguments():
+    """The analysis entry points clear the thread's last error on entry and name what
+    was wrong when they refuse their arguments."""
+    lib = _lib.load()
+    dp = ctypes.POINTER(ctypes.c_double)
+    x = np.zeros(4)
+    out = np.zeros(8)
+    lib.crs_forward_host(0, x.ctypes.data_as(dp), x.ctypes.data_as(dp), x.ctypes.data_as(dp),
+                         -1, 1, None, 0, None, out.ctypes.data_as(dp))  # leaves a message
+    assert lib.crs_analyze_focus(x.ctypes.data_as(dp), x.ctypes.data_as(dp), x.ctypes.data_as(dp),
+                                 x.ctypes.data_as(dp), 0, 50.0, 0.5, 20.0, 0,
+                                 out.ctypes.data_as(dp), None) == 2
+    assert "crs_analyze_focus" in _lib.last_error()
+    assert lib.crs_polar_focus_check(x.ctypes.data_as(dp), x.ctypes.data_as(dp),
+                                     x.ctypes.data_as(dp), x.ctypes.data_as(dp), 2, 0.5, 50.0,
+                                     60.0, 0, 0, out.ctypes.data_as(dp)) == 2
+    assert "nz < 1" in _lib.last_error()
