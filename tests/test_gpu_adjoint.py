@@ -144,7 +144,7 @@ def test_config4_gradient_fp32c_vs_fp64(torch_cuda):
         t = torch.polar(amp, phi)[None]
         loss = _ee_loss(torch, rs_field(t, px, py, [plane], mode=mode), plane, r0)
         loss.backward()
-        grads[mode] = (phi.grad.cpu().numpy(), amp.grad.cpu().numpy(), float(loss))
+        grads[mode] = (phi.grad.cpu().numpy(), amp.grad.cpu().numpy(), float(loss.detach()))
     ep = rel_l2(grads["fp32c"][0], grads["fp64"][0])
     ea = rel_l2(grads["fp32c"][1], grads["fp64"][1])
     print(f"config4 grad relL2 phase {ep:.3e} amp {ea:.3e} loss {grads['fp64'][2]:.6f}")
