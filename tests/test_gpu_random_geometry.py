@@ -143,3 +143,72 @@ def test_fp32c_vs_fp64_random_geometry(torch_cuda, rsctx, seed):
     assert ge <= BOUND
     print(f"[random geometry {seed}] field relL2 {rel_l2(E32, E64):.2e}, gradient relL2 "
           f"{rel_l2(g32, g64):.2e}")
+
+
+def batched_geometry(seed):
+    """Several transmission sets and one wavelength per plane, as the batched analyses
+    use them (chromatic scans, tolerance trials, field points)."""
+    rng = np.random.default_rng(5000 + seed)
+    p = rng.uniform(0.25, 0.5)
+    n_side = int(rng.integers(8, 48))
+    c = (np.arange(n_side) - (n_side - 1) / 2.0) * p
+    X, Y = np.meshgrid(c, c)
+    px, py = X.ravel(), Y.ravel()
+    nsets = int(rng.integers(1, 4))
+    t = rng.uniform(0.3, 1.0, (nsets, px.size)) * np.exp(1j * rng.uniform(-np.pi, np.pi, (nsets, px.size)))
+    half = n_side * p / 2.0
+    planes = []
+    for _ in range(int(rng.integers(2, 6))):
+        lam = rng.uniform(0.4, 0.9)
+        k = 2.0 * math.pi / lam
+        tset = int(rng.integers(0, nsets))
+        n = int(rng.integers(2, 60))
+        z = float(np.exp(rng.uniform(np.log(1.0), np.log(200.0))))
+        W = 0.5 * (n - 1) * lam * float(np.exp(rng.uniform(np.log(0.03), np.log(0.8))))
+        cx, cy = rng.uniform(-half, half, 2)
+        if rng.random() < 0.8:
+            planes.append(Plane.psf(float(cx), float(cy), z, k, n, W, tset=tset))
+        else:
+            planes.append(Plane.zx(float(cx) - W, float(cx) + W, n, z, z + (n - 1) * 0.3 * lam, n,
+                                   k, tset=tset))
+    return px, py, t, planes
+
+
+@pytest.mark.parametrize("seed", list(range(16)))
+def test_fp32c_vs_fp64_random_batches(torch_cuda, rsctx, seed):
+    """Batched calls: per-plane wavelength and transmission set; forward per element
+    against FP64 with the bound above (each plane against its own set's |t|), adjoint per
+    set and pillar against sum_p |u_p| / R_pq over that set's planes."""
+    torch = torch_cuda
+    px, py, t, planes = batched_geometry(seed)
+    nsets = t.shape[0]
+    print(f"[random batch {seed}] npil={px.size} sets={nsets} planes="
+          + ", ".join(f"{'zx' if P.sz else 'psf'} {P.nx}x{P.ny} set {P.tset}" for P in planes),
+          flush=True)
+    dev = torch.device("cuda:0")
+    dpx = torch.from_numpy(px).to(dev)
+    dpy = torch.from_numpy(py).to(dev)
+    dt = torch.from_numpy(np.ascontiguousarray(t)).to(dev)
+    E64 = rsctx.forward(dpx, dpy, dt, planes, mode="fp64")[0].cpu().numpy()
+    E32 = rsctx.forward(dpx, dpy, dt, planes, mode="fp32c")[0].cpu().numpy()
+    rng = np.random.default_rng(99 + seed)
+    u = rng.standard_normal(E64.size) + 1j * rng.standard_normal(E64.size)
+    du = torch.from_numpy(u).to(dev)
+    g64 = rsctx.adjoint(dpx, dpy, du, planes, nsets=nsets, mode="fp64").cpu().numpy()
+    g32 = rsctx.adjoint(dpx, dpy, du, planes, nsets=nsets, mode="fp32c").cpu().numpy()
+    fe, ge = 0.0, 0.0
+    T = np.zeros((nsets, px.size))
+    for P in crs.resolve_offsets(planes):
+        sl = slice(P.offset, P.offset + P.npix)
+        fx, fy, fz = pixel_coords([P])
+        S, Tp = magnitude_sums(torch, px, py, np.abs(t[P.tset]), fx, fy, fz, np.abs(u[sl]))
+        fe = max(fe, float(np.max(np.abs(E32[sl] - E64[sl]) / S)))
+        T[P.tset] += Tp
+    used = sorted({P.tset for P in planes})
+    ge = float(np.max(np.abs(g32[used] - g64[used]) / T[used]))
+    print(f"[random batch {seed}] {rsctx.last_launch(False)[0]}: forward max {fe:.2e}, "
+          f"adjoint max {ge:.2e}", flush=True)
+    assert fe <= BOUND
+    assert ge <= BOUND
+    unused = [s for s in range(nsets) if s not in used]
+    assert np.all(g32[unused] == 0) and np.all(g64[unused] == 0)
