@@ -149,6 +149,16 @@ def test_fp32c_vs_fp64_random_geometry(torch_cuda, rsctx, seed):
     assert ge <= BOUND
     print(f"[random geometry {seed}] field relL2 {rel_l2(E32, E64):.2e}, gradient relL2 "
           f"{rel_l2(g32, g64):.2e}")
+    # Plain FP32 (the reference kernel's arithmetic: k R formed and reduced in float):
+    # each pair's phase is off by a few float roundings of k R (~2^-24 k R each) and its
+    # 1/R by ~1e-7, so |dE_p| <= (4e-7 k R_max + 1e-6) sum_q |t_q| / R_pq.
+    Ef = rsctx.forward(dpx, dpy, dt, planes, mode="fp32", field=True)[0].cpu().numpy()
+    kR = max(P.k for P in planes) * float(np.max(np.sqrt(
+        (np.max(np.abs(fx)) + np.max(np.abs(px))) ** 2
+        + (np.max(np.abs(fy)) + np.max(np.abs(py))) ** 2 + np.max(fz ** 2))))
+    e32 = float(np.max(np.abs(Ef - E64) / S))
+    print(f"[random geometry {seed}] plain FP32 (k R_max {kR:.0f}): max {e32:.2e}")
+    assert e32 <= 4e-7 * kR + 1e-6
 
 
 def batched_geometry(seed):
