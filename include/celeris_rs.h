@@ -44,10 +44,14 @@ typedef enum crs_status {
 /* Arithmetic modes. */
 typedef enum crs_mode {
     /* FP32 pair arithmetic; each pixel block's phase reference k*(R0 - z) is formed
-     * and reduced mod 2*pi in FP64.  Forward: float sums over at most 64 pillars,
-     * folded into float running sums over at most 4,096 pillars per wave, then FP64
-     * across waves and pillar chunks.  Adjoint: float sums per pixel block, Kahan-
-     * compensated across pixel tiles, then FP64 across runs.  Default and fastest. */
+     * and reduced mod 2*pi in FP64, and each pixel's offset from it is expanded to the
+     * order that keeps every pair within 1e-6 rad of phase and 4e-6 of amplitude; a
+     * plane too close or too coarsely sampled for that is evaluated pair by pair, each
+     * pair at its own FP64 distance (decided per plane).  Forward: float sums over at
+     * most 64 pillars, folded into float running sums over at most 4,096 pillars per
+     * wave, then FP64 across waves and pillar chunks.  Adjoint: float sums per pixel
+     * block, Kahan-compensated across pixel tiles, then FP64 across runs.  Default and
+     * fastest. */
     CRS_MODE_FP32C = 0,
     /* The reference algorithm in FP64 throughout (sqrt, sincos, 1/R, complex MAC). */
     CRS_MODE_FP64 = 1,
@@ -327,8 +331,10 @@ void crs_graph_destroy(crs_graph *graph);
 
 /* Kernel timing (no reference counterpart).  With timing enabled, each crs_forward /
  * crs_adjoint records HIP events on its stream immediately before and after the
- * kernel that does the pair sums; crs_last_kernel_ms waits for the last such event
- * and returns that kernel's duration in milliseconds (adjoint = 0: forward). */
+ * kernel that does the pair sums (an adjoint that mixes planes evaluated pair by pair
+ * with blocked ones runs two such kernels, and the events bracket both);
+ * crs_last_kernel_ms waits for the last such event and returns that duration in
+ * milliseconds (adjoint = 0: forward). */
 crs_status crs_set_timing(crs_context *ctx, int enable);
 crs_status crs_last_kernel_ms(crs_context *ctx, int adjoint, float *ms);
 
@@ -338,7 +344,8 @@ const char *crs_kernel_name(int mode, int adjoint);
 
 /* The pair-sum kernel the last successful crs_forward (adjoint = 0) or crs_adjoint
  * (adjoint = 1) on ctx launched: its name as a profiler lists it (demangled, without
- * the argument list, e.g. "crs::fwd_fp32c<4>") and its number of work-groups.
+ * the argument list, e.g. "crs::fwd_fp32c<4>") and its number of work-groups (for a
+ * mixed adjoint, the first of its two launches).
  * CRS_ERR_INVALID before the first such call. */
 crs_status crs_last_launch(crs_context *ctx, int adjoint, const char **kernel,
                            int *workgroups);
