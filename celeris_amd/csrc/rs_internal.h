@@ -52,22 +52,16 @@ __host__ __device__ inline double plane_min_abs_z(double z0, double sz, int ny) 
     return 0.0;
 }
 
-// FP32C staged pillar buffer (forward): positions (double2, npil), {|t|, arg t / 2 pi}
-// (float2, nsets x npil), the staging blocks' maxima of |x|, |y| (double2 per block of
-// kThreads staged elements) and their overall maximum (one double2, um).
+// FP32C staged pillar buffer (forward): positions (double2, npil) and {|t|, arg t / 2 pi}
+// (float2, nsets x npil).
 struct StagedLayout {
-    size_t pos, ap, bmax, ext, bytes;
-    int64_t nbmax;
+    size_t pos, ap, bytes;
 };
 inline StagedLayout fp32c_staged_layout(int64_t npil, int64_t nsets) {
-    auto align16 = [](size_t v) { return (v + 15) & ~static_cast<size_t>(15); };
     StagedLayout L;
-    L.nbmax = (npil * nsets + kThreads - 1) / kThreads;
     L.pos = 0;
     L.ap = static_cast<size_t>(npil) * 16;
-    L.bmax = align16(L.ap + static_cast<size_t>(npil * nsets) * 8);
-    L.ext = L.bmax + static_cast<size_t>(L.nbmax) * 16;
-    L.bytes = L.ext + 16;
+    L.bytes = L.ap + static_cast<size_t>(npil * nsets) * 8;
     return L;
 }
 

@@ -76,72 +76,17 @@ struct alignas(16) PillarRec {
 
 // FP32C: positions stay FP64 (pos[npil], shared by all sets; each plane scales them
 // by its own k/(2 pi) when it loads a tile) and {|t|, arg(t)/(2 pi)} per pillar and set.
-// t == NULL stages positions only.  Each block also writes the maximum |x|, |y| of the
-// positions it staged (0 when it staged none) for extent_reduce.
+// t == NULL stages positions only.
 __global__ __launch_bounds__(kThreads) void stage_pillars_fp32c(
     const double* __restrict__ px, const double* __restrict__ py, const double2* __restrict__ t,
-    int npil, int64_t total, double2* __restrict__ pos, float2* __restrict__ ap,
-    double2* __restrict__ bmax) {
-    __shared__ double2 red[kWaves];
+    int npil, int64_t total, double2* __restrict__ pos, float2* __restrict__ ap) {
     const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    double mx = 0.0, my = 0.0;
-    if (idx < total) {
-        const int q = static_cast<int>(idx % npil);
-        if (idx < npil) {
-            const double x = px[q], y = py[q];
-            pos[q] = make_double2(x, y);
-            mx = fabs(x);
-            my = fabs(y);
-        }
-        if (t) {
-            const double2 v = t[idx];
-            ap[idx] = make_float2(static_cast<float>(hypot(v.x, v.y)),
-                                  static_cast<float>(atan2(v.y, v.x) * kInv2Pi));
-        }
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        mx = fmax(mx, __shfl_xor(mx, o));
-        my = fmax(my, __shfl_xor(my, o));
-    }
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = make_double2(mx, my);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double2 m = red[0];
-#pragma unroll
-        for (int w = 1; w < kWaves; ++w) {
-            m.x = fmax(m.x, red[w].x);
-            m.y = fmax(m.y, red[w].y);
-        }
-        bmax[blockIdx.x] = m;
-    }
-}
-
-// One work-group: ext = max over the staging blocks' maxima (the pillars' extent, um).
-__global__ __launch_bounds__(kThreads) void extent_reduce(const double2* __restrict__ bmax,
-                                                          int n, double2* __restrict__ ext) {
-    __shared__ double2 red[kWaves];
-    double mx = 0.0, my = 0.0;
-    for (int i = threadIdx.x; i < n; i += kThreads) {
-        const double2 v = bmax[i];
-        mx = fmax(mx, v.x);
-        my = fmax(my, v.y);
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        mx = fmax(mx, __shfl_xor(mx, o));
-        my = fmax(my, __shfl_xor(my, o));
-    }
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = make_double2(mx, my);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double2 m = red[0];
-#pragma unroll
-        for (int w = 1; w < kWaves; ++w) {
-            m.x = fmax(m.x, red[w].x);
-            m.y = fmax(m.y, red[w].y);
-        }
-        ext[0] = m;
+    if (idx >= total) return;
+    if (idx < npil) pos[idx] = make_double2(px[idx], py[idx]);
+    if (t) {
+        const double2 v = t[idx];
+        ap[idx] = make_float2(static_cast<float>(hypot(v.x, v.y)),
+                              static_cast<float>(atan2(v.y, v.x) * kInv2Pi));
     }
 }
 
@@ -343,8 +288,8 @@ __device__ inline void tile_origin(const DevPlane& P, int shape, int t, int& ox,
 template <int BX, int BY>
 __device__ __forceinline__ void fwd_tiles(
     const double2* __restrict__ pos, const float2* __restrict__ ap, int npil,
-    const DevPlane& P, const FwdBlock& B, const double2* __restrict__ ext,
-    PillarRec* tile, float2* run, float2* __restrict__ partial) {
+    const DevPlane& P, const FwdBlock& B, PillarRec* tile, float2* run,
+    float2* __restrict__ partial) {
     constexpr int N = BX * BY;
     const int nx = P.nx;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -354,9 +299,38 @@ __device__ __forceinline__ void fwd_tiles(
     tile_origin(P, B.shape, active ? my_tile : B.tile0, ox, oy);
     const double kt = P.kturn;
     const double zref = P.zref;
-    // Uniform across the work-group; readfirstlane makes the branch on it scalar.
-    const bool gen =
-        __builtin_amdgcn_readfirstlane(N > 1 && !fp32c_first_order(P, ext[0], BX, BY));
+
+    // Series order from the work-group's own pillar chunk: its extent max |x|, max |y|
+    // (um), reduced through the running-sum buffer before that is cleared.  Uniform
+    // across the work-group; readfirstlane makes the branch on it scalar.
+    bool gen = false;
+    if constexpr (N > 1) {
+        double mx = 0.0, my = 0.0;
+        for (int q = B.q0 + tid; q < B.q1; q += kThreads) {
+            const double2 xy = pos[q];
+            mx = fmax(mx, fabs(xy.x));
+            my = fmax(my, fabs(xy.y));
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            mx = fmax(mx, __shfl_xor(mx, o));
+            my = fmax(my, __shfl_xor(my, o));
+        }
+        double* red = reinterpret_cast<double*>(run);
+        if (lane == 0) {
+            red[2 * wave] = mx;
+            red[2 * wave + 1] = my;
+        }
+        __syncthreads();
+        double2 E = make_double2(red[0], red[1]);
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w) {
+            E.x = fmax(E.x, red[2 * w]);
+            E.y = fmax(E.y, red[2 * w + 1]);
+        }
+        gen = __builtin_amdgcn_readfirstlane(!fp32c_first_order(P, E, BX, BY));
+        __syncthreads();  // every wave has read the maxima before run is cleared
+    }
 
     BlockGeom<BX, BY> G;
     {
@@ -449,7 +423,7 @@ template <int BLK>
 __global__ __launch_bounds__(kThreads, 4) void fwd_fp32c(
     const double2* __restrict__ pos, const float2* __restrict__ ap, int npil,
     const DevPlane* __restrict__ planes, const FwdBlock* __restrict__ blocks,
-    const double2* __restrict__ ext, float2* __restrict__ partial) {
+    float2* __restrict__ partial) {
     constexpr int NMAX = BLK * BLK;
     __shared__ PillarRec tile[kThreads];
     __shared__ float2 run[kWaves * NMAX * 64];
@@ -457,13 +431,13 @@ __global__ __launch_bounds__(kThreads, 4) void fwd_fp32c(
     const DevPlane& P = planes[B.plane];
     if constexpr (BLK == kBlk) {
         switch (B.shape) {
-            case 0: fwd_tiles<kBlk, kBlk>(pos, ap, npil, P, B, ext, tile, run, partial); break;
-            case 1: fwd_tiles<1, kBlk>(pos, ap, npil, P, B, ext, tile, run, partial); break;
-            case 2: fwd_tiles<kBlk, 1>(pos, ap, npil, P, B, ext, tile, run, partial); break;
-            default: fwd_tiles<1, 1>(pos, ap, npil, P, B, ext, tile, run, partial); break;
+            case 0: fwd_tiles<kBlk, kBlk>(pos, ap, npil, P, B, tile, run, partial); break;
+            case 1: fwd_tiles<1, kBlk>(pos, ap, npil, P, B, tile, run, partial); break;
+            case 2: fwd_tiles<kBlk, 1>(pos, ap, npil, P, B, tile, run, partial); break;
+            default: fwd_tiles<1, 1>(pos, ap, npil, P, B, tile, run, partial); break;
         }
     } else {
-        fwd_tiles<1, 1>(pos, ap, npil, P, B, ext, tile, run, partial);
+        fwd_tiles<1, 1>(pos, ap, npil, P, B, tile, run, partial);
     }
 }
 
@@ -1036,12 +1010,9 @@ hipError_t launch_stage_pillars(int mode, const double* px, const double* py, co
     if (mode == 0) {
         const StagedLayout L = fp32c_staged_layout(npil, nsets);
         unsigned char* base = static_cast<unsigned char*>(out);
-        double2* bmax = reinterpret_cast<double2*>(base + L.bmax);
         stage_pillars_fp32c<<<grid_for(total), kThreads, 0, s>>>(
             px, py, tt, npil, total, reinterpret_cast<double2*>(base + L.pos),
-            reinterpret_cast<float2*>(base + L.ap), bmax);
-        extent_reduce<<<1, kThreads, 0, s>>>(bmax, static_cast<int>(L.nbmax),
-                                             reinterpret_cast<double2*>(base + L.ext));
+            reinterpret_cast<float2*>(base + L.ap));
     } else if (mode == 2) {
         stage_pillars_fp32<<<grid_for(total), kThreads, 0, s>>>(px, py, tt, npil, total,
                                                                  static_cast<float4*>(out));
@@ -1062,13 +1033,12 @@ hipError_t launch_forward(int mode, int order, const void* pil, int npil, int ns
         const unsigned char* base = static_cast<const unsigned char*>(pil);
         const double2* pos = reinterpret_cast<const double2*>(base + L.pos);
         const float2* ap = reinterpret_cast<const float2*>(base + L.ap);
-        const double2* ext = reinterpret_cast<const double2*>(base + L.ext);
         // order kExactOrder: 1 x 1 blocks (every pair at its own reference distance);
-        // otherwise 4 x 4 blocks, series order chosen per work-group from ext.
+        // otherwise 4 x 4 blocks, series order chosen per pillar tile from its extent.
         if (order == kExactOrder)
-            fwd_fp32c<1><<<nblocks, kThreads, 0, s>>>(pos, ap, npil, planes, blocks, ext, partial);
+            fwd_fp32c<1><<<nblocks, kThreads, 0, s>>>(pos, ap, npil, planes, blocks, partial);
         else
-            fwd_fp32c<kBlk><<<nblocks, kThreads, 0, s>>>(pos, ap, npil, planes, blocks, ext, partial);
+            fwd_fp32c<kBlk><<<nblocks, kThreads, 0, s>>>(pos, ap, npil, planes, blocks, partial);
     } else if (mode == 2) {
         fwd_fp32<<<nblocks, kThreads, 0, s>>>(static_cast<const float4*>(pil), npil, planes,
                                               blocks, static_cast<double2*>(partial_v));
