@@ -67,6 +67,8 @@ def angle_diff(a, b) -> np.ndarray:
     (metalens.cpp:15-20), element-wise."""
     d = np.asarray(a, np.float64) - np.asarray(b, np.float64)
     d = np.array(d, dtype=np.float64, copy=True)
+    if np.isinf(d).any():  # one turn at a time never reaches (-pi, pi] from +-inf
+        raise ValueError("angle_diff: infinite angle")
     two_pi = 2.0 * math.pi
     while True:
         m = d > math.pi

@@ -58,3 +58,11 @@ def test_psfbench_lens_inputs():
     assert L.lens.n_cells == 343 and L.px.size == 92361
     assert L.half_window_um == 4.0 and L.grid == 161
     assert np.all(np.abs(L.t) > 0.8)
+
+
+def test_angle_diff_refuses_infinite_angles():
+    """Wrapping one turn at a time (metalens.cpp:15-20) cannot finish from +-inf; NaN
+    passes through as NaN."""
+    with pytest.raises(ValueError, match="infinite"):
+        angle_diff(np.array([0.0, np.inf]), 0.0)
+    assert np.isnan(angle_diff(np.nan, 0.0))
