@@ -70,6 +70,10 @@ def cpu_model() -> str:
     return platform.processor()
 
 
+def _every(stride: int, what: str) -> str:
+    return f"every {what}" if stride == 1 else f"every {stride}th {what}"
+
+
 def cpu_baseline(lens, u_np, budget_s: float) -> dict:
     """The oracle (FP64 restatement of focal.cpp:205-238 + its transposed adjoint) on
     the host cores: rows striped over threads for the forward, pillars for the
@@ -111,9 +115,9 @@ def cpu_baseline(lens, u_np, budget_s: float) -> dict:
         "unit": "G pairs/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"oracle FP64 forward on every {row_stride}th focal row ({p_fwd:.3e} pairs, "
-                   f"{t_fwd:.2f} s) + adjoint on every {pil_stride}th pillar ({p_adj:.3e} pairs, "
-                   f"{t_adj:.2f} s), {threads} threads, scaled per pair"),
+        "sample": (f"oracle FP64 forward on {_every(row_stride, 'focal row')} ({p_fwd:.3e} pairs, "
+                   f"{t_fwd:.2f} s) + adjoint on {_every(pil_stride, 'pillar')} ({p_adj:.3e} "
+                   f"pairs, {t_adj:.2f} s), {threads} threads, scaled per pair"),
         "cpu_model": cpu_model(),
         "fwd_s_per_pair": t_fwd / p_fwd,
         "adj_s_per_pair": t_adj / p_adj,
