@@ -71,7 +71,10 @@ def cpu_model() -> str:
 
 
 def _every(stride: int, what: str) -> str:
-    return f"every {what}" if stride == 1 else f"every {stride}th {what}"
+    if stride == 1:
+        return f"every {what}"
+    suffix = "th" if 10 <= stride % 100 <= 20 else {1: "st", 2: "nd", 3: "rd"}.get(stride % 10, "th")
+    return f"every {stride}{suffix} {what}"
 
 
 def cpu_baseline(lens, u_np, budget_s: float) -> dict:
