@@ -181,7 +181,7 @@ def run_psfbench(args) -> None:
     cpu_ms, threads, cpu_I, stride = psfbench_cpu_baseline(L)
     rows = slice(0, L.grid, stride)  # agreement over the rows the CPU leg computed
     print("")
-    sample = "" if stride == 1 else f", every {stride}th row, scaled"
+    sample = "" if stride == 1 else f", {_every(stride, 'row')}, scaled"
     print(f"  CPU (double, parallel) : {cpu_ms:8.1f} ms   ({threads} threads, "
           f"{pairs / cpu_ms / 1e6:.2f} G pairs/s{sample})")
     rec = {"workload": "psfbench", "pillars": int(L.px.size), "grid": L.grid,
