@@ -246,7 +246,8 @@ def main():
         from celeris_amd.tune import autotune
 
         tuning = autotune(ctx, px, py, t, [plane], u, mode=args.mode, use_cache=False)
-        print(f"[bench] tuned work-groups: forward {tuning['fwd']}, adjoint {tuning.get('adj')}",
+        print(f"[bench] tuned: forward {tuning.get('fwd_shape')} x {tuning['fwd']} work-groups, "
+              f"adjoint {tuning.get('adj_shape')} x {tuning.get('adj')}",
               file=sys.stderr, flush=True)
 
     def step(mode):
@@ -347,7 +348,9 @@ def main():
             "mode": args.mode,
             "parallelism": "single GPU",
             "tuning": tuning and {"fwd_target_blocks": tuning["fwd"],
-                                  "adj_target_blocks": tuning.get("adj")},
+                                  "adj_target_blocks": tuning.get("adj"),
+                                  "fwd_block": tuning.get("fwd_shape"),
+                                  "adj_block": tuning.get("adj_shape")},
         },
         "roofline": {
             "bound": "valu",

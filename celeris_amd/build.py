@@ -53,6 +53,20 @@ def build_library() -> str:
     return LIB
 
 
+DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "dropin_driver.cpp")
+DRIVER = os.path.join(ROOT, "tests", "_build", "dropin_driver")
+
+
+def build_dropin_driver() -> str:
+    """The C++ drop-in test caller: plain g++ against include/celeris/cuda/propagate.hpp,
+    linked to the built library (found next to the binary's tree through its rpath)."""
+    os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), DRIVER_SRC,
+          "-L" + LIBDIR, "-lceleris_rs", "-Wl,-rpath,$ORIGIN/../../celeris_amd/_lib", "-pthread",
+          "-o", DRIVER])
+    return DRIVER
+
+
 def build_oracle() -> str:
     _run(["make", "-s", "-C", ORACLE])
     return os.path.join(ORACLE, "_build", "liboracle_rs.so")
@@ -70,6 +84,7 @@ def build_reference() -> bool:
 
 def build_all() -> None:
     build_library()
+    build_dropin_driver()
     build_oracle()
     build_reference()
 

@@ -18,12 +18,12 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -31,18 +31,28 @@
 namespace crs {
 hipError_t launch_stage_pillars(int mode, const double* px, const double* py, const double* t,
                                 int npil, int nsets, void* out, hipStream_t s);
-hipError_t launch_forward(int mode, int order, const void* pil, int npil, int nsets,
-                          const DevPlane* planes, const FwdBlock* blocks, int nblocks,
-                          void* partial, hipStream_t s);
-hipError_t launch_fwd_reduce(int mode, const DevPlane* planes, const PixBlock* pblocks,
-                             int nblocks, const void* partial, double2* field,
-                             double* intensity, hipStream_t s);
+hipError_t launch_forward_fp32c(int bx, int by, const Fp32cFwdArgs& a, hipStream_t s);
+hipError_t launch_fwd_reduce_fp32c(const DevPlane* planes, const FwdTile* tiles,
+                                   const FwdRed* reds, int nred, const float2* partial,
+                                   int64_t slot_elems, double2* field, double* intensity,
+                                   hipStream_t s);
+hipError_t launch_adjoint_fp32c(int bx, int by, const Fp32cAdjArgs& a, hipStream_t s);
+hipError_t launch_adj_reduce_fp32c(const float2* partial, const int32_t* slot_off,
+                                   const int32_t* slots, int npil, int ngroups, int64_t n,
+                                   double2* g, hipStream_t s);
+int fp32c_resident_per_cu(bool adjoint, int bx, int by);
+const char* fp32c_kernel_name(bool adjoint, int bx, int by);
+hipError_t launch_forward(int mode, const void* pil, int npil, const DevPlane* planes,
+                          const FwdBlock* blocks, int nblocks, void* partial, hipStream_t s);
+hipError_t launch_fwd_reduce(const DevPlane* planes, const PixBlock* pblocks, int nblocks,
+                             const void* partial, double2* field, double* intensity,
+                             hipStream_t s);
 hipError_t launch_stage_pixels(const DevPlane* planes, const PixBlock* pblocks, int nblocks,
                                const double2* u, float4* pix, hipStream_t s);
-hipError_t launch_adjoint(int mode, int order, const void* pil, int npil,
-                          const DevPlane* planes, const AdjQuad* quads, const AdjBlock* blocks,
-                          int nblocks, const void* pix_or_u, void* partial, hipStream_t s);
-hipError_t launch_adj_reduce(int mode, const void* partial, int nslots, int64_t n, double2* g,
+hipError_t launch_adjoint(int mode, const void* pil, int npil, const DevPlane* planes,
+                          const AdjQuad* quads, const AdjBlock* blocks, int nblocks,
+                          const void* u, void* partial, hipStream_t s);
+hipError_t launch_adj_reduce(const void* partial, int nslots, int64_t n, double2* g,
                              hipStream_t s);
 hipError_t launch_mtf(const double2* P, const double* P0, int n, int ns, double p, double f,
                       double lam, double2* acc, double* acc0, float* freq, float* mtf,
@@ -91,10 +101,6 @@ bool stream_capturing(hipStream_t s) {
     return st != hipStreamCaptureStatusNone;
 }
 
-}  // namespace
-
-namespace {
-
 // Host-built descriptor tables travel in one pinned buffer and one async copy.
 struct TableBuilder {
     std::vector<unsigned char> bytes;
@@ -106,27 +112,45 @@ struct TableBuilder {
     }
 };
 
+// One FP32C pair-kernel launch of a plan: block shape and its work-group table.
+struct Fp32cLaunch {
+    int bx = 1, by = 1;
+    int nwg = 0;
+    size_t o_wg = 0;  // int32 wg_seg[nwg + 1]
+};
+
 struct FwdPlan {
     int mode = 0;
-    int order = 0;
     int npil = 0, nsets = 0;
+    TableBuilder tb;
+    size_t o_planes = 0;
+    // FP32 / FP64: uniform split.
     int64_t part_elems = 0;
     int nblocks = 0, npblocks = 0;
-    TableBuilder tb;
-    size_t o_planes = 0, o_blocks = 0, o_pblocks = 0;
+    size_t o_blocks = 0, o_pblocks = 0;
+    // FP32C: stream-K segments.
+    std::vector<Fp32cLaunch> launches;
+    int64_t nseg = 0, slot_elems = 0;
+    int nred = 0;
+    size_t o_tiles = 0, o_segs = 0, o_reds = 0;
 };
 
 struct AdjPlan {
     int mode = 0;
-    int order = 1;
     int npil = 0, nsets = 0;
-    int64_t nsplit = 1, n_out = 0, staged_records = 0;  // nsplit: partial slots, all launches
-    int nblocks = 0, npblocks = 0;
-    // FP32C calls mixing planes flagged exact with blocked ones: a second launch, of the
-    // 1 x 1 kernel over the flagged planes' quads, into partial slots after the first's.
-    int nblocks_exact = 0;
+    int64_t n_out = 0, staged_records = 0;
     TableBuilder tb;
-    size_t o_planes = 0, o_quads = 0, o_blocks = 0, o_blocks_exact = 0, o_pblocks = 0;
+    size_t o_planes = 0, o_pblocks = 0;
+    int npblocks = 0;
+    // FP32 / FP64: uniform split over quad runs.
+    int64_t nsplit = 1;
+    int nblocks = 0;
+    size_t o_quads = 0, o_blocks = 0;
+    // FP32C: stream-K segments with per-(set, group) slot lists.
+    std::vector<Fp32cLaunch> launches;
+    int64_t nseg = 0;
+    int ngroups = 0;
+    size_t o_strips = 0, o_segs = 0, o_slot_off = 0, o_slots = 0;
 };
 
 // A context remembers the last plan of each direction; a call with the same shape
@@ -143,6 +167,7 @@ constexpr int kPinnedRing = 4;
 
 struct crs_context {
     int device = 0;
+    int cus = 0;
     DevBuffer pil, pix, partial;
     PlanSlot slot[2];  // [0] forward, [1] adjoint
     FwdPlan fwd_plan;
@@ -156,26 +181,44 @@ struct crs_context {
     int ring_next = 0;
     int fwd_target[3] = {0, 0, 0};
     int adj_target[3] = {0, 0, 0};
+    int fwd_shape[2] = {0, 0};  // FP32C block shape (0, 0: built-in default)
+    int adj_shape[2] = {0, 0};
     bool timing = false;
     hipEvent_t ev[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [adjoint][start/stop]
     bool ev_valid[2] = {false, false};
     const char* last_kernel[2] = {nullptr, nullptr};  // [adjoint]: last pair kernel launched
     int last_groups[2] = {0, 0};
+    // Completion of the last call's work and its stream: a call on another stream first
+    // waits for it (the scratch and tables are shared), and scratch is freed only after it.
+    hipEvent_t done_ev = nullptr;
+    hipStream_t last_stream = nullptr;
+    bool done_valid = false;
 };
 
 namespace {
 
-// Built-in launch targets: work-groups per forward/adjoint launch.  256 CUs x a few
-// resident work-groups, several waves deep so the tail of uneven splits is short.  On
-// the 91809-pillar, 161 x 161 case both FP32C pair kernels measure within 2% of their
-// best between 3072 and 6144.
-int default_target(int mode, bool adjoint) {
-    (void)mode;
-    (void)adjoint;
-    return 4096;
+// Built-in launch targets of the FP32 / FP64 kernels: work-groups per launch.  256 CUs
+// x a few resident work-groups, several waves deep so the tail of uneven splits is short.
+int default_target() { return 4096; }
+
+// Built-in FP32C block shapes: 4 x 4 pixels per forward lane (4 x 8 needs ~220 VGPRs,
+// 2 waves per SIMD, and measured no faster), 4 x 8 per adjoint block.
+constexpr int kDefFwdBX = 4, kDefFwdBY = 4;
+constexpr int kDefAdjBX = 4, kDefAdjBY = 8;
+// Built-in FP32C launch sizes, in multiples of the resident work-groups: the forward at
+// most kFwdRounds x (and at most one LDS tile of pillars per work-group), the adjoint
+// kAdjRounds x.  Several rounds let the CUs that finish early take more work; measured
+// at 91,809 pillars x 161^2 both kernels sit within a few % of their best there.
+constexpr int kFwdRounds = 16;
+constexpr int kAdjRounds = 8;
+
+// The previous call's work must be finished before its scratch is freed.
+void wait_previous(crs_context* ctx) {
+    if (ctx && ctx->done_valid) hipEventSynchronize(ctx->done_ev);
 }
 
-crs_status ensure(DevBuffer& b, size_t bytes, hipStream_t stream, const char* what) {
+crs_status ensure(crs_context* ctx, DevBuffer& b, size_t bytes, hipStream_t stream,
+                  const char* what) {
     if (bytes == 0) bytes = 256;
     if (b.cap >= bytes) return CRS_OK;
     if (stream_capturing(stream)) {
@@ -188,8 +231,10 @@ crs_status ensure(DevBuffer& b, size_t bytes, hipStream_t stream, const char* wh
         return CRS_ERR_ALLOC;
     }
     if (b.ptr) {
-        // The old buffer may still be read by queued work on this stream.
+        // The old buffer may still be read by queued work on this stream or, through the
+        // context, on the stream of its previous call.
         hipStreamSynchronize(stream);
+        wait_previous(ctx);
         hipFree(b.ptr);
         b.ptr = nullptr;
         b.cap = 0;
@@ -205,11 +250,10 @@ crs_status ensure(DevBuffer& b, size_t bytes, hipStream_t stream, const char* wh
     return CRS_OK;
 }
 
-
 crs_status upload_tables(crs_context* ctx, DevBuffer& dst, const TableBuilder& tb,
                          hipStream_t stream) {
     const size_t n = tb.bytes.size();
-    crs_status st = ensure(dst, n, stream, "descriptor tables");
+    crs_status st = ensure(ctx, dst, n, stream, "descriptor tables");
     if (st != CRS_OK) return st;
     const int r = ctx->ring_next;
     ctx->ring_next = (r + 1) % kPinnedRing;
@@ -248,14 +292,13 @@ crs_status upload_tables(crs_context* ctx, DevBuffer& dst, const TableBuilder& t
 
 // Key of a call's plan: everything the tables depend on (not the data pointers).
 std::vector<unsigned char> plan_key(int mode, int npil, int nsets, const crs_plane* planes,
-                                    int nplanes, int target) {
-    std::vector<unsigned char> key(5 * sizeof(int) + static_cast<size_t>(nplanes) * sizeof(crs_plane));
-    const int head[5] = {mode, npil, nsets, nplanes, target};
+                                    int nplanes, int target, const int shape[2]) {
+    std::vector<unsigned char> key(7 * sizeof(int) + static_cast<size_t>(nplanes) * sizeof(crs_plane));
+    const int head[7] = {mode, npil, nsets, nplanes, target, shape[0], shape[1]};
     std::memcpy(key.data(), head, sizeof head);
     if (nplanes) std::memcpy(key.data() + sizeof head, planes, static_cast<size_t>(nplanes) * sizeof(crs_plane));
     return key;
 }
-
 
 // Arguments every batched entry point shares, checked before any device work.
 crs_status check_args(int mode, int npil, int nsets, const crs_plane* planes, int nplanes) {
@@ -301,7 +344,25 @@ crs_status check_common(crs_context* ctx, int mode, int npil, int nsets, const c
     return CRS_OK;
 }
 
-DevPlane make_dev_plane(const crs_plane& P, int px_per_thread, int py_per_thread) {
+// Orders a call on `stream` after the context's previous call when that ran on another
+// stream: the scratch buffers and descriptor tables it reads are shared by the calls.
+void order_after_previous(crs_context* ctx, hipStream_t stream) {
+    if (ctx->done_valid && ctx->last_stream != stream && !stream_capturing(stream))
+        hipStreamWaitEvent(stream, ctx->done_ev, 0);
+}
+
+// Marks the end of a call's device work on `stream` (outside capture).
+void record_done(crs_context* ctx, hipStream_t stream) {
+    if (stream_capturing(stream)) return;
+    if (hipEventRecord(ctx->done_ev, stream) == hipSuccess) {
+        ctx->last_stream = stream;
+        ctx->done_valid = true;
+    } else {
+        hipGetLastError();
+    }
+}
+
+DevPlane make_dev_plane(const crs_plane& P) {
     DevPlane d{};
     d.k = P.k;
     d.kturn = P.k * kInv2Pi;
@@ -310,69 +371,50 @@ DevPlane make_dev_plane(const crs_plane& P, int px_per_thread, int py_per_thread
     d.z0 = P.z0; d.sz = P.sz;
     d.zref = d.kturn * P.z0;
     d.nx = P.nx; d.ny = P.ny;
-    d.tx = static_cast<int32_t>(ceil_div(P.nx, px_per_thread));
-    d.ty = static_cast<int32_t>(ceil_div(P.ny, py_per_thread));
+    d.tx = 0;
+    d.ty = 0;
     d.tset = P.tset;
     d.nslots = 1;
     d.offset = P.offset;
     d.part = 0;
     d.sbase = 0;
-    d.nxp = static_cast<int32_t>(ceil_div(P.nx, kQuad) * kQuad);
+    d.nxp = P.nx;
     d.nquads = static_cast<int32_t>(ceil_div(P.ny, kQuad));
+    d.bx = 1;
+    d.by = 1;
+    d.obl1 = 1.0;
+    d.obl2 = 1.0;
     return d;
 }
 
-// Series order 4 x 4 blocks need on one plane, with no knowledge of where the pillars
-// are (the kernels refine it per work-group from the pillars' extent).  A block spans at
-// most e turns around its centre; at distance >= z' the expansion parameter is
-// r = e / z' (y <= r/2).  Dropped terms, at worst (a pillar seen at grazing angle): phase
-// order 1 leaves 2 x1 y^2 <= e r^2 / 2 turns, order 2 leaves ~5 x1 y^3 <= 0.625 e r^3;
-// amplitude order 1 leaves 6 y^2 <= 1.5 r^2, order 2 leaves ~20 y^3 <= 2.5 r^3, against
-// budgets of 1.6e-7 turns (1e-6 rad) / 4e-6 of amplitude.  Returns kExactOrder when even
-// order 2 would exceed them (the plane's pairs are then evaluated one by one at their
-// own reference distances), else 2 * (phase order - 1) + (amplitude order - 1).  Planes
-// with non-finite geometry (n == 1 windows) give NaN either way and return 0.
-int fp32c_plane_order(const DevPlane& d) {
-    const double ex = 1.5 * std::fabs(d.sx) * d.kturn;
-    const double ey = 1.5 * std::fabs(d.sy) * d.kturn;
-    const double ez = 1.5 * std::fabs(d.sz) * d.kturn;
-    const double e = std::sqrt(ex * ex + ey * ey) + ez;
-    const double zmin = d.kturn * plane_min_abs_z(d.z0, d.sz, d.ny);
-    if (!std::isfinite(e)) return 0;
-    if (!(zmin > 0.0)) return kExactOrder;  // a z = 0 row: no distance bound
-    int po = 1, ao = 1;
-    const double r = e / zmin;
-    if (0.5 * e * r * r > 1.6e-7) {
-        if (0.625 * e * r * r * r > 1.6e-7) return kExactOrder;
-        po = 2;
-    }
-    if (1.5 * r * r > 4e-6) {
-        if (2.5 * r * r * r > 4e-6) return kExactOrder;
-        ao = 2;
-    }
-    return 2 * (po - 1) + (ao - 1);
+// Whether bx x by blocks of plane d meet the FP32C pair budget at the highest series
+// order for pillars at any obliquity (the host does not know where the pillars are;
+// the kernels then pick the cheapest order each work item's pillars admit).  ny_rows:
+// rows whose depth counts (the adjoint's padded strips reach beyond ny).
+bool fp32c_block_admitted(const DevPlane& d, int bx, int by, int ny_rows) {
+    DevPlane q = d;
+    q.ny = ny_rows;
+    const SeriesGeom g = series_geom(q, bx, by);
+    if (!std::isfinite(g.a) || !std::isfinite(g.c) || !(g.z > 0.0)) return false;
+    return fp32c_pair_error(g, 1.0, kP2A2) <= kPairBudget;
 }
 
-// Flags (exact[p]) each plane that needs pair-by-pair evaluation and returns the call's
-// order code: kExactOrder when every plane does (the 1 x 1 kernels), else the highest
-// order of the others (informational; any code below kExactOrder selects the 4 x 4
-// kernels, and the flagged planes of such a call are taken pair by pair).
-int fp32c_order(const std::vector<DevPlane>& dp, std::vector<char>& exact) {
-    int order = 0;
-    bool all_exact = !dp.empty();
-    exact.assign(dp.size(), 0);
-    for (size_t p = 0; p < dp.size(); ++p) {
-        const int o = fp32c_plane_order(dp[p]);
-        exact[p] = o == kExactOrder;
-        if (exact[p]) continue;
-        all_exact = false;
-        order = std::max(order, o);
-    }
-    return all_exact ? kExactOrder : order;
+// FP32C relative cost of one pillar against one lane of a thread tile of nel pixels
+// (pair ops plus the per-pillar FP64 distance and state), for the stream-K cuts.
+double fwd_tile_rate(int nel) { return nel == 1 ? 48.0 : 7.0 * nel + 40.0; }
+
+// Work-groups of an FP32C launch: the tuned target, else `rounds` x every CU's resident
+// count.
+int64_t fp32c_target(const crs_context* ctx, int tuned, bool adjoint, int bx, int by, int rounds) {
+    if (tuned > 0) return tuned;
+    int per_cu = fp32c_resident_per_cu(adjoint, bx, by);
+    if (per_cu <= 0) per_cu = 2;
+    return static_cast<int64_t>(per_cu) * std::max(1, ctx->cus) * rounds;
 }
 
 // Bytes of staged pillar data: FP32C keeps FP64 positions once plus {|t|, arg t} per
-// set; FP32 and FP64 keep one {x, y, Re t, Im t} record per pillar and set.
+// set and per-tile extents; FP32 and FP64 keep one {x, y, Re t, Im t} record per
+// pillar and set.
 size_t staged_pillar_bytes(int mode, int64_t npil, int64_t nsets) {
     if (mode == CRS_MODE_FP32C) return fp32c_staged_layout(npil, nsets).bytes;
     return npil * nsets * (mode == CRS_MODE_FP64 ? sizeof(double4) : sizeof(float4));
@@ -382,6 +424,11 @@ crs_status launch_status(hipError_t e, const char* what) {
     if (e == hipSuccess) return CRS_OK;
     set_error("%s: %s", what, hipGetErrorString(e));
     return CRS_ERR_LAUNCH;
+}
+
+const char* plain_kernel(int mode, bool adjoint) {
+    if (mode == CRS_MODE_FP64) return adjoint ? "crs::adj_fp64" : "crs::fwd_fp64";
+    return adjoint ? "crs::adj_fp32" : "crs::fwd_fp32";
 }
 
 }  // namespace
@@ -409,6 +456,11 @@ extern "C" crs_status crs_context_create(int device, crs_context** out) {
     }
     crs_context* ctx = new crs_context();
     ctx->device = device;
+    if (hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, device) !=
+        hipSuccess) {
+        hipGetLastError();
+        ctx->cus = 256;
+    }
     for (hipEvent_t& e : ctx->tables_ev) {
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
             hipGetLastError();
@@ -416,6 +468,12 @@ extern "C" crs_status crs_context_create(int device, crs_context** out) {
             set_error("hipEventCreate failed");
             return CRS_ERR_LAUNCH;
         }
+    }
+    if (hipEventCreateWithFlags(&ctx->done_ev, hipEventDisableTiming) != hipSuccess) {
+        hipGetLastError();
+        crs_context_destroy(ctx);
+        set_error("hipEventCreate failed");
+        return CRS_ERR_LAUNCH;
     }
     *out = ctx;
     return CRS_OK;
@@ -435,6 +493,7 @@ extern "C" void crs_context_destroy(crs_context* ctx) {
         if (ctx->host_tables[r]) hipHostFree(ctx->host_tables[r]);
         if (ctx->tables_ev[r]) hipEventDestroy(ctx->tables_ev[r]);
     }
+    if (ctx->done_ev) hipEventDestroy(ctx->done_ev);
     delete ctx;
 }
 
@@ -448,6 +507,30 @@ extern "C" crs_status crs_set_tuning(crs_context* ctx, int mode, int fwd_target_
     }
     ctx->fwd_target[mode] = fwd_target_blocks;
     ctx->adj_target[mode] = adj_target_blocks;
+    return CRS_OK;
+}
+
+namespace {
+bool fwd_shape_ok(int bx, int by) {
+    return (bx == 0 && by == 0) || (bx == 4 && by == 4) || (bx == 4 && by == 8) ||
+           (bx == 8 && by == 4);
+}
+bool adj_shape_ok(int bx, int by) {
+    return (bx == 0 && by == 0) || ((bx == 4 || bx == 8) && (by == 4 || by == 8));
+}
+}  // namespace
+
+extern "C" crs_status crs_set_block_shape(crs_context* ctx, int fwd_bx, int fwd_by, int adj_bx,
+                                          int adj_by) {
+    if (!ctx || !fwd_shape_ok(fwd_bx, fwd_by) || !adj_shape_ok(adj_bx, adj_by)) {
+        set_error("bad block shape (forward %d x %d, adjoint %d x %d)", fwd_bx, fwd_by, adj_bx,
+                  adj_by);
+        return CRS_ERR_INVALID;
+    }
+    ctx->fwd_shape[0] = fwd_bx;
+    ctx->fwd_shape[1] = fwd_by;
+    ctx->adj_shape[0] = adj_bx;
+    ctx->adj_shape[1] = adj_by;
     return CRS_OK;
 }
 
@@ -480,18 +563,6 @@ extern "C" crs_status crs_last_kernel_ms(crs_context* ctx, int adjoint, float* m
     return CRS_OK;
 }
 
-namespace {
-// Profiler name of the pair kernel a plan launches (launch_forward / launch_adjoint).
-const char* pair_kernel(int mode, int order, bool adjoint) {
-    if (mode == CRS_MODE_FP32C) {
-        if (order == kExactOrder) return adjoint ? "crs::adj_fp32c<1, 1>" : "crs::fwd_fp32c<1>";
-        return adjoint ? "crs::adj_fp32c<4, 4>" : "crs::fwd_fp32c<4>";
-    }
-    if (mode == CRS_MODE_FP64) return adjoint ? "crs::adj_fp64" : "crs::fwd_fp64";
-    return adjoint ? "crs::adj_fp32" : "crs::fwd_fp32";
-}
-}  // namespace
-
 extern "C" crs_status crs_last_launch(crs_context* ctx, int adjoint, const char** kernel,
                                       int* workgroups) {
     if (!ctx || !kernel || !workgroups || adjoint < 0 || adjoint > 1) {
@@ -523,65 +594,171 @@ struct Scratch {  // where one call's device data lives (a context's or a graph'
     DevBuffer* pix;
     DevBuffer* partial;
     unsigned char* tables;  // device copy of the plan's table bytes
+    crs_context* owner;     // the context whose previous call may still use the buffers
 };
 
-
-crs_status plan_forward(const crs_context* ctx, int mode, int npil, int nsets,
-                        const crs_plane* planes, int nplanes, FwdPlan& plan) {
-    // Output ranges [offset, offset + nx*ny) must not overlap: each element has one writer.
-    {
-        std::vector<std::pair<int64_t, int64_t>> r(nplanes);
-        for (int p = 0; p < nplanes; ++p)
-            r[p] = {planes[p].offset, planes[p].offset + static_cast<int64_t>(planes[p].nx) * planes[p].ny};
-        std::sort(r.begin(), r.end());
-        for (int p = 1; p < nplanes; ++p)
-            if (r[p].first < r[p - 1].second) {
-                set_error("planes' output ranges overlap at element %lld", (long long)r[p].first);
-                return CRS_ERR_INVALID;
+// Stream-K cut of a work line into work-groups.  The line is a sequence of runs (run r
+// has len[r] units of cost rate[r] each); `nwg` equal-cost pieces are cut at unit
+// positions that are multiples of `gran` within a run (no empty segment; a piece that
+// would be empty merges into its neighbour, so fewer pieces may result).  emit(run, u0,
+// u1) is called for each segment in order; wg_start receives the index of each piece's
+// first segment followed by the total segment count.
+template <class Emit>
+void streamk_cut(const std::vector<int64_t>& len, const std::vector<double>& rate, int64_t nwg,
+                 int64_t gran, Emit emit, std::vector<int32_t>& wg_start) {
+    double total = 0.0;
+    for (size_t r = 0; r < len.size(); ++r) total += rate[r] * static_cast<double>(len[r]);
+    nwg = std::max<int64_t>(1, nwg);
+    const double per = total / static_cast<double>(nwg);
+    int64_t nseg = 0;
+    wg_start.clear();
+    wg_start.push_back(0);
+    int64_t next = 1;  // index of the next piece boundary, at cost per * next
+    double cum = 0.0;
+    for (size_t r = 0; r < len.size(); ++r) {
+        const int64_t n = len[r];
+        if (n <= 0) continue;
+        const double rt = rate[r];
+        const double end_cost = cum + rt * static_cast<double>(n);
+        int64_t u = 0;
+        while (u < n) {
+            int64_t uend = n;
+            bool cut = false;
+            if (next < nwg) {
+                const double b = per * static_cast<double>(next);
+                if (b < end_cost) {
+                    int64_t uc = static_cast<int64_t>(std::llround((b - cum) / rt / gran)) * gran;
+                    if (uc <= u) uc = u + gran;
+                    if (uc < n) {
+                        uend = uc;
+                        cut = true;
+                    }
+                }
             }
-    }
-    plan.mode = mode;
-    plan.npil = npil;
-    plan.nsets = nsets;
-    std::vector<DevPlane> dp(nplanes);
-    for (int p = 0; p < nplanes; ++p) dp[p] = make_dev_plane(planes[p], 1, 1);
-    int blk = 1;
-    std::vector<char> exact(nplanes, 0);
-    if (mode == CRS_MODE_FP32C) {
-        plan.order = fp32c_order(dp, exact);
-        blk = plan.order == kExactOrder ? 1 : kBlk;
-    }
-    // Split over pillar chunks: enough work-groups to fill the chip, chunks of whole
-    // LDS tiles (FP32C: no longer than kMaxChunk).  FP32C work-groups take kFwdTiles
-    // thread tiles of a plane, FP32 / FP64 ones kThreads pixels.
-    // FP32C tiles come in four classes (kernel: tile_count / tile_origin): 4 x 4 tiles
-    // over the largest multiple-of-4 part of the plane, then 1 x 4, 4 x 1 and 1 x 1
-    // tiles for the leftover columns, rows and corner, so no lane computes a pixel
-    // outside the plane.  With 1 x 1 blocks (tx = ty = 0: the 1 x 1 kernels, or a plane
-    // flagged exact in a call of the 4 x 4 kernels) the last class is the plane.
-    const int per_wg = mode == CRS_MODE_FP32C ? kFwdTiles : kThreads;
-    const int nclass = mode == CRS_MODE_FP32C ? 4 : 1;
-    std::vector<std::array<int64_t, 4>> counts(nplanes);
-    int64_t base_blocks = 0;
-    for (int p = 0; p < nplanes; ++p) {
-        std::array<int64_t, 4>& c = counts[p];
-        if (mode == CRS_MODE_FP32C) {
-            const bool blocked = blk > 1 && !exact[p];
-            dp[p].tx = blocked ? dp[p].nx / blk : 0;
-            dp[p].ty = blocked ? dp[p].ny / blk : 0;
-            const int64_t tx = dp[p].tx, ty = dp[p].ty;
-            const int64_t rx = dp[p].nx - 4 * tx, ry = dp[p].ny - 4 * ty;
-            c = {tx * ty, rx * ty, tx * ry, rx * ry};
-        } else {
-            c = {static_cast<int64_t>(dp[p].nx) * dp[p].ny, 0, 0, 0};
+            emit(static_cast<int>(r), u, uend);
+            ++nseg;
+            u = uend;
+            if (cut) {
+                wg_start.push_back(static_cast<int32_t>(nseg));
+                ++next;
+            }
         }
-        for (int k = 0; k < nclass; ++k) base_blocks += ceil_div(c[k], per_wg);
+        cum = end_cost;
+        // Boundaries at or before this run's end that fell on its last unit start the
+        // next piece at the next run.
+        while (next < nwg && per * static_cast<double>(next) <= cum * (1.0 + 1e-12)) {
+            if (wg_start.back() != nseg) wg_start.push_back(static_cast<int32_t>(nseg));
+            ++next;
+        }
     }
-    const int target = ctx->fwd_target[mode] > 0 ? ctx->fwd_target[mode] : default_target(mode, false);
+    if (wg_start.back() == nseg && wg_start.size() > 1) wg_start.pop_back();
+    wg_start.push_back(static_cast<int32_t>(nseg));
+}
+
+// FP32C forward: block shape per plane (the tuned shape where it meets the pair budget,
+// else 4 x 4, else exact), tiles per launch, stream-K segments per launch.
+crs_status plan_forward_fp32c(const crs_context* ctx, int npil, std::vector<DevPlane>& dp,
+                              FwdPlan& plan) {
+    const int nplanes = static_cast<int>(dp.size());
+    int want_bx = ctx->fwd_shape[0] ? ctx->fwd_shape[0] : kDefFwdBX;
+    int want_by = ctx->fwd_shape[1] ? ctx->fwd_shape[1] : kDefFwdBY;
+    const int shapes[3][2] = {{want_bx, want_by}, {4, 4}, {1, 1}};
+    std::vector<int> pick(nplanes, 2);
+    for (int p = 0; p < nplanes; ++p) {
+        if (fp32c_block_admitted(dp[p], want_bx, want_by, dp[p].ny)) pick[p] = 0;
+        else if (fp32c_block_admitted(dp[p], 4, 4, dp[p].ny)) pick[p] = 1;
+    }
+    if (want_bx == 4 && want_by == 4)
+        for (int& k : pick) k = k == 1 ? 0 : k;
+    std::vector<FwdTile> tiles;
+    std::vector<FwdSeg> segs;
+    std::vector<int32_t> wg_all;
+    int64_t slot_elems = 64;
+    for (int L = 0; L < 3; ++L) {
+        const int kbx = shapes[L][0], kby = shapes[L][1];
+        std::vector<int64_t> len;
+        std::vector<double> rate;
+        const size_t tile0 = tiles.size();
+        for (int p = 0; p < nplanes; ++p) {
+            if (pick[p] != L) continue;
+            DevPlane& d = dp[p];
+            d.bx = kbx;
+            d.by = kby;
+            d.tx = kbx > 1 ? d.nx / kbx : 0;
+            d.ty = kby > 1 ? d.ny / kby : 0;
+            d.obl1 = kbx > 1 ? fp32c_order_obliquity(d, kbx, kby, kP1A1) : 1.0;
+            d.obl2 = kbx > 1 ? std::max(d.obl1, fp32c_order_obliquity(d, kbx, kby, kP1A2)) : 1.0;
+            for (int k = 0; k < 4; ++k) {
+                const int64_t cnt = tile_count(d, k, kbx, kby);
+                const int nel = k == 0 ? kbx * kby : k == 1 ? kby : k == 2 ? kbx : 1;
+                for (int64_t b = 0; b < cnt; b += kFwdTiles) {
+                    tiles.push_back({p, k, static_cast<int32_t>(b), nel, 0, 0});
+                    len.push_back(npil);
+                    rate.push_back(fwd_tile_rate(nel));
+                }
+            }
+        }
+        if (tiles.size() == tile0) continue;
+        slot_elems = std::max<int64_t>(slot_elems, 64 * kbx * kby);
+        // Enough pieces to fill the chip, but each with a few LDS tiles of work.
+        int64_t units = 0;
+        for (int64_t l : len) units += ceil_div(l, kThreads);
+        const int64_t nwg = std::min<int64_t>(
+            fp32c_target(ctx, ctx->fwd_target[CRS_MODE_FP32C], false, kbx, kby, kFwdRounds),
+            std::max<int64_t>(1, units));
+        std::vector<int32_t> wg_start;
+        const int64_t seg_base = static_cast<int64_t>(segs.size());
+        int last_tile = -1;
+        streamk_cut(len, rate, nwg, 16,
+                    [&](int r, int64_t u0, int64_t u1) {
+                        const int t = static_cast<int>(tile0) + r;
+                        if (t != last_tile) {
+                            tiles[t].seg0 = static_cast<int32_t>(segs.size());
+                            last_tile = t;
+                        }
+                        ++tiles[t].nseg;
+                        segs.push_back({t, static_cast<int32_t>(u0), static_cast<int32_t>(u1), 0});
+                    },
+                    wg_start);
+        Fp32cLaunch La;
+        La.bx = kbx;
+        La.by = kby;
+        La.nwg = static_cast<int>(wg_start.size()) - 1;
+        for (int32_t& w : wg_start) w += static_cast<int32_t>(seg_base);
+        La.o_wg = wg_all.size();
+        wg_all.insert(wg_all.end(), wg_start.begin(), wg_start.end());
+        plan.launches.push_back(La);
+    }
+    if (segs.size() > static_cast<size_t>(INT32_MAX) ||
+        static_cast<double>(segs.size()) * slot_elems * 8.0 > 64e9) {
+        set_error("launch too large (%zu segments)", segs.size());
+        return CRS_ERR_INVALID;
+    }
+    std::vector<FwdRed> reds;
+    for (size_t t = 0; t < tiles.size(); ++t)
+        for (int el = 0; el < tiles[t].nel * 64; el += kThreads)
+            reds.push_back({static_cast<int32_t>(t), el});
+    plan.nseg = static_cast<int64_t>(segs.size());
+    plan.slot_elems = slot_elems;
+    plan.nred = static_cast<int>(reds.size());
+    plan.o_planes = plan.tb.add(dp.data(), dp.size() * sizeof(DevPlane));
+    plan.o_tiles = plan.tb.add(tiles.data(), tiles.size() * sizeof(FwdTile));
+    plan.o_segs = plan.tb.add(segs.data(), segs.size() * sizeof(FwdSeg));
+    plan.o_reds = plan.tb.add(reds.data(), reds.size() * sizeof(FwdRed));
+    const size_t o_wg = plan.tb.add(wg_all.data(), wg_all.size() * sizeof(int32_t));
+    for (Fp32cLaunch& La : plan.launches) La.o_wg = o_wg + La.o_wg * sizeof(int32_t);
+    return CRS_OK;
+}
+
+// FP32 / FP64 forward: pillar chunks split uniformly; work-groups of kThreads pixels.
+crs_status plan_forward_plain(const crs_context* ctx, int mode, int npil,
+                              std::vector<DevPlane>& dp, FwdPlan& plan) {
+    const int nplanes = static_cast<int>(dp.size());
+    int64_t base_blocks = 0;
+    for (int p = 0; p < nplanes; ++p)
+        base_blocks += ceil_div(static_cast<int64_t>(dp[p].nx) * dp[p].ny, kThreads);
+    const int target = ctx->fwd_target[mode] > 0 ? ctx->fwd_target[mode] : default_target();
     int64_t nsplit = std::max<int64_t>(1, ceil_div(target, base_blocks));
-    if (mode == CRS_MODE_FP32C) nsplit = std::max<int64_t>(nsplit, ceil_div(npil, kMaxChunk));
-    // Chunks are whole LDS tiles of kThreads pillars, or, when a problem is too small to
-    // fill the chip that way, multiples of kMinChunk (each wave then takes kMinChunk / 4).
     nsplit = std::min<int64_t>(nsplit, std::max<int64_t>(1, npil / kMinChunk));
     int64_t chunk = ceil_div(npil, nsplit);
     chunk = chunk >= kThreads ? ceil_div(chunk, kThreads) * kThreads : ceil_div(chunk, kMinChunk) * kMinChunk;
@@ -603,27 +780,9 @@ crs_status plan_forward(const crs_context* ctx, int mode, int npil, int nsets,
         const int32_t q1 = static_cast<int32_t>(std::min<int64_t>(npil, (s + 1) * chunk));
         for (int p = 0; p < nplanes; ++p) {
             const int64_t npix = static_cast<int64_t>(dp[p].nx) * dp[p].ny;
-            for (int k = 0; k < nclass; ++k)
-                for (int64_t b = 0; b < counts[p][k]; b += per_wg)
-                    blocks.push_back({p, static_cast<int32_t>(b), q0, q1, dp[p].part + s * npix,
-                                      k, 0});
+            for (int64_t b = 0; b < npix; b += kThreads)
+                blocks.push_back({p, static_cast<int32_t>(b), q0, q1, dp[p].part + s * npix, 0, 0});
         }
-    }
-    // Heaviest work-groups first.  The dispatcher hands work-groups to the 8 XCDs round
-    // robin in blockIdx order, so a run of equal heavy groups spreads evenly over the
-    // XCDs whatever the groups per split, and the light edge and short-chunk groups
-    // fill the tail instead of landing on the same XCDs split after split.  Each block
-    // names its own output slot, so the order does not change any result.
-    {
-        const int64_t pix_per_tile[4] = {int64_t(blk) * blk, blk, blk, 1};
-        auto cost = [&](const FwdBlock& B) {
-            const int64_t tiles = std::min<int64_t>(per_wg, counts[B.plane][B.shape] - B.tile0);
-            const int64_t pix = mode == CRS_MODE_FP32C ? pix_per_tile[B.shape] : 1;
-            return static_cast<int64_t>(B.q1 - B.q0) * tiles * (pix + 2);
-        };
-        std::stable_sort(blocks.begin(), blocks.end(), [&](const FwdBlock& a, const FwdBlock& b) {
-            return cost(a) > cost(b);
-        });
     }
     std::vector<PixBlock> pblocks;
     for (int p = 0; p < nplanes; ++p) {
@@ -639,17 +798,41 @@ crs_status plan_forward(const crs_context* ctx, int mode, int npil, int nsets,
     return CRS_OK;
 }
 
-// Bytes of one split-sum partial: FP32C keeps each chunk's (or quad run's) sum as a
-// float pair, rounded once from its FP64 wave combine; FP64 and FP32 keep doubles.
+crs_status plan_forward(const crs_context* ctx, int mode, int npil, int nsets,
+                        const crs_plane* planes, int nplanes, FwdPlan& plan) {
+    // Output ranges [offset, offset + nx*ny) must not overlap: each element has one writer.
+    {
+        std::vector<std::pair<int64_t, int64_t>> r(nplanes);
+        for (int p = 0; p < nplanes; ++p)
+            r[p] = {planes[p].offset, planes[p].offset + static_cast<int64_t>(planes[p].nx) * planes[p].ny};
+        std::sort(r.begin(), r.end());
+        for (int p = 1; p < nplanes; ++p)
+            if (r[p].first < r[p - 1].second) {
+                set_error("planes' output ranges overlap at element %lld", (long long)r[p].first);
+                return CRS_ERR_INVALID;
+            }
+    }
+    plan.mode = mode;
+    plan.npil = npil;
+    plan.nsets = nsets;
+    std::vector<DevPlane> dp(nplanes);
+    for (int p = 0; p < nplanes; ++p) dp[p] = make_dev_plane(planes[p]);
+    if (mode == CRS_MODE_FP32C) return plan_forward_fp32c(ctx, npil, dp, plan);
+    return plan_forward_plain(ctx, mode, npil, dp, plan);
+}
+
+// Bytes of one split-sum partial: FP32C keeps each work item's sum as a float pair,
+// rounded once from its FP64 combine; FP64 and FP32 keep doubles.
 size_t partial_bytes(int mode) {
     return mode == CRS_MODE_FP32C ? sizeof(float2) : sizeof(double2);
 }
 
 crs_status size_forward(const FwdPlan& plan, const Scratch& sc, hipStream_t stream) {
-    crs_status st = ensure(*sc.pil, staged_pillar_bytes(plan.mode, plan.npil, plan.nsets), stream,
-                           "staged pillars");
+    crs_status st = ensure(sc.owner, *sc.pil, staged_pillar_bytes(plan.mode, plan.npil, plan.nsets),
+                           stream, "staged pillars");
     if (st != CRS_OK) return st;
-    return ensure(*sc.partial, static_cast<size_t>(plan.part_elems) * partial_bytes(plan.mode),
+    const int64_t elems = plan.mode == CRS_MODE_FP32C ? plan.nseg * plan.slot_elems : plan.part_elems;
+    return ensure(sc.owner, *sc.partial, static_cast<size_t>(elems) * partial_bytes(plan.mode),
                   stream, "forward partials");
 }
 
@@ -664,154 +847,274 @@ crs_status enqueue_forward(const FwdPlan& plan, const Scratch& sc, const double*
                             "stage pillars")) != CRS_OK)
         return st;
     if (ev0) hipEventRecord(ev0, stream);
-    if ((st = launch_status(launch_forward(mode, plan.order, sc.pil->ptr, plan.npil, plan.nsets, d_planes,
+    if (mode == CRS_MODE_FP32C) {
+        const StagedLayout L = fp32c_staged_layout(plan.npil, plan.nsets);
+        unsigned char* base = static_cast<unsigned char*>(sc.pil->ptr);
+        Fp32cFwdArgs a;
+        a.pos = reinterpret_cast<const double2*>(base + L.pos);
+        a.ap = reinterpret_cast<const float2*>(base + L.ap);
+        a.ext = reinterpret_cast<const double2*>(base + L.ext);
+        a.npil = plan.npil;
+        a.planes = d_planes;
+        a.tiles = reinterpret_cast<const FwdTile*>(sc.tables + plan.o_tiles);
+        a.segs = reinterpret_cast<const FwdSeg*>(sc.tables + plan.o_segs);
+        a.partial = static_cast<float2*>(sc.partial->ptr);
+        a.slot_elems = plan.slot_elems;
+        for (const Fp32cLaunch& La : plan.launches) {
+            a.wg_seg = reinterpret_cast<const int32_t*>(sc.tables + La.o_wg);
+            a.nwg = La.nwg;
+            if ((st = launch_status(launch_forward_fp32c(La.bx, La.by, a, stream), "forward")) != CRS_OK)
+                return st;
+        }
+        if (ev1) hipEventRecord(ev1, stream);
+        return launch_status(
+            launch_fwd_reduce_fp32c(d_planes, a.tiles,
+                                    reinterpret_cast<const FwdRed*>(sc.tables + plan.o_reds),
+                                    plan.nred, a.partial, plan.slot_elems,
+                                    reinterpret_cast<double2*>(field), intensity, stream),
+            "forward reduce");
+    }
+    if ((st = launch_status(launch_forward(mode, sc.pil->ptr, plan.npil, d_planes,
                                            reinterpret_cast<const FwdBlock*>(sc.tables + plan.o_blocks),
                                            plan.nblocks, sc.partial->ptr, stream),
                             "forward")) != CRS_OK)
         return st;
     if (ev1) hipEventRecord(ev1, stream);
-    return launch_status(launch_fwd_reduce(mode, d_planes,
+    return launch_status(launch_fwd_reduce(d_planes,
                                            reinterpret_cast<const PixBlock*>(sc.tables + plan.o_pblocks),
                                            plan.npblocks, sc.partial->ptr,
                                            reinterpret_cast<double2*>(field), intensity, stream),
                          "forward reduce");
 }
 
+// FP32C adjoint: strips of kStrip rows; block shape per plane as for the forward (with
+// the strips' padded rows counted for depth); per launch a strip list per set and
+// stream-K segments over the (set, pillar group, strip columns) work line.
+crs_status plan_adjoint_fp32c(const crs_context* ctx, int npil, int nsets,
+                              std::vector<DevPlane>& dp, AdjPlan& plan) {
+    const int nplanes = static_cast<int>(dp.size());
+    int64_t srec = 0;
+    for (int p = 0; p < nplanes; ++p) {
+        dp[p].nxp = static_cast<int32_t>(ceil_div(dp[p].nx, kColAlign) * kColAlign);
+        dp[p].nquads = static_cast<int32_t>(ceil_div(dp[p].ny, kStrip));
+        dp[p].sbase = srec;
+        srec += static_cast<int64_t>(dp[p].nquads) * dp[p].nxp * kStrip;
+    }
+    if (srec > (int64_t(1) << 31)) { set_error("too many pixels"); return CRS_ERR_INVALID; }
+    const int want_bx = ctx->adj_shape[0] ? ctx->adj_shape[0] : kDefAdjBX;
+    const int want_by = ctx->adj_shape[1] ? ctx->adj_shape[1] : kDefAdjBY;
+    const int shapes[3][2] = {{want_bx, want_by}, {4, 4}, {1, 1}};
+    std::vector<int> pick(nplanes, 2);
+    for (int p = 0; p < nplanes; ++p) {
+        const int rows = dp[p].nquads * kStrip;
+        if (fp32c_block_admitted(dp[p], want_bx, want_by, rows)) pick[p] = 0;
+        else if (fp32c_block_admitted(dp[p], 4, 4, rows)) pick[p] = 1;
+    }
+    if (want_bx == 4 && want_by == 4)
+        for (int& k : pick) k = k == 1 ? 0 : k;
+    for (int p = 0; p < nplanes; ++p) {
+        DevPlane& d = dp[p];
+        d.bx = shapes[pick[p]][0];
+        d.by = shapes[pick[p]][1];
+        DevPlane q = d;
+        q.ny = d.nquads * kStrip;  // the strips' padded rows count for the nearest depth
+        d.obl1 = d.bx > 1 ? fp32c_order_obliquity(q, d.bx, d.by, kP1A1) : 1.0;
+        d.obl2 = d.obl1;
+    }
+    const int ngroups = static_cast<int>(ceil_div(npil, kThreads));
+    std::vector<AdjStrip> strips;
+    std::vector<AdjSeg> segs;
+    std::vector<int32_t> wg_all;
+    // Slots of each (set, group), in launch order then segment order.
+    std::vector<std::vector<int32_t>> gslots(static_cast<size_t>(nsets) * ngroups);
+    for (int L = 0; L < 3; ++L) {
+        const int bx = shapes[L][0], by = shapes[L][1];
+        // Per set: its strips (planes in order, strips in order) and their block columns.
+        std::vector<int64_t> set_u0(nsets), set_nb(nsets, 0);
+        std::vector<std::vector<int64_t>> prefix(nsets);  // block-column prefix per set's strips
+        bool any = false;
+        for (int s = 0; s < nsets; ++s) {
+            set_u0[s] = static_cast<int64_t>(strips.size());
+            prefix[s].push_back(0);
+            for (int p = 0; p < nplanes; ++p) {
+                if (pick[p] != L || dp[p].tset != s) continue;
+                any = true;
+                const int64_t nb = ceil_div(dp[p].nx, bx);
+                for (int j = 0; j < dp[p].ny; j += kStrip) {
+                    strips.push_back({p, j});
+                    prefix[s].push_back(prefix[s].back() + nb);
+                }
+            }
+            set_nb[s] = prefix[s].back();
+        }
+        if (!any) continue;
+        // Runs: (set, group) in order, each the set's block columns; cost per block column:
+        // its pairs and per-block state (1 x 1: the rows a strip really has).
+        std::vector<int64_t> len;
+        std::vector<double> rate;
+        std::vector<std::pair<int, int>> run_sg;
+        const double r_blk = bx * by == 1 ? 8.0 * 48.0 : 8.0 * bx * 7.0 + (kStrip / by) * 60.0;
+        for (int s = 0; s < nsets; ++s)
+            for (int g = 0; g < ngroups; ++g) {
+                if (set_nb[s] == 0) continue;
+                len.push_back(set_nb[s]);
+                rate.push_back(r_blk);
+                run_sg.push_back({s, g});
+            }
+        int64_t units = 0;
+        for (int64_t l : len) units += l;
+        const int64_t nwg = std::min<int64_t>(
+            fp32c_target(ctx, ctx->adj_target[CRS_MODE_FP32C], true, bx, by, kAdjRounds),
+            std::max<int64_t>(1, units));
+        std::vector<int32_t> wg_start;
+        const int64_t seg_base = static_cast<int64_t>(segs.size());
+        streamk_cut(len, rate, nwg, 1,
+                    [&](int r, int64_t b0, int64_t b1) {
+                        const int s = run_sg[r].first, g = run_sg[r].second;
+                        const std::vector<int64_t>& pre = prefix[s];
+                        // strip holding block column b: last k with pre[k] <= b
+                        auto strip_of = [&](int64_t b) {
+                            return static_cast<int64_t>(std::upper_bound(pre.begin(), pre.end(), b) - pre.begin()) - 1;
+                        };
+                        const int64_t k0 = strip_of(b0), k1 = strip_of(b1 - 1);
+                        AdjSeg S;
+                        S.tset = s;
+                        S.q0 = g * kThreads;
+                        S.u0 = static_cast<int32_t>(set_u0[s] + k0);
+                        S.c0 = static_cast<int32_t>((b0 - pre[k0]) * bx);
+                        S.u1 = static_cast<int32_t>(set_u0[s] + k1);
+                        S.c1 = static_cast<int32_t>((b1 - pre[k1]) * bx);
+                        S.slot = static_cast<int32_t>(segs.size());
+                        S.pad = 0;
+                        gslots[static_cast<size_t>(s) * ngroups + g].push_back(S.slot);
+                        segs.push_back(S);
+                    },
+                    wg_start);
+        Fp32cLaunch La;
+        La.bx = bx;
+        La.by = by;
+        La.nwg = static_cast<int>(wg_start.size()) - 1;
+        for (int32_t& w : wg_start) w += static_cast<int32_t>(seg_base);
+        La.o_wg = wg_all.size();
+        wg_all.insert(wg_all.end(), wg_start.begin(), wg_start.end());
+        plan.launches.push_back(La);
+    }
+    if (segs.size() > static_cast<size_t>(INT32_MAX) || strips.size() > static_cast<size_t>(INT32_MAX)) {
+        set_error("launch too large");
+        return CRS_ERR_INVALID;
+    }
+    std::vector<int32_t> slot_off(1, 0), slots;
+    for (const std::vector<int32_t>& v : gslots) {
+        slots.insert(slots.end(), v.begin(), v.end());
+        slot_off.push_back(static_cast<int32_t>(slots.size()));
+    }
+    std::vector<PixBlock> pblocks;
+    for (int p = 0; p < nplanes; ++p) {
+        const int64_t nrec = static_cast<int64_t>(dp[p].nquads) * dp[p].nxp * kStrip;
+        for (int64_t b = 0; b < nrec; b += kThreads) pblocks.push_back({p, static_cast<int32_t>(b)});
+    }
+    plan.staged_records = srec;
+    plan.nseg = static_cast<int64_t>(segs.size());
+    plan.ngroups = ngroups;
+    plan.npblocks = static_cast<int>(pblocks.size());
+    plan.o_planes = plan.tb.add(dp.data(), dp.size() * sizeof(DevPlane));
+    plan.o_strips = plan.tb.add(strips.data(), strips.size() * sizeof(AdjStrip));
+    plan.o_segs = plan.tb.add(segs.data(), segs.size() * sizeof(AdjSeg));
+    plan.o_slot_off = plan.tb.add(slot_off.data(), slot_off.size() * sizeof(int32_t));
+    plan.o_slots = plan.tb.add(slots.data(), std::max<size_t>(1, slots.size()) * sizeof(int32_t));
+    plan.o_pblocks = plan.tb.add(pblocks.data(), pblocks.size() * sizeof(PixBlock));
+    const size_t o_wg = plan.tb.add(wg_all.data(), wg_all.size() * sizeof(int32_t));
+    for (Fp32cLaunch& La : plan.launches) La.o_wg = o_wg + La.o_wg * sizeof(int32_t);
+    return CRS_OK;
+}
+
+// FP32 / FP64 adjoint: quads of 4 rows, runs split uniformly.
+crs_status plan_adjoint_plain(const crs_context* ctx, int mode, int npil, int nsets,
+                              std::vector<DevPlane>& dp, AdjPlan& plan) {
+    const int nplanes = static_cast<int>(dp.size());
+    const int target = ctx->adj_target[mode] > 0 ? ctx->adj_target[mode] : default_target();
+    const int64_t groups = ceil_div(npil, kThreads);
+    const int64_t base_blocks = groups * nsets;
+    const int64_t n_out = static_cast<int64_t>(nsets) * npil;
+    std::vector<AdjQuad> quads;
+    std::vector<int64_t> set_q0(nsets + 1, 0);
+    std::vector<std::vector<int64_t>> work_prefix(nsets);
+    for (int s = 0; s < nsets; ++s) {
+        set_q0[s] = static_cast<int64_t>(quads.size());
+        std::vector<int64_t>& pre = work_prefix[s];
+        pre.push_back(0);
+        for (int p = 0; p < nplanes; ++p) {
+            if (dp[p].tset != s) continue;
+            for (int j = 0; j < dp[p].ny; j += kQuad) {
+                quads.push_back({p, j});
+                pre.push_back(pre.back() + static_cast<int64_t>(std::min(kQuad, dp[p].ny - j)) * dp[p].nx);
+            }
+        }
+    }
+    set_q0[nsets] = static_cast<int64_t>(quads.size());
+    int64_t max_quads = 1;
+    for (int s = 0; s < nsets; ++s) max_quads = std::max<int64_t>(max_quads, set_q0[s + 1] - set_q0[s]);
+    int64_t nsplit = std::max<int64_t>(1, ceil_div(target, base_blocks));
+    nsplit = std::min<int64_t>(nsplit, max_quads);
+    std::vector<AdjBlock> blocks;
+    for (int64_t c = 0; c < nsplit; ++c) {
+        for (int s = 0; s < nsets; ++s) {
+            const std::vector<int64_t>& pre = work_prefix[s];
+            const int64_t nq = static_cast<int64_t>(pre.size()) - 1;
+            const int64_t total = pre.back();
+            auto cut = [&](int64_t k) -> int64_t {
+                if (k >= nsplit) return nq;
+                const int64_t want = (total * k) / nsplit;
+                return static_cast<int64_t>(std::lower_bound(pre.begin(), pre.end(), want) - pre.begin());
+            };
+            const int64_t u0 = std::min(cut(c), nq), u1 = std::min(cut(c + 1), nq);
+            for (int64_t gq = 0; gq < groups; ++gq)
+                blocks.push_back({s, static_cast<int32_t>(gq * kThreads),
+                                  static_cast<int32_t>(set_q0[s] + u0),
+                                  static_cast<int32_t>(set_q0[s] + std::max(u0, u1)),
+                                  c * n_out + static_cast<int64_t>(s) * npil});
+        }
+    }
+    if (quads.size() > static_cast<size_t>(INT32_MAX) || blocks.size() > static_cast<size_t>(INT32_MAX)) {
+        set_error("launch too large");
+        return CRS_ERR_INVALID;
+    }
+    plan.nsplit = nsplit;
+    plan.nblocks = static_cast<int>(blocks.size());
+    plan.o_planes = plan.tb.add(dp.data(), dp.size() * sizeof(DevPlane));
+    plan.o_quads = plan.tb.add(quads.data(), quads.size() * sizeof(AdjQuad));
+    plan.o_blocks = plan.tb.add(blocks.data(), blocks.size() * sizeof(AdjBlock));
+    return CRS_OK;
+}
 
 crs_status plan_adjoint(const crs_context* ctx, int mode, int npil, int nsets,
                         const crs_plane* planes, int nplanes, AdjPlan& plan) {
     plan.mode = mode;
     plan.npil = npil;
     plan.nsets = nsets;
+    plan.n_out = static_cast<int64_t>(nsets) * npil;
     std::vector<DevPlane> dp(nplanes);
-    int64_t srec = 0;
-    for (int p = 0; p < nplanes; ++p) {
-        dp[p] = make_dev_plane(planes[p], 1, 1);
-        dp[p].sbase = srec;
-        srec += static_cast<int64_t>(dp[p].nquads) * dp[p].nxp * kQuad;
-    }
-    if (srec > (int64_t(1) << 31)) { set_error("too many pixels"); return CRS_ERR_INVALID; }
-    std::vector<char> exact(nplanes, 0);
-    if (mode == CRS_MODE_FP32C) plan.order = fp32c_order(dp, exact);
-    // A mixed FP32C call (some planes flagged exact, the order below kExactOrder) gets
-    // two launches: the 4 x 4 kernel over the other planes, then the 1 x 1 kernel over
-    // the flagged ones, so the 4 x 4 kernel carries no pair-by-pair path.
-    bool mixed = false;
-    if (mode == CRS_MODE_FP32C && plan.order != kExactOrder)
-        for (char e : exact) mixed = mixed || e;
-
-    const int target = ctx->adj_target[mode] > 0 ? ctx->adj_target[mode] : default_target(mode, true);
-    const int PQ = mode == CRS_MODE_FP32C ? kPQ : 1;
-    const int64_t groups = ceil_div(npil, static_cast<int64_t>(kThreads) * PQ);
-    const int64_t base_blocks = groups * nsets;
-    const int64_t n_out = static_cast<int64_t>(nsets) * npil;
-    std::vector<AdjQuad> quads;
-
-    // One launch's work-groups over the planes `take` selects: per-set quad lists (planes
-    // in order, quads in order, appended to `quads`), each set's quads cut into nsplit
-    // runs of about equal work (in pixel records: FP32C computes the padded 4 x nxp
-    // records of a quad), results to partial slots slot0, slot0 + 1, ...  Returns nsplit.
-    auto add_launch = [&](auto take, int64_t slot0, std::vector<AdjBlock>& blocks) -> int64_t {
-        std::vector<int64_t> set_q0(nsets + 1, 0);
-        std::vector<std::vector<int64_t>> work_prefix(nsets);
-        for (int s = 0; s < nsets; ++s) {
-            set_q0[s] = static_cast<int64_t>(quads.size());
-            std::vector<int64_t>& pre = work_prefix[s];
-            pre.push_back(0);
-            for (int p = 0; p < nplanes; ++p) {
-                if (dp[p].tset != s || !take(p)) continue;
-                for (int j = 0; j < dp[p].ny; j += kQuad) {
-                    quads.push_back({p, j});
-                    const int64_t w = mode == CRS_MODE_FP32C
-                                          ? static_cast<int64_t>(kQuad) * dp[p].nxp
-                                          : static_cast<int64_t>(std::min(kQuad, dp[p].ny - j)) * dp[p].nx;
-                    pre.push_back(pre.back() + w);
-                }
-            }
-        }
-        set_q0[nsets] = static_cast<int64_t>(quads.size());
-        int64_t max_quads = 1;
-        for (int s = 0; s < nsets; ++s) max_quads = std::max<int64_t>(max_quads, set_q0[s + 1] - set_q0[s]);
-        int64_t nsplit = std::max<int64_t>(1, ceil_div(target, base_blocks));
-        nsplit = std::min<int64_t>(nsplit, max_quads);
-        std::vector<int64_t> block_cost;
-        const size_t first = blocks.size();
-        for (int64_t c = 0; c < nsplit; ++c) {
-            for (int s = 0; s < nsets; ++s) {
-                const std::vector<int64_t>& pre = work_prefix[s];
-                const int64_t nq = static_cast<int64_t>(pre.size()) - 1;
-                const int64_t total = pre.back();
-                auto cut = [&](int64_t k) -> int64_t {  // first quad whose prefix >= k*total/nsplit
-                    if (k >= nsplit) return nq;
-                    const int64_t want = (total * k) / nsplit;
-                    return static_cast<int64_t>(std::lower_bound(pre.begin(), pre.end(), want) - pre.begin());
-                };
-                const int64_t u0 = std::min(cut(c), nq), u1 = std::min(cut(c + 1), nq);
-                for (int64_t gq = 0; gq < groups; ++gq) {
-                    blocks.push_back({s, static_cast<int32_t>(gq * kThreads * PQ),
-                                      static_cast<int32_t>(set_q0[s] + u0),
-                                      static_cast<int32_t>(set_q0[s] + std::max(u0, u1)),
-                                      (slot0 + c) * n_out + static_cast<int64_t>(s) * npil});
-                    const int64_t pil = std::min<int64_t>(int64_t(kThreads) * PQ, npil - gq * kThreads * PQ);
-                    block_cost.push_back(pil * (pre[std::max(u0, u1)] - pre[u0]));
-                }
-            }
-        }
-        // Heaviest work-groups first (quad runs hold whole quads, so runs differ by up to
-        // one quad): the XCDs take work-groups round robin in blockIdx order, and the
-        // light ones then fill the tail.  Each block names its own output slot.
-        std::vector<int32_t> order(block_cost.size());
-        for (size_t i = 0; i < order.size(); ++i) order[i] = static_cast<int32_t>(i);
-        std::stable_sort(order.begin(), order.end(),
-                         [&](int32_t a, int32_t b) { return block_cost[a] > block_cost[b]; });
-        std::vector<AdjBlock> sorted(order.size());
-        for (size_t i = 0; i < order.size(); ++i) sorted[i] = blocks[first + order[i]];
-        std::copy(sorted.begin(), sorted.end(), blocks.begin() + static_cast<std::ptrdiff_t>(first));
-        return nsplit;
-    };
-
-    std::vector<AdjBlock> blocks, blocks_exact;
-    int64_t nsplit = mixed ? add_launch([&](int p) { return !exact[p]; }, 0, blocks)
-                           : add_launch([](int) { return true; }, 0, blocks);
-    if (mixed) nsplit += add_launch([&](int p) { return exact[p] != 0; }, nsplit, blocks_exact);
-    if (quads.size() > static_cast<size_t>(INT32_MAX) ||
-        blocks.size() + blocks_exact.size() > static_cast<size_t>(INT32_MAX)) {
-        set_error("launch too large");
-        return CRS_ERR_INVALID;
-    }
-    // Staging blocks walk each plane's staged records.
-    std::vector<PixBlock> pblocks;
-    for (int p = 0; p < nplanes; ++p) {
-        const int64_t nrec = static_cast<int64_t>(dp[p].nquads) * dp[p].nxp * kQuad;
-        for (int64_t b = 0; b < nrec; b += kThreads) pblocks.push_back({p, static_cast<int32_t>(b)});
-    }
-    plan.nsplit = nsplit;
-    plan.n_out = n_out;
-    plan.staged_records = srec;
-    plan.nblocks = static_cast<int>(blocks.size());
-    plan.nblocks_exact = static_cast<int>(blocks_exact.size());
-    plan.npblocks = static_cast<int>(pblocks.size());
-    plan.o_planes = plan.tb.add(dp.data(), dp.size() * sizeof(DevPlane));
-    plan.o_quads = plan.tb.add(quads.data(), quads.size() * sizeof(AdjQuad));
-    plan.o_blocks = plan.tb.add(blocks.data(), blocks.size() * sizeof(AdjBlock));
-    plan.o_blocks_exact = plan.tb.add(blocks_exact.data(), blocks_exact.size() * sizeof(AdjBlock));
-    plan.o_pblocks = plan.tb.add(pblocks.data(), pblocks.size() * sizeof(PixBlock));
-    return CRS_OK;
+    for (int p = 0; p < nplanes; ++p) dp[p] = make_dev_plane(planes[p]);
+    if (mode == CRS_MODE_FP32C) return plan_adjoint_fp32c(ctx, npil, nsets, dp, plan);
+    return plan_adjoint_plain(ctx, mode, npil, nsets, dp, plan);
 }
 
 crs_status size_adjoint(const AdjPlan& plan, const Scratch& sc, hipStream_t stream) {
     crs_status st;
-    // Positions only (FP32 / FP64; FP32C reads the caller's px, py directly).
-    if (plan.mode != CRS_MODE_FP32C &&
-        (st = ensure(*sc.pil, staged_pillar_bytes(plan.mode, plan.npil, 1), stream,
-                     "staged pillars")) != CRS_OK)
+    if (plan.mode != CRS_MODE_FP32C) {
+        // Positions only (FP32C reads the caller's px, py directly).
+        if ((st = ensure(sc.owner, *sc.pil, staged_pillar_bytes(plan.mode, plan.npil, 1), stream,
+                         "staged pillars")) != CRS_OK)
+            return st;
+        return ensure(sc.owner, *sc.partial,
+                      static_cast<size_t>(plan.nsplit * plan.n_out) * partial_bytes(plan.mode),
+                      stream, "adjoint partials");
+    }
+    if ((st = ensure(sc.owner, *sc.pix, static_cast<size_t>(plan.staged_records) * sizeof(float4),
+                     stream, "staged pixels")) != CRS_OK)
         return st;
-    if (plan.mode == CRS_MODE_FP32C &&
-        (st = ensure(*sc.pix, static_cast<size_t>(plan.staged_records) * sizeof(float4), stream,
-                     "staged pixels")) != CRS_OK)
-        return st;
-    return ensure(*sc.partial,
-                  static_cast<size_t>(plan.nsplit * plan.n_out) * partial_bytes(plan.mode), stream,
+    return ensure(sc.owner, *sc.partial,
+                  static_cast<size_t>(plan.nseg) * kThreads * partial_bytes(plan.mode), stream,
                   "adjoint partials");
 }
 
@@ -821,17 +1124,6 @@ crs_status enqueue_adjoint(const AdjPlan& plan, const Scratch& sc, const double*
     crs_status st;
     const int mode = plan.mode;
     const DevPlane* d_planes = reinterpret_cast<const DevPlane*>(sc.tables + plan.o_planes);
-    // FP32 / FP64: t = NULL stages positions with zero transmissions.
-    const double* const xy[2] = {px, py};
-    const void* pil = xy;
-    if (mode != CRS_MODE_FP32C) {
-        if ((st = launch_status(launch_stage_pillars(mode, px, py, nullptr, plan.npil, 1,
-                                                     sc.pil->ptr, stream),
-                                "stage positions")) != CRS_OK)
-            return st;
-        pil = sc.pil->ptr;
-    }
-    const void* pix_or_u = u;
     if (mode == CRS_MODE_FP32C) {
         if ((st = launch_status(
                  launch_stage_pixels(d_planes,
@@ -840,26 +1132,61 @@ crs_status enqueue_adjoint(const AdjPlan& plan, const Scratch& sc, const double*
                                      static_cast<float4*>(sc.pix->ptr), stream),
                  "stage pixels")) != CRS_OK)
             return st;
-        pix_or_u = sc.pix->ptr;
+        if (ev0) hipEventRecord(ev0, stream);
+        Fp32cAdjArgs a;
+        a.px = px;
+        a.py = py;
+        a.npil = plan.npil;
+        a.planes = d_planes;
+        a.strips = reinterpret_cast<const AdjStrip*>(sc.tables + plan.o_strips);
+        a.segs = reinterpret_cast<const AdjSeg*>(sc.tables + plan.o_segs);
+        a.pix = static_cast<const float4*>(sc.pix->ptr);
+        a.partial = static_cast<float2*>(sc.partial->ptr);
+        for (const Fp32cLaunch& La : plan.launches) {
+            a.wg_seg = reinterpret_cast<const int32_t*>(sc.tables + La.o_wg);
+            a.nwg = La.nwg;
+            if ((st = launch_status(launch_adjoint_fp32c(La.bx, La.by, a, stream), "adjoint")) != CRS_OK)
+                return st;
+        }
+        if (ev1) hipEventRecord(ev1, stream);
+        return launch_status(
+            launch_adj_reduce_fp32c(a.partial,
+                                    reinterpret_cast<const int32_t*>(sc.tables + plan.o_slot_off),
+                                    reinterpret_cast<const int32_t*>(sc.tables + plan.o_slots),
+                                    plan.npil, plan.ngroups, plan.n_out,
+                                    reinterpret_cast<double2*>(g), stream),
+            "adjoint reduce");
     }
+    // FP32 / FP64: t = NULL stages positions with zero transmissions.
+    if ((st = launch_status(launch_stage_pillars(mode, px, py, nullptr, plan.npil, 1, sc.pil->ptr,
+                                                 stream),
+                            "stage positions")) != CRS_OK)
+        return st;
     if (ev0) hipEventRecord(ev0, stream);
-    const AdjQuad* d_quads = reinterpret_cast<const AdjQuad*>(sc.tables + plan.o_quads);
-    if ((st = launch_status(launch_adjoint(mode, plan.order, pil, plan.npil, d_planes, d_quads,
+    if ((st = launch_status(launch_adjoint(mode, sc.pil->ptr, plan.npil, d_planes,
+                                           reinterpret_cast<const AdjQuad*>(sc.tables + plan.o_quads),
                                            reinterpret_cast<const AdjBlock*>(sc.tables + plan.o_blocks),
-                                           plan.nblocks, pix_or_u, sc.partial->ptr, stream),
+                                           plan.nblocks, u, sc.partial->ptr, stream),
                             "adjoint")) != CRS_OK)
         return st;
-    if (plan.nblocks_exact > 0 &&
-        (st = launch_status(launch_adjoint(mode, kExactOrder, pil, plan.npil, d_planes, d_quads,
-                                           reinterpret_cast<const AdjBlock*>(sc.tables + plan.o_blocks_exact),
-                                           plan.nblocks_exact, pix_or_u, sc.partial->ptr, stream),
-                            "adjoint (planes evaluated pair by pair)")) != CRS_OK)
-        return st;
     if (ev1) hipEventRecord(ev1, stream);
-    return launch_status(launch_adj_reduce(mode, sc.partial->ptr,
-                                           static_cast<int>(plan.nsplit), plan.n_out,
-                                           reinterpret_cast<double2*>(g), stream),
+    return launch_status(launch_adj_reduce(sc.partial->ptr, static_cast<int>(plan.nsplit),
+                                           plan.n_out, reinterpret_cast<double2*>(g), stream),
                          "adjoint reduce");
+}
+
+// Profiler name and work-group count of a plan's (first) pair-kernel launch.
+template <class Plan>
+void note_launch(crs_context* ctx, const Plan& plan, bool adjoint) {
+    const int a = adjoint ? 1 : 0;
+    if (plan.mode == CRS_MODE_FP32C) {
+        if (plan.launches.empty()) return;
+        ctx->last_kernel[a] = fp32c_kernel_name(adjoint, plan.launches[0].bx, plan.launches[0].by);
+        ctx->last_groups[a] = plan.launches[0].nwg;
+    } else {
+        ctx->last_kernel[a] = plain_kernel(plan.mode, adjoint);
+        ctx->last_groups[a] = plan.nblocks;
+    }
 }
 
 }  // namespace
@@ -874,8 +1201,10 @@ extern "C" crs_status crs_forward(crs_context* ctx, void* stream_, int mode, con
     if (!px || !py || !t) { set_error("null pillar array"); return CRS_ERR_INVALID; }
     if (nplanes == 0 || (!field && !intensity)) return CRS_OK;
     hipStream_t stream = static_cast<hipStream_t>(stream_);
+    order_after_previous(ctx, stream);
     PlanSlot& slot = ctx->slot[0];
-    std::vector<unsigned char> key = plan_key(mode, npil, nsets, planes, nplanes, ctx->fwd_target[mode]);
+    std::vector<unsigned char> key =
+        plan_key(mode, npil, nsets, planes, nplanes, ctx->fwd_target[mode], ctx->fwd_shape);
     const bool reuse = slot.valid && slot.key == key;
     if (!reuse) {
         slot.valid = false;
@@ -884,7 +1213,7 @@ extern "C" crs_status crs_forward(crs_context* ctx, void* stream_, int mode, con
             return st;
     }
     const FwdPlan& plan = ctx->fwd_plan;
-    Scratch sc{&ctx->pil, &ctx->pix, &ctx->partial, nullptr};
+    Scratch sc{&ctx->pil, &ctx->pix, &ctx->partial, nullptr, ctx};
     if ((st = size_forward(plan, sc, stream)) != CRS_OK) return st;
     if (!reuse) {
         if ((st = upload_tables(ctx, slot.tables, plan.tb, stream)) != CRS_OK) return st;
@@ -896,11 +1225,9 @@ extern "C" crs_status crs_forward(crs_context* ctx, void* stream_, int mode, con
     const bool timed = ctx->timing && !stream_capturing(stream);
     st = enqueue_forward(plan, sc, px, py, t, field, intensity, stream,
                          timed ? ctx->ev[0][0] : nullptr, timed ? ctx->ev[0][1] : nullptr);
+    record_done(ctx, stream);
     if (st == CRS_OK && timed) ctx->ev_valid[0] = true;
-    if (st == CRS_OK) {
-        ctx->last_kernel[0] = pair_kernel(mode, plan.order, false);
-        ctx->last_groups[0] = plan.nblocks;
-    }
+    if (st == CRS_OK) note_launch(ctx, plan, false);
     return st;
 }
 
@@ -912,8 +1239,10 @@ extern "C" crs_status crs_adjoint(crs_context* ctx, void* stream_, int mode, con
     if (st != CRS_OK) return st;
     if (!px || !py || !g || (nplanes > 0 && !u)) { set_error("null array argument"); return CRS_ERR_INVALID; }
     hipStream_t stream = static_cast<hipStream_t>(stream_);
+    order_after_previous(ctx, stream);
     PlanSlot& slot = ctx->slot[1];
-    std::vector<unsigned char> key = plan_key(mode, npil, nsets, planes, nplanes, ctx->adj_target[mode]);
+    std::vector<unsigned char> key =
+        plan_key(mode, npil, nsets, planes, nplanes, ctx->adj_target[mode], ctx->adj_shape);
     const bool reuse = slot.valid && slot.key == key;
     if (!reuse) {
         slot.valid = false;
@@ -922,7 +1251,7 @@ extern "C" crs_status crs_adjoint(crs_context* ctx, void* stream_, int mode, con
             return st;
     }
     const AdjPlan& plan = ctx->adj_plan;
-    Scratch sc{&ctx->pil, &ctx->pix, &ctx->partial, nullptr};
+    Scratch sc{&ctx->pil, &ctx->pix, &ctx->partial, nullptr, ctx};
     if ((st = size_adjoint(plan, sc, stream)) != CRS_OK) return st;
     if (!reuse) {
         if ((st = upload_tables(ctx, slot.tables, plan.tb, stream)) != CRS_OK) return st;
@@ -934,11 +1263,9 @@ extern "C" crs_status crs_adjoint(crs_context* ctx, void* stream_, int mode, con
     const bool timed = ctx->timing && !stream_capturing(stream);
     st = enqueue_adjoint(plan, sc, px, py, u, g, stream, timed ? ctx->ev[1][0] : nullptr,
                          timed ? ctx->ev[1][1] : nullptr);
+    record_done(ctx, stream);
     if (st == CRS_OK && timed) ctx->ev_valid[1] = true;
-    if (st == CRS_OK) {
-        ctx->last_kernel[1] = pair_kernel(mode, plan.order, true);
-        ctx->last_groups[1] = plan.nblocks;
-    }
+    if (st == CRS_OK) note_launch(ctx, plan, true);
     return st;
 }
 
@@ -959,7 +1286,7 @@ crs_status build_graph(crs_context* ctx, Plan& plan, SizeFn size_fn, EnqueueFn e
                        crs_graph** out) {
     crs_graph* gr = new crs_graph();
     gr->device = ctx->device;
-    Scratch sc{&gr->pil, &gr->pix, &gr->partial, nullptr};
+    Scratch sc{&gr->pil, &gr->pix, &gr->partial, nullptr, nullptr};
     hipStream_t cap = nullptr;
     crs_status st = CRS_OK;
     auto fail = [&](crs_status s) {
@@ -978,7 +1305,7 @@ crs_status build_graph(crs_context* ctx, Plan& plan, SizeFn size_fn, EnqueueFn e
     }
     if ((st = size_fn(plan, sc, cap)) != CRS_OK) return fail(st);
     const size_t nt = plan.tb.bytes.size();
-    if ((st = ensure(gr->tables, nt, cap, "graph tables")) != CRS_OK) return fail(st);
+    if ((st = ensure(nullptr, gr->tables, nt, cap, "graph tables")) != CRS_OK) return fail(st);
     // Tables are copied once, before capture: the graph holds kernel nodes only.
     if (hipMemcpy(gr->tables.ptr, plan.tb.bytes.data(), nt, hipMemcpyHostToDevice) != hipSuccess) {
         hipGetLastError();
@@ -1143,9 +1470,9 @@ crs_status host_run(int mode, const double* px, const double* py, const double* 
     const size_t t_bytes = 2 * pil_bytes * static_cast<size_t>(nsets);
     const size_t f_elems = field ? 2 * static_cast<size_t>(total) : 0;
     const size_t i_elems = intensity ? static_cast<size_t>(total) : 0;
-    crs_status st = ensure(s->in, 2 * pil_bytes + t_bytes, s->stream, "host inputs");
+    crs_status st = ensure(nullptr, s->in, 2 * pil_bytes + t_bytes, s->stream, "host inputs");
     if (st == CRS_OK)
-        st = ensure(s->out, (f_elems + i_elems) * sizeof(double), s->stream, "host outputs");
+        st = ensure(nullptr, s->out, (f_elems + i_elems) * sizeof(double), s->stream, "host outputs");
     if (st == CRS_OK) {
         double* d_px = static_cast<double*>(s->in.ptr);
         double* d_py = d_px + npil;
@@ -1258,9 +1585,9 @@ extern "C" crs_status crs_analyze_mtf(const double* pupil, const double* pupil0,
     DropinState* st_ = acquire_state();
     if (!st_) return CRS_ERR_NO_DEVICE;
     const size_t nn = static_cast<size_t>(n) * n;
-    crs_status st = ensure(st_->in, nn * 3 * sizeof(double), st_->stream, "MTF pupil");
+    crs_status st = ensure(nullptr, st_->in, nn * 3 * sizeof(double), st_->stream, "MTF pupil");
     if (st == CRS_OK)
-        st = ensure(st_->out, static_cast<size_t>(ns) * (3 * sizeof(double) + 3 * sizeof(float)),
+        st = ensure(nullptr, st_->out, static_cast<size_t>(ns) * (3 * sizeof(double) + 3 * sizeof(float)),
                     st_->stream, "MTF sums");
     if (st == CRS_OK) {
         double2* d_P = static_cast<double2*>(st_->in.ptr);

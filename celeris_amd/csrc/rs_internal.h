@@ -1,9 +1,11 @@
 // rs_internal.h — device-side descriptors shared by the HIP kernels (rs_kernels.hip)
-// and the host planner (rs_api.cpp).  Not part of the public ABI.
+// and the host planner (rs_api.cpp), and the FP32C per-pair error bound both sides
+// evaluate.  Not part of the public ABI.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 
 namespace crs {
@@ -12,34 +14,42 @@ namespace crs {
 constexpr int kThreads = 256;
 // Waves (wave64) per work-group.
 constexpr int kWaves = kThreads / 64;
-// FP32C: pixels are handled in blocks of 4 x 4 (4 columns x 4 rows) that share one
-// reference distance, or 1 x 1 where a plane's pixel pitch is too coarse for the
-// block expansion.  The forward gives each lane one block ("thread tile") and a
-// work-group kFwdTiles tiles, its 4 waves splitting the pillar chunk; the adjoint
-// walks a plane in strips of kQuad rows ("quads") and each quad in 4-column groups.
-constexpr int kBlk = 4;
-constexpr int kQuad = 4;
-constexpr int kFwdTiles = 64;
-// Outputs per work-group of the partial-sum reductions (each wave adds a quarter of
-// the slots for kReduceCols outputs).
+// Outputs per work-group of the FP32 / FP64 partial-sum reductions (each wave adds a
+// quarter of the slots for kReduceCols outputs).
 constexpr int kReduceCols = 64;
-// Longest pillar chunk one FP32C forward work-group sums (each wave a quarter of it,
-// in float, folded every 64 pillars): bounds the float accumulation error.
-constexpr int kMaxChunk = 16384;
-// Shortest pillar chunk of a forward split (small problems only; 16 pillars per wave).
+// FP32 / FP64 adjoint: rows per strip ("quad") of a plane.
+constexpr int kQuad = 4;
+// FP32 / FP64 forward: shortest pillar chunk of a split.
 constexpr int kMinChunk = 64;
-// Plan order code of the 1 x 1 (unexpanded) FP32C kernels, used when every plane of a
-// call needs each pair at its own reference distance (fp32c_order); codes 0..3 select
-// the 4 x 4 kernels, which choose their series order per work-group; planes flagged
-// exact in such a call are evaluated pair by pair (forward: 1 x 1 tiles; adjoint: a
-// second launch of the 1 x 1 kernel).
-constexpr int kExactOrder = 4;
-// FP32C adjoint: pillars per thread (q0 + tid + k*kThreads, k < kPQ).  One: four per
-// thread (sharing each block's broadcast pixel records) measured 5-6% slower at 91,809
-// pillars x 161^2 and 321^2 and at 1,002,001 x 1024^2.
-constexpr int kPQ = 1;
-// FP32C adjoint: columns per LDS tile of a quad (kAdjCols x 4 pixel records).
-constexpr int kAdjCols = kThreads / kQuad;
+
+// ---- FP32C ------------------------------------------------------------------------
+// Pixels are handled in blocks of BX columns x BY rows (BX, BY in {4, 8}) that share
+// one FP64 reference distance per pillar, or 1 x 1 where a plane is too close or too
+// coarsely sampled for the block expansion ("exact" planes).  The forward gives each
+// lane one block ("thread tile") and a work-group 64 tiles of one plane, its 4 waves
+// splitting the pillars; the adjoint gives each thread one pillar and walks a plane in
+// strips of kStrip rows, each strip in BX-column blocks.
+constexpr int kFwdTiles = 64;
+// FP32C adjoint strip height (rows); BY = 4 kernels take a strip as two halves.
+constexpr int kStrip = 8;
+// FP32C adjoint: columns per LDS tile of a strip (kAdjCols x kStrip pixel records).
+constexpr int kAdjCols = 64;
+// FP32C staged-pixel columns are padded to a multiple of this (the widest block).
+constexpr int kColAlign = 8;
+// FP32C forward: pillars one wave sums in float before the work-group folds the four
+// waves' sums into FP64 running sums (so a float sum never runs over more than this).
+constexpr int kWaveFold = 1024;
+// Pillars per FP32C extent record (one per LDS tile of staged pillars).
+constexpr int kExtTile = kThreads;
+
+// Series orders of the FP32C pair evaluation: phase order / amplitude order.
+//   kP1A1: R - R0 = x1 (1 - y),            R0/R = 1 - 2y             6 FP32 ops per pair
+//   kP1A2: R - R0 = x1 (1 - y),            R0/R = 1 - 2y + 6y^2      7
+//   kP2A2: R - R0 = x1 (1 - y + 2y^2),     R0/R = 1 - 2y + 6y^2      9
+// with x1 = D / (2 R0), y = D / (4 R0^2), D = R^2 - R0^2 (the adjoint has kP1A1 and kP2A2).
+constexpr int kP1A1 = 0;
+constexpr int kP1A2 = 1;
+constexpr int kP2A2 = 2;
 
 // Lower bound on |z| of every row z0 + j sz (0 <= j < ny) of a plane and of every
 // pixel block's centre: the nearer end when all rows lie on one side of the lens
@@ -52,16 +62,17 @@ __host__ __device__ inline double plane_min_abs_z(double z0, double sz, int ny) 
     return 0.0;
 }
 
-// FP32C staged pillar buffer (forward): positions (double2, npil) and {|t|, arg t / 2 pi}
-// (float2, nsets x npil).
+// FP32C staged pillar buffer (forward): positions (double2, npil), {|t|, arg t / 2 pi}
+// (float2, nsets x npil) and, per kExtTile pillars, their extent {max |x|, max |y|}.
 struct StagedLayout {
-    size_t pos, ap, bytes;
+    size_t pos, ap, ext, bytes;
 };
 inline StagedLayout fp32c_staged_layout(int64_t npil, int64_t nsets) {
     StagedLayout L;
     L.pos = 0;
     L.ap = static_cast<size_t>(npil) * 16;
-    L.bytes = L.ap + static_cast<size_t>(npil * nsets) * 8;
+    L.ext = L.ap + ((static_cast<size_t>(npil * nsets) * 8 + 255) & ~size_t(255));
+    L.bytes = L.ext + static_cast<size_t>((npil + kExtTile - 1) / kExtTile) * 16;
     return L;
 }
 
@@ -74,24 +85,30 @@ struct DevPlane {
     double z0, sz;  // z  = z0 + j*sz
     double zref;    // kturn * z0: FP32C phase reference, restored as exp(2 pi i zref)
     int32_t nx, ny;
-    int32_t tx;     // forward thread tiles across a row (FP32C: floor(nx / 4) 4 x 4 tiles)
-    int32_t ty;     // forward thread tiles down a column (FP32C: floor(ny / 4))
+    int32_t tx;     // FP32C forward: whole BX-wide blocks across a row (0: exact plane)
+    int32_t ty;     // FP32C forward: whole BY-high blocks down a column (0: exact plane)
     int32_t tset;
-    int32_t nslots; // pillar chunks the forward sum is split into
+    int32_t nslots; // FP32 / FP64 forward: pillar chunks the sum is split into
     int64_t offset; // first output element of the plane
-    int64_t part;   // first forward partial element of slot 0
+    int64_t part;   // FP32 / FP64 forward: first partial element of slot 0
     int64_t sbase;  // FP32C adjoint: first staged pixel record of the plane
-    int32_t nxp;    // FP32C adjoint: nx rounded up to a multiple of 4
-    int32_t nquads; // ceil(ny / 4)
+    int32_t nxp;    // FP32C adjoint: nx rounded up to kColAlign; FP32/FP64: unused
+    int32_t nquads; // FP32 / FP64 adjoint: ceil(ny / 4); FP32C: strips ceil(ny / kStrip)
+    int32_t bx, by; // FP32C forward: block shape of the plane's launch (1 x 1 if exact)
+    // FP32C: steepest pair obliquity (sin of the angle from the z axis) at which the
+    // series orders kP1A1 / kP1A2 meet the pair budget for the plane's block shape (-1:
+    // never); beyond obl2, kP2A2, which the host admitted at any obliquity.
+    double obl1, obl2;
 };
 
-// Forward work-group: thread tiles [tile0, tile0 + kFwdTiles) of class `shape`
-// (FP32C) or pixels [tile0, tile0 + kThreads) (FP32, FP64) of `plane`, summed over
+// ---- FP32 / FP64 work decomposition (split sums, uniform slots) --------------------
+
+// Forward work-group: pixels [tile0, tile0 + kThreads) of `plane`, summed over
 // pillars [q0, q1); results go to partial[part + pixel].
 struct FwdBlock {
     int32_t plane, tile0, q0, q1;
     int64_t part;
-    int32_t shape;  // FP32C tile class: 0 = 4 x 4, 1 = 1 x 4, 2 = 4 x 1, 3 = 1 x 1
+    int32_t shape;
     int32_t pad;
 };
 
@@ -106,12 +123,207 @@ struct AdjQuad {
     int32_t plane, j0;
 };
 
-// Adjoint work-group: pillars q0 + tid + k*kThreads (k < pillars per thread) of
-// transmission set `tset`, summed over quads [u0, u1) of the quad list; results go
-// to partial[part + q].
+// Adjoint work-group: pillars q0 + tid of transmission set `tset`, summed over quads
+// [u0, u1) of the quad list; results go to partial[part + q].
 struct AdjBlock {
     int32_t tset, q0, u0, u1;
     int64_t part;
 };
+
+// ---- FP32C work decomposition (stream-K: equal work per work-group) ----------------
+// Forward: a "tile" is 64 thread tiles of one class of one plane (class 0 = BX x BY
+// blocks, 1 = 1 x BY, 2 = BX x 1, 3 = 1 x 1).  A "segment" is one tile against a pillar
+// range; each segment writes its own partial slot (64 x BX*BY float pairs).  Work-group
+// w runs segments [wg_seg[w], wg_seg[w+1]), a contiguous run of the tile-major
+// (tile, pillar) work line cut into equal-cost pieces.
+struct FwdTile {
+    int32_t plane, shape, tile0;
+    int32_t nel;    // pixels per lane of this class (BX*BY, BY, BX or 1)
+    int32_t seg0, nseg;  // its segments = partial slots seg0 .. seg0+nseg-1
+};
+struct FwdSeg {
+    int32_t tile, q0, q1, pad;
+};
+// Forward reduction work-group: elements [el0, el0 + kThreads) (e*64 + lane) of a tile.
+struct FwdRed {
+    int32_t tile, el0;
+};
+
+// Adjoint: a segment is one group of kThreads pillars (q0 ...) of set `tset` against a
+// run of its set's strip list, from column c0 of strip u0 to column c1 (exclusive) of
+// strip u1 (columns in blocks of the launch's BX); it writes its own partial slot
+// (kThreads float pairs).  Each (set, group) lists its slots (CSR over all launches).
+struct AdjStrip {
+    int32_t plane, j0;
+};
+struct AdjSeg {
+    int32_t tset, q0, u0, c0;
+    int32_t u1, c1, slot, pad;
+};
+
+// FP32C pixel tiles.  A plane blocked KBX x KBY has four classes of thread tiles, so
+// no lane computes a pixel outside the plane: class 0 = KBX x KBY blocks over the
+// largest multiple-of-block corner (tx x ty of them), 1 = 1 x KBY tiles of the
+// rx = nx - KBX tx leftover columns, 2 = KBX x 1 tiles of the ry = ny - KBY ty leftover
+// rows, 3 = 1 x 1 tiles of the leftover corner.  An exact plane (tx = ty = 0) is all
+// class 3.
+__host__ __device__ inline int tile_count(const DevPlane& P, int shape, int kbx, int kby) {
+    const int rx = P.nx - kbx * P.tx, ry = P.ny - kby * P.ty;
+    switch (shape) {
+        case 0: return P.tx * P.ty;
+        case 1: return rx * P.ty;
+        case 2: return P.tx * ry;
+        default: return rx * ry;
+    }
+}
+
+__host__ __device__ inline void tile_origin(const DevPlane& P, int shape, int t, int kbx, int kby,
+                                            int& ox, int& oy) {
+    const int rx = P.nx - kbx * P.tx;
+    switch (shape) {
+        case 0: ox = kbx * (t % P.tx); oy = kby * (t / P.tx); break;
+        case 1: ox = kbx * P.tx + t % rx; oy = kby * (t / rx); break;
+        case 2: ox = kbx * (t % P.tx); oy = kby * P.ty + t / P.tx; break;
+        default: ox = kbx * P.tx + t % rx; oy = kby * P.ty + t / rx; break;
+    }
+}
+
+// Device arguments of one FP32C forward / adjoint pair-kernel launch.
+struct Fp32cFwdArgs {
+    const double2* pos;
+    const float2* ap;
+    const double2* ext;
+    int npil;
+    const DevPlane* planes;
+    const FwdTile* tiles;
+    const FwdSeg* segs;
+    const int32_t* wg_seg;  // nwg + 1 entries
+    int nwg;
+    float2* partial;
+    int64_t slot_elems;
+};
+struct Fp32cAdjArgs {
+    const double* px;
+    const double* py;
+    int npil;
+    const DevPlane* planes;
+    const AdjStrip* strips;
+    const AdjSeg* segs;
+    const int32_t* wg_seg;  // nwg + 1 entries
+    int nwg;
+    const float4* pix;
+    float2* partial;
+};
+
+// ---- FP32C per-pair error bound ------------------------------------------------------
+// A pair's computed term differs from t_q exp(i k R) / R by at most eps |t_q| / R_pq,
+// with eps the sum of:
+//   * the dropped series terms (turns of phase x 2 pi, plus relative amplitude):
+//       order 1 phase 2 x1 y^2 <= B^3 / (16 R0^2),  order 2 phase 5 x1 y^3 <= 5 B^4 / (128 R0^3)
+//       order 1 amplitude 6 y^2 <= 3 B^2 / (8 R0^2),  order 2 amplitude 20 y^3 <= 5 B^3 / (16 R0^3)
+//     where |D| / R0 <= B(s) = a s + c (a = 2 e', c = 2 e_z' + (e'^2 + e_z'^2) / z', e' the
+//     block's half diagonal, e_z' its half depth, z' the nearest depth, all in turns) for
+//     a pair of obliquity s = rho / R0, and R0 >= z' / sqrt(1 - s^2);
+//   * float rounding of the phase, <= 2^-24 (2 + 6 (e' + e_z')) turns (the final sum at
+//     magnitude < 1 + |x1|, the staged arg t and th0, D's terms and the pixel offsets);
+//   * float rounding of the amplitude (v_rsq_f32 1.6 ulp, the staged |t|, two products),
+//     <= 2^-21;
+//   * v_sin_f32 / v_cos_f32, <= 1.25e-7 absolute each on gfx950 over [-2, 2] turns
+//     (measured), taken as 2.5e-7.
+// A series order is admitted where eps <= kPairBudget.  The float accumulation of the
+// admitted terms (at most kWaveFold per float sum, then FP64) has its own error, ~1e-6
+// of sum |t|/R at worst for a coherent pixel, so the total stays below 1e-5 of it.
+constexpr double kPairBudget = 8e-6;
+constexpr double kTwoPiD = 6.28318530717958647693;
+
+struct SeriesGeom {
+    double a, c, z, e;  // B(s) = a s + c; z: nearest depth (turns); e: e' + e_z'
+};
+
+// Geometry of bx x by blocks of plane P (lengths in turns).
+__host__ __device__ inline SeriesGeom series_geom(const DevPlane& P, int bx, int by) {
+    const double hx = 0.5 * (bx - 1) * fabs(P.sx) * P.kturn;
+    const double hy = 0.5 * (by - 1) * fabs(P.sy) * P.kturn;
+    const double hz = 0.5 * (by - 1) * fabs(P.sz) * P.kturn;
+    const double e = sqrt(hx * hx + hy * hy);
+    SeriesGeom g;
+    g.z = P.kturn * plane_min_abs_z(P.z0, P.sz, P.ny);
+    g.a = 2.0 * e;
+    g.c = 2.0 * hz + (e * e + hz * hz) / g.z;
+    g.e = e + hz;
+    return g;
+}
+
+// max over 0 <= s <= sn of (a s + c)^k (1 - s^2)^m (m = 1 or 3/2, given as 2m): the
+// function rises to the positive root of its derivative, ((k + 2m) a s^2 + 2m c s - k a
+// = 0), and falls after it.
+__host__ __device__ inline double series_peak(double a, double c, double sn, int k, int twom) {
+    double s = 0.0;
+    if (a > 0.0) {
+        const double A = (k + twom) * a, Bq = twom * c, C = -k * a;
+        s = (-Bq + sqrt(Bq * Bq - 4.0 * A * C)) / (2.0 * A);
+    }
+    s = fmin(fmax(s, 0.0), sn);
+    const double B = a * s + c, om = 1.0 - s * s;
+    const double f = twom == 2 ? om : om * sqrt(om);
+    return pow(B, k) * f;
+}
+
+// eps (fraction of |t|/R) of series order `order` for pairs of obliquity <= sn.
+__host__ __device__ inline double fp32c_pair_error(const SeriesGeom& g, double sn, int order) {
+    if (!(g.z > 0.0)) return INFINITY;
+    const double z2 = g.z * g.z, z3 = z2 * g.z;
+    double phase, amp;
+    if (order == kP2A2) {
+        phase = 5.0 * series_peak(g.a, g.c, sn, 4, 3) / (128.0 * z3);
+    } else {
+        phase = series_peak(g.a, g.c, sn, 3, 2) / (16.0 * z2);
+    }
+    if (order == kP1A1) {
+        amp = 3.0 * series_peak(g.a, g.c, sn, 2, 2) / (8.0 * z2);
+    } else {
+        amp = 5.0 * series_peak(g.a, g.c, sn, 3, 3) / (16.0 * z3);
+    }
+    const double round_phase = 5.9604644775390625e-8 * (2.0 + 6.0 * g.e);
+    const double round_amp = 4.76837158203125e-7;
+    const double trig = 2.5e-7;
+    return kTwoPiD * (phase + round_phase) + amp + round_amp + trig;
+}
+
+// Steepest pillar-to-pixel obliquity of plane P against pillars within |x| <= ex,
+// |y| <= ey (um): sin of the angle from the z axis.
+__host__ __device__ inline double plane_obliquity(const DevPlane& P, double ex, double ey) {
+    const double xl = P.x0 + (P.nx - 1) * P.sx, yl = P.y0 + (P.ny - 1) * P.sy;
+    const double rx = fmax(fabs(P.x0), fabs(xl)) + ex;
+    const double ry = fmax(fabs(P.y0), fabs(yl)) + ey;
+    const double zmin = plane_min_abs_z(P.z0, P.sz, P.ny);
+    const double rho2 = rx * rx + ry * ry;
+    if (!(rho2 > 0.0)) return 0.0;
+    return sqrt(rho2 / (rho2 + zmin * zmin));
+}
+
+// Largest obliquity sn in [0, 1] at which series order `order` of bx x by blocks of P
+// meets the pair budget (the error grows with sn), or -1 if it never does.
+inline double fp32c_order_obliquity(const DevPlane& P, int bx, int by, int order) {
+    const SeriesGeom g = series_geom(P, bx, by);
+    if (!(fp32c_pair_error(g, 0.0, order) <= kPairBudget)) return -1.0;
+    if (fp32c_pair_error(g, 1.0, order) <= kPairBudget) return 1.0;
+    double lo = 0.0, hi = 1.0;
+    for (int i = 0; i < 60; ++i) {
+        const double mid = 0.5 * (lo + hi);
+        if (fp32c_pair_error(g, mid, order) <= kPairBudget) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Cheapest series order admitted for P's blocks against pillars within the extent
+// (ex, ey), from the plane's precomputed obliquity limits.
+__host__ __device__ inline int fp32c_series_order(const DevPlane& P, double ex, double ey) {
+    const double sn = plane_obliquity(P, ex, ey);
+    if (sn <= P.obl1) return kP1A1;
+    if (sn <= P.obl2) return kP1A2;
+    return kP2A2;
+}
 
 }  // namespace crs

@@ -6,39 +6,37 @@
 // the FP64 CPU loop src/celeris/analysis/focal.cpp:229-233) and its adjoint
 // g_q = sum_p conj(G_pq) u_p (no reference counterpart).
 //
-// Three arithmetic modes share one work decomposition: a work-group of 256 threads
-// (4 wave64s) owns a set of pixels (forward) or pillars (adjoint) and streams the
-// other side through LDS in tiles that every lane reads by broadcast.  Sums are split
-// over pillar chunks (forward) or pixel quads (adjoint) so that small focal grids
-// still launch thousands of work-groups across the 256 CUs; each split writes a
-// partial (float64, or for FP32C a float pair rounded once from its FP64 combine) and a
-// second kernel adds the partials in FP64 in a fixed order, so results are bitwise
-// repeatable run to run.
+// Three arithmetic modes.  A work-group of 256 threads (4 wave64s) owns a set of
+// pixels (forward) or pillars (adjoint) and streams the other side through LDS in tiles
+// that every lane reads by broadcast.  Partial sums land in per-work-item slots and a
+// second kernel adds them in FP64 in a fixed order, so results are bitwise repeatable.
 //
 // FP32C ("FP32-compensated").  Lengths are measured in turns (x' = k x / 2 pi), so a
-// distance is directly a phase.  Pixels are taken in 4 x 4 blocks.  For each pillar q
-// and block b the distance R0 from q to the block centre and the phase
-// frac(R0 - z'_ref + arg t_q / 2 pi) are formed in FP64 (one v_rsq_f32 plus one FP64
-// Newton step).  Every pixel p of the block sits at a small, exactly known offset e
+// distance is directly a phase.  Pixels are taken in BX x BY blocks (4 or 8 each way).
+// For each pillar q and block b the distance R0 from q to the block centre and the
+// phase frac(R0 - z'_ref + arg t_q / 2 pi) are formed in FP64 (one v_rsq_f32 plus one
+// FP64 Newton step).  Every pixel p of the block sits at a small, exactly known offset e
 // from the centre, so
 //     R^2 = R0^2 + D,   D = e_x (2 dx0 + e_x) + e_y (2 dy0 + e_y) + (z_p'^2 - z_c'^2)
-// where D is tiny next to R0^2 and is formed in float without cancellation (each
-// term is one fma against per-block constants).  Then, with h = 1/(2 R0),
-// x1 = D h = a[c] + b[r] (separable per column / row) and y = x1 h = D / (4 R0^2):
-//     R - R0 = x1 (1 - y + 2 y^2 - ...)          (first order keeps x1 w, w = 1 - y)
-//     R0 / R = 1 - 2 y + 6 y^2 - ...              (first order keeps 2 w - 1)
+// where D is tiny next to R0^2 and is formed in float without cancellation (each term
+// is one fma against per-block constants, separable into a column part a[c] and a row
+// part b[r]).  With h = 1/(2 R0), x1 = D h and y = x1 h = D / (4 R0^2):
+//     R - R0 = x1 (1 - y + 2 y^2 - ...),      R0 / R = 1 - 2 y + 6 y^2 - ...
 // and the pair is
 //     E += |t| (k/2 pi) / R0 * (R0/R) * (cos 2 pi th, sin 2 pi th),  th = th0 + (R - R0)
-// with v_cos_f32 / v_sin_f32 (which take turns): 6 VALU ops and 2 transcendentals per
-// pair.  No float ever holds kR (~10^3 rad) or R^2 (~10^4 turns^2): the phase is carried
-// to ~1e-7 turns.  Each work-group checks, from its plane and the pillars' extent,
-// that the first-order terms meet the error budget and otherwise takes the second
-// order (fp32c_first_order); the host flags each plane where even that would not, and
-// such a plane is taken in 1 x 1 blocks (every pair at its own FP64 R0).  The factor
-// exp(2 pi i z'_ref) is restored in FP64 by the reduction; split partials are added in
-// FP64.  Storing the FP32C partials as floats halves the reductions' HBM reads (they
-// are bandwidth-bound) for one more float rounding of each chunk's sum, ~6e-8 of a
-// partial.
+// with v_cos_f32 / v_sin_f32 (which take turns): 6 FP32 ops and 2 transcendentals per
+// pair at the lowest series order.  No float ever holds kR (~10^3 rad) or R^2 (~10^4
+// turns^2).  Each work item chooses, from its plane and the pillars' extent, the
+// cheapest series order whose worst-case pair error stays within kPairBudget of
+// |t|/R (rs_internal.h: truncation, float rounding and the hardware sin/cos error); the
+// host sends a plane to 1 x 1 blocks (every pair at its own FP64 R0) where even the
+// highest order would not.  The factor exp(2 pi i z'_ref) is restored in FP64 by the
+// reduction.
+//
+// FP32C work is split stream-K style: the (pixel tile, pillar) work of a forward call
+// and the (pillar group, pixel strip column) work of an adjoint call are laid on one
+// line and cut into equal-cost pieces, one per work-group, so a launch of exactly the
+// resident work-group count keeps every CU busy to the end and writes few partials.
 #include "rs_internal.h"
 
 #include <hip/hip_runtime.h>
@@ -69,24 +67,54 @@ struct alignas(16) PillarRec {
     float amp, pad;
 };
 
+// Wave index as a scalar (every lane of a wave has the same one).
+__device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
 }  // namespace
 
 // ---------------------------------------------------------------------------------
 // Staging: float64 pillar lists -> the layout each mode's pair loop reads.
 
 // FP32C: positions stay FP64 (pos[npil], shared by all sets; each plane scales them
-// by its own k/(2 pi) when it loads a tile) and {|t|, arg(t)/(2 pi)} per pillar and set.
-// t == NULL stages positions only.
+// by its own k/(2 pi) when it loads a tile), {|t|, arg(t)/(2 pi)} per pillar and set,
+// and per kExtTile pillars their extent {max |x|, max |y|} (one work-group's max).
+// t == NULL stages positions (and extents) only.
 __global__ __launch_bounds__(kThreads) void stage_pillars_fp32c(
     const double* __restrict__ px, const double* __restrict__ py, const double2* __restrict__ t,
-    int npil, int64_t total, double2* __restrict__ pos, float2* __restrict__ ap) {
+    int npil, int64_t total, double2* __restrict__ pos, float2* __restrict__ ap,
+    double2* __restrict__ ext) {
+    __shared__ double2 red[kWaves];
     const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (idx >= total) return;
-    if (idx < npil) pos[idx] = make_double2(px[idx], py[idx]);
-    if (t) {
+    double mx = 0.0, my = 0.0;
+    if (idx < npil) {
+        const double x = px[idx], y = py[idx];
+        pos[idx] = make_double2(x, y);
+        mx = fabs(x);
+        my = fabs(y);
+    }
+    if (t && idx < total) {
         const double2 v = t[idx];
         ap[idx] = make_float2(static_cast<float>(hypot(v.x, v.y)),
                               static_cast<float>(atan2(v.y, v.x) * kInv2Pi));
+    }
+    // Work-groups of the first npil / kThreads ... blocks cover the positions; each
+    // writes its kExtTile pillars' extent.
+    if (static_cast<int64_t>(blockIdx.x) * blockDim.x >= npil) return;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        mx = fmax(mx, __shfl_xor(mx, o));
+        my = fmax(my, __shfl_xor(my, o));
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = make_double2(mx, my);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double2 E = red[0];
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w) {
+            E.x = fmax(E.x, red[w].x);
+            E.y = fmax(E.y, red[w].y);
+        }
+        ext[blockIdx.x] = E;
     }
 }
 
@@ -115,11 +143,12 @@ __global__ void stage_pillars_fp64(const double* __restrict__ px, const double* 
 }
 
 // ---------------------------------------------------------------------------------
-// Forward, FP32C.  A work-group owns kFwdTiles thread tiles (one BX x BY pixel block per
-// lane) of one plane; its 4 waves split the pillar chunk: pillars stream through LDS in
-// tiles of 256, scaled to turns by the plane's k/(2 pi) as they are loaded, and wave w
-// takes entries [64w, 64w + 64) of each.  The waves' sums are added in FP64 at the end.
-// BX = BY = 1 evaluates every pair at its own FP64 reference distance (no expansion).
+// Forward, FP32C.  A work-group runs its segments (a tile of 64 thread tiles against a
+// pillar range) one after the other.  Within a segment its 4 waves split each LDS tile
+// of 256 pillars (wave w takes entries [n w / 4, n (w+1) / 4)), each lane summing its
+// BX x BY pixels in float registers; every kWaveFold pillars per wave (and at the end)
+// the four waves' sums are added, wave by wave in order, into FP64 running sums in LDS,
+// which the segment finally stores as float pairs in its own partial slot.
 
 // Per-lane constants of a pixel block: its centre (turns, FP64) and the offsets of its
 // columns / rows, te = 2 e, e2 = e^2 (+ z_row'^2 - z_centre'^2 for the rows).
@@ -130,18 +159,19 @@ struct BlockGeom {
 };
 
 // One pillar against one block: th0 = frac(R0 - zeta), h = 1/(2 R0), c1 = |t| (k/2pi)
-// / R0', and the separable parts of D (first order) or of x1 = D h (second order):
-// a[c] = 2 e_c dx0 + e_c^2, b[r] = 2 e_r dy0 + e_r^2 + dz2_r, times h for GEN = 1.
-//   GEN = 0:  p = h^3, q = -2 c1 h^2, c = c1   (block_pairs folds h into them)
-//   GEN = 1:  p = 2 c1, q = -c1
+// / R0', and the separable parts of D (orders kP1A1 / kP1A2) or of x1 = D h (kP2A2):
+// a[c] = 2 e_c dx0 + e_c^2, b[r] = 2 e_r dy0 + e_r^2 + dz2_r (times h for kP2A2).
+//   kP1A1:  p = h^3, q = -2 c1 h^2, c = c1
+//   kP1A2:  as kP1A1, and r = 6 c1 h^4
+//   kP2A2:  p = 2 c1, q = -c1
 //   1 x 1 blocks: c = c1 only (R = R0).
 template <int BX, int BY>
 struct PairState {
-    float th0, h, p, q, c;
+    float th0, h, p, q, c, r;
     float a[BX], b[BY];
 };
 
-template <int GEN, int BX, int BY>
+template <int ORD, int BX, int BY>
 __device__ __forceinline__ void pair_state(const PillarRec& T, const BlockGeom<BX, BY>& G,
                                            PairState<BX, BY>& S) {
     const double dX0 = G.FXc - T.x, dY0 = G.FYc - T.y;
@@ -152,7 +182,7 @@ __device__ __forceinline__ void pair_state(const PillarRec& T, const BlockGeom<B
     const float c1 = T.amp * rf;
     if constexpr (BX * BY == 1) {
         S.c = c1;
-    } else if constexpr (GEN) {
+    } else if constexpr (ORD == kP2A2) {
         S.p = c1 + c1;
         S.q = -c1;
     } else {
@@ -160,177 +190,111 @@ __device__ __forceinline__ void pair_state(const PillarRec& T, const BlockGeom<B
         S.p = h2 * S.h;
         S.q = -2.0f * c1 * h2;
         S.c = c1;
+        if constexpr (ORD == kP1A2) S.r = 6.0f * c1 * (h2 * h2);
     }
     if constexpr (BX * BY > 1) {
         const float dx0 = static_cast<float>(dX0), dy0 = static_cast<float>(dY0);
 #pragma unroll
         for (int c = 0; c < BX; ++c) {
             S.a[c] = fmaf(G.tex[c], dx0, G.ex2[c]);
-            if constexpr (GEN) S.a[c] *= S.h;
+            if constexpr (ORD == kP2A2) S.a[c] *= S.h;
         }
 #pragma unroll
         for (int r = 0; r < BY; ++r) {
             S.b[r] = fmaf(G.tey[r], dy0, G.eyz[r]);
-            if constexpr (GEN) S.b[r] *= S.h;
+            if constexpr (ORD == kP2A2) S.b[r] *= S.h;
         }
     }
 }
 
-// The BX x BY pairs of one pillar.  With x1 = D h, y = x1 h = D / (4 R0^2):
-//   GEN = 0:  R - R0 = x1 w,  w = 1 - y;             R0/R = 2 w - 1 = 1 - 2 y
-//   GEN = 1:  R - R0 = x1 w,  w = 1 - y (1 - 2 y);   R0/R = 1 + y (6 y - 2)
-// and E += c1 (R0/R) (cos 2 pi th, sin 2 pi th), th = th0 + (R - R0), in turns.
-// GEN = 0 works on D itself: th = th0 + D (h - D h^3), A = c1 + (-2 c1 h^2) D (h folded
-// into per-pillar constants); GEN = 1 on x1 in Horner form.  6 and 9 FP32 ops per pair
-// besides the two transcendentals.
-template <int GEN, int BX, int BY>
+// The BX x BY pairs of one pillar (element e = r BX + c).
+//   kP1A1 (on D):  th = th0 + D (h - D h^3),  A = c1 + (-2 c1 h^2) D
+//   kP1A2 (on D):  th as kP1A1,                A = c1 + D (-2 c1 h^2 + 6 c1 h^4 D)
+//   kP2A2 (on x1): th = th0 + x1 w, w = 1 - y (1 - 2 y),  A = c1 (1 + y (6 y - 2))
+// and E += A (cos 2 pi th, sin 2 pi th): 6, 7 and 9 FP32 ops per pair besides the two
+// transcendentals.
+// Phase and amplitude of pair e of a block (ORD as above).
+template <int ORD, int BX, int BY>
+__device__ __forceinline__ void pair_phase(const PairState<BX, BY>& S, int e, float& th, float& A) {
+    const int r = e / BX, c = e % BX;
+    const float d = S.a[c] + S.b[r];
+    if constexpr (ORD == kP2A2) {
+        const float y = d * S.h;
+        const float w = fmaf(-y, fmaf(-2.0f, y, 1.0f), 1.0f);
+        A = fmaf(fmaf(3.0f * S.p, y, -S.p), y, -S.q);
+        th = fmaf(d, w, S.th0);
+    } else {
+        th = fmaf(d, fmaf(-d, S.p, S.h), S.th0);
+        if constexpr (ORD == kP1A2)
+            A = fmaf(fmaf(S.r, d, S.q), d, S.c);
+        else
+            A = fmaf(S.q, d, S.c);
+    }
+}
+
+template <int ORD, int BX, int BY>
 __device__ __forceinline__ void block_pairs(const PairState<BX, BY>& S, float (&ar)[BX * BY],
                                             float (&ai)[BX * BY]) {
-    if constexpr (BX * BY == 1) {
+    constexpr int N = BX * BY;
+    if constexpr (N == 1) {
         const float A = S.c;
         ar[0] = fmaf(A, __builtin_amdgcn_cosf(S.th0), ar[0]);
         ai[0] = fmaf(A, __builtin_amdgcn_sinf(S.th0), ai[0]);
     } else {
+        // Groups of G pairs: the group's phases, then its transcendentals, then its sums,
+        // with scheduling barriers so that no sum waits on a transcendental issued just
+        // before it (left to itself the compiler chains each pair's sin and cos straight
+        // into its sums, measured 12% slower for 4 x 4 blocks).
+        constexpr int G = N < 8 ? N : 8;
 #pragma unroll
-        for (int r = 0; r < BY; ++r) {
+        for (int g0 = 0; g0 < N; g0 += G) {
+            float th[G], A[G], cs[G], sn[G];
 #pragma unroll
-            for (int c = 0; c < BX; ++c) {
-                const float d = S.a[c] + S.b[r];  // D (GEN = 0) or x1 (GEN = 1)
-                float th, A;
-                if constexpr (GEN) {
-                    const float y = d * S.h;
-                    const float w = fmaf(-y, fmaf(-2.0f, y, 1.0f), 1.0f);
-                    A = fmaf(fmaf(3.0f * S.p, y, -S.p), y, -S.q);
-                    th = fmaf(d, w, S.th0);
-                } else {
-                    th = fmaf(d, fmaf(-d, S.p, S.h), S.th0);
-                    A = fmaf(S.q, d, S.c);
-                }
-                const int e = r * BX + c;
-                ar[e] = fmaf(A, __builtin_amdgcn_cosf(th), ar[e]);
-                ai[e] = fmaf(A, __builtin_amdgcn_sinf(th), ai[e]);
+            for (int k = 0; k < G; ++k) pair_phase<ORD>(S, g0 + k, th[k], A[k]);
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                cs[k] = __builtin_amdgcn_cosf(th[k]);
+                sn[k] = __builtin_amdgcn_sinf(th[k]);
             }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                ar[g0 + k] = fmaf(A[k], cs[k], ar[g0 + k]);
+                ai[g0 + k] = fmaf(A[k], sn[k], ai[g0 + k]);
+            }
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
 }
 
-// Pillars tile[k0 .. k1) against one block, one pillar state at a time.  (Issuing the
-// next pillar's FP64 distance work alongside the current pillar's pairs, with two
-// states alternating, took 143 VGPRs and 3 work-groups per CU and ran 3% slower.)
-template <int GEN, int BX, int BY>
+// Pillars tile[k0 .. k1) against one block, one pillar state at a time.
+template <int ORD, int BX, int BY>
 __device__ __forceinline__ void pillar_run(const PillarRec* tile, int k0, int k1,
-                                                 const BlockGeom<BX, BY>& G,
-                                                 float (&ar)[BX * BY], float (&ai)[BX * BY]) {
+                                           const BlockGeom<BX, BY>& G, float (&ar)[BX * BY],
+                                           float (&ai)[BX * BY]) {
     for (int k = k0; k < k1; ++k) {
         PairState<BX, BY> st;
-        pair_state<GEN>(tile[k], G, st);
-        block_pairs<GEN>(st, ar, ai);
+        pair_state<ORD>(tile[k], G, st);
+        block_pairs<ORD>(st, ar, ai);
     }
 }
 
-// Whether the first-order pair terms meet the error budget for every 4 x 4 block of
-// plane P against pillars within |x| <= ext.x, |y| <= ext.y (um).  With e' the block's
-// half diagonal, e_z' its half depth, z' the nearest row and s = rho / R0 a pair's
-// obliquity (at most sn, the steepest pillar-to-pixel direction): |D| / R0 <= B(s) =
-// a s + c, a = 2 e', c = 2 e_z' + (e'^2 + e_z'^2) / z', and the dropped terms are
-// 2 x1 y^2 <= B^3 / (16 R0^2) turns and 6 y^2 <= 3 B^2 / (8 R0^2).  An oblique pair is
-// also a far one, R0^2 >= z'^2 / (1 - s^2), so the bounds are (a s + c)^k (1 - s^2) /
-// z'^2 (k = 3, 2), which rise up to the root s_k of their derivative,
-// s_3 = (sqrt(c^2 + 15 a^2) - c) / (5 a) and s_2 = (sqrt(c^2 + 8 a^2) - c) / (4 a), and
-// fall after it: each is checked at min(s_k, sn).
-__device__ inline bool fp32c_first_order(const DevPlane& P, double2 ext, int bx, int by) {
-    const double xl = P.x0 + (P.nx - 1) * P.sx, yl = P.y0 + (P.ny - 1) * P.sy;
-    const double rx = fmax(fabs(P.x0), fabs(xl)) + ext.x;
-    const double ry = fmax(fabs(P.y0), fabs(yl)) + ext.y;
-    const double zmin = plane_min_abs_z(P.z0, P.sz, P.ny);
-    const double rho2 = rx * rx + ry * ry;
-    const double sn = sqrt(rho2 / (rho2 + zmin * zmin));
-    const double hx = 0.5 * (bx - 1) * fabs(P.sx) * P.kturn;
-    const double hy = 0.5 * (by - 1) * fabs(P.sy) * P.kturn;
-    const double hz = 0.5 * (by - 1) * fabs(P.sz) * P.kturn;
-    const double e = sqrt(hx * hx + hy * hy);
-    const double z = zmin * P.kturn;
-    const double a = 2.0 * e, c = 2.0 * hz + (e * e + hz * hz) / z;
-    const double s3 = a > 0.0 ? fmin(sn, (sqrt(c * c + 15.0 * a * a) - c) / (5.0 * a)) : 0.0;
-    const double s2 = a > 0.0 ? fmin(sn, (sqrt(c * c + 8.0 * a * a) - c) / (4.0 * a)) : 0.0;
-    const double B3 = a * s3 + c, B2 = a * s2 + c;
-    const double z2 = z * z;
-    return B3 * B3 * B3 * (1.0 - s3 * s3) / (16.0 * z2) <= 1.6e-7 &&
-           3.0 * B2 * B2 * (1.0 - s2 * s2) / (8.0 * z2) <= 4e-6;
-}
-
-// Pixel origin of thread tile t of a plane's tile class (shape): 0 = the 4 x 4 tiles
-// of the largest multiple-of-4 corner of the plane (tx x ty of them), 1 = 1 x 4 tiles
-// of the rx = nx - 4 tx leftover columns, 2 = 4 x 1 tiles of the ry = ny - 4 ty
-// leftover rows, 3 = 1 x 1 tiles of the leftover corner.  With tx = ty = 0 (no
-// blocks) class 3 covers the whole plane.  No tile reaches outside the plane.
-__device__ inline int tile_count(const DevPlane& P, int shape) {
-    const int rx = P.nx - 4 * P.tx, ry = P.ny - 4 * P.ty;
-    switch (shape) {
-        case 0: return P.tx * P.ty;
-        case 1: return rx * P.ty;
-        case 2: return P.tx * ry;
-        default: return rx * ry;
-    }
-}
-
-__device__ inline void tile_origin(const DevPlane& P, int shape, int t, int& ox, int& oy) {
-    const int rx = P.nx - 4 * P.tx;
-    switch (shape) {
-        case 0: ox = 4 * (t % P.tx); oy = 4 * (t / P.tx); break;
-        case 1: ox = 4 * P.tx + t % rx; oy = 4 * (t / rx); break;
-        case 2: ox = 4 * (t % P.tx); oy = 4 * P.ty + t / P.tx; break;
-        default: ox = 4 * P.tx + t % rx; oy = 4 * P.ty + t / rx; break;
-    }
-}
-
-// One work-group's pixel tiles of one class, all BX x BY, against its pillar chunk.
-template <int BX, int BY>
-__device__ __forceinline__ void fwd_tiles(
-    const double2* __restrict__ pos, const float2* __restrict__ ap, int npil,
-    const DevPlane& P, const FwdBlock& B, PillarRec* tile, float2* run,
-    float2* __restrict__ partial) {
+// One segment: tile T (class of block shape BX x BY within a plane blocked KBX x KBY)
+// against pillars [q0, q1) at series order `order`; partial sums to `slot`
+// (element e * 64 + lane).
+template <int BX, int BY, int KBX, int KBY>
+__device__ __forceinline__ void fwd_segment(
+    const double2* __restrict__ pos, const float2* __restrict__ aps, const DevPlane& P,
+    const FwdTile& T, int q0, int q1, int order, PillarRec* tile, double2* run,
+    float2* __restrict__ slot) {
     constexpr int N = BX * BY;
-    const int nx = P.nx;
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int my_tile = B.tile0 + lane;
-    const bool active = my_tile < tile_count(P, B.shape);
+    const int tid = threadIdx.x, wave = wave_id(), lane = tid & 63;
+    const int my_tile = T.tile0 + lane;
+    const bool active = my_tile < tile_count(P, T.shape, KBX, KBY);
     int ox, oy;
-    tile_origin(P, B.shape, active ? my_tile : B.tile0, ox, oy);
+    tile_origin(P, T.shape, active ? my_tile : T.tile0, KBX, KBY, ox, oy);
     const double kt = P.kturn;
     const double zref = P.zref;
-
-    // Series order from the work-group's own pillar chunk: its extent max |x|, max |y|
-    // (um), reduced through the running-sum buffer before that is cleared.  Uniform
-    // across the work-group; readfirstlane makes the branch on it scalar.
-    bool gen = false;
-    if constexpr (N > 1) {
-        double mx = 0.0, my = 0.0;
-        for (int q = B.q0 + tid; q < B.q1; q += kThreads) {
-            const double2 xy = pos[q];
-            mx = fmax(mx, fabs(xy.x));
-            my = fmax(my, fabs(xy.y));
-        }
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            mx = fmax(mx, __shfl_xor(mx, o));
-            my = fmax(my, __shfl_xor(my, o));
-        }
-        double* red = reinterpret_cast<double*>(run);
-        if (lane == 0) {
-            red[2 * wave] = mx;
-            red[2 * wave + 1] = my;
-        }
-        __syncthreads();
-        double2 E = make_double2(red[0], red[1]);
-#pragma unroll
-        for (int w = 1; w < kWaves; ++w) {
-            E.x = fmax(E.x, red[2 * w]);
-            E.y = fmax(E.y, red[2 * w + 1]);
-        }
-        gen = __builtin_amdgcn_readfirstlane(!fp32c_first_order(P, E, BX, BY));
-        __syncthreads();  // every wave has read the maxima before run is cleared
-    }
 
     BlockGeom<BX, BY> G;
     {
@@ -354,15 +318,14 @@ __device__ __forceinline__ void fwd_tiles(
         }
     }
     const float amp_scale = static_cast<float>(kt);  // 1/R = (k/2pi) / R'
-    const float2* __restrict__ aps = ap + static_cast<int64_t>(P.tset) * npil;
 
-    // Each lane's running sums live in LDS (float, one slot per wave, pixel and lane),
-    // fed by register sums over at most 64 pillars per LDS tile.
+    float ar[N], ai[N];
 #pragma unroll
-    for (int e = 0; e < N; ++e) run[(wave * N + e) * 64 + lane] = make_float2(0.0f, 0.0f);
-
-    for (int base = B.q0; base < B.q1; base += kThreads) {
-        const int n = min(kThreads, B.q1 - base);
+    for (int e = 0; e < N; ++e) ar[e] = ai[e] = 0.0f;
+    bool first = true;
+    int since_fold = 0;
+    for (int base = q0; base < q1; base += kThreads) {
+        const int n = min(kThreads, q1 - base);
         __syncthreads();
         if (tid < n) {
             const double2 xy = pos[base + tid];
@@ -377,67 +340,116 @@ __device__ __forceinline__ void fwd_tiles(
         }
         __syncthreads();
         const int k0 = (wave * n) / kWaves, k1 = ((wave + 1) * n) / kWaves;
-        if (k0 < k1) {
-            float ar[N], ai[N];
-#pragma unroll
-            for (int e = 0; e < N; ++e) ar[e] = ai[e] = 0.0f;
-            if (gen)
-                pillar_run<1>(tile, k0, k1, G, ar, ai);
+        if constexpr (N == 1) {
+            pillar_run<kP1A1>(tile, k0, k1, G, ar, ai);
+        } else {
+            if (order == kP1A1)
+                pillar_run<kP1A1>(tile, k0, k1, G, ar, ai);
+            else if (order == kP1A2)
+                pillar_run<kP1A2>(tile, k0, k1, G, ar, ai);
             else
-                pillar_run<0>(tile, k0, k1, G, ar, ai);
-#pragma unroll
-            for (int e = 0; e < N; ++e) {
-                float2& v = run[(wave * N + e) * 64 + lane];
-                v.x += ar[e];
-                v.y += ai[e];
-            }
+                pillar_run<kP2A2>(tile, k0, k1, G, ar, ai);
         }
-    }
-
-    // Wave 0 adds the 4 waves' sums in FP64, in wave order, and stores the float pair.
-    __syncthreads();
-    if (wave != 0 || !active) return;
-#pragma unroll
-    for (int r = 0; r < BY; ++r) {
-#pragma unroll
-        for (int c = 0; c < BX; ++c) {
-            const int e = r * BX + c;
-            double vr = 0.0, vi = 0.0;
+        since_fold += kThreads / kWaves;
+        if (since_fold >= kWaveFold || base + kThreads >= q1) {
+            // Fold the four waves' float sums into the FP64 running sums, in wave order.
 #pragma unroll
             for (int w = 0; w < kWaves; ++w) {
-                const float2 o = run[(w * N + e) * 64 + lane];
-                vr += o.x;
-                vi += o.y;
+                __syncthreads();
+                if (wave == w) {
+#pragma unroll
+                    for (int e = 0; e < N; ++e) {
+                        double2 v = make_double2(0.0, 0.0);
+                        if (!first || w != 0) v = run[e * 64 + lane];
+                        v.x += static_cast<double>(ar[e]);
+                        v.y += static_cast<double>(ai[e]);
+                        run[e * 64 + lane] = v;
+                    }
+                }
             }
-            partial[B.part + static_cast<int64_t>(oy + r) * nx + (ox + c)] =
-                make_float2(static_cast<float>(vr), static_cast<float>(vi));
+#pragma unroll
+            for (int e = 0; e < N; ++e) ar[e] = ai[e] = 0.0f;
+            first = false;
+            since_fold = 0;
         }
+    }
+    __syncthreads();
+    for (int i = tid; i < N * 64; i += kThreads) {
+        const double2 v = run[i];
+        slot[i] = make_float2(static_cast<float>(v.x), static_cast<float>(v.y));
     }
 }
 
-// BLK = 4: 4 x 4 tiles plus the edge classes (1 x 4, 4 x 1, 1 x 1), chosen per
-// work-group from its descriptor (a plane flagged exact has 1 x 1 tiles only); BLK = 1,
-// for calls whose planes are all flagged: 1 x 1 tiles only (every pair at its own
-// FP64 reference distance).  123 VGPRs and 40 KB of LDS: 4 work-groups per CU.
-template <int BLK>
-__global__ __launch_bounds__(kThreads, 4) void fwd_fp32c(
-    const double2* __restrict__ pos, const float2* __restrict__ ap, int npil,
-    const DevPlane* __restrict__ planes, const FwdBlock* __restrict__ blocks,
-    float2* __restrict__ partial) {
-    constexpr int NMAX = BLK * BLK;
+// Waves per SIMD each forward instance is compiled for (a register budget of 512 /
+// waves VGPRs): the accumulators alone take 2 BX BY registers; 4 x 8 blocks need ~220
+// (forcing 168 spills), so they run 2 waves per SIMD.
+constexpr int fwd_waves_per_simd(int n) { return n <= 16 ? 4 : 2; }
+
+// KBX x KBY: the launch's block shape; its work-groups take the four tile classes of
+// the planes blocked that way (and the 1 x 1 tiles of exact planes).  <1, 1>: calls
+// whose planes are all exact.
+template <int KBX, int KBY>
+__global__ __launch_bounds__(kThreads, fwd_waves_per_simd(KBX * KBY)) void fwd_fp32c(
+    const double2* __restrict__ pos, const float2* __restrict__ ap,
+    const double2* __restrict__ ext, int npil, const DevPlane* __restrict__ planes,
+    const FwdTile* __restrict__ tiles, const FwdSeg* __restrict__ segs,
+    const int32_t* __restrict__ wg_seg, float2* __restrict__ partial, int64_t slot_elems) {
     __shared__ PillarRec tile[kThreads];
-    __shared__ float2 run[kWaves * NMAX * 64];
-    const FwdBlock B = blocks[blockIdx.x];
-    const DevPlane& P = planes[B.plane];
-    if constexpr (BLK == kBlk) {
-        switch (B.shape) {
-            case 0: fwd_tiles<kBlk, kBlk>(pos, ap, npil, P, B, tile, run, partial); break;
-            case 1: fwd_tiles<1, kBlk>(pos, ap, npil, P, B, tile, run, partial); break;
-            case 2: fwd_tiles<kBlk, 1>(pos, ap, npil, P, B, tile, run, partial); break;
-            default: fwd_tiles<1, 1>(pos, ap, npil, P, B, tile, run, partial); break;
+    __shared__ double2 run[KBX * KBY * 64];
+    __shared__ double2 red[kWaves];
+    const int s0 = wg_seg[blockIdx.x], s1 = wg_seg[blockIdx.x + 1];
+    for (int s = s0; s < s1; ++s) {
+        const FwdSeg S = segs[s];
+        const FwdTile T = tiles[S.tile];
+        const DevPlane& P = planes[T.plane];
+        const float2* __restrict__ aps = ap + static_cast<int64_t>(P.tset) * npil;
+        float2* __restrict__ slot = partial + static_cast<int64_t>(s) * slot_elems;
+        if constexpr (KBX * KBY == 1) {
+            fwd_segment<1, 1, 1, 1>(pos, aps, P, T, S.q0, S.q1, kP1A1, tile, run, slot);
+        } else {
+            // Series order: class 0 from the segment's pillar extent (the staged extents
+            // of the kExtTile runs it covers); the smaller edge blocks take kP2A2, which
+            // the host admitted for the full block shape at any obliquity.
+            int order = kP2A2;
+            if (T.shape == 0) {
+                double mx = 0.0, my = 0.0;
+                const int e0 = S.q0 / kExtTile, e1 = (S.q1 - 1) / kExtTile;
+                for (int i = e0 + static_cast<int>(threadIdx.x); i <= e1; i += kThreads) {
+                    const double2 v = ext[i];
+                    mx = fmax(mx, v.x);
+                    my = fmax(my, v.y);
+                }
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) {
+                    mx = fmax(mx, __shfl_xor(mx, o));
+                    my = fmax(my, __shfl_xor(my, o));
+                }
+                __syncthreads();  // the previous segment has read red
+                if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = make_double2(mx, my);
+                __syncthreads();
+                double2 E = red[0];
+#pragma unroll
+                for (int w = 1; w < kWaves; ++w) {
+                    E.x = fmax(E.x, red[w].x);
+                    E.y = fmax(E.y, red[w].y);
+                }
+                order = __builtin_amdgcn_readfirstlane(fp32c_series_order(P, E.x, E.y));
+            }
+            switch (T.shape) {
+                case 0:
+                    fwd_segment<KBX, KBY, KBX, KBY>(pos, aps, P, T, S.q0, S.q1, order, tile, run, slot);
+                    break;
+                case 1:
+                    fwd_segment<1, KBY, KBX, KBY>(pos, aps, P, T, S.q0, S.q1, kP2A2, tile, run, slot);
+                    break;
+                case 2:
+                    fwd_segment<KBX, 1, KBX, KBY>(pos, aps, P, T, S.q0, S.q1, kP2A2, tile, run, slot);
+                    break;
+                default:
+                    fwd_segment<1, 1, KBX, KBY>(pos, aps, P, T, S.q0, S.q1, kP1A1, tile, run, slot);
+                    break;
+            }
         }
-    } else {
-        fwd_tiles<1, 1>(pos, ap, npil, P, B, tile, run, partial);
     }
 }
 
@@ -525,12 +537,12 @@ __global__ __launch_bounds__(kThreads) void fwd_fp64(
 }
 
 // ---------------------------------------------------------------------------------
-// Forward reduction: add the plane's pillar-chunk partials, restore the factor
-// exp(2 pi i z'_ref) that FP32C leaves out, write field and/or |E|^2.  A work-group
+// Forward reduction (FP32 / FP64): add the plane's pillar-chunk partials, write field
+// and/or |E|^2 (restore_phase: turn by exp(2 pi i z'_ref) first).  A work-group
 // takes kReduceCols pixels; wave w adds the w-th quarter of the slots, and wave 0 adds
 // the four quarter sums in order (a fixed order: results are bitwise repeatable).
 
-// T: float2 (FP32C partials, each one chunk's sum rounded once) or double2.
+// T: the partial type (double2).
 template <typename T>
 __global__ __launch_bounds__(kThreads) void fwd_reduce(
     const DevPlane* __restrict__ planes, const PixBlock* __restrict__ pblocks,
@@ -577,23 +589,61 @@ __global__ __launch_bounds__(kThreads) void fwd_reduce(
 }
 
 // ---------------------------------------------------------------------------------
-// Adjoint staging (FP32C).  Pixel records are laid out quad by quad (4 rows), column
-// by column, 4 records per column: record (quad u, column i, row r) sits at
-// sbase + (u*nxp + i)*4 + r, so an LDS tile of 64 columns x 4 rows is one contiguous
-// run.  Record = {2U, -U, arg(u)/2pi - z'_ref mod 1, 0} with U = |u| k/2pi; padding
-// (rows >= ny, columns >= nx) has U = 0 and contributes nothing.
+// Forward reduction, FP32C: for every element (e * 64 + lane) of a tile, the sum over
+// the tile's segments' partial slots in segment order (FP64), turned by exp(2 pi i
+// z'_ref) and written at the element's pixel.  A work-group takes 256 elements of one
+// tile.
+
+__global__ __launch_bounds__(kThreads) void fwd_reduce_fp32c(
+    const DevPlane* __restrict__ planes, const FwdTile* __restrict__ tiles,
+    const FwdRed* __restrict__ reds, const float2* __restrict__ partial, int64_t slot_elems,
+    double2* __restrict__ field, double* __restrict__ intensity) {
+    const FwdRed R = reds[blockIdx.x];
+    const FwdTile T = tiles[R.tile];
+    const DevPlane& P = planes[T.plane];
+    const int i = R.el0 + static_cast<int>(threadIdx.x);
+    if (i >= T.nel * 64) return;
+    const int e = i >> 6, lane = i & 63;
+    const int t = T.tile0 + lane;
+    if (t >= tile_count(P, T.shape, P.bx, P.by)) return;
+    int ox, oy;
+    tile_origin(P, T.shape, t, P.bx, P.by, ox, oy);
+    const int bw = (T.shape == 0 || T.shape == 2) ? P.bx : 1;
+    const int c = e % bw, r = e / bw;
+    double er = 0.0, ei = 0.0;
+    const float2* __restrict__ src = partial + static_cast<int64_t>(T.seg0) * slot_elems + i;
+    for (int k = 0; k < T.nseg; ++k) {
+        const float2 v = src[static_cast<int64_t>(k) * slot_elems];
+        er += v.x;
+        ei += v.y;
+    }
+    double sn, cs;
+    sincos(kTwoPi * (P.zref - floor(P.zref)), &sn, &cs);
+    const double xr = er * cs - ei * sn;
+    const double xi = er * sn + ei * cs;
+    const int64_t o = P.offset + static_cast<int64_t>(oy + r) * P.nx + (ox + c);
+    if (field) field[o] = make_double2(xr, xi);
+    if (intensity) intensity[o] = xr * xr + xi * xi;
+}
+
+// ---------------------------------------------------------------------------------
+// Adjoint staging (FP32C).  Pixel records are laid out strip by strip (kStrip rows),
+// column by column, kStrip records per column: record (strip u, column i, row r) sits at
+// sbase + (u*nxp + i)*kStrip + r, so an LDS tile of 64 columns x kStrip rows is one
+// contiguous run.  Record = {2U, -U, arg(u)/2pi - z'_ref mod 1, 0} with U = |u| k/2pi;
+// padding (rows >= ny, columns >= nx) has U = 0 and contributes nothing.
 
 __global__ __launch_bounds__(kThreads) void stage_pixels_fp32c(
     const DevPlane* __restrict__ planes, const PixBlock* __restrict__ pblocks,
     const double2* __restrict__ u, float4* __restrict__ pix) {
     const PixBlock B = pblocks[blockIdx.x];
     const DevPlane& P = planes[B.plane];
-    const int nrec = P.nquads * P.nxp * kQuad;
+    const int nrec = P.nquads * P.nxp * kStrip;
     const int rix = B.pix0 + static_cast<int>(threadIdx.x);
     if (rix >= nrec) return;
-    const int quad = rix / (P.nxp * kQuad);
-    const int rem = rix - quad * P.nxp * kQuad;
-    const int i = rem / kQuad, j = quad * kQuad + rem % kQuad;
+    const int strip = rix / (P.nxp * kStrip);
+    const int rem = rix - strip * P.nxp * kStrip;
+    const int i = rem / kStrip, j = strip * kStrip + rem % kStrip;
     float4 rec = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (i < P.nx && j < P.ny) {
         const double2 v = u[P.offset + static_cast<int64_t>(j) * P.nx + i];
@@ -606,16 +656,17 @@ __global__ __launch_bounds__(kThreads) void stage_pixels_fp32c(
 }
 
 // ---------------------------------------------------------------------------------
-// Adjoint, FP32C.  Thread owns kPQ = 1 pillar (FP64 position in registers); a quad's pixel
-// records stream through LDS 64 columns at a time and are consumed in BX x BY blocks
-// (4 x 4; 1 x 1 for planes flagged exact, where every pair needs its own reference
-// distance, which the host sends to the <1, 1> instance).  Per pair
+// Adjoint, FP32C.  A thread owns one pillar (FP64 position in registers) of a group of
+// 256; a work-group runs its segments (a group against a run of strip columns) one after
+// the other.  A strip's records stream through LDS 64 columns at a time (as three float
+// arrays, so a column of records is a few vector reads) and are consumed in BX x BY
+// blocks (BY = 4 takes a strip as two halves; 1 x 1 for exact planes, in a launch of
+// their own).  Per pair
 //     conj(G) u = U / R0' (R0/R) exp(2 pi i (chi - th0 - (R - R0))),   U = |u| k/2pi,
-// with the same expansion of R - R0 and R0/R as the forward; the block's sum is scaled
-// by 1/R0' once.
+// with the forward's expansion of R - R0 and R0/R; each block's sum (kept per column,
+// so the accumulation chains stay short) is turned by exp(-2 pi i th0) and scaled by
+// 1/R0' once.  Sums over LDS tiles are Kahan-compensated.
 
-// One pillar against one block: th0 = frac(R0 - z'_ref), h = 1/(2 R0), rf = 1/R0 and
-// the separable parts a[c], b[r] of x1 = D h.
 template <int BX, int BY>
 struct AdjState {
     float th0, h, rf;
@@ -639,53 +690,65 @@ __device__ __forceinline__ void adj_state(double dX0, double dY0, double zref,
     }
 }
 
-// The pairs of one pillar with one block's records W (column-major: W[c*BY + r] =
-// {2U, -U, chi}); adds the block's sum to (gr, gi).  GEN as in block_pairs.  For 4 x 4
-// blocks the pair phases are chi - (R - R0) and the block's sum is turned by
-// exp(-2 pi i th0) and scaled by 1/R0' once; 1 x 1 blocks use chi - th0 directly.
-template <int GEN, int BX, int BY>
-__device__ __forceinline__ void adj_pairs(const AdjState<BX, BY>& S, const float3 (&W)[BX * BY],
+// The pairs of one pillar with one block whose first record (column 0, row 0 of the
+// block) is LDS record k0 (record of column c, row r: k0 + c kStrip + r); adds the
+// block's turned sum to (gr, gi).
+//   kP1A1:  w = 1 - x1 h,             A = 2U w - U,                phase chi - x1 w
+//   kP2A2:  w = 1 - y (1 - 2 y),      A = U (1 + y (6 y - 2)),     phase chi - x1 w
+// (y = x1 h); 1 x 1 blocks use chi - th0 and A = U directly.
+template <int ORD, int BX, int BY>
+__device__ __forceinline__ void adj_pairs(const AdjState<BX, BY>& S, const float* sU2,
+                                          const float* sUm, const float* sChi, int k0,
                                           float& gr, float& gi) {
     if constexpr (BX * BY == 1) {
-        const float phi = W[0].z - S.th0;
-        const float A = S.rf * (W[0].x + W[0].y);
+        const float phi = sChi[k0] - S.th0;
+        const float A = S.rf * (sU2[k0] + sUm[k0]);
         gr = fmaf(A, __builtin_amdgcn_cosf(phi), gr);
         gi = fmaf(A, __builtin_amdgcn_sinf(phi), gi);
     } else {
-        float br = 0.0f, bi = 0.0f;
+        float br[BX], bi[BX];
 #pragma unroll
         for (int c = 0; c < BX; ++c) {
+            br[c] = 0.0f;
+            bi[c] = 0.0f;
 #pragma unroll
             for (int r = 0; r < BY; ++r) {
-                const float3 w3 = W[c * BY + r];
+                const int k = k0 + c * kStrip + r;
+                const float U2 = sU2[k], Um = sUm[k], chi = sChi[k];
                 const float x1 = S.a[c] + S.b[r];
                 float w, A;
-                if constexpr (GEN) {  // U = -w3.y, 2U = w3.x: A = U (1 + y (6 y - 2))
+                if constexpr (ORD == kP2A2) {
                     const float y = x1 * S.h;
                     w = fmaf(-y, fmaf(-2.0f, y, 1.0f), 1.0f);
-                    A = fmaf(fmaf(3.0f * w3.x, y, -w3.x), y, -w3.y);
+                    A = fmaf(fmaf(3.0f * U2, y, -U2), y, -Um);
                 } else {
                     w = fmaf(-x1, S.h, 1.0f);
-                    A = fmaf(w3.x, w, w3.y);
+                    A = fmaf(U2, w, Um);
                 }
-                const float phi = fmaf(-x1, w, w3.z);
-                br = fmaf(A, __builtin_amdgcn_cosf(phi), br);
-                bi = fmaf(A, __builtin_amdgcn_sinf(phi), bi);
+                const float phi = fmaf(-x1, w, chi);
+                br[c] = fmaf(A, __builtin_amdgcn_cosf(phi), br[c]);
+                bi[c] = fmaf(A, __builtin_amdgcn_sinf(phi), bi[c]);
             }
+        }
+        float sr = br[0], si = bi[0];
+#pragma unroll
+        for (int c = 1; c < BX; ++c) {
+            sr += br[c];
+            si += bi[c];
         }
         const float cr = S.rf * __builtin_amdgcn_cosf(S.th0);
         const float sn = S.rf * __builtin_amdgcn_sinf(S.th0);
-        gr = fmaf(br, cr, fmaf(bi, sn, gr));
-        gi = fmaf(bi, cr, fmaf(-br, sn, gi));
+        gr = fmaf(sr, cr, fmaf(si, sn, gr));
+        gi = fmaf(si, cr, fmaf(-sr, sn, gi));
     }
 }
 
-// One 64-column LDS tile of a quad against the thread's pillars.
-template <int GEN, int BX, int BY>
-__device__ __forceinline__ void adj_tile(const float4* tile, int ncol, const DevPlane& P, int j0,
-                                         int cb, const double (&qx)[kPQ],
-                                         const double (&qy)[kPQ], float (&ar)[kPQ],
-                                         float (&ai)[kPQ]) {
+// Columns [0, ncol) of one LDS tile (strip starting at row j0, tile starting at plane
+// column cb) against the thread's pillar (qx, qy in um).
+template <int ORD, int BX, int BY>
+__device__ __forceinline__ void adj_tile(const float* sU2, const float* sUm, const float* sChi,
+                                         int ncol, const DevPlane& P, int j0, int cb, double qx,
+                                         double qy, float& ar, float& ai) {
     const double kt = P.kturn;
     BlockGeom<BX, BY> G;
 #pragma unroll
@@ -694,8 +757,8 @@ __device__ __forceinline__ void adj_tile(const float4* tile, int ncol, const Dev
         G.tex[c] = 2.0f * e;
         G.ex2[c] = e * e;
     }
-#pragma unroll
-    for (int sr0 = 0; sr0 < kQuad; sr0 += BY) {
+    for (int sr0 = 0; sr0 < kStrip; sr0 += BY) {
+        if (BY < kStrip && j0 + sr0 >= P.ny) break;  // all-padding rows (uniform)
         const double jc = j0 + sr0 + (BY - 1) * 0.5;
         G.FYc = kt * (P.y0 + jc * P.sy);
         const double zc = kt * (P.z0 + jc * P.sz);
@@ -707,107 +770,98 @@ __device__ __forceinline__ void adj_tile(const float4* tile, int ncol, const Dev
             G.tey[r] = 2.0f * e;
             G.eyz[r] = e * e + static_cast<float>(zr * zr - G.ZC2);
         }
-        double dY0[kPQ];
-#pragma unroll
-        for (int k = 0; k < kPQ; ++k) dY0[k] = fma(-kt, qy[k], G.FYc);
+        const double dY0 = fma(-kt, qy, G.FYc);
         for (int g = 0; g < ncol; g += BX) {
             G.FXc = kt * (P.x0 + (cb + g + (BX - 1) * 0.5) * P.sx);
-            float3 W[BX * BY];
-#pragma unroll
-            for (int c = 0; c < BX; ++c)
-#pragma unroll
-                for (int r = 0; r < BY; ++r) {
-                    const float4 v = tile[(g + c) * kQuad + sr0 + r];
-                    W[c * BY + r] = make_float3(v.x, v.y, v.z);
-                }
-#pragma unroll
-            for (int k = 0; k < kPQ; ++k) {
-                AdjState<BX, BY> S;
-                adj_state(fma(-kt, qx[k], G.FXc), dY0[k], P.zref, G, S);
-                adj_pairs<GEN>(S, W, ar[k], ai[k]);
-            }
+            AdjState<BX, BY> S;
+            adj_state(fma(-kt, qx, G.FXc), dY0, P.zref, G, S);
+            adj_pairs<ORD>(S, sU2, sUm, sChi, g * kStrip + sr0, ar, ai);
         }
     }
 }
 
 template <int BX, int BY>
-__global__ __launch_bounds__(kThreads, 3) void adj_fp32c(
+__global__ __launch_bounds__(kThreads) void adj_fp32c(
     const double* __restrict__ px, const double* __restrict__ py, int npil,
-    const DevPlane* __restrict__ planes, const AdjQuad* __restrict__ quads,
-    const AdjBlock* __restrict__ blocks, const float4* __restrict__ pix,
-    float2* __restrict__ partial) {
-    __shared__ float4 tile[kThreads];
+    const DevPlane* __restrict__ planes, const AdjStrip* __restrict__ strips,
+    const AdjSeg* __restrict__ segs, const int32_t* __restrict__ wg_seg,
+    const float4* __restrict__ pix, float2* __restrict__ partial) {
+    __shared__ float sU2[kAdjCols * kStrip], sUm[kAdjCols * kStrip], sChi[kAdjCols * kStrip];
     __shared__ double2 red[kWaves];
-    const AdjBlock B = blocks[blockIdx.x];
     const int tid = threadIdx.x;
-    double qx[kPQ], qy[kPQ];
-    double mx = 0.0, my = 0.0;
+    const int s0 = wg_seg[blockIdx.x], s1 = wg_seg[blockIdx.x + 1];
+    for (int s = s0; s < s1; ++s) {
+        const AdjSeg S = segs[s];
+        const int q = min(S.q0 + tid, npil - 1);
+        const double qx = px[q], qy = py[q];
+        // The group's own pillar extent decides the series order of each plane.
+        double2 E = make_double2(0.0, 0.0);
+        if constexpr (BX * BY > 1) {
+            double mx = fabs(qx), my = fabs(qy);
 #pragma unroll
-    for (int k = 0; k < kPQ; ++k) {
-        const int q = min(B.q0 + tid + k * kThreads, npil - 1);
-        qx[k] = px[q];
-        qy[k] = py[q];
-        mx = fmax(mx, fabs(qx[k]));
-        my = fmax(my, fabs(qy[k]));
-    }
-    // The work-group's own pillar extent decides the series order (no global pass).
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        mx = fmax(mx, __shfl_xor(mx, o));
-        my = fmax(my, __shfl_xor(my, o));
-    }
-    if ((tid & 63) == 0) red[tid >> 6] = make_double2(mx, my);
-    __syncthreads();
-    double2 E = red[0];
-#pragma unroll
-    for (int w = 1; w < kWaves; ++w) {
-        E.x = fmax(E.x, red[w].x);
-        E.y = fmax(E.y, red[w].y);
-    }
-    // Kahan-compensated running sums, fed by one float sum per LDS tile.
-    float sr[kPQ], si[kPQ], cr[kPQ], ci[kPQ];
-#pragma unroll
-    for (int k = 0; k < kPQ; ++k) sr[k] = si[k] = cr[k] = ci[k] = 0.0f;
-
-    for (int u = B.u0; u < B.u1; ++u) {
-        const AdjQuad Q = quads[u];
-        const DevPlane& P = planes[Q.plane];
-        const bool gen =
-            __builtin_amdgcn_readfirstlane(BX * BY > 1 && !fp32c_first_order(P, E, BX, kQuad));
-        const float4* __restrict__ prow =
-            pix + P.sbase + static_cast<int64_t>(Q.j0 / kQuad) * P.nxp * kQuad;
-        for (int cb = 0; cb < P.nxp; cb += kAdjCols) {
-            const int ncol = min(kAdjCols, P.nxp - cb);
+            for (int o = 32; o > 0; o >>= 1) {
+                mx = fmax(mx, __shfl_xor(mx, o));
+                my = fmax(my, __shfl_xor(my, o));
+            }
+            __syncthreads();  // the previous segment has read red
+            if ((tid & 63) == 0) red[tid >> 6] = make_double2(mx, my);
             __syncthreads();
-            if (tid < ncol * kQuad) tile[tid] = prow[cb * kQuad + tid];
-            __syncthreads();
-            float ar[kPQ], ai[kPQ];
+            E = red[0];
 #pragma unroll
-            for (int k = 0; k < kPQ; ++k) ar[k] = ai[k] = 0.0f;
-            if (gen)
-                adj_tile<1, BX, BY>(tile, ncol, P, Q.j0, cb, qx, qy, ar, ai);
-            else
-                adj_tile<0, BX, BY>(tile, ncol, P, Q.j0, cb, qx, qy, ar, ai);
-#pragma unroll
-            for (int k = 0; k < kPQ; ++k) {
-                const float yr = ar[k] - cr[k];
-                const float tr = sr[k] + yr;
-                cr[k] = (tr - sr[k]) - yr;
-                sr[k] = tr;
-                const float yi = ai[k] - ci[k];
-                const float tq = si[k] + yi;
-                ci[k] = (tq - si[k]) - yi;
-                si[k] = tq;
+            for (int w = 1; w < kWaves; ++w) {
+                E.x = fmax(E.x, red[w].x);
+                E.y = fmax(E.y, red[w].y);
             }
         }
-    }
-#pragma unroll
-    for (int k = 0; k < kPQ; ++k) {
-        const int q = B.q0 + tid + k * kThreads;
-        if (q < npil)
-            partial[B.part + q] =
-                make_float2(static_cast<float>(static_cast<double>(sr[k]) - static_cast<double>(cr[k])),
-                            static_cast<float>(static_cast<double>(si[k]) - static_cast<double>(ci[k])));
+        float sr = 0.0f, si = 0.0f, cr = 0.0f, ci = 0.0f;
+        int prev_plane = -1, order = kP1A1;
+        for (int u = S.u0; u <= S.u1; ++u) {
+            const AdjStrip st = strips[u];
+            const DevPlane& P = planes[st.plane];
+            const int ncols = (P.nx + BX - 1) / BX * BX;
+            const int cs = u == S.u0 ? S.c0 : 0;
+            const int ce = u == S.u1 ? S.c1 : ncols;
+            if (cs >= ce) continue;
+            if constexpr (BX * BY > 1) {
+                if (st.plane != prev_plane) {
+                    // The adjoint plan's limits count rows up to the strips' padded end
+                    // (obl2 = obl1: no kP1A2 in the adjoint).
+                    DevPlane Q = P;
+                    Q.ny = P.nquads * kStrip;
+                    order = __builtin_amdgcn_readfirstlane(fp32c_series_order(Q, E.x, E.y));
+                    prev_plane = st.plane;
+                }
+            }
+            const float4* __restrict__ prow =
+                pix + P.sbase + static_cast<int64_t>(st.j0 / kStrip) * P.nxp * kStrip;
+            for (int cb = cs; cb < ce; cb += kAdjCols) {
+                const int ncol = min(kAdjCols, ce - cb);
+                __syncthreads();
+                for (int k = tid; k < ncol * kStrip; k += kThreads) {
+                    const float4 v = prow[cb * kStrip + k];
+                    sU2[k] = v.x;
+                    sUm[k] = v.y;
+                    sChi[k] = v.z;
+                }
+                __syncthreads();
+                float ar = 0.0f, ai = 0.0f;
+                if (order == kP1A1)
+                    adj_tile<kP1A1, BX, BY>(sU2, sUm, sChi, ncol, P, st.j0, cb, qx, qy, ar, ai);
+                else
+                    adj_tile<kP2A2, BX, BY>(sU2, sUm, sChi, ncol, P, st.j0, cb, qx, qy, ar, ai);
+                const float yr = ar - cr;
+                const float tr = sr + yr;
+                cr = (tr - sr) - yr;
+                sr = tr;
+                const float yi = ai - ci;
+                const float tq = si + yi;
+                ci = (tq - si) - yi;
+                si = tq;
+            }
+        }
+        partial[static_cast<int64_t>(S.slot) * kThreads + tid] =
+            make_float2(static_cast<float>(static_cast<double>(sr) - static_cast<double>(cr)),
+                        static_cast<float>(static_cast<double>(si) - static_cast<double>(ci)));
     }
 }
 
@@ -903,7 +957,7 @@ __global__ __launch_bounds__(kThreads) void adj_fp64(
     if (q < npil) partial[B.part + q] = make_double2(gr, gi);
 }
 
-// Adjoint reduction: g[idx] = sum over slots of partial[slot*n + idx]; kReduceCols
+// Adjoint reduction (FP32 / FP64): g[idx] = sum over slots of partial[slot*n + idx]; kReduceCols
 // outputs per work-group, slot quarters per wave, added in order as in fwd_reduce.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void adj_reduce(const T* __restrict__ partial,
@@ -932,6 +986,28 @@ __global__ __launch_bounds__(kThreads) void adj_reduce(const T* __restrict__ par
     for (int w = 0; w < kWaves; ++w) {
         gr += red[w][lane].x;
         gi += red[w][lane].y;
+    }
+    g[idx] = make_double2(gr, gi);
+}
+
+// Adjoint reduction, FP32C: g[s npil + q] = sum over the slots listed for q's group
+// (slot_off[s ngroups + grp] .. [+1]) of partial[slot * kThreads + q - grp kThreads],
+// in list order (FP64).
+__global__ __launch_bounds__(kThreads) void adj_reduce_fp32c(
+    const float2* __restrict__ partial, const int32_t* __restrict__ slot_off,
+    const int32_t* __restrict__ slots, int npil, int ngroups, int64_t n,
+    double2* __restrict__ g) {
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (idx >= n) return;
+    const int s = static_cast<int>(idx / npil);
+    const int q = static_cast<int>(idx - static_cast<int64_t>(s) * npil);
+    const int grp = q / kThreads, lane = q - grp * kThreads;
+    const int k0 = slot_off[s * ngroups + grp], k1 = slot_off[s * ngroups + grp + 1];
+    double gr = 0.0, gi = 0.0;
+    for (int k = k0; k < k1; ++k) {
+        const float2 v = partial[static_cast<int64_t>(slots[k]) * kThreads + lane];
+        gr += v.x;
+        gi += v.y;
     }
     g[idx] = make_double2(gr, gi);
 }
@@ -1001,6 +1077,18 @@ __global__ __launch_bounds__(kThreads) void mtf_finish(const double2* __restrict
 
 namespace {
 inline unsigned grid_for(int64_t n) { return static_cast<unsigned>((n + kThreads - 1) / kThreads); }
+
+template <int KBX, int KBY>
+void launch_fwd_shape(const Fp32cFwdArgs& a, hipStream_t s) {
+    fwd_fp32c<KBX, KBY><<<a.nwg, kThreads, 0, s>>>(a.pos, a.ap, a.ext, a.npil, a.planes, a.tiles,
+                                                   a.segs, a.wg_seg, a.partial, a.slot_elems);
+}
+
+template <int BX, int BY>
+void launch_adj_shape(const Fp32cAdjArgs& a, hipStream_t s) {
+    adj_fp32c<BX, BY><<<a.nwg, kThreads, 0, s>>>(a.px, a.py, a.npil, a.planes, a.strips, a.segs,
+                                                 a.wg_seg, a.pix, a.partial);
+}
 }  // namespace
 
 hipError_t launch_stage_pillars(int mode, const double* px, const double* py, const double* t,
@@ -1012,7 +1100,7 @@ hipError_t launch_stage_pillars(int mode, const double* px, const double* py, co
         unsigned char* base = static_cast<unsigned char*>(out);
         stage_pillars_fp32c<<<grid_for(total), kThreads, 0, s>>>(
             px, py, tt, npil, total, reinterpret_cast<double2*>(base + L.pos),
-            reinterpret_cast<float2*>(base + L.ap));
+            reinterpret_cast<float2*>(base + L.ap), reinterpret_cast<double2*>(base + L.ext));
     } else if (mode == 2) {
         stage_pillars_fp32<<<grid_for(total), kThreads, 0, s>>>(px, py, tt, npil, total,
                                                                  static_cast<float4*>(out));
@@ -1023,23 +1111,93 @@ hipError_t launch_stage_pillars(int mode, const double* px, const double* py, co
     return hipGetLastError();
 }
 
-hipError_t launch_forward(int mode, int order, const void* pil, int npil, int nsets,
-                          const DevPlane* planes, const FwdBlock* blocks, int nblocks,
-                          void* partial_v, hipStream_t s) {
+// FP32C forward pair kernel of block shape bx x by: 4 x 4, 4 x 8, 8 x 4, or 1 x 1
+// (exact planes only).
+hipError_t launch_forward_fp32c(int bx, int by, const Fp32cFwdArgs& a, hipStream_t s) {
+    if (a.nwg <= 0) return hipSuccess;
+    if (bx == 4 && by == 4) launch_fwd_shape<4, 4>(a, s);
+    else if (bx == 4 && by == 8) launch_fwd_shape<4, 8>(a, s);
+    else if (bx == 8 && by == 4) launch_fwd_shape<8, 4>(a, s);
+    else if (bx == 1 && by == 1) launch_fwd_shape<1, 1>(a, s);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_fwd_reduce_fp32c(const DevPlane* planes, const FwdTile* tiles,
+                                   const FwdRed* reds, int nred, const float2* partial,
+                                   int64_t slot_elems, double2* field, double* intensity,
+                                   hipStream_t s) {
+    if (nred <= 0) return hipSuccess;
+    fwd_reduce_fp32c<<<nred, kThreads, 0, s>>>(planes, tiles, reds, partial, slot_elems, field,
+                                               intensity);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_fp32c(int bx, int by, const Fp32cAdjArgs& a, hipStream_t s) {
+    if (a.nwg <= 0) return hipSuccess;
+    if (bx == 4 && by == 4) launch_adj_shape<4, 4>(a, s);
+    else if (bx == 4 && by == 8) launch_adj_shape<4, 8>(a, s);
+    else if (bx == 8 && by == 4) launch_adj_shape<8, 4>(a, s);
+    else if (bx == 8 && by == 8) launch_adj_shape<8, 8>(a, s);
+    else if (bx == 1 && by == 1) launch_adj_shape<1, 1>(a, s);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_reduce_fp32c(const float2* partial, const int32_t* slot_off,
+                                   const int32_t* slots, int npil, int ngroups, int64_t n,
+                                   double2* g, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    adj_reduce_fp32c<<<grid_for(n), kThreads, 0, s>>>(partial, slot_off, slots, npil, ngroups, n,
+                                                      g);
+    return hipGetLastError();
+}
+
+// Work-groups of the FP32C pair kernel of this shape that one CU holds at once (0 if
+// unknown): the stream-K planner sizes its launches from it.
+int fp32c_resident_per_cu(bool adjoint, int bx, int by) {
+    int n = 0;
+    hipError_t e = hipErrorInvalidValue;
+    auto occ = [&](const void* fn) { e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kThreads, 0); };
+    if (!adjoint) {
+        if (bx == 4 && by == 4) occ(reinterpret_cast<const void*>(fwd_fp32c<4, 4>));
+        else if (bx == 4 && by == 8) occ(reinterpret_cast<const void*>(fwd_fp32c<4, 8>));
+        else if (bx == 8 && by == 4) occ(reinterpret_cast<const void*>(fwd_fp32c<8, 4>));
+        else occ(reinterpret_cast<const void*>(fwd_fp32c<1, 1>));
+    } else {
+        if (bx == 4 && by == 4) occ(reinterpret_cast<const void*>(adj_fp32c<4, 4>));
+        else if (bx == 4 && by == 8) occ(reinterpret_cast<const void*>(adj_fp32c<4, 8>));
+        else if (bx == 8 && by == 4) occ(reinterpret_cast<const void*>(adj_fp32c<8, 4>));
+        else if (bx == 8 && by == 8) occ(reinterpret_cast<const void*>(adj_fp32c<8, 8>));
+        else occ(reinterpret_cast<const void*>(adj_fp32c<1, 1>));
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+// Profiler names of the FP32C pair kernels.
+const char* fp32c_kernel_name(bool adjoint, int bx, int by) {
+    static const char* fwd[] = {"crs::fwd_fp32c<4, 4>", "crs::fwd_fp32c<4, 8>",
+                                "crs::fwd_fp32c<8, 4>", "crs::fwd_fp32c<8, 8>",
+                                "crs::fwd_fp32c<1, 1>"};  // <8, 8>: adjoint only
+    static const char* adj[] = {"crs::adj_fp32c<4, 4>", "crs::adj_fp32c<4, 8>",
+                                "crs::adj_fp32c<8, 4>", "crs::adj_fp32c<8, 8>",
+                                "crs::adj_fp32c<1, 1>"};
+    int i = 4;
+    if (bx == 4 && by == 4) i = 0;
+    else if (bx == 4 && by == 8) i = 1;
+    else if (bx == 8 && by == 4) i = 2;
+    else if (bx == 8 && by == 8) i = 3;
+    return adjoint ? adj[i] : fwd[i];
+}
+
+hipError_t launch_forward(int mode, const void* pil, int npil, const DevPlane* planes,
+                          const FwdBlock* blocks, int nblocks, void* partial_v, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
-    if (mode == 0) {
-        float2* partial = static_cast<float2*>(partial_v);
-        const StagedLayout L = fp32c_staged_layout(npil, nsets);
-        const unsigned char* base = static_cast<const unsigned char*>(pil);
-        const double2* pos = reinterpret_cast<const double2*>(base + L.pos);
-        const float2* ap = reinterpret_cast<const float2*>(base + L.ap);
-        // order kExactOrder: 1 x 1 blocks (every pair at its own reference distance);
-        // otherwise 4 x 4 blocks, series order chosen per pillar tile from its extent.
-        if (order == kExactOrder)
-            fwd_fp32c<1><<<nblocks, kThreads, 0, s>>>(pos, ap, npil, planes, blocks, partial);
-        else
-            fwd_fp32c<kBlk><<<nblocks, kThreads, 0, s>>>(pos, ap, npil, planes, blocks, partial);
-    } else if (mode == 2) {
+    if (mode == 2) {
         fwd_fp32<<<nblocks, kThreads, 0, s>>>(static_cast<const float4*>(pil), npil, planes,
                                               blocks, static_cast<double2*>(partial_v));
     } else {
@@ -1049,16 +1207,12 @@ hipError_t launch_forward(int mode, int order, const void* pil, int npil, int ns
     return hipGetLastError();
 }
 
-hipError_t launch_fwd_reduce(int mode, const DevPlane* planes, const PixBlock* pblocks,
-                             int nblocks, const void* partial, double2* field,
-                             double* intensity, hipStream_t s) {
+hipError_t launch_fwd_reduce(const DevPlane* planes, const PixBlock* pblocks, int nblocks,
+                             const void* partial, double2* field, double* intensity,
+                             hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
-    if (mode == 0)  // FP32C: float partials; the reduction restores exp(2 pi i z'_ref)
-        fwd_reduce<float2><<<nblocks, kThreads, 0, s>>>(
-            planes, pblocks, static_cast<const float2*>(partial), 1, field, intensity);
-    else
-        fwd_reduce<double2><<<nblocks, kThreads, 0, s>>>(
-            planes, pblocks, static_cast<const double2*>(partial), 0, field, intensity);
+    fwd_reduce<double2><<<nblocks, kThreads, 0, s>>>(
+        planes, pblocks, static_cast<const double2*>(partial), 0, field, intensity);
     return hipGetLastError();
 }
 
@@ -1069,41 +1223,28 @@ hipError_t launch_stage_pixels(const DevPlane* planes, const PixBlock* pblocks, 
     return hipGetLastError();
 }
 
-hipError_t launch_adjoint(int mode, int order, const void* pil, int npil,
-                          const DevPlane* planes, const AdjQuad* quads, const AdjBlock* blocks,
-                          int nblocks, const void* pix_or_u, void* partial_v, hipStream_t s) {
+hipError_t launch_adjoint(int mode, const void* pil, int npil, const DevPlane* planes,
+                          const AdjQuad* quads, const AdjBlock* blocks, int nblocks,
+                          const void* u, void* partial_v, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
-    if (mode == 0) {
-        float2* partial = static_cast<float2*>(partial_v);
-        // FP32C reads the caller's positions directly: pil = {px, py}.
-        const double* const* xy = static_cast<const double* const*>(pil);
-        const float4* pix = static_cast<const float4*>(pix_or_u);
-        if (order == kExactOrder)
-            adj_fp32c<1, 1><<<nblocks, kThreads, 0, s>>>(xy[0], xy[1], npil, planes, quads, blocks, pix, partial);
-        else
-            adj_fp32c<kBlk, kQuad><<<nblocks, kThreads, 0, s>>>(xy[0], xy[1], npil, planes, quads, blocks, pix, partial);
-    } else if (mode == 2) {
+    if (mode == 2) {
         adj_fp32<<<nblocks, kThreads, 0, s>>>(static_cast<const float4*>(pil), npil, planes, quads,
-                                              blocks, static_cast<const double2*>(pix_or_u),
+                                              blocks, static_cast<const double2*>(u),
                                               static_cast<double2*>(partial_v));
     } else {
         adj_fp64<<<nblocks, kThreads, 0, s>>>(static_cast<const double4*>(pil), npil, planes, quads,
-                                              blocks, static_cast<const double2*>(pix_or_u),
+                                              blocks, static_cast<const double2*>(u),
                                               static_cast<double2*>(partial_v));
     }
     return hipGetLastError();
 }
 
-hipError_t launch_adj_reduce(int mode, const void* partial, int nslots, int64_t n, double2* g,
+hipError_t launch_adj_reduce(const void* partial, int nslots, int64_t n, double2* g,
                              hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const unsigned grid = static_cast<unsigned>((n + kReduceCols - 1) / kReduceCols);
-    if (mode == 0)
-        adj_reduce<float2><<<grid, kThreads, 0, s>>>(static_cast<const float2*>(partial), nslots,
-                                                     n, g);
-    else
-        adj_reduce<double2><<<grid, kThreads, 0, s>>>(static_cast<const double2*>(partial),
-                                                      nslots, n, g);
+    adj_reduce<double2><<<grid, kThreads, 0, s>>>(static_cast<const double2*>(partial), nslots, n,
+                                                  g);
     return hipGetLastError();
 }
 

@@ -216,6 +216,12 @@ class Context:
         _lib.check(_lib.load().crs_set_tuning(self._h, _mode(mode), int(fwd_target_blocks),
                                               int(adj_target_blocks)), "crs_set_tuning")
 
+    def set_block_shape(self, fwd=(0, 0), adj=(0, 0)) -> None:
+        """FP32C pixels per forward lane and per adjoint block, (columns, rows); (0, 0)
+        restores the built-in 4 x 8."""
+        _lib.check(_lib.load().crs_set_block_shape(self._h, int(fwd[0]), int(fwd[1]), int(adj[0]),
+                                                   int(adj[1])), "crs_set_block_shape")
+
     def set_timing(self, enable: bool = True) -> None:
         _lib.check(_lib.load().crs_set_timing(self._h, int(bool(enable))), "crs_set_timing")
 

@@ -1,23 +1,32 @@
-"""Autotuner for the launch split of the forward and adjoint sums.
+"""Autotuner for the FP32C block shape and the launch size of the forward and adjoint sums.
 
-The forward splits its pillar sum into chunks and the adjoint its pixel sum into quad
-runs so that a launch has enough work-groups for the 256 CUs; too few leaves CUs
-idle in the last wave, too many adds partial sums (more traffic for the reduction
-kernel) and per-block setup.  The best count depends on the shape (pillars, pixels
-per plane, planes, mode), so it is measured: each candidate's whole call (staging,
-pair kernel, reduction) is timed with stream events over back-to-back calls, and the
-fastest is installed on the context (crs_set_tuning) and remembered in a JSON cache
-keyed by the shape.
+Two knobs decide how a call maps onto the 256 CUs:
+
+* the block shape (FP32C only): pixels per forward lane (4 x 4, 4 x 8 or 8 x 4) and per
+  adjoint block (4 or 8 each way).  Larger blocks spread each pillar's FP64 distance
+  work over more pairs but hold more accumulators (the 4 x 8 forward runs 2 waves per
+  SIMD instead of 4); a plane whose pair-error bound does not admit a shape falls back to
+  4 x 4, then to pair-by-pair evaluation, so every candidate is exact to the same bound;
+* the work-group count: the pieces a call's work is cut into (FP32C: equal-cost pieces;
+  FP32 / FP64: pillar chunks or pixel-row runs).  Few pieces write few partial sums;
+  more pieces let the CUs that finish early take more work.
+
+Each candidate's whole call (staging, pair kernels, reduction) is timed with stream
+events over back-to-back calls, the fastest is installed on the context
+(crs_set_block_shape, crs_set_tuning) and remembered in a JSON cache keyed by the shape
+of the problem.
 """
 from __future__ import annotations
 
 import json
 import os
-from typing import Dict, Optional, Sequence
+from typing import Dict, Sequence
 
 from .rs import Context, Plane
 
-CANDIDATES = (1024, 2048, 3072, 4096, 6144, 8192, 12288, 16384, 24576)
+TARGETS = (0, 2048, 4096, 8192, 16384)  # 0: the library's built-in size
+FWD_SHAPES = ((4, 4), (4, 8), (8, 4))
+ADJ_SHAPES = ((4, 4), (4, 8), (8, 4), (8, 8))
 CACHE = os.environ.get("CELERIS_RS_TUNE_CACHE",
                        os.path.join(os.path.expanduser("~"), ".cache", "celeris_rs_tune.json"))
 
@@ -46,12 +55,19 @@ def _store_cache(cache: Dict[str, dict]) -> None:
         pass
 
 
+def _install(ctx: Context, mode: str, best: dict) -> None:
+    if mode == "fp32c":
+        ctx.set_block_shape(fwd=tuple(best["fwd_shape"]), adj=tuple(best.get("adj_shape", (0, 0))))
+    ctx.set_tuning(mode, best["fwd"], best.get("adj", 0))
+
+
 def autotune(ctx: Context, px, py, t, planes: Sequence[Plane], u=None, mode: str = "fp32c",
-             candidates: Sequence[int] = CANDIDATES, reps: int = 10,
-             use_cache: bool = True) -> dict:
-    """Time the forward (and, if a cotangent u is given, the adjoint) of this shape for
-    each candidate work-group target, install the fastest on ctx, and return
-    {"fwd": target, "adj": target, "fwd_ms": {...}, "adj_ms": {...}}."""
+             targets: Sequence[int] = TARGETS, reps: int = 10, use_cache: bool = True) -> dict:
+    """Time the forward (and, if a cotangent u is given, the adjoint) of this problem for
+    each candidate (block shape x work-group target; targets only for FP32 / FP64),
+    install the fastest on ctx and return {"fwd": target, "fwd_shape": (bx, by),
+    "adj": target, "adj_shape": (bx, by), "fwd_ms": {...}, "adj_ms": {...}}; the timing
+    tables are keyed "BXxBY/target" and name the kernel that ran."""
     import torch
 
     npil = px.numel()
@@ -61,9 +77,8 @@ def autotune(ctx: Context, px, py, t, planes: Sequence[Plane], u=None, mode: str
     cache = _load_cache() if use_cache else {}
     if use_cache and key in cache:
         best = cache[key]
-        ctx.set_tuning(mode, best["fwd"], best.get("adj", 0))
+        _install(ctx, mode, best)
         return best
-    fwd_ms, adj_ms = {}, {}
 
     def timed(fn):
         fn()  # plan
@@ -77,24 +92,40 @@ def autotune(ctx: Context, px, py, t, planes: Sequence[Plane], u=None, mode: str
         b.synchronize()
         return a.elapsed_time(b) / reps
 
+    fwd_shapes = FWD_SHAPES if mode == "fp32c" else ((0, 0),)
+    adj_shapes = ADJ_SHAPES if mode == "fp32c" else ((0, 0),)
     # The clocks ramp up over the first few milliseconds of work; without a warm-up the
     # first candidate reads several percent slow.
-    ctx.set_tuning(mode, candidates[0], candidates[0])
+    ctx.set_tuning(mode, 0, 0)
     for _ in range(20):
         ctx.forward(px, py, t, planes, mode=mode)
     torch.cuda.synchronize()
-    for cand in candidates:
-        ctx.set_tuning(mode, cand, cand)
-        fwd_ms[cand] = timed(lambda: ctx.forward(px, py, t, planes, mode=mode))
-        if u is not None:
-            adj_ms[cand] = timed(lambda: ctx.adjoint(px, py, u, planes, nsets=nsets, mode=mode))
-    best = {"fwd": min(fwd_ms, key=fwd_ms.get), "fwd_ms": fwd_ms}
+    fwd_ms, adj_ms = {}, {}
+    for shape in fwd_shapes:
+        for target in targets:
+            if mode == "fp32c":
+                ctx.set_block_shape(fwd=shape, adj=(0, 0))
+            ctx.set_tuning(mode, target, 0)
+            ms = timed(lambda: ctx.forward(px, py, t, planes, mode=mode))
+            fwd_ms[f"{shape[0]}x{shape[1]}/{target}"] = {"ms": ms, "kernel": ctx.last_launch(False)[0],
+                                                          "shape": shape, "target": target}
+    if u is not None:
+        for shape in adj_shapes:
+            for target in targets:
+                if mode == "fp32c":
+                    ctx.set_block_shape(fwd=(0, 0), adj=shape)
+                ctx.set_tuning(mode, 0, target)
+                ms = timed(lambda: ctx.adjoint(px, py, u, planes, nsets=nsets, mode=mode))
+                adj_ms[f"{shape[0]}x{shape[1]}/{target}"] = {"ms": ms, "kernel": ctx.last_launch(True)[0],
+                                                              "shape": shape, "target": target}
+    bf = min(fwd_ms.values(), key=lambda r: r["ms"])
+    best = {"fwd": bf["target"], "fwd_shape": list(bf["shape"]), "fwd_ms": fwd_ms}
     if adj_ms:
-        best["adj"] = min(adj_ms, key=adj_ms.get)
-        best["adj_ms"] = adj_ms
-    ctx.set_tuning(mode, best["fwd"], best.get("adj", 0))
+        ba = min(adj_ms.values(), key=lambda r: r["ms"])
+        best.update({"adj": ba["target"], "adj_shape": list(ba["shape"]), "adj_ms": adj_ms})
+    _install(ctx, mode, best)
     if use_cache:
         cache = _load_cache()
-        cache[key] = {k: v for k, v in best.items()}
+        cache[key] = best
         _store_cache(cache)
     return best

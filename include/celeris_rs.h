@@ -45,13 +45,12 @@ typedef enum crs_status {
 typedef enum crs_mode {
     /* FP32 pair arithmetic; each pixel block's phase reference k*(R0 - z) is formed
      * and reduced mod 2*pi in FP64, and each pixel's offset from it is expanded to the
-     * order that keeps every pair within 1e-6 rad of phase and 4e-6 of amplitude; a
-     * plane too close or too coarsely sampled for that is evaluated pair by pair, each
-     * pair at its own FP64 distance (decided per plane).  Forward: float sums over at
-     * most 64 pillars, folded into float running sums over at most 4,096 pillars per
-     * wave, then FP64 across waves and pillar chunks.  Adjoint: float sums per pixel
-     * block, Kahan-compensated across pixel tiles, then FP64 across runs.  Default and
-     * fastest. */
+     * lowest series order whose worst-case error per pair (truncation, float rounding,
+     * the hardware sin/cos) stays within 8e-6 of |t|/R; a plane too close or too
+     * coarsely sampled for that is evaluated pair by pair, each pair at its own FP64
+     * distance (decided per plane).  Forward: float sums over at most 1,024 pillars per
+     * wave, then FP64.  Adjoint: float sums per pixel block, Kahan-compensated across
+     * pixel tiles, then FP64 across work items.  Default and fastest. */
     CRS_MODE_FP32C = 0,
     /* The reference algorithm in FP64 throughout (sqrt, sincos, 1/R, complex MAC). */
     CRS_MODE_FP64 = 1,
@@ -120,7 +119,10 @@ int crs_set_default_mode(int mode);
 
 /* Create a context bound to HIP device `device`.  A context owns device scratch
  * (staged pillars, split-sum partials, descriptor tables) that grows on demand and
- * is reused; one context must not be used by two host threads at once. */
+ * is reused; one context must not be used by two host threads at once.  Calls on one
+ * context may use different streams: a call on a stream other than the previous
+ * call's first makes its stream wait for the previous call's work (outside stream
+ * capture), and scratch is freed only after that work has finished. */
 crs_status crs_context_create(int device, crs_context **ctx);
 void crs_context_destroy(crs_context *ctx);
 
@@ -308,11 +310,20 @@ crs_status crs_polar_focus_check(const double *px, const double *py, const doubl
                                  double *out4);
 
 /* Launch-shape tuning (no reference counterpart).  target_blocks: work-groups a
- * forward/adjoint launch aims for through its split over pillar chunks (forward)
- * or pixel rows (adjoint); 0 restores the built-in default.  Applies to the
- * context's later calls. */
+ * forward/adjoint launch aims for; 0 restores the built-in default.  FP32C cuts each
+ * call's work into that many equal-cost pieces (default: the pair kernel's resident
+ * work-groups on every CU); FP32 / FP64 split their sums over pillar chunks (forward)
+ * or pixel rows (adjoint) to reach it.  Applies to the context's later calls. */
 crs_status crs_set_tuning(crs_context *ctx, int mode, int fwd_target_blocks,
                           int adj_target_blocks);
+
+/* FP32C block shape (no reference counterpart): pixels per lane of the forward
+ * (fwd_bx columns x fwd_by rows: 4 x 4, 4 x 8 or 8 x 4) and per pixel block of the
+ * adjoint (adj_bx x adj_by, each 4 or 8); 0, 0 restores the built-in 4 x 8.  A plane
+ * whose pair-error bound does not admit the shape falls back to 4 x 4, then to pair by
+ * pair.  CRS_ERR_INVALID for other shapes.  Applies to the context's later calls. */
+crs_status crs_set_block_shape(crs_context *ctx, int fwd_bx, int fwd_by, int adj_bx,
+                               int adj_by);
 
 /* HIP graphs (no reference counterpart).  Capture one crs_forward / crs_adjoint of
  * this exact shape and these exact device pointers into an instantiated graph.  The
@@ -331,8 +342,8 @@ void crs_graph_destroy(crs_graph *graph);
 
 /* Kernel timing (no reference counterpart).  With timing enabled, each crs_forward /
  * crs_adjoint records HIP events on its stream immediately before and after the
- * kernel that does the pair sums (an adjoint that mixes planes evaluated pair by pair
- * with blocked ones runs two such kernels, and the events bracket both);
+ * kernels that do the pair sums (a call whose planes need several block shapes runs
+ * one such kernel per shape, and the events bracket them all);
  * crs_last_kernel_ms waits for the last such event and returns that duration in
  * milliseconds (adjoint = 0: forward). */
 crs_status crs_set_timing(crs_context *ctx, int enable);
@@ -344,8 +355,8 @@ const char *crs_kernel_name(int mode, int adjoint);
 
 /* The pair-sum kernel the last successful crs_forward (adjoint = 0) or crs_adjoint
  * (adjoint = 1) on ctx launched: its name as a profiler lists it (demangled, without
- * the argument list, e.g. "crs::fwd_fp32c<4>") and its number of work-groups (for a
- * mixed adjoint, the first of its two launches).
+ * the argument list, e.g. "crs::fwd_fp32c<4, 8>") and its number of work-groups (for a
+ * call whose planes need several block shapes, the first of its launches).
  * CRS_ERR_INVALID before the first such call. */
 crs_status crs_last_launch(crs_context *ctx, int adjoint, const char **kernel,
                            int *workgroups);

@@ -127,10 +127,19 @@ def test_autograd_fd_encircled_energy(torch_cuda):
 
 
 def test_config4_gradient_fp32c_vs_fp64(torch_cuda):
-    """Config 4: -EE within 1.22 lambda/NA at z = f, 91,809 pillars, 161^2 grid; the
-    FP32C adjoint gradient against the FP64 one, and against FD on 16 seeded pillars
-    (FD exploits linearity: E(phi_q +- h) = E + t_q (e^{+-ih} - 1) G_q)."""
+    """Config 4: -EE within 1.22 lambda/NA at z = f, 91,809 pillars, 161^2 grid.  The
+    FP32C gradient with respect to every pillar's phase and amplitude (adjoint kernel
+    through the autograd Function) against
+      * the FP64 GPU gradient, over all pillars;
+      * central finite differences, h = 1e-4 rad (phase) and 1e-4 (amplitude), on 64
+        seeded pillars, for both (FD exploits linearity: E(t_q + dt) = E + dt G_q with
+        G_q the FP64 field of pillar q alone);
+      * the FP64 CPU gradient of the oracle's adjoint on those 64 pillars,
+        dL/dphi_q = -Im(conj(g_q) t_q), dL/da_q = Re(conj(g_q) e^{i phi_q}),
+        g_q = sum_p conj(G_pq) u_p with u = dL/dRe E + i dL/dIm E at the FP64 field."""
     torch = torch_cuda
+    import oracle
+    from celeris_amd import Context
     from celeris_amd.autograd import rs_field
 
     L = configs.config2(n=161)
@@ -150,24 +159,46 @@ def test_config4_gradient_fp32c_vs_fp64(torch_cuda):
     print(f"config4 grad relL2 phase {ep:.3e} amp {ea:.3e} loss {grads['fp64'][2]:.6f}")
     assert ep <= 1e-5 and ea <= 1e-5
 
-    # FD on seeded pillars with exact column updates (FP64 forward).
-    from celeris_amd import Context
-
     ctx = Context(0)
     E, _ = ctx.forward(px, py, dev(torch, L.t), [plane], mode="fp64")
-    pick = (configs.uniform(configs.SEED, configs.STREAM_FD_PILLARS, 16) * L.npil).astype(int)
+    pick = np.unique((configs.uniform(configs.SEED, configs.STREAM_FD_PILLARS, 64) * L.npil).astype(int))
+    assert pick.size >= 60
     h = 1e-4
-    gphi = grads["fp64"][0]
+    gphi32, gamp32 = grads["fp32c"][0], grads["fp32c"][1]
+    sp, sa = np.max(np.abs(grads["fp64"][0])), np.max(np.abs(grads["fp64"][1]))
+    one = torch.ones((1, 1), dtype=torch.complex128, device="cuda")
+    worst_p = worst_a = 0.0
     for q in pick:
-        one = torch.ones((1, 1), dtype=torch.complex128, device="cuda")
         Gq, _ = ctx.forward(px[q:q + 1].contiguous(), py[q:q + 1].contiguous(), one, [plane],
                             mode="fp64")
         tq = complex(L.t[0, q])
+        eq = tq / abs(tq)
         lp = float(_ee_loss(torch, E + tq * (np.exp(1j * h) - 1) * Gq, plane, r0))
         lm = float(_ee_loss(torch, E + tq * (np.exp(-1j * h) - 1) * Gq, plane, r0))
-        fd = (lp - lm) / (2 * h)
-        print(q, gphi[q], fd)
-        assert abs(gphi[q] - fd) <= 1e-6 * np.max(np.abs(gphi)) + 1e-9
+        fd_phi = (lp - lm) / (2 * h)
+        lp = float(_ee_loss(torch, E + h * eq * Gq, plane, r0))
+        lm = float(_ee_loss(torch, E - h * eq * Gq, plane, r0))
+        fd_amp = (lp - lm) / (2 * h)
+        worst_p = max(worst_p, abs(gphi32[q] - fd_phi) / sp)
+        worst_a = max(worst_a, abs(gamp32[q] - fd_amp) / sa)
+    print(f"config4 FD on {pick.size} pillars: worst |fp32c - FD| / max|g|: phase {worst_p:.2e}, "
+          f"amplitude {worst_a:.2e}")
+    assert worst_p <= 1e-5 and worst_a <= 1e-5
+
+    # FP64 CPU gradient (the oracle's adjoint) on the same pillars.
+    Ev = E.clone().requires_grad_(True)
+    _ee_loss(torch, Ev, plane, r0).backward()
+    u = Ev.grad.detach().cpu().numpy()
+    g = oracle.adjoint(L.px[pick], L.py[pick], u, [plane])[0]
+    tq = L.t[0, pick]
+    cpu_phi = -np.imag(np.conj(g) * tq)
+    cpu_amp = np.real(np.conj(g) * tq / np.abs(tq))
+    dp = np.max(np.abs(gphi32[pick] - cpu_phi)) / sp
+    da = np.max(np.abs(gamp32[pick] - cpu_amp)) / sa
+    dp64 = np.max(np.abs(grads["fp64"][0][pick] - cpu_phi)) / sp
+    print(f"config4 vs FP64 CPU gradient on {pick.size} pillars: fp32c phase {dp:.2e} amp {da:.2e}; "
+          f"fp64 GPU phase {dp64:.2e}")
+    assert dp <= 1e-5 and da <= 1e-5 and dp64 <= 1e-10
     ctx.close()
 
 
@@ -246,7 +277,8 @@ def test_small_problem_work_splits(torch_cuda, rsctx):
         name, groups = rsctx.last_launch(True)
         fname, fgroups = rsctx.last_launch(False)
         print(f"  {fname} x {fgroups}, {name} x {groups}")
-        assert name == "crs::adj_fp32c<4, 4>" and fname == "crs::fwd_fp32c<4>"
+        assert name.startswith("crs::adj_fp32c<") and fname.startswith("crs::fwd_fp32c<")
+        assert "<1, 1>" not in name and "<1, 1>" not in fname
         out[pq] = (f.cpu().numpy(), g.cpu().numpy(), fgroups)
         assert rel_l2(out[pq][0], ref_f) <= 1e-5 and rel_l2(out[pq][1], ref_g) <= 1e-5
     rsctx.set_tuning("fp32c", 0, 0)

@@ -174,6 +174,44 @@ def test_config2_headline_rows_vs_oracle(torch_cuda, rsctx):
     assert efull <= 1e-5 and eint <= 2e-5
 
 
+@pytest.mark.parametrize("n,stride", [(321, 32), (1024, 128)])
+def test_config2_large_grids_vs_oracle(torch_cuda, rsctx, n, stride):
+    """The headline lens (91,809 pillars) on its other two grids, 321^2 and 1024^2: FP32C
+    and FP64 against the FP64 oracle on every stride-th row, FP32C against FP64 on the
+    whole grid (field and |E|^2), and the FP32C adjoint of a whole-grid cotangent
+    against the FP64 adjoint."""
+    L = configs.config2(n=n)
+    ref = oracle.forward(L.px, L.py, L.t, L.planes, row_stride=stride)
+    rows = np.arange(0, n, stride)
+    mask = np.zeros((n, n), bool)
+    mask[rows] = True
+    mask = mask.ravel()
+    f64, _ = gpu_forward(torch_cuda, rsctx, L.px, L.py, L.t, L.planes, "fp64")
+    f32c, i32c = gpu_forward(torch_cuda, rsctx, L.px, L.py, L.t, L.planes, "fp32c")
+    e64 = rel_l2(f64[mask], ref[mask])
+    e32 = rel_l2(f32c[mask], ref[mask])
+    efull = rel_l2(f32c, f64)
+    eint = rel_l2(i32c, np.abs(f64) ** 2)
+    print(f"config2 {n}: fp64 vs oracle {e64:.3e}; fp32c vs oracle {e32:.3e}; "
+          f"fp32c vs fp64 full {efull:.3e}, |E|^2 {eint:.3e}", flush=True)
+    assert e64 <= 1e-12
+    assert e32 <= 1e-5
+    assert efull <= 1e-5 and eint <= 2e-5
+    rng = np.random.default_rng(n)
+    u = rng.standard_normal(n * n) + 1j * rng.standard_normal(n * n)
+    du = dev(torch_cuda, u)
+    dpx, dpy = dev(torch_cuda, L.px), dev(torch_cuda, L.py)
+    g32 = rsctx.adjoint(dpx, dpy, du, L.planes, nsets=1, mode="fp32c").cpu().numpy()
+    g64 = rsctx.adjoint(dpx, dpy, du, L.planes, nsets=1, mode="fp64").cpu().numpy()
+    eg = rel_l2(g32, g64)
+    # the FP64 adjoint against the oracle on a pillar sample
+    pick = np.arange(0, L.npil, L.npil // 48)
+    gref = oracle.adjoint(L.px[pick], L.py[pick], u, L.planes)
+    eg64 = rel_l2(g64[0, pick], gref[0])
+    print(f"config2 {n}: adjoint fp32c vs fp64 {eg:.3e}; fp64 vs oracle (sample) {eg64:.3e}")
+    assert eg <= 1e-5 and eg64 <= 1e-12
+
+
 def test_dropin_psf_matches_oracle(torch_cuda):
     L = configs.config1()
     p = L.planes[0]
@@ -264,7 +302,7 @@ def test_mixed_exact_and_blocked_planes(torch_cuda, rsctx):
               L.planes[0]]
     ref = oracle.forward(L.px, L.py, L.t, planes)
     f, _ = gpu_forward(torch_cuda, rsctx, L.px, L.py, L.t, planes, "fp32c")
-    assert rsctx.last_launch(False)[0] == "crs::fwd_fp32c<4>"
+    assert rsctx.last_launch(False)[0] == "crs::fwd_fp32c<4, 4>"  # first of its launches
     for p in crs.resolve_offsets(planes):
         sl = slice(p.offset, p.offset + p.npix)
         e = rel_l2(f[sl], ref[sl])
@@ -275,36 +313,54 @@ def test_mixed_exact_and_blocked_planes(torch_cuda, rsctx):
     gref = oracle.adjoint(L.px, L.py, u, planes)
     g = rsctx.adjoint(dev(torch_cuda, L.px), dev(torch_cuda, L.py), dev(torch_cuda, u), planes,
                       nsets=1, mode="fp32c")
-    assert rsctx.last_launch(True)[0] == "crs::adj_fp32c<4, 4>"
+    assert rsctx.last_launch(True)[0] == "crs::adj_fp32c<4, 8>"
     eg = rel_l2(g.cpu().numpy(), gref)
     print(f"mixed exact/blocked adjoint relL2 {eg:.3e}")
     assert eg <= 1e-5
     # The near plane alone still takes the 1 x 1 kernels.
     gpu_forward(torch_cuda, rsctx, L.px, L.py, L.t, planes[1:2], "fp32c")
-    assert rsctx.last_launch(False)[0] == "crs::fwd_fp32c<1>"
+    assert rsctx.last_launch(False)[0] == "crs::fwd_fp32c<1, 1>"
 
 
-@pytest.mark.parametrize("mode,tol", [("fp64", 1e-12), ("fp32c", 1e-5)])
-def test_forward_ragged_edges_blocked(torch_cuda, rsctx, mode, tol):
-    """4 x 4 blocks with every kind of leftover: nx, ny = 4k + 1, 2, 3 (the 1 x 4, 4 x 1
-    and 1 x 1 edge tiles), rectangular planes, a plane narrower than one block."""
+@pytest.mark.parametrize("mode,tol,shape,ashape", [
+    ("fp64", 1e-12, None, None), ("fp32c", 1e-5, (4, 4), (4, 4)), ("fp32c", 1e-5, (4, 8), (4, 8)),
+    ("fp32c", 1e-5, (8, 4), (8, 4)), ("fp32c", 1e-5, (4, 4), (8, 8))])
+def test_forward_ragged_edges_blocked(torch_cuda, rsctx, mode, tol, shape, ashape):
+    """Blocks of every forward shape with every kind of leftover: nx, ny = 4k + 1, 2, 3 and
+    8k + 1 ... 7 (the 1 x BY, BX x 1 and 1 x 1 edge tiles), rectangular planes, planes
+    narrower or shorter than one block."""
     L = configs.config1()
     k = L.planes[0].k
     planes = [
-        Plane.psf(0.0, 0.0, 50.0, k, 42, 1.0),                           # 42 = 4*10 + 2
-        Plane(k, -1.1, 0.05, -0.9, 0.05, 50.0, 0.0, 43, 41),             # 43 = 4*10 + 3, 41 = 4*10 + 1
+        Plane.psf(0.0, 0.0, 50.0, k, 42, 1.0),                           # 42 = 4*10 + 2 = 8*5 + 2
+        Plane(k, -1.1, 0.05, -0.9, 0.05, 50.0, 0.0, 43, 41),             # 43 = 8*5 + 3, 41 = 8*5 + 1
         Plane(k, 0.3, 0.05, -0.5, 0.05, 49.0, 0.0, 3, 22),               # narrower than a block
         Plane(k, -0.4, 0.04, 0.2, 0.04, 51.0, 0.0, 17, 2),               # shorter than a block
+        Plane(k, -0.6, 0.03, -0.3, 0.03, 50.5, 0.0, 15, 23),             # 15 = 8 + 7, 23 = 16 + 7
     ]
     ref = oracle.forward(L.px, L.py, L.t, planes)
-    f, _ = gpu_forward(torch_cuda, rsctx, L.px, L.py, L.t, planes, mode)
-    for p in crs.resolve_offsets(planes):
-        sl = slice(p.offset, p.offset + p.npix)
-        e = rel_l2(f[sl], ref[sl])
-        print(f"ragged {mode} {p.nx}x{p.ny} relL2 {e:.3e}")
-        assert e <= tol
-    if mode == "fp32c":
-        assert rsctx.last_launch(False)[0] == "crs::fwd_fp32c<4>"
+    if shape:
+        rsctx.set_block_shape(fwd=shape, adj=ashape)
+    try:
+        f, _ = gpu_forward(torch_cuda, rsctx, L.px, L.py, L.t, planes, mode)
+        for p in crs.resolve_offsets(planes):
+            sl = slice(p.offset, p.offset + p.npix)
+            e = rel_l2(f[sl], ref[sl])
+            print(f"ragged {mode} {shape} {p.nx}x{p.ny} relL2 {e:.3e}")
+            assert e <= tol
+        if mode == "fp32c":
+            assert rsctx.last_launch(False)[0] == f"crs::fwd_fp32c<{shape[0]}, {shape[1]}>"
+            rng = np.random.default_rng(5)
+            u = rng.standard_normal(ref.size) + 1j * rng.standard_normal(ref.size)
+            gref = oracle.adjoint(L.px, L.py, u, planes)
+            g = rsctx.adjoint(dev(torch_cuda, L.px), dev(torch_cuda, L.py), dev(torch_cuda, u),
+                              planes, nsets=1, mode="fp32c")
+            assert rsctx.last_launch(True)[0] == f"crs::adj_fp32c<{ashape[0]}, {ashape[1]}>"
+            eg = rel_l2(g.cpu().numpy(), gref)
+            print(f"ragged adjoint {ashape} relL2 {eg:.3e}")
+            assert eg <= tol
+    finally:
+        rsctx.set_block_shape()
 
 
 @pytest.mark.parametrize("mode,tol", [("fp64", 1e-12), ("fp32c", 1e-5)])

@@ -12,8 +12,10 @@ import os
 import numpy as np
 import pytest
 
+import oracle
 from celeris_amd import Context, Plane, configs
 from celeris_amd import rs as crs
+from conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -185,8 +187,60 @@ def test_tuning_timing_and_launch_info(torch_cuda, tmp_path, monkeypatch):
 
     monkeypatch.setattr(tune, "CACHE", str(tmp_path / "tune.json"))
     u = dev(torch, np.ones(sum(p.npix for p in planes), np.complex128))
-    best = tune.autotune(ctx, dpx, dpy, dt, planes, u, mode="fp32c", candidates=(512, 2048), reps=1)
+    best = tune.autotune(ctx, dpx, dpy, dt, planes, u, mode="fp32c", targets=(512, 2048), reps=1)
     assert best["fwd"] in (512, 2048) and best["adj"] in (512, 2048)
-    again = tune.autotune(ctx, dpx, dpy, dt, planes, u, mode="fp32c", candidates=(512, 2048), reps=1)
+    assert tuple(best["fwd_shape"]) in tune.FWD_SHAPES and tuple(best["adj_shape"]) in tune.ADJ_SHAPES
+    # every candidate was timed, and names the kernel that ran
+    assert len(best["fwd_ms"]) == 2 * len(tune.FWD_SHAPES) and len(best["adj_ms"]) == 2 * len(tune.ADJ_SHAPES)
+    assert all(r["kernel"].startswith("crs::fwd_fp32c<") for r in best["fwd_ms"].values())
+    again = tune.autotune(ctx, dpx, dpy, dt, planes, u, mode="fp32c", targets=(512, 2048), reps=1)
     assert again["fwd"] == best["fwd"] and again["adj"] == best["adj"]  # from the cache
+    assert again["fwd_shape"] == best["fwd_shape"] and again["adj_shape"] == best["adj_shape"]
+    # the installed choice gives the same field as the built-in one, to roundoff
+    f_tuned, _ = ctx.forward(dpx, dpy, dt, planes, mode="fp32c")
+    ctx.set_block_shape()
+    ctx.set_tuning("fp32c", 0, 0)
+    f_def, _ = ctx.forward(dpx, dpy, dt, planes, mode="fp32c")
+    assert torch.linalg.vector_norm(f_tuned - f_def) <= 2e-6 * torch.linalg.vector_norm(f_def)
+    ctx.close()
+
+
+def test_one_context_alternating_streams(torch_cuda):
+    """One context, forward and adjoint calls alternating between two streams with no
+    host synchronisation between them, on shapes that change the plan and grow the
+    scratch from call to call (the context orders each call after the previous one and
+    frees scratch only once that work is done): every result against the oracle."""
+    torch = torch_cuda
+    L = configs.config1()
+    k = L.planes[0].k
+    ctx = Context(0)
+    dpx, dpy, dt = dev(torch, L.px), dev(torch, L.py), dev(torch, L.t)
+    shapes = [[Plane.psf(0.0, 0.0, 50.0, k, 21, 2.0)],
+              [Plane.psf(0.0, 0.0, 50.0, k, 81, 5.0), Plane.psf(1.0, -1.0, 48.0, k, 41, 3.0)],
+              [Plane.psf(0.5, 0.5, 52.0, k, 121, 6.0)]]
+    rng = np.random.default_rng(21)
+    us = [rng.standard_normal(sum(p.npix for p in pl)) + 1j * rng.standard_normal(sum(p.npix for p in pl))
+          for pl in shapes]
+    dus = [dev(torch, u) for u in us]
+    torch.cuda.synchronize()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    out = []
+    for i in range(9):
+        pl = shapes[i % 3]
+        if i % 2 == 0:
+            f, _ = ctx.forward(dpx, dpy, dt, pl, mode="fp32c", stream=s1)
+            out.append(("fwd", i % 3, f))
+        else:
+            g = ctx.adjoint(dpx, dpy, dus[i % 3], pl, nsets=1, mode="fp32c", stream=s2)
+            out.append(("adj", i % 3, g))
+    torch.cuda.synchronize()
+    for kind, si, res in out:
+        pl = shapes[si]
+        if kind == "fwd":
+            ref = oracle.forward(L.px, L.py, L.t, pl)
+        else:
+            ref = oracle.adjoint(L.px, L.py, us[si], pl)
+        e = rel_l2(res.cpu().numpy(), ref)
+        print(f"{kind} shape {si}: relL2 {e:.2e}")
+        assert e <= 1e-5
     ctx.close()
