@@ -84,6 +84,9 @@ SIGNATURES = [
     ("crs_adjoint", c_int,
      [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_int,
       POINTER(crs_plane), c_int, c_void_p, c_void_p]),
+    ("crs_forward_noalloc", c_int,
+     [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_int, POINTER(crs_plane), c_int,
+      c_void_p, c_void_p]),
     ("crs_set_tuning", c_int, [c_void_p, c_int, c_int, c_int]),
     ("crs_set_block_shape", c_int, [c_void_p, c_int, c_int, c_int, c_int]),
     ("crs_graph_forward", c_int,

@@ -57,6 +57,7 @@ hipError_t launch_adj_reduce(const void* partial, int nslots, int64_t n, double2
 hipError_t launch_mtf(const double2* P, const double* P0, int n, int ns, double p, double f,
                       double lam, double2* acc, double* acc0, float* freq, float* mtf,
                       float* ideal, hipStream_t s);
+hipError_t noalloc_scratch(void** ptr);
 }  // namespace crs
 
 using namespace crs;
@@ -1407,6 +1408,118 @@ extern "C" void crs_graph_destroy(crs_graph* g) {
     for (DevBuffer* b : {&g->pil, &g->pix, &g->tables, &g->partial})
         if (b->ptr) hipFree(b->ptr);
     delete g;
+}
+
+// ------------------------------------------------------------------------------------
+// Allocation-free forward: crs_forward with its scratch carved from the static device
+// array `scratch` (rs_kernels.hip) instead of device allocations.
+
+namespace {
+
+size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
+
+// Per-device state of the allocation-free entry point, set up on its first call on a
+// device: the scratch address, the device's CU count, a host staging buffer for the
+// descriptor tables and an event marking when the last upload from it was read.
+struct NoAllocDevice {
+    std::mutex mu;
+    bool ready = false;
+    unsigned char* base = nullptr;
+    int cus = 0;
+    std::vector<unsigned char> host_tables;
+    hipEvent_t uploaded = nullptr;
+    bool pending = false;
+};
+
+NoAllocDevice* noalloc_device(int dev) {
+    static NoAllocDevice devs[64];
+    return dev >= 0 && dev < 64 ? &devs[dev] : nullptr;
+}
+
+}  // namespace
+
+extern "C" crs_status crs_forward_noalloc(void* stream_, int mode, const double* px,
+                                          const double* py, const double* t, int npil,
+                                          int nsets, const crs_plane* planes, int nplanes,
+                                          double* field, double* intensity) {
+    g_last_error.clear();
+    crs_status st = check_args(mode, npil, nsets, planes, nplanes);
+    if (st != CRS_OK) return st;
+    if (!px || !py || !t) { set_error("null pillar array"); return CRS_ERR_INVALID; }
+    if (nplanes == 0 || (!field && !intensity)) return CRS_OK;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        hipGetLastError();
+        set_error("no HIP device");
+        return CRS_ERR_NO_DEVICE;
+    }
+    NoAllocDevice* D = noalloc_device(dev);
+    if (!D) { set_error("device %d outside the supported range", dev); return CRS_ERR_INVALID; }
+    std::lock_guard<std::mutex> lk(D->mu);
+    if (!D->ready) {
+        void* p = nullptr;
+        if (noalloc_scratch(&p) != hipSuccess || !p) {
+            hipGetLastError();
+            set_error("hipGetSymbolAddress of the scratch array failed");
+            return CRS_ERR_LAUNCH;
+        }
+        D->base = static_cast<unsigned char*>(p);
+        if (hipDeviceGetAttribute(&D->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+            hipGetLastError();
+            D->cus = 256;
+        }
+        if (hipEventCreateWithFlags(&D->uploaded, hipEventDisableTiming) != hipSuccess) {
+            hipGetLastError();
+            set_error("hipEventCreate failed");
+            return CRS_ERR_LAUNCH;
+        }
+        D->ready = true;
+    }
+    hipStream_t stream = static_cast<hipStream_t>(stream_);
+    // Plan with the built-in shapes and sizes; if the partial sums do not fit the
+    // scratch, cut the call into fewer, longer pieces (fewer partial slots).
+    crs_context pc;
+    pc.device = dev;
+    pc.cus = D->cus;
+    FwdPlan plan;
+    size_t staged = 0, partial = 0, tables = 0;
+    for (int attempt = 0;; ++attempt) {
+        plan = FwdPlan();
+        if ((st = plan_forward(&pc, mode, npil, nsets, planes, nplanes, plan)) != CRS_OK) return st;
+        staged = align256(staged_pillar_bytes(mode, npil, nsets));
+        const int64_t elems = mode == CRS_MODE_FP32C ? plan.nseg * plan.slot_elems : plan.part_elems;
+        partial = align256(static_cast<size_t>(elems) * partial_bytes(mode));
+        tables = align256(plan.tb.bytes.size());
+        if (staged + partial + tables <= kNoAllocScratchBytes) break;
+        const int groups = mode == CRS_MODE_FP32C ? (plan.launches.empty() ? 0 : plan.launches[0].nwg)
+                                                  : plan.nblocks;
+        if (attempt >= 12 || groups <= 64) {
+            set_error("crs_forward_noalloc: the call needs %zu bytes of scratch, more than the "
+                      "%zu-byte allocation-free maximum (use crs_forward)",
+                      staged + partial + tables, kNoAllocScratchBytes);
+            return CRS_ERR_INVALID;
+        }
+        pc.fwd_target[mode] = std::max(1, groups / 2);
+    }
+    DevBuffer pil_buf{D->base, staged}, part_buf{D->base + staged, partial};
+    unsigned char* d_tables = D->base + staged + partial;
+    // The host staging buffer may still be read by the previous call's upload.
+    if (D->pending) {
+        hipEventSynchronize(D->uploaded);
+        D->pending = false;
+    }
+    D->host_tables.assign(plan.tb.bytes.begin(), plan.tb.bytes.end());
+    if (hipMemcpyAsync(d_tables, D->host_tables.data(), D->host_tables.size(), hipMemcpyHostToDevice,
+                       stream) != hipSuccess) {
+        hipGetLastError();
+        set_error("descriptor tables: hipMemcpyAsync failed");
+        return CRS_ERR_LAUNCH;
+    }
+    if (!stream_capturing(stream) && hipEventRecord(D->uploaded, stream) == hipSuccess)
+        D->pending = true;
+    if (inject("launch")) { set_error("injected launch failure"); return CRS_ERR_LAUNCH; }
+    Scratch sc{&pil_buf, nullptr, &part_buf, d_tables, nullptr};
+    return enqueue_forward(plan, sc, px, py, t, field, intensity, stream, nullptr, nullptr);
 }
 
 // ------------------------------------------------------------------------------------

@@ -42,6 +42,11 @@ constexpr int kWaveFold = 1024;
 // Pillars per FP32C extent record (one per LDS tile of staged pillars).
 constexpr int kExtTile = kThreads;
 
+// Bytes of the file-scope device array the allocation-free forward entry point
+// (crs_forward_noalloc) carves its scratch from: staged pillars, partial sums and
+// descriptor tables of one call.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+
 // Series orders of the FP32C pair evaluation: phase order / amplitude order.
 //   kP1A1: R - R0 = x1 (1 - y),            R0/R = 1 - 2y             6 FP32 ops per pair
 //   kP1A2: R - R0 = x1 (1 - y),            R0/R = 1 - 2y + 6y^2      7

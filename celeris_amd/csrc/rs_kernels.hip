@@ -42,8 +42,36 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
+
+// Scratch of the allocation-free forward entry point (crs_forward_noalloc).  Each
+// device holds its own copy for the life of the process; the host's name for it is a
+// stand-in, so host code uses only the device address noalloc_scratch returns.  Calls
+// that overlap on one device share it and race.
+static __device__ __align__(256) unsigned char scratch[crs::kNoAllocScratchBytes];
 
 namespace crs {
+
+// Device address of the current device's copy of `scratch`, looked up once per device
+// with that device current (hipGetSymbolAddress), then cached.
+hipError_t noalloc_scratch(void** ptr) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void* addr[kMaxDevices] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addr[dev] = p;
+    }
+    *ptr = addr[dev];
+    return hipSuccess;
+}
 
 namespace {
 

@@ -152,6 +152,21 @@ crs_status crs_adjoint(crs_context *ctx, void *stream, int mode,
                        const crs_plane *planes, int nplanes,
                        const double *u, double *g);
 
+/* crs_forward with no device allocation (no reference counterpart; for callers that
+ * cannot allocate device memory: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, results and stream semantics as
+ * crs_forward (built-in block shapes and launch sizes), on the current HIP device and
+ * without a context: staged pillars, partial sums and descriptor tables are carved from
+ * a 64 MiB device array each device holds for the life of the process (set up on the
+ * first call on a device).  A call whose scratch would not fit is cut into fewer work
+ * items; if it still does not fit, CRS_ERR_INVALID (use crs_forward).  The array keeps
+ * the last call's contents and nothing in it needs to start at zero.  Calls that overlap
+ * on one device (on different streams or host threads without ordering) share the array
+ * and race: serialise them. */
+crs_status crs_forward_noalloc(void *stream, int mode, const double *px, const double *py,
+                               const double *t, int npil, int nsets, const crs_plane *planes,
+                               int nplanes, double *field, double *intensity);
+
 /* crs_forward on HOST arrays (no reference counterpart): inputs are copied to the
  * device, the planes run in one launch, and the outputs (field: complex, and/or
  * intensity, total-pixel length) are copied back before the call returns; they are
