@@ -105,6 +105,8 @@ SIGNATURES = [
      [_DP, _DP, _DP, c_int, c_double, c_double, c_double, c_int, c_int, _DP, _DP, _DP, _DP]),
     ("crs_analyze_chromatic", c_int,
      [_DP, _DP, _DP, c_int, c_double, c_double, c_double, c_double, c_int, c_int, _DP]),
+    ("crs_analyze_chromatic_dispersive", c_int,
+     [_DP, _DP, _DP, _DP, c_int, c_double, c_double, c_double, c_double, c_int, c_int, _DP]),
     ("crs_analyze_field_grid", c_int,
      [_DP, _DP, _DP, c_int, c_double, c_double, c_double, c_double, c_int, c_int, c_int, _DP]),
     ("crs_analyze_field_of_view", c_int,

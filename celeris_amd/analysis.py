@@ -9,6 +9,7 @@ module (src/celeris/analysis/*.hpp; bindings/python/celeris_module.cpp:415-440):
 * :func:`compute_psf_field`     focal.cpp:240-311     -> :class:`FieldPsf`
 * :func:`analyze_through_focus` throughfocus.cpp:15-132 -> :class:`ThroughFocus`
 * :func:`analyze_chromatic`     chromatic.cpp:9-58    -> list of :class:`ChromaticPoint`
+* :func:`analyze_chromatic_dispersive`  chromatic.cpp:60-127  -> list of :class:`ChromaticPoint`
 * :func:`analyze_field_grid`    field.cpp:150-233     -> :class:`FieldGrid`
 * :func:`analyze_field_of_view` field.cpp:12-76       -> list of :class:`FieldPoint`
 * :func:`analyze_wide_fov`      field.cpp:78-148      -> list of :class:`FieldPoint`
@@ -31,14 +32,15 @@ from __future__ import annotations
 import ctypes
 import math
 from dataclasses import dataclass, field
-from typing import List
+from typing import Callable, List
 
 import numpy as np
 
 from . import _lib
 from .design import (AchromaticDesign, DispersiveLibrary, DispersivePbLibrary, MetalensDesign,
                      PbAchromaticDesign, UnitCellLibrary, achromatic_band_transmissions,
-                     aperture_pillars, exit_pupil, pb_achromatic_band_transmissions)
+                     aperture_fills, aperture_pillars, exit_pupil,
+                     pb_achromatic_band_transmissions)
 from .rs import _mode, crs_plane
 
 _DP = ctypes.POINTER(ctypes.c_double)
@@ -264,6 +266,40 @@ def analyze_chromatic_pillars(px, py, t, design_focal, design_wavelength, lambda
     _check(_lib.load().crs_analyze_chromatic(_p(px), _p(py), _p(t), px.size, design_focal,
                                              design_wavelength, lambda_min, lambda_max, int(n),
                                              _mode(mode), _p(out)), "crs_analyze_chromatic")
+    return [ChromaticPoint(*map(float, out[3 * i:3 * i + 3])) for i in range(n)]
+
+
+def analyze_chromatic_dispersive(lens: MetalensDesign, build_lib_at: Callable[[float], UnitCellLibrary],
+                                 design_focal: float, design_wavelength: float, diameter: float,
+                                 lambda_min: float, lambda_max: float, n: int,
+                                 mode="fp32c") -> List[ChromaticPoint]:
+    """chromatic.cpp:60-127: the unit-cell library is rebuilt at every wavelength by
+    build_lib_at(lambda) (the caller's library builder; the RCWA sweep behind it stays on
+    the CPU), each aperture cell takes its fill's transmission from that library, and the
+    n x 120 on-axis samples plus the design peak run in one launch.  build_lib_at is called
+    as the reference calls it: at the design wavelength first, then at each lambda_i."""
+    if n < 2:
+        raise ValueError("n must be at least 2")
+    px, py, t0 = aperture_pillars(lens, build_lib_at(design_wavelength), diameter)
+    fills = aperture_fills(lens, diameter)
+    bands = []
+    for i in range(n):
+        lam = lambda_min + (lambda_max - lambda_min) * i / (n - 1)
+        bands.append(build_lib_at(lam).transmission_for_fill(fills))
+    return analyze_chromatic_dispersive_pillars(px, py, t0, np.array(bands), design_focal,
+                                                design_wavelength, lambda_min, lambda_max, n, mode)
+
+
+def analyze_chromatic_dispersive_pillars(px, py, t_design, t_bands, design_focal, design_wavelength,
+                                         lambda_min, lambda_max, n, mode="fp32c") -> List[ChromaticPoint]:
+    """The dispersive chromatic scan on given pillars: t_design (npil) at the design
+    wavelength, t_bands (n x npil) at lambda_min + (lambda_max - lambda_min) i / (n - 1)."""
+    px, py, t0 = _pillars(px, py, _c(t_design))
+    tb = np.ascontiguousarray(_c(t_bands).reshape(n, px.size))
+    out = np.zeros(3 * n)
+    _check(_lib.load().crs_analyze_chromatic_dispersive(
+        _p(px), _p(py), _p(t0), _p(tb), px.size, design_focal, design_wavelength, lambda_min,
+        lambda_max, int(n), _mode(mode), _p(out)), "crs_analyze_chromatic_dispersive")
     return [ChromaticPoint(*map(float, out[3 * i:3 * i + 3])) for i in range(n)]
 
 

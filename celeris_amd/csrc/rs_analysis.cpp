@@ -4,6 +4,7 @@
 //   crs_analyze_focus          analysis/focal.cpp:32-146   (analyze_focus)
 //   crs_analyze_through_focus  analysis/throughfocus.cpp:15-132
 //   crs_analyze_chromatic      analysis/chromatic.cpp:9-58
+//   crs_analyze_chromatic_dispersive  analysis/chromatic.cpp:60-127
 //   crs_analyze_field_grid     analysis/field.cpp:150-233
 //   crs_analyze_field_of_view  analysis/field.cpp:12-76
 //   crs_analyze_wide_fov       analysis/field.cpp:78-148
@@ -204,31 +205,32 @@ extern "C" crs_status crs_analyze_through_focus(const double* px, const double* 
     return CRS_OK;
 }
 
-extern "C" crs_status crs_analyze_chromatic(const double* px, const double* py, const double* t,
-                                            int npil, double design_focal, double design_wavelength,
-                                            double lambda_min, double lambda_max, int n, int mode,
-                                            double* out3n) {
-    crs::clear_error();
-    if (!px || !py || !t || !out3n || npil <= 0 || n < 2) {
-        crs::set_error("crs_analyze_chromatic: null array, npil <= 0 or n < 2");
-        return CRS_ERR_INVALID;
-    }
-    const int NZ = 120;  // chromatic.cpp:48
+namespace {
+
+// The chromatic scan of chromatic.cpp:9-58 and :60-127: the design-wavelength peak at
+// the design focus (:40, :100-102) and, for each of n wavelengths, NZ = 120 on-axis
+// samples over [0.55, 1.6] f0 lambda0 / lambda (:44-56, :106-124), all as planes of one
+// launch.  set_of(i) is the transmission set of wavelength i (-1: the design peak).
+template <class SetOf>
+crs_status chromatic_scan(const double* px, const double* py, const double* t, int npil, int nsets,
+                          SetOf set_of, double design_focal, double design_wavelength,
+                          double lambda_min, double lambda_max, int n, int mode, double* out3n) {
+    const int NZ = 120;
     std::vector<crs_plane> planes;
     planes.reserve(n + 1);
-    // chromatic.cpp:40: the design-wavelength peak at the design focus.
-    planes.push_back(axis_plane(design_focal, design_focal, 1, 2.0 * kPi / design_wavelength, 0, 0));
+    planes.push_back(axis_plane(design_focal, design_focal, 1, 2.0 * kPi / design_wavelength,
+                                set_of(-1), 0));
     std::vector<double> lam(n), zlo(n), zhi(n);
     for (int i = 0; i < n; ++i) {
         lam[i] = lambda_min + (lambda_max - lambda_min) * i / (n - 1);
         const double expected = design_focal * design_wavelength / lam[i];
         zlo[i] = 0.55 * expected;
         zhi[i] = 1.6 * expected;
-        planes.push_back(axis_plane(zlo[i], zhi[i], NZ, 2.0 * kPi / lam[i], 0,
+        planes.push_back(axis_plane(zlo[i], zhi[i], NZ, 2.0 * kPi / lam[i], set_of(i),
                                     1 + static_cast<int64_t>(i) * NZ));
     }
     std::vector<double> I(1 + static_cast<size_t>(n) * NZ);
-    crs_status st = crs_forward_host(mode, px, py, t, npil, 1, planes.data(),
+    crs_status st = crs_forward_host(mode, px, py, t, npil, nsets, planes.data(),
                                      static_cast<int>(planes.size()), nullptr, I.data());
     if (st != CRS_OK) return st;
     const double design_peak = I[0];
@@ -245,6 +247,43 @@ extern "C" crs_status crs_analyze_chromatic(const double* px, const double* py, 
         out3n[3 * i + 2] = design_peak > 0 ? best_I / design_peak : 0.0;
     }
     return CRS_OK;
+}
+
+}  // namespace
+
+extern "C" crs_status crs_analyze_chromatic(const double* px, const double* py, const double* t,
+                                            int npil, double design_focal, double design_wavelength,
+                                            double lambda_min, double lambda_max, int n, int mode,
+                                            double* out3n) {
+    crs::clear_error();
+    if (!px || !py || !t || !out3n || npil <= 0 || n < 2) {
+        crs::set_error("crs_analyze_chromatic: null array, npil <= 0 or n < 2");
+        return CRS_ERR_INVALID;
+    }
+    return chromatic_scan(px, py, t, npil, 1, [](int) { return 0; }, design_focal,
+                          design_wavelength, lambda_min, lambda_max, n, mode, out3n);
+}
+
+extern "C" crs_status crs_analyze_chromatic_dispersive(
+    const double* px, const double* py, const double* t_design, const double* t_bands, int npil,
+    double design_focal, double design_wavelength, double lambda_min, double lambda_max, int n,
+    int mode, double* out3n) {
+    crs::clear_error();
+    if (!px || !py || !t_design || !t_bands || !out3n || npil <= 0 || n < 2) {
+        crs::set_error("crs_analyze_chromatic_dispersive: null array, npil <= 0 or n < 2");
+        return CRS_ERR_INVALID;
+    }
+    if (static_cast<int64_t>(npil) * (n + 1) > (int64_t(1) << 31) - 1) {
+        crs::set_error("crs_analyze_chromatic_dispersive: (n + 1) * npil exceeds 2^31 - 1");
+        return CRS_ERR_INVALID;
+    }
+    // Set 0: the design wavelength's transmissions; set 1 + i: wavelength i's.
+    const size_t per = 2 * static_cast<size_t>(npil);
+    std::vector<double> t(per * (n + 1));
+    std::copy(t_design, t_design + per, t.begin());
+    std::copy(t_bands, t_bands + per * n, t.begin() + per);
+    return chromatic_scan(px, py, t.data(), npil, n + 1, [](int i) { return i + 1; },
+                          design_focal, design_wavelength, lambda_min, lambda_max, n, mode, out3n);
 }
 
 extern "C" crs_status crs_analyze_field_grid(const double* px, const double* py, const double* t,

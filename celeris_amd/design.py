@@ -140,6 +140,20 @@ def aperture_pillars(lens: MetalensDesign, lib: UnitCellLibrary, diameter_um: fl
     return tuple(out)
 
 
+def aperture_fills(lens: MetalensDesign, diameter_um: float) -> np.ndarray:
+    """Fill factors of the cells aperture_pillars keeps, in the same order (the per-cell
+    input of a library lookup, e.g. chromatic.cpp:75-81 keeps fills and looks them up in
+    each wavelength's library)."""
+    n = lens.n_cells
+    p = lens.period_um
+    c = (n - 1) / 2.0
+    iy, ix = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    x = ((ix - c) * p).ravel()
+    y = ((iy - c) * p).ravel()
+    keep = ~(np.sqrt(x * x + y * y) > diameter_um / 2.0)
+    return np.asarray(lens.fill_map, np.float64).ravel()[keep]
+
+
 def exit_pupil(lens: MetalensDesign, lib: UnitCellLibrary, focal_length_um: float,
                wavelength_um: float, diameter_um: float):
     """The exit pupil analyze_mtf correlates (mtf.cpp:20-36): on the full n_cells^2 grid,

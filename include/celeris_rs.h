@@ -207,6 +207,20 @@ crs_status crs_analyze_chromatic(const double *px, const double *py, const doubl
                                  double lambda_min, double lambda_max, int n, int mode,
                                  double *out3n);
 
+/* analyze_chromatic_dispersive (chromatic.cpp:60-127): as crs_analyze_chromatic, but each
+ * wavelength has its own transmissions, from the unit-cell library rebuilt at that
+ * wavelength (the reference's build_lib_at(lambda); the library build itself is outside
+ * this library, so the caller passes its result).  t_design: the aperture cells'
+ * transmissions at design_wavelength (the design peak, chromatic.cpp:100-102); t_bands:
+ * n x npil (band-major, interleaved re/im), band i at lambda_min + (lambda_max -
+ * lambda_min) i / (n - 1).  n wavelengths x 120 on-axis samples plus the design peak in
+ * one launch.  out3n = n x {lambda, best focus z, peak / design peak}. */
+crs_status crs_analyze_chromatic_dispersive(const double *px, const double *py,
+                                            const double *t_design, const double *t_bands,
+                                            int npil, double design_focal,
+                                            double design_wavelength, double lambda_min,
+                                            double lambda_max, int n, int mode, double *out3n);
+
 /* analyze_field_grid (field.cpp:150-233): tilted-illumination spots on a
  * (2 n_half + 1)^2 grid of field angles up to max_angle_deg, each psf_n^2, plus the
  * on-axis reference spot, all in one launch (one transmission set per field point).

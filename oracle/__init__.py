@@ -190,6 +190,27 @@ def verify_achromatic_focus(px, py, t_bands, wavelengths, center_index, center_w
                      for lam, z, i in zip(wavelengths, bz, bi)])
 
 
+def chromatic(px, py, t_sets, design_set, band_sets, design_focal, design_wavelength,
+              lambda_min, lambda_max, n):
+    """analyze_chromatic (chromatic.cpp:9-58; every band_sets entry the same set) and
+    analyze_chromatic_dispersive (chromatic.cpp:60-127; one set per wavelength): the design
+    peak I(f0) at lambda0 from set design_set, then per wavelength lambda_i = lmin + (lmax -
+    lmin) i / (n - 1) the first maximum of 120 on-axis samples over [0.55, 1.6] f0 lambda0 /
+    lambda_i (default z: the expected focus).  Returns n x (lambda, best z, peak / design peak)."""
+    design_peak = axis_intensity(px, py, t_sets, [(2.0 * math.pi / design_wavelength, design_focal,
+                                                   design_set)])[0]
+    scans, lams = [], []
+    for i in range(n):
+        lam = lambda_min + (lambda_max - lambda_min) * i / (n - 1)
+        expected = design_focal * design_wavelength / lam
+        zlo, zhi = 0.55 * expected, 1.6 * expected
+        scans.append((band_sets[i], 120, lam, zlo, zhi - zlo, expected))
+        lams.append(lam)
+    bz, bi, _ = axis_scan(px, py, t_sets, scans)
+    return np.array([[lam, z, i / design_peak if design_peak > 0 else 0.0]
+                     for lam, z, i in zip(lams, bz, bi)])
+
+
 def polar_focus_check(px, py, tx, ty, wavelength, fx, fy, nz=240):
     """The polarisation check of cmd_polar.cpp:85-126 (nz = 240; gui/workers.cpp:181-197
     uses 200): each channel's on-axis focus over [0.5 min(fx, fy), 1.5 max(fx, fy)] and

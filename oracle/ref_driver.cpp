@@ -22,6 +22,9 @@
 //                                             (analyze_through_focus, throughfocus.cpp:15)
 //   chromatic    : lens, f, lambda, D, lmin, lmax, n -> n x (lambda, f, rel)
 //                                             (analyze_chromatic, chromatic.cpp:9)
+//   chromatic_disp : lens, f, lambda, D, lmin, lmax, n, nlibs, nlibs x (lambda,
+//                  fill[nlib], phase[nlib], amplitude[nlib]) -> n x (lambda, f, rel)
+//                                             (analyze_chromatic_dispersive, chromatic.cpp:60)
 //   mtf          : lens, f, lambda, D      -> m, freq[m], mtf[m], ideal[m], cutoff
 //                                             (analyze_mtf, mtf.cpp:10)
 //   fieldgrid    : lens, f, lambda, D, max_angle, n_half, psf_n -> n, then n*n x
@@ -59,6 +62,7 @@
 #include "celeris/io/image.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -242,6 +246,33 @@ int main(int argc, char** argv) {
             out.push_back(tf.caustic_nx);
             out.push_back(tf.caustic_nz);
             for (float v : tf.caustic) out.push_back(v);
+        } else if (cmd == "chromatic_disp") {
+            // lmin, lmax, n, nlibs, then nlibs x (lambda, fill[nlib], phase[nlib],
+            // amplitude[nlib]) on the design library's fill grid; build_lib_at(lambda)
+            // returns the table whose lambda matches (analyze_chromatic_dispersive asks
+            // for the design wavelength and then each lambda_i).
+            const double lmin = r.next(), lmax = r.next();
+            const int n = r.next_int(), nlibs = r.next_int();
+            std::vector<UnitCellLibrary> tables(nlibs);
+            for (UnitCellLibrary& t : tables) {
+                t = lib;
+                t.wavelength_um = r.next();
+                for (double& x : t.fill) x = r.next();
+                for (double& x : t.phase) x = r.next();
+                for (double& x : t.amplitude) x = r.next();
+            }
+            auto build_lib_at = [&](double wl) -> UnitCellLibrary {
+                for (const UnitCellLibrary& t : tables)
+                    if (std::fabs(t.wavelength_um - wl) <= 1e-12 * wl) return t;
+                std::fprintf(stderr, "ref_driver: no library table at lambda %.17g\n", wl);
+                std::exit(3);
+            };
+            auto pts = analyze_chromatic_dispersive(lens, build_lib_at, f, lam, D, lmin, lmax, n);
+            for (const auto& p : pts) {
+                out.push_back(p.wavelength_um);
+                out.push_back(p.focal_length_um);
+                out.push_back(p.rel_peak);
+            }
         } else if (cmd == "chromatic") {
             const double lmin = r.next(), lmax = r.next();
             const int n = r.next_int();

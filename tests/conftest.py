@@ -67,3 +67,25 @@ def achromatic_golden():
         px, py, t = dz.pb_achromatic_band_transmissions(plib, pdes)
         out.append((name, "pbachro", px, py, t, lam, ci, lc, f, g("pbachro"), plib, pdes))
     return out
+
+
+def dispersive_golden():
+    """(lens, build_lib_at, f, lam, D, lmin, lmax, n, reference result, calls) from
+    tests/golden/chromatic_dispersive.npz (written by the reference's
+    analyze_chromatic_dispersive with a table-backed library builder); calls records the
+    wavelengths build_lib_at was asked for."""
+    from celeris_amd.design import MetalensDesign, UnitCellLibrary
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                             "chromatic_dispersive.npz"))
+    lens = MetalensDesign(int(d["n_cells"]), float(d["period"]), d["fill_map"])
+    calls = []
+
+    def build_lib_at(wl):
+        calls.append(wl)
+        i = int(np.argmin(np.abs(d["lib_wavelengths"] - wl)))
+        assert abs(d["lib_wavelengths"][i] - wl) <= 1e-12 * wl
+        return UnitCellLibrary(float(d["period"]), d["lib_fill"], d["lib_phases"][i], d["lib_amps"][i])
+
+    lmin, lmax, n = d["args"]
+    return (lens, build_lib_at, float(d["f"]), float(d["lam"]), float(d["D"]), float(lmin),
+            float(lmax), int(n), d["result"].reshape(-1, 3), calls)

@@ -64,6 +64,42 @@ def lens_case():
                 lib_phase=lib_phase, lib_amp=lib_amp, fill_map=fill_map)
 
 
+def dispersive_libraries(L, wavelengths):
+    """Per-wavelength unit-cell libraries on the design library's fill grid, standing in
+    for build_lib_at(lambda)'s RCWA sweeps: each entry's phase scales with lambda0 / lambda
+    plus a fill-dependent waveguide term, its amplitude drifts with lambda (deterministic
+    formulas, no RCWA)."""
+    lam0 = L["lam"]
+    out = []
+    for wl in wavelengths:
+        d = (wl - lam0) / lam0
+        phase = L["lib_phase"] * (lam0 / wl) + 0.8 * d * np.sin(2 * math.pi * L["lib_fill"])
+        amp = np.clip(L["lib_amp"] * (1.0 - 0.6 * d * (L["lib_fill"] - 0.5)), 0.05, 1.0)
+        out.append((wl, L["lib_fill"].copy(), phase, amp))
+    return out
+
+
+def write_chromatic_dispersive():
+    """analyze_chromatic_dispersive (chromatic.cpp:60-127) on the lens case, with the
+    table-backed library builder of dispersive_libraries."""
+    L = lens_case()
+    lmin, lmax, n = 0.45, 0.65, 6
+    wls = [L["lam"]] + [lmin + (lmax - lmin) * i / (n - 1) for i in range(n)]
+    libs = dispersive_libraries(L, wls)
+    blk = oracle.lens_block(L["n_cells"], L["period"], L["fill_map"], L["lib_fill"], L["lib_phase"],
+                            L["lib_amp"])
+    vals = [blk, [L["f"], L["lam"], L["D"], lmin, lmax, n, len(libs)]]
+    for wl, fill, phase, amp in libs:
+        vals += [[wl], fill, phase, amp]
+    out = {k: np.asarray(v) for k, v in L.items()}
+    out["args"] = np.array([lmin, lmax, n])
+    out["lib_wavelengths"] = np.array([w for w, *_ in libs])
+    out["lib_phases"] = np.array([p for _, _, p, _ in libs])
+    out["lib_amps"] = np.array([a for *_, a in libs])
+    out["result"] = oracle.ref_run("chromatic_disp", np.concatenate(vals))
+    np.savez_compressed(os.path.join(HERE, "chromatic_dispersive.npz"), **out)
+
+
 def mtf_case():
     """A larger lens for analyze_mtf alone: 81 x 81 cells at 0.35 um with seeded random
     fills (a poor lens, so the MTF curve is far from the ideal one) inside a D = 27 um
@@ -182,13 +218,15 @@ def write_design():
 def main():
     if not oracle.ref_available():
         sys.exit("oracle/_ref/ref_driver is not built (make -C oracle ref)")
-    only = {"achromatic": write_achromatic, "tolerance": write_tolerance, "design": write_design}
+    only = {"achromatic": write_achromatic, "tolerance": write_tolerance, "design": write_design,
+            "dispersive": write_chromatic_dispersive}
     if sys.argv[1:] and sys.argv[1] in only:
         only[sys.argv[1]]()
         return
     write_achromatic()
     write_tolerance()
     write_design()
+    write_chromatic_dispersive()
     out = {}
     for name, px, py, t, cx, cy, z, lam, n, W in pillars_cases():
         I = oracle.ref_propagate_pillars(px, py, t, cx, cy, z, lam, n, W)

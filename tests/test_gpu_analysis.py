@@ -14,6 +14,7 @@ import pytest
 
 from celeris_amd import analysis
 from celeris_amd.design import MetalensDesign, UnitCellLibrary
+from conftest import dispersive_golden
 
 pytestmark = pytest.mark.gpu
 
@@ -83,6 +84,22 @@ def test_analyze_chromatic(lens, mode, tol):
     print(mode, got, ref)
     assert np.array_equal(got[:, 0], ref[:, 0])
     assert np.allclose(got[:, 1], ref[:, 1], rtol=1e-12, atol=0)  # argmax on the same z grid
+    assert np.allclose(got[:, 2], ref[:, 2], rtol=tol, atol=0)
+
+
+@pytest.mark.parametrize("mode,tol", [("fp64", 1e-9), ("fp32c", 1e-5)])
+def test_analyze_chromatic_dispersive(torch_cuda, mode, tol):
+    """analyze_chromatic_dispersive (chromatic.cpp:60-127) against the reference's own output
+    (tests/golden/chromatic_dispersive.npz): per-wavelength libraries from the caller's
+    builder, asked for in the reference's order (design wavelength first), the whole scan in
+    one launch."""
+    lens_, build, f, lam, D, lmin, lmax, n, ref, calls = dispersive_golden()
+    pts = analysis.analyze_chromatic_dispersive(lens_, build, f, lam, D, lmin, lmax, n, mode=mode)
+    got = np.array([[p.wavelength_um, p.focal_length_um, p.rel_peak] for p in pts])
+    print(mode, got, ref)
+    assert calls[0] == lam and np.allclose(calls[1:], [lmin + (lmax - lmin) * i / (n - 1) for i in range(n)])
+    assert np.array_equal(got[:, 0], ref[:, 0])
+    assert np.allclose(got[:, 1], ref[:, 1], rtol=1e-12, atol=0)
     assert np.allclose(got[:, 2], ref[:, 2], rtol=tol, atol=0)
 
 

@@ -1,21 +1,26 @@
 """FP32C against FP64 over random geometries: forward and adjoint, per output element.
 
-The FP32-compensated kernels choose, per work-group, between first-order and
-second-order expansions of R about a 4 x 4 block centre, and the host falls back to
-every pair at its own FP64 distance where neither bound holds.  The fixed workloads
-exercise a few of these choices; this sweep draws wavelength, pillar pitch and count,
-transmission (a focusing lens or random complex values), plane kind (psf windows
-anywhere in front of the lens, zx slices from near the lens plane outwards), grid
-sizes (ragged against the 4 x 4 blocks), window sizes and offsets from a seeded
-generator, so the choices and their boundaries are met in many combinations.
+The FP32-compensated kernels choose, per work item, among series orders of the
+expansion of R about a pixel block's centre (the cheapest whose worst-case pair error,
+truncation plus float rounding plus the hardware sin/cos, stays within 8e-6 of |t|/R),
+fall back to 4 x 4 blocks where the tuned shape is not admitted, and to every pair at
+its own FP64 distance where no block is.  The fixed workloads exercise a few of these
+choices; this sweep draws wavelength, pillar pitch and count, transmission (a focusing
+lens or random complex values), plane kind (psf windows anywhere in front of the lens,
+zx slices from near the lens plane outwards), grid sizes (ragged against the blocks),
+window sizes and offsets from a seeded generator, so the choices and their boundaries
+are met in many combinations.  Seeds 40-59 draw coarse pixel pitches (1 to 10
+wavelengths) and far planes (up to 3 mm), where the blocks span several turns and the
+float rounding of the block offsets is largest.
 
-Bound, per output element: every pair's error is held to 1e-6 rad of phase and 4e-6 of
-amplitude by the expansion checks, and float accumulation over at most 64 terms per
-register sum adds at most ~64 * 2^-24 of the summed magnitudes.  So
+Bound, per output element: every pair's error is held to 8e-6 of |t_q|/R_pq by the
+admission bound, and the float sums (at most 1,024 terms per wave before the FP64 fold)
+add ~1e-6 of the summed magnitudes for a coherent pixel.  So
     |E_fp32c - E_fp64|_p  <=  1e-5 * sum_q |t_q| / R_pq
     |g_fp32c - g_fp64|_q  <=  1e-5 * sum_p |u_p| / R_pq
 with the right-hand sides computed here in FP64 from the geometry; the FP64 kernels are
-held to the oracle on the same inputs, per element, at 1e-12 of the same scale.  (Relative L2 is
+held to the oracle on the same inputs, per element, at (1e-12 + 1e-15 k R_max) of the
+same scale (the FP64 rounding of k R).  (Relative L2 is
 printed, not asserted: a window in a dark region, where the terms cancel, can have any
 ratio of error to field.)
 """
@@ -31,7 +36,7 @@ from conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
 
-SEEDS = list(range(40))
+SEEDS = list(range(60))
 BOUND = 1e-5
 
 
@@ -57,21 +62,25 @@ def geometry(seed):
         f = None
         t = rng.standard_normal(px.size) + 1j * rng.standard_normal(px.size)
     # Pixel pitches from 0.02 to 1 wavelength (focal sampling is usually well below one),
-    # so that first-order blocks, second-order blocks and exact pairs all occur.
+    # so that first-order blocks, second-order blocks and exact pairs all occur; the
+    # coarse seeds take 1 to 10 wavelengths and planes up to 3 mm away.
     logu = lambda lo, hi: float(np.exp(rng.uniform(np.log(lo), np.log(hi))))  # noqa: E731
+    coarse = seed >= 40
+    pitch = (1.0, 10.0) if coarse else (0.02, 1.0)
+    zrange = (100.0, 3000.0) if coarse else (0.5, 300.0)
     planes = []
     for _ in range(int(rng.integers(1, 4))):
         if rng.random() < 0.7:
-            z = f if (f is not None and rng.random() < 0.5) else logu(0.5, 300.0)
+            z = f if (f is not None and rng.random() < 0.5 and not coarse) else logu(*zrange)
             n = int(rng.integers(2, 98))
-            W = 0.5 * (n - 1) * lam * logu(0.02, 1.0)
+            W = 0.5 * (n - 1) * lam * logu(*pitch)
             cx, cy = rng.uniform(-1.5 * half, 1.5 * half, 2)
             planes.append(Plane.psf(float(cx), float(cy), z, k, n, W))
         else:
             nx, nz = int(rng.integers(2, 98)), int(rng.integers(2, 98))
-            z1 = logu(0.3, 150.0)
-            z2 = z1 + (nz - 1) * lam * logu(0.02, 1.0)
-            xw = 0.5 * (nx - 1) * lam * logu(0.02, 1.0)
+            z1 = logu(zrange[0] * 0.6, zrange[1] * 0.5)
+            z2 = z1 + (nz - 1) * lam * logu(*pitch)
+            xw = 0.5 * (nx - 1) * lam * logu(*pitch)
             x0 = float(rng.uniform(-half, half))
             planes.append(Plane.zx(x0 - xw, x0 + xw, nx, z1, z2, nz, k))
     return dict(lam=lam, p=p, px=px, py=py, t=t, kind=kind, f=f, planes=planes)
@@ -136,11 +145,17 @@ def test_fp32c_vs_fp64_random_geometry(torch_cuda, rsctx, seed):
 
     fx, fy, fz = pixel_coords(crs.resolve_offsets(planes))
     S, T = magnitude_sums(torch, px, py, np.abs(t), fx, fy, fz, np.abs(u))
-    # The FP64 kernels against the oracle on the same inputs, to FP64 roundoff.
+    # The FP64 kernels against the oracle on the same inputs, to FP64 roundoff: k R is
+    # rounded once in each (relative 2^-53, so ~1.1e-16 k R of phase per pair, and the two
+    # sincos implementations differ by an ulp of it), hence 1e-12 + 1e-15 k R_max.
+    kR = max(P.k for P in planes) * float(np.max(np.sqrt(
+        (np.max(np.abs(fx)) + np.max(np.abs(px))) ** 2
+        + (np.max(np.abs(fy)) + np.max(np.abs(py))) ** 2 + np.max(fz ** 2))))
     Eor = oracle.forward(px, py, t, planes)
     gor = oracle.adjoint(px, py, u, planes)
-    assert float(np.max(np.abs(E64 - Eor) / S)) <= 1e-12
-    assert float(np.max(np.abs(g64.ravel() - np.ravel(gor)) / T)) <= 1e-12
+    tol64 = 1e-12 + 1e-15 * kR
+    assert float(np.max(np.abs(E64 - Eor) / S)) <= tol64
+    assert float(np.max(np.abs(g64.ravel() - np.ravel(gor)) / T)) <= tol64
     fe = float(np.max(np.abs(E32 - E64) / S))
     ge = float(np.max(np.abs(g32 - g64) / T))
     print(f"[random geometry {seed}] {kernel}: forward max |dE|/sum|t|/R {fe:.2e}, "
@@ -153,9 +168,6 @@ def test_fp32c_vs_fp64_random_geometry(torch_cuda, rsctx, seed):
     # each pair's phase is off by a few float roundings of k R (~2^-24 k R each) and its
     # 1/R by ~1e-7, so |dE_p| <= (4e-7 k R_max + 1e-6) sum_q |t_q| / R_pq.
     Ef = rsctx.forward(dpx, dpy, dt, planes, mode="fp32", field=True)[0].cpu().numpy()
-    kR = max(P.k for P in planes) * float(np.max(np.sqrt(
-        (np.max(np.abs(fx)) + np.max(np.abs(px))) ** 2
-        + (np.max(np.abs(fy)) + np.max(np.abs(py))) ** 2 + np.max(fz ** 2))))
     e32 = float(np.max(np.abs(Ef - E64) / S))
     print(f"[random geometry {seed}] plain FP32 (k R_max {kR:.0f}): max {e32:.2e}")
     assert e32 <= 4e-7 * kR + 1e-6

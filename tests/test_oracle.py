@@ -21,7 +21,7 @@ import pytest
 import oracle
 from celeris_amd import Plane, configs
 from celeris_amd.design import MetalensDesign, UnitCellLibrary, aperture_pillars
-from conftest import achromatic_golden, rel_l2
+from conftest import achromatic_golden, dispersive_golden, rel_l2
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -393,3 +393,22 @@ def test_oracle_tolerance_jitter_properties():
     sf = 0.010 / 0.4
     assert abs(a.mean()) < 4 * sf / math.sqrt(a.size)
     assert abs(a.std() / sf - 1) < 4 / math.sqrt(2 * a.size)
+
+
+def test_oracle_chromatic_dispersive_matches_reference():
+    """analyze_chromatic_dispersive (chromatic.cpp:60-127) through the reference's own code
+    (tests/golden/chromatic_dispersive.npz): the restated scan, on the transmissions each
+    wavelength's library gives the aperture cells, picks the reference's z sample in every
+    band and its relative peak to 1e-12."""
+    from celeris_amd.design import aperture_fills, aperture_pillars
+    lens, build, f, lam, D, lmin, lmax, n, ref, calls = dispersive_golden()
+    px, py, t0 = aperture_pillars(lens, build(lam), D)
+    fills = aperture_fills(lens, D)
+    bands = [build(lmin + (lmax - lmin) * i / (n - 1)).transmission_for_fill(fills) for i in range(n)]
+    t_sets = np.vstack([t0[None], np.array(bands)])
+    got = oracle.chromatic(px, py, t_sets, 0, list(range(1, n + 1)), f, lam, lmin, lmax, n)
+    assert np.array_equal(got[:, 0], ref[:, 0]) and np.array_equal(got[:, 1], ref[:, 1])
+    assert np.allclose(got[:, 2], ref[:, 2], rtol=1e-12, atol=0)
+    # the fixed-transmission scan differs (the libraries really disperse)
+    fixed = oracle.chromatic(px, py, t0[None], 0, [0] * n, f, lam, lmin, lmax, n)
+    assert np.max(np.abs(fixed[:, 2] - ref[:, 2])) > 1e-3
