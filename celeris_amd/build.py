@@ -67,6 +67,19 @@ def build_dropin_driver() -> str:
     return DRIVER
 
 
+PROBE_SRC = os.path.join(CSRC, "issue_probe.hip")
+PROBE = os.path.join(LIBDIR, "issue_probe")
+
+
+def build_issue_probe() -> str:
+    """The VALU issue-cost probe (shader-clock cycles per wave64 FP32 FMA, v_sin_f32 and
+    FP64 FMA per SIMD), the measured side of the pair kernels' roofline."""
+    os.makedirs(LIBDIR, exist_ok=True)
+    flags = [f for f in HIP_FLAGS if f not in ("-fPIC", "-shared")]
+    _run([HIPCC, *flags, PROBE_SRC, "-o", PROBE])
+    return PROBE
+
+
 def build_oracle() -> str:
     _run(["make", "-s", "-C", ORACLE])
     return os.path.join(ORACLE, "_build", "liboracle_rs.so")
@@ -84,6 +97,7 @@ def build_reference() -> bool:
 
 def build_all() -> None:
     build_library()
+    build_issue_probe()
     build_dropin_driver()
     build_oracle()
     build_reference()

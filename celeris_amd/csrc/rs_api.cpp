@@ -206,12 +206,18 @@ int default_target() { return 4096; }
 // 2 waves per SIMD, and measured no faster), 4 x 8 per adjoint block.
 constexpr int kDefFwdBX = 4, kDefFwdBY = 4;
 constexpr int kDefAdjBX = 4, kDefAdjBY = 8;
-// Built-in FP32C launch sizes, in multiples of the resident work-groups: the forward at
-// most kFwdRounds x (and at most one LDS tile of pillars per work-group), the adjoint
-// kAdjRounds x.  Several rounds let the CUs that finish early take more work; measured
-// at 91,809 pillars x 161^2 both kernels sit within a few % of their best there.
-constexpr int kFwdRounds = 16;
+// Built-in FP32C launch sizes, in multiples of the resident work-groups (4 per CU for
+// both default shapes).  Several rounds let the CUs that finish early take more work:
+// at 91,809 pillars x 161^2 (whole call, pair kernel and reduction) the forward takes
+// 0.795 / 0.745 / 0.720 / 0.715 ms at 1 / 2 / 4 / 8 rounds, the adjoint 0.751 / 0.711 /
+// 0.693 / 0.687 ms.
+constexpr int kFwdRounds = 8;
 constexpr int kAdjRounds = 8;
+// Fewest pairs per FP32C work-group piece: below this a piece's fixed costs (pillar
+// loads, state, the partial slot and its reduction) outweigh its pairs, so small calls
+// launch fewer, larger pieces (measured at 4,096 pillars x 81^2: 512 adjoint pieces take
+// 0.6x the time of 3,696).
+constexpr int64_t kMinPiecePairs = 65536;
 
 // The previous call's work must be finished before its scratch is freed.
 void wait_previous(crs_context* ctx) {
@@ -701,12 +707,15 @@ crs_status plan_forward_fp32c(const crs_context* ctx, int npil, std::vector<DevP
         }
         if (tiles.size() == tile0) continue;
         slot_elems = std::max<int64_t>(slot_elems, 64 * kbx * kby);
-        // Enough pieces to fill the chip, but each with a few LDS tiles of work.
-        int64_t units = 0;
-        for (int64_t l : len) units += ceil_div(l, kThreads);
+        // Enough pieces to fill the chip, each at least 64 pillars and kMinPiecePairs pairs.
+        int64_t units = 0, pairs = 0;
+        for (size_t r = 0; r < len.size(); ++r) {
+            units += ceil_div(len[r], 64);
+            pairs += len[r] * 64 * tiles[tile0 + r].nel;
+        }
         const int64_t nwg = std::min<int64_t>(
             fp32c_target(ctx, ctx->fwd_target[CRS_MODE_FP32C], false, kbx, kby, kFwdRounds),
-            std::max<int64_t>(1, units));
+            std::max<int64_t>(1, std::min(units, pairs / kMinPiecePairs)));
         std::vector<int32_t> wg_start;
         const int64_t seg_base = static_cast<int64_t>(segs.size());
         int last_tile = -1;
@@ -737,7 +746,7 @@ crs_status plan_forward_fp32c(const crs_context* ctx, int npil, std::vector<DevP
     }
     std::vector<FwdRed> reds;
     for (size_t t = 0; t < tiles.size(); ++t)
-        for (int el = 0; el < tiles[t].nel * 64; el += kThreads)
+        for (int el = 0; el < tiles[t].nel * 64; el += 64)
             reds.push_back({static_cast<int32_t>(t), el});
     plan.nseg = static_cast<int64_t>(segs.size());
     plan.slot_elems = slot_elems;
@@ -964,9 +973,10 @@ crs_status plan_adjoint_fp32c(const crs_context* ctx, int npil, int nsets,
             }
         int64_t units = 0;
         for (int64_t l : len) units += l;
+        const int64_t pairs = units * bx * kStrip * kThreads;
         const int64_t nwg = std::min<int64_t>(
             fp32c_target(ctx, ctx->adj_target[CRS_MODE_FP32C], true, bx, by, kAdjRounds),
-            std::max<int64_t>(1, units));
+            std::max<int64_t>(1, std::min(units, pairs / kMinPiecePairs)));
         std::vector<int32_t> wg_start;
         const int64_t seg_base = static_cast<int64_t>(segs.size());
         streamk_cut(len, rate, nwg, 1,

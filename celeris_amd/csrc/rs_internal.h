@@ -26,18 +26,21 @@ constexpr int kMinChunk = 64;
 // Pixels are handled in blocks of BX columns x BY rows (BX, BY in {4, 8}) that share
 // one FP64 reference distance per pillar, or 1 x 1 where a plane is too close or too
 // coarsely sampled for the block expansion ("exact" planes).  The forward gives each
-// lane one block ("thread tile") and a work-group 64 tiles of one plane, its 4 waves
-// splitting the pillars; the adjoint gives each thread one pillar and walks a plane in
+// lane one block ("thread tile") and a work-group 64 tiles of one plane, its
+// kFwdWaves waves splitting the pillars; the adjoint gives each thread one pillar and walks a plane in
 // strips of kStrip rows, each strip in BX-column blocks.
 constexpr int kFwdTiles = 64;
+// FP32C forward work-group: waves that split the pillars of its 64 thread tiles.
+constexpr int kFwdWaves = 4;
+constexpr int kFwdThreads = 64 * kFwdWaves;
 // FP32C adjoint strip height (rows); BY = 4 kernels take a strip as two halves.
 constexpr int kStrip = 8;
 // FP32C adjoint: columns per LDS tile of a strip (kAdjCols x kStrip pixel records).
 constexpr int kAdjCols = 64;
 // FP32C staged-pixel columns are padded to a multiple of this (the widest block).
 constexpr int kColAlign = 8;
-// FP32C forward: pillars one wave sums in float before the work-group folds the four
-// waves' sums into FP64 running sums (so a float sum never runs over more than this).
+// FP32C forward: pillars one wave sums in float before the work-group folds its waves'
+// sums into FP64 running sums (so a float sum never runs over more than this).
 constexpr int kWaveFold = 1024;
 // Pillars per FP32C extent record (one per LDS tile of staged pillars).
 constexpr int kExtTile = kThreads;
@@ -149,7 +152,7 @@ struct FwdTile {
 struct FwdSeg {
     int32_t tile, q0, q1, pad;
 };
-// Forward reduction work-group: elements [el0, el0 + kThreads) (e*64 + lane) of a tile.
+// Forward reduction work-group: elements [el0, el0 + 64) (e*64 + lane) of a tile.
 struct FwdRed {
     int32_t tile, el0;
 };

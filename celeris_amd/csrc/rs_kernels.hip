@@ -352,8 +352,8 @@ __device__ __forceinline__ void fwd_segment(
     for (int e = 0; e < N; ++e) ar[e] = ai[e] = 0.0f;
     bool first = true;
     int since_fold = 0;
-    for (int base = q0; base < q1; base += kThreads) {
-        const int n = min(kThreads, q1 - base);
+    for (int base = q0; base < q1; base += kFwdThreads) {
+        const int n = min(kFwdThreads, q1 - base);
         __syncthreads();
         if (tid < n) {
             const double2 xy = pos[base + tid];
@@ -367,7 +367,7 @@ __device__ __forceinline__ void fwd_segment(
             tile[tid] = rec;
         }
         __syncthreads();
-        const int k0 = (wave * n) / kWaves, k1 = ((wave + 1) * n) / kWaves;
+        const int k0 = (wave * n) / kFwdWaves, k1 = ((wave + 1) * n) / kFwdWaves;
         if constexpr (N == 1) {
             pillar_run<kP1A1>(tile, k0, k1, G, ar, ai);
         } else {
@@ -378,11 +378,11 @@ __device__ __forceinline__ void fwd_segment(
             else
                 pillar_run<kP2A2>(tile, k0, k1, G, ar, ai);
         }
-        since_fold += kThreads / kWaves;
-        if (since_fold >= kWaveFold || base + kThreads >= q1) {
+        since_fold += kFwdThreads / kFwdWaves;
+        if (since_fold >= kWaveFold || base + kFwdThreads >= q1) {
             // Fold the four waves' float sums into the FP64 running sums, in wave order.
 #pragma unroll
-            for (int w = 0; w < kWaves; ++w) {
+            for (int w = 0; w < kFwdWaves; ++w) {
                 __syncthreads();
                 if (wave == w) {
 #pragma unroll
@@ -402,9 +402,13 @@ __device__ __forceinline__ void fwd_segment(
         }
     }
     __syncthreads();
-    for (int i = tid; i < N * 64; i += kThreads) {
-        const double2 v = run[i];
-        slot[i] = make_float2(static_cast<float>(v.x), static_cast<float>(v.y));
+#pragma unroll
+    for (int k = 0; k < (N * 64 + kFwdThreads - 1) / kFwdThreads; ++k) {
+        const int i = tid + k * kFwdThreads;
+        if (N * 64 % kFwdThreads == 0 || i < N * 64) {
+            const double2 v = run[i];
+            slot[i] = make_float2(static_cast<float>(v.x), static_cast<float>(v.y));
+        }
     }
 }
 
@@ -417,14 +421,13 @@ constexpr int fwd_waves_per_simd(int n) { return n <= 16 ? 4 : 2; }
 // the planes blocked that way (and the 1 x 1 tiles of exact planes).  <1, 1>: calls
 // whose planes are all exact.
 template <int KBX, int KBY>
-__global__ __launch_bounds__(kThreads, fwd_waves_per_simd(KBX * KBY)) void fwd_fp32c(
+__global__ __launch_bounds__(kFwdThreads, fwd_waves_per_simd(KBX * KBY)) void fwd_fp32c(
     const double2* __restrict__ pos, const float2* __restrict__ ap,
     const double2* __restrict__ ext, int npil, const DevPlane* __restrict__ planes,
     const FwdTile* __restrict__ tiles, const FwdSeg* __restrict__ segs,
     const int32_t* __restrict__ wg_seg, float2* __restrict__ partial, int64_t slot_elems) {
-    __shared__ PillarRec tile[kThreads];
+    __shared__ PillarRec tile[kFwdThreads];
     __shared__ double2 run[KBX * KBY * 64];
-    __shared__ double2 red[kWaves];
     const int s0 = wg_seg[blockIdx.x], s1 = wg_seg[blockIdx.x + 1];
     for (int s = s0; s < s1; ++s) {
         const FwdSeg S = segs[s];
@@ -438,30 +441,18 @@ __global__ __launch_bounds__(kThreads, fwd_waves_per_simd(KBX * KBY)) void fwd_f
             // Series order: class 0 from the segment's pillar extent (the staged extents
             // of the kExtTile runs it covers); the smaller edge blocks take kP2A2, which
             // the host admitted for the full block shape at any obliquity.
+            // The extent loop runs on uniform addresses (scalar loads, no lane shuffles:
+            // shuffle indices held across the segment loop used to spill to scratch).
             int order = kP2A2;
             if (T.shape == 0) {
                 double mx = 0.0, my = 0.0;
                 const int e0 = S.q0 / kExtTile, e1 = (S.q1 - 1) / kExtTile;
-                for (int i = e0 + static_cast<int>(threadIdx.x); i <= e1; i += kThreads) {
+                for (int i = e0; i <= e1; ++i) {
                     const double2 v = ext[i];
                     mx = fmax(mx, v.x);
                     my = fmax(my, v.y);
                 }
-#pragma unroll
-                for (int o = 32; o > 0; o >>= 1) {
-                    mx = fmax(mx, __shfl_xor(mx, o));
-                    my = fmax(my, __shfl_xor(my, o));
-                }
-                __syncthreads();  // the previous segment has read red
-                if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = make_double2(mx, my);
-                __syncthreads();
-                double2 E = red[0];
-#pragma unroll
-                for (int w = 1; w < kWaves; ++w) {
-                    E.x = fmax(E.x, red[w].x);
-                    E.y = fmax(E.y, red[w].y);
-                }
-                order = __builtin_amdgcn_readfirstlane(fp32c_series_order(P, E.x, E.y));
+                order = __builtin_amdgcn_readfirstlane(fp32c_series_order(P, mx, my));
             }
             switch (T.shape) {
                 case 0:
@@ -618,33 +609,67 @@ __global__ __launch_bounds__(kThreads) void fwd_reduce(
 
 // ---------------------------------------------------------------------------------
 // Forward reduction, FP32C: for every element (e * 64 + lane) of a tile, the sum over
-// the tile's segments' partial slots in segment order (FP64), turned by exp(2 pi i
-// z'_ref) and written at the element's pixel.  A work-group takes 256 elements of one
-// tile.
+// the tile's segments' partial slots (FP64), turned by exp(2 pi i z'_ref) and written at
+// the element's pixel.  A work-group takes 64 elements of one tile; wave w sums the w-th
+// quarter of the tile's slots (eight loads in flight), and wave 0 adds the four quarter
+// sums in wave order, so the result does not depend on timing.
+
+namespace {
+__device__ __forceinline__ void sum_slots(const float2* __restrict__ src, int64_t stride, int k0,
+                                          int k1, double& er, double& ei) {
+    constexpr int U = 8;
+    int k = k0;
+    for (; k + U <= k1; k += U) {
+        float2 v[U];
+#pragma unroll
+        for (int j = 0; j < U; ++j) v[j] = src[static_cast<int64_t>(k + j) * stride];
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+            er += v[j].x;
+            ei += v[j].y;
+        }
+    }
+    for (; k < k1; ++k) {
+        const float2 v = src[static_cast<int64_t>(k) * stride];
+        er += v.x;
+        ei += v.y;
+    }
+}
+}  // namespace
 
 __global__ __launch_bounds__(kThreads) void fwd_reduce_fp32c(
     const DevPlane* __restrict__ planes, const FwdTile* __restrict__ tiles,
     const FwdRed* __restrict__ reds, const float2* __restrict__ partial, int64_t slot_elems,
     double2* __restrict__ field, double* __restrict__ intensity) {
+    __shared__ double2 red[kWaves][64];
     const FwdRed R = reds[blockIdx.x];
     const FwdTile T = tiles[R.tile];
     const DevPlane& P = planes[T.plane];
-    const int i = R.el0 + static_cast<int>(threadIdx.x);
-    if (i >= T.nel * 64) return;
-    const int e = i >> 6, lane = i & 63;
+    const int wave = wave_id(), lane = threadIdx.x & 63;
+    const int i = R.el0 + lane;  // element of the tile (el0 is a multiple of 64)
+    double er = 0.0, ei = 0.0;
+    if (i < T.nel * 64) {
+        const int k0 = (wave * T.nseg) / kWaves, k1 = ((wave + 1) * T.nseg) / kWaves;
+        sum_slots(partial + static_cast<int64_t>(T.seg0) * slot_elems + i, slot_elems, k0, k1,
+                  er, ei);
+    }
+    red[wave][lane] = make_double2(er, ei);
+    __syncthreads();
+    if (wave != 0 || i >= T.nel * 64) return;
+    er = red[0][lane].x;
+    ei = red[0][lane].y;
+#pragma unroll
+    for (int w = 1; w < kWaves; ++w) {
+        er += red[w][lane].x;
+        ei += red[w][lane].y;
+    }
+    const int e = i >> 6;  // element e of thread tile T.tile0 + lane
     const int t = T.tile0 + lane;
     if (t >= tile_count(P, T.shape, P.bx, P.by)) return;
     int ox, oy;
     tile_origin(P, T.shape, t, P.bx, P.by, ox, oy);
     const int bw = (T.shape == 0 || T.shape == 2) ? P.bx : 1;
     const int c = e % bw, r = e / bw;
-    double er = 0.0, ei = 0.0;
-    const float2* __restrict__ src = partial + static_cast<int64_t>(T.seg0) * slot_elems + i;
-    for (int k = 0; k < T.nseg; ++k) {
-        const float2 v = src[static_cast<int64_t>(k) * slot_elems];
-        er += v.x;
-        ei += v.y;
-    }
     double sn, cs;
     sincos(kTwoPi * (P.zref - floor(P.zref)), &sn, &cs);
     const double xr = er * cs - ei * sn;
@@ -1032,7 +1057,23 @@ __global__ __launch_bounds__(kThreads) void adj_reduce_fp32c(
     const int grp = q / kThreads, lane = q - grp * kThreads;
     const int k0 = slot_off[s * ngroups + grp], k1 = slot_off[s * ngroups + grp + 1];
     double gr = 0.0, gi = 0.0;
-    for (int k = k0; k < k1; ++k) {
+    // Eight slot indices, then their eight values, in flight at once; summed in list order.
+    constexpr int U = 8;
+    int k = k0;
+    for (; k + U <= k1; k += U) {
+        int sl[U];
+        float2 v[U];
+#pragma unroll
+        for (int j = 0; j < U; ++j) sl[j] = slots[k + j];
+#pragma unroll
+        for (int j = 0; j < U; ++j) v[j] = partial[static_cast<int64_t>(sl[j]) * kThreads + lane];
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+            gr += v[j].x;
+            gi += v[j].y;
+        }
+    }
+    for (; k < k1; ++k) {
         const float2 v = partial[static_cast<int64_t>(slots[k]) * kThreads + lane];
         gr += v.x;
         gi += v.y;
@@ -1108,7 +1149,7 @@ inline unsigned grid_for(int64_t n) { return static_cast<unsigned>((n + kThreads
 
 template <int KBX, int KBY>
 void launch_fwd_shape(const Fp32cFwdArgs& a, hipStream_t s) {
-    fwd_fp32c<KBX, KBY><<<a.nwg, kThreads, 0, s>>>(a.pos, a.ap, a.ext, a.npil, a.planes, a.tiles,
+    fwd_fp32c<KBX, KBY><<<a.nwg, kFwdThreads, 0, s>>>(a.pos, a.ap, a.ext, a.npil, a.planes, a.tiles,
                                                    a.segs, a.wg_seg, a.partial, a.slot_elems);
 }
 
@@ -1186,18 +1227,20 @@ hipError_t launch_adj_reduce_fp32c(const float2* partial, const int32_t* slot_of
 int fp32c_resident_per_cu(bool adjoint, int bx, int by) {
     int n = 0;
     hipError_t e = hipErrorInvalidValue;
-    auto occ = [&](const void* fn) { e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kThreads, 0); };
+    auto occ = [&](const void* fn, int threads) {
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, threads, 0);
+    };
     if (!adjoint) {
-        if (bx == 4 && by == 4) occ(reinterpret_cast<const void*>(fwd_fp32c<4, 4>));
-        else if (bx == 4 && by == 8) occ(reinterpret_cast<const void*>(fwd_fp32c<4, 8>));
-        else if (bx == 8 && by == 4) occ(reinterpret_cast<const void*>(fwd_fp32c<8, 4>));
-        else occ(reinterpret_cast<const void*>(fwd_fp32c<1, 1>));
+        if (bx == 4 && by == 4) occ(reinterpret_cast<const void*>(fwd_fp32c<4, 4>), kFwdThreads);
+        else if (bx == 4 && by == 8) occ(reinterpret_cast<const void*>(fwd_fp32c<4, 8>), kFwdThreads);
+        else if (bx == 8 && by == 4) occ(reinterpret_cast<const void*>(fwd_fp32c<8, 4>), kFwdThreads);
+        else occ(reinterpret_cast<const void*>(fwd_fp32c<1, 1>), kFwdThreads);
     } else {
-        if (bx == 4 && by == 4) occ(reinterpret_cast<const void*>(adj_fp32c<4, 4>));
-        else if (bx == 4 && by == 8) occ(reinterpret_cast<const void*>(adj_fp32c<4, 8>));
-        else if (bx == 8 && by == 4) occ(reinterpret_cast<const void*>(adj_fp32c<8, 4>));
-        else if (bx == 8 && by == 8) occ(reinterpret_cast<const void*>(adj_fp32c<8, 8>));
-        else occ(reinterpret_cast<const void*>(adj_fp32c<1, 1>));
+        if (bx == 4 && by == 4) occ(reinterpret_cast<const void*>(adj_fp32c<4, 4>), kThreads);
+        else if (bx == 4 && by == 8) occ(reinterpret_cast<const void*>(adj_fp32c<4, 8>), kThreads);
+        else if (bx == 8 && by == 4) occ(reinterpret_cast<const void*>(adj_fp32c<8, 4>), kThreads);
+        else if (bx == 8 && by == 8) occ(reinterpret_cast<const void*>(adj_fp32c<8, 8>), kThreads);
+        else occ(reinterpret_cast<const void*>(adj_fp32c<1, 1>), kThreads);
     }
     if (e != hipSuccess) {
         (void)hipGetLastError();

@@ -12,9 +12,10 @@ Two knobs decide how a call maps onto the 256 CUs:
   more pieces let the CUs that finish early take more work.
 
 Each candidate's whole call (staging, pair kernels, reduction) is timed with stream
-events over back-to-back calls, the fastest is installed on the context
-(crs_set_block_shape, crs_set_tuning) and remembered in a JSON cache keyed by the shape
-of the problem.
+events over back-to-back calls.  Among the candidates within TIE of the fastest, the one
+with the fewest work-groups wins (fewer partial slots to write and reduce for the same
+time); it is installed on the context (crs_set_block_shape, crs_set_tuning) and
+remembered in a JSON cache keyed by the shape of the problem.
 """
 from __future__ import annotations
 
@@ -25,6 +26,7 @@ from typing import Dict, Sequence
 from .rs import Context, Plane
 
 TARGETS = (0, 2048, 4096, 8192, 16384)  # 0: the library's built-in size
+TIE = 0.005  # candidates this close to the fastest count as equally fast
 FWD_SHAPES = ((4, 4), (4, 8), (8, 4))
 ADJ_SHAPES = ((4, 4), (4, 8), (8, 4), (8, 8))
 CACHE = os.environ.get("CELERIS_RS_TUNE_CACHE",
@@ -59,6 +61,14 @@ def _install(ctx: Context, mode: str, best: dict) -> None:
     if mode == "fp32c":
         ctx.set_block_shape(fwd=tuple(best["fwd_shape"]), adj=tuple(best.get("adj_shape", (0, 0))))
     ctx.set_tuning(mode, best["fwd"], best.get("adj", 0))
+
+
+def pick(table: Dict[str, dict], tie: float = TIE) -> dict:
+    """The candidate with the fewest work-groups among those within `tie` of the fastest
+    (then the fastest of those)."""
+    fastest = min(r["ms"] for r in table.values())
+    near = [r for r in table.values() if r["ms"] <= fastest * (1.0 + tie)]
+    return min(near, key=lambda r: (r.get("groups", 0), r["ms"]))
 
 
 def autotune(ctx: Context, px, py, t, planes: Sequence[Plane], u=None, mode: str = "fp32c",
@@ -107,7 +117,8 @@ def autotune(ctx: Context, px, py, t, planes: Sequence[Plane], u=None, mode: str
                 ctx.set_block_shape(fwd=shape, adj=(0, 0))
             ctx.set_tuning(mode, target, 0)
             ms = timed(lambda: ctx.forward(px, py, t, planes, mode=mode))
-            fwd_ms[f"{shape[0]}x{shape[1]}/{target}"] = {"ms": ms, "kernel": ctx.last_launch(False)[0],
+            name, groups = ctx.last_launch(False)
+            fwd_ms[f"{shape[0]}x{shape[1]}/{target}"] = {"ms": ms, "kernel": name, "groups": groups,
                                                           "shape": shape, "target": target}
     if u is not None:
         for shape in adj_shapes:
@@ -116,12 +127,13 @@ def autotune(ctx: Context, px, py, t, planes: Sequence[Plane], u=None, mode: str
                     ctx.set_block_shape(fwd=(0, 0), adj=shape)
                 ctx.set_tuning(mode, 0, target)
                 ms = timed(lambda: ctx.adjoint(px, py, u, planes, nsets=nsets, mode=mode))
-                adj_ms[f"{shape[0]}x{shape[1]}/{target}"] = {"ms": ms, "kernel": ctx.last_launch(True)[0],
+                name, groups = ctx.last_launch(True)
+                adj_ms[f"{shape[0]}x{shape[1]}/{target}"] = {"ms": ms, "kernel": name, "groups": groups,
                                                               "shape": shape, "target": target}
-    bf = min(fwd_ms.values(), key=lambda r: r["ms"])
+    bf = pick(fwd_ms)
     best = {"fwd": bf["target"], "fwd_shape": list(bf["shape"]), "fwd_ms": fwd_ms}
     if adj_ms:
-        ba = min(adj_ms.values(), key=lambda r: r["ms"])
+        ba = pick(adj_ms)
         best.update({"adj": ba["target"], "adj_shape": list(ba["shape"]), "adj_ms": adj_ms})
     _install(ctx, mode, best)
     if use_cache:

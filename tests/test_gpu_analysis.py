@@ -124,8 +124,10 @@ def test_analyze_field_grid(lens, mode, tol):
     assert np.allclose(got[:, 2], ref[:, 2], rtol=tol, atol=0)
     # selftest [21] invariants (selftest.cpp:908-929): centre node is the reference,
     # 4-corner symmetry of a rotationally symmetric lens
+    # (FP32C: the reference peak and the centre node come from calls of different shapes,
+    # whose pillar ranges are cut differently, so they agree to the mode's tolerance.)
     c = g.n // 2
-    assert got[c * g.n + c, 2] == pytest.approx(1.0, abs=1e-9)
+    assert got[c * g.n + c, 2] == pytest.approx(1.0, abs=1e-9 if mode == "fp64" else tol)
     corners = got[[0, g.n - 1, (g.n - 1) * g.n, g.n * g.n - 1], 2]
     assert corners.max() - corners.min() < 1e-3
 
