@@ -36,7 +36,8 @@ constexpr int kThreadsPerWg = 256;
 
 // MODE 0: 48 FP32 FMAs per iteration (16 chains of 3); 1: 16 v_sin_f32 (16 chains);
 // 2: 16 v_sin_f32 then 48 FP32 FMAs (grouped, as the pair loops group them); 3: 48 FP64
-// FMAs (8 chains of 6).  At most 64 VGPRs, so that kW waves fit on a SIMD.
+// FMAs (8 chains of 6); 4: 48 FP32 adds; 5: 48 FP32 multiplies.  At most 64 VGPRs, so
+// that kW waves fit on a SIMD.
 template <int MODE>
 __global__ __launch_bounds__(kThreadsPerWg, kW) void probe(long long* cycles, float* sink,
                                                            int iters, float s) {
@@ -78,6 +79,20 @@ __global__ __launch_bounds__(kThreadsPerWg, kW) void probe(long long* cycles, fl
                 acc[i & 7] = fmaf(t[i], s, acc[i & 7]);
                 x[i] = fmaf(x[i], s, 0.001f);
                 acc[i & 7] = fmaf(x[i], acc[i & 7], s);
+            }
+        } else if constexpr (MODE == 4) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                acc[i] = acc[i] + x[i];
+                x[i] = x[i] + s;
+                acc[i] = acc[i] + x[i];
+            }
+        } else if constexpr (MODE == 5) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                acc[i] = acc[i] * x[i];
+                x[i] = x[i] * s;
+                acc[i] = acc[i] * x[i];
             }
         } else {
 #pragma unroll
@@ -166,6 +181,8 @@ int main(int argc, char** argv) {
     const Result sin32 = run<1>(nwg, iters, 16, d_cyc, d_sink);
     const Result mix = run<2>(nwg, iters, 64, d_cyc, d_sink);
     const Result fma64 = run<3>(nwg, iters, 48, d_cyc, d_sink);
+    const Result add32 = run<4>(nwg, iters, 48, d_cyc, d_sink);
+    const Result mul32 = run<5>(nwg, iters, 48, d_cyc, d_sink);
     CHECK(hipFree(d_cyc));
     CHECK(hipFree(d_sink));
     // mix: cycles per 64-instruction iteration against the sum of its parts
@@ -181,6 +198,10 @@ int main(int argc, char** argv) {
     item("v_sin_f32", sin32);
     std::printf(", ");
     item("v_fma_f64", fma64);
+    std::printf(", ");
+    item("v_add_f32", add32);
+    std::printf(", ");
+    item("v_mul_f32", mul32);
     std::printf(", ");
     item("mix_16sin_48fma", mix);
     std::printf(", \"mix_iteration_cycles\": %.1f, \"sum_of_parts_cycles\": %.1f}\n", mix_iter, parts);
