@@ -132,6 +132,35 @@ def cpu_baseline(lens, u_np, budget_s: float) -> dict:
     }
 
 
+def reference_cpu_forward(lens, budget_s: float):
+    """The reference's own CPU forward, celeris::propagate_pillars (focal.cpp:205-238,
+    rows over std::thread::hardware_concurrency() workers; built from its sources into
+    oracle/_ref by oracle/ref.mk with the reference's Release flags), on every k-th pillar
+    of the workload over the full focal grid, k sized from a calibration run so the timed
+    run takes about budget_s.  Returns (pairs, seconds, k) or None where that build is
+    absent."""
+    import oracle
+
+    if not oracle.ref_available():
+        return None
+    plane = lens.planes[0]
+    n = plane.nx
+    W = plane.sx * (n - 1) / 2.0
+    cx, cy = plane.x0 + W, plane.y0 + W
+    lam = lens.wavelengths[0]
+
+    def run(k):
+        px, py, t = lens.px[::k], lens.py[::k], lens.t.reshape(-1)[: lens.npil][::k]
+        t0 = time.perf_counter()
+        oracle.ref_propagate_pillars(px, py, t, cx, cy, plane.z0, lam, n, W)
+        return px.size * n * n, time.perf_counter() - t0
+
+    p_cal, t_cal = run(64)
+    k = max(1, int(round(64 * t_cal / max(budget_s, 1e-3))))
+    pairs, secs = run(k)
+    return pairs, secs, k
+
+
 def profiled_traffic(launches):
     """HBM bytes of one forward + one adjoint pair-kernel launch, from the rocprofv3
     FETCH_SIZE / WRITE_SIZE summary in profiles/ (scaled by its copy-kernel calibration)
@@ -229,6 +258,7 @@ def main():
     dev = torch.device("cuda:0")
     lens = configs.config2(n=args.grid) if args.lens == "headline" else configs.config1()
     plane = lens.planes[0]
+    n_grid = plane.nx
     npix = plane.npix
     pairs = lens.npil * npix
     px = torch.from_numpy(lens.px).to(dev)
@@ -316,9 +346,33 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(lens, u.cpu().numpy(), args.cpu_seconds)
-        print(f"[bench] cpu baseline {cpu['value']:.3f} G pairs/s on {cpu['cores']} threads",
-              file=sys.stderr, flush=True)
+        port = cpu_baseline(lens, u.cpu().numpy(), args.cpu_seconds)
+        cpu = port
+        ref = reference_cpu_forward(lens, 0.5 * args.cpu_seconds)
+        if ref is not None:
+            # The reference's forward per pair, with the adjoint leg (which the reference
+            # does not have) timed on the oracle's FP64 restatement.
+            p_ref, t_ref, k_ref = ref
+            tau = t_ref / p_ref + port["adj_s_per_pair"]
+            cpu = {
+                "value": 1.0 / tau / 1e9,
+                "unit": "G pairs/s",
+                "cores": port["cores"],
+                "kind": "reference",
+                "sample": (f"reference propagate_pillars (focal.cpp:205-238, oracle/_ref build, "
+                           f"{port['cores']} hardware threads) on {_every(k_ref, 'pillar')} x "
+                           f"{n_grid}^2 ({p_ref:.3e} pairs, {t_ref:.2f} s) for the forward; the "
+                           f"adjoint, absent from the reference, from the oracle's FP64 "
+                           f"restatement ({port['adj_s_per_pair'] * 1e9:.3f} ns per pair); "
+                           f"scaled per pair"),
+                "cpu_model": port["cpu_model"],
+                "fwd_s_per_pair": t_ref / p_ref,
+                "adj_s_per_pair": port["adj_s_per_pair"],
+                "port": {k: port[k] for k in ("value", "kind", "sample", "fwd_s_per_pair",
+                                               "adj_s_per_pair", "single_thread_G_pairs_s")},
+            }
+        print(f"[bench] cpu baseline {cpu['value']:.3f} G pairs/s ({cpu['kind']}) on {cpu['cores']} "
+              f"threads; port {port['value']:.3f}", file=sys.stderr, flush=True)
 
     if args.calibrate_copy > 0:
         n = args.calibrate_copy // 4

@@ -5,6 +5,8 @@ The hot path of the reference's focal, through-focus and chromatic analyses
 hand-written HIP kernels for gfx950 behind a C ABI (include/celeris_rs.h).
 """
 from .rs import (  # noqa: F401
+    KERNEL_REF,
+    KERNEL_RSI,
     Context,
     Plane,
     available,
@@ -17,6 +19,8 @@ from .rs import (  # noqa: F401
 )
 
 __all__ = [
+    "KERNEL_REF",
+    "KERNEL_RSI",
     "Context",
     "Plane",
     "available",

@@ -41,7 +41,7 @@ class crs_plane(ctypes.Structure):
         ("nx", c_int32),
         ("ny", c_int32),
         ("tset", c_int32),
-        ("reserved", c_int32),
+        ("kernel", c_int32),
         ("offset", c_int64),
     ]
 

@@ -330,8 +330,16 @@ crs_status check_args(int mode, int npil, int nsets, const crs_plane* planes, in
             set_error("plane %d: tset %d outside [0, %d)", p, P.tset, nsets);
             return CRS_ERR_INVALID;
         }
-        if (P.offset < 0 || P.reserved != 0) {
-            set_error("plane %d: negative offset or nonzero reserved field", p);
+        if (P.offset < 0) {
+            set_error("plane %d: negative offset", p);
+            return CRS_ERR_INVALID;
+        }
+        if (P.kernel != CRS_KERNEL_REF && P.kernel != CRS_KERNEL_RSI) {
+            set_error("plane %d: unknown kernel %d", p, P.kernel);
+            return CRS_ERR_INVALID;
+        }
+        if (P.kernel == CRS_KERNEL_RSI && mode == CRS_MODE_FP32C) {
+            set_error("plane %d: the RS-I kernel runs in CRS_MODE_FP64 or CRS_MODE_FP32 only", p);
             return CRS_ERR_INVALID;
         }
     }
@@ -391,6 +399,7 @@ DevPlane make_dev_plane(const crs_plane& P) {
     d.by = 1;
     d.obl1 = 1.0;
     d.obl2 = 1.0;
+    d.kernel = P.kernel;
     return d;
 }
 

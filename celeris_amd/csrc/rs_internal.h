@@ -22,6 +22,9 @@ constexpr int kQuad = 4;
 // FP32 / FP64 forward: shortest pillar chunk of a split.
 constexpr int kMinChunk = 64;
 
+// Plane kernel selector (crs_kernel): RS-I, exp(ikR)/R (z/R)(1/R - ik).
+constexpr int32_t kKernelRsi = 1;
+
 // ---- FP32C ------------------------------------------------------------------------
 // Pixels are handled in blocks of BX columns x BY rows (BX, BY in {4, 8}) that share
 // one FP64 reference distance per pillar, or 1 x 1 where a plane is too close or too
@@ -107,6 +110,8 @@ struct DevPlane {
     // series orders kP1A1 / kP1A2 meet the pair budget for the plane's block shape (-1:
     // never); beyond obl2, kP2A2, which the host admitted at any obliquity.
     double obl1, obl2;
+    int32_t kernel;  // crs_kernel: 0 exp(ikR)/R, 1 RS-I (FP32 / FP64 kernels only)
+    int32_t pad;
 };
 
 // ---- FP32 / FP64 work decomposition (split sums, uniform slots) --------------------

@@ -473,6 +473,122 @@ __global__ __launch_bounds__(kFwdThreads, fwd_waves_per_simd(KBX * KBY)) void fw
 }
 
 // ---------------------------------------------------------------------------------
+// Pair kernels G(R) of the FP32 / FP64 paths.  RSI = false: exp(ikR)/R, the reference's
+// (focal.cpp:232, propagate.cu:71-77).  RSI = true: Rayleigh-Sommerfeld I,
+// exp(ikR)/R * (z/R) * (1/R - ik) = exp(ikR) * (z/R^2) * (1/R - ik), opt-in per plane.
+
+template <bool RSI>
+__device__ __forceinline__ void green64(double dx, double dy, double z, double k, double& gr,
+                                        double& gi) {
+    const double R = sqrt(dx * dx + dy * dy + z * z);
+    double s, c;
+    sincos(k * R, &s, &c);
+    const double rho = 1.0 / R;
+    if constexpr (RSI) {
+        const double a = z * rho * rho;           // z / R^2
+        const double fr = a * rho, fi = -a * k;   // (z / R^2) (1/R - ik)
+        gr = c * fr - s * fi;
+        gi = s * fr + c * fi;
+    } else {
+        gr = rho * c;
+        gi = rho * s;
+    }
+}
+
+template <bool RSI>
+__device__ __forceinline__ void green32(float dx, float dy, float z, float k, float& gr,
+                                        float& gi) {
+    const float R = sqrtf(dx * dx + dy * dy + z * z);
+    float s, c;
+    sincosf(k * R, &s, &c);
+    const float inv = 1.0f / R;
+    if constexpr (RSI) {
+        const float a = z * inv * inv;
+        const float fr = a * inv, fi = -a * k;
+        gr = c * fr - s * fi;
+        gi = s * fr + c * fi;
+    } else {
+        gr = c * inv;
+        gi = s * inv;
+    }
+}
+
+// One LDS tile of pillars against one pixel (forward): E += t G; the reference
+// kernel in the reference's own operation order, (t_r c - t_i s) / R (propagate.cu:76-77).
+template <bool RSI>
+__device__ __forceinline__ void fwd_tile32(const float4* tile, int n, float fx, float fy, float z,
+                                           float k, float& er, float& ei) {
+    for (int q = 0; q < n; ++q) {
+        const float4 Q = tile[q];
+        if constexpr (RSI) {
+            float gr, gi;
+            green32<true>(fx - Q.x, fy - Q.y, z, k, gr, gi);
+            er += Q.z * gr - Q.w * gi;
+            ei += Q.z * gi + Q.w * gr;
+        } else {
+            const float dx = fx - Q.x, dy = fy - Q.y;
+            const float R = sqrtf(dx * dx + dy * dy + z * z);
+            float s, c;
+            sincosf(k * R, &s, &c);
+            const float inv = 1.0f / R;
+            er += (Q.z * c - Q.w * s) * inv;
+            ei += (Q.z * s + Q.w * c) * inv;
+        }
+    }
+}
+
+template <bool RSI>
+__device__ __forceinline__ void fwd_tile64(const double4* tile, int n, double fx, double fy,
+                                           double z, double k, double& er, double& ei) {
+    for (int q = 0; q < n; ++q) {
+        const double4 Q = tile[q];
+        double gr, gi;
+        green64<RSI>(fx - Q.x, fy - Q.y, z, k, gr, gi);
+        er += Q.z * gr - Q.w * gi;
+        ei += Q.z * gi + Q.w * gr;
+    }
+}
+
+// One LDS tile of pixel cotangents against one pillar (adjoint): g += conj(G) u.
+template <bool RSI>
+__device__ __forceinline__ void adj_tile32(const float2* tile, int n, float x0, float sx, int base,
+                                           float qx, float dy, float z, float k, float& gr,
+                                           float& gi) {
+    for (int p = 0; p < n; ++p) {
+        const float2 U = tile[p];
+        const float fx = x0 + static_cast<float>(base + p) * sx;
+        if constexpr (RSI) {
+            float cr, ci;
+            green32<true>(fx - qx, dy, z, k, cr, ci);
+            gr += cr * U.x + ci * U.y;
+            gi += cr * U.y - ci * U.x;
+        } else {
+            const float dx = fx - qx;
+            const float R = sqrtf(dx * dx + dy * dy + z * z);
+            float s, c;
+            sincosf(k * R, &s, &c);
+            const float inv = 1.0f / R;
+            gr += (c * U.x + s * U.y) * inv;
+            gi += (c * U.y - s * U.x) * inv;
+        }
+    }
+}
+
+template <bool RSI>
+__device__ __forceinline__ void adj_tile64(const double2* tile, int n, double x0, double sx,
+                                           int base, double qx, double dy, double z, double k,
+                                           double& gr, double& gi) {
+    for (int p = 0; p < n; ++p) {
+        const double2 U = tile[p];
+        const double fx = x0 + (base + p) * sx;
+        double cr, ci;
+        green64<RSI>(fx - qx, dy, z, k, cr, ci);
+        gr += cr * U.x + ci * U.y;
+        gi += cr * U.y - ci * U.x;
+    }
+}
+
+// ---------------------------------------------------------------------------------
 // Forward, FP32 (the reference kernel's arithmetic, propagate.cu:68-78).  One pixel
 // per thread.
 
@@ -499,16 +615,10 @@ __global__ __launch_bounds__(kThreads) void fwd_fp32(
         __syncthreads();
         if (tid < n) tile[tid] = src[base + tid];
         __syncthreads();
-        for (int q = 0; q < n; ++q) {
-            const float4 Q = tile[q];
-            const float dx = fx - Q.x, dy = fy - Q.y;
-            const float R = sqrtf(dx * dx + dy * dy + z * z);
-            float s, c;
-            sincosf(k * R, &s, &c);
-            const float inv = 1.0f / R;
-            er += (Q.z * c - Q.w * s) * inv;
-            ei += (Q.z * s + Q.w * c) * inv;
-        }
+        if (P.kernel == kKernelRsi)
+            fwd_tile32<true>(tile, n, fx, fy, z, k, er, ei);
+        else
+            fwd_tile32<false>(tile, n, fx, fy, z, k, er, ei);
     }
     if (active) partial[B.part + pix] = make_double2(er, ei);
 }
@@ -540,17 +650,10 @@ __global__ __launch_bounds__(kThreads) void fwd_fp64(
         __syncthreads();
         if (tid < n) tile[tid] = src[base + tid];
         __syncthreads();
-        for (int q = 0; q < n; ++q) {
-            const double4 Q = tile[q];
-            const double dx = fx - Q.x, dy = fy - Q.y;
-            const double R = sqrt(dx * dx + dy * dy + z * z);
-            double s, c;
-            sincos(k * R, &s, &c);
-            const double rho = 1.0 / R;
-            const double pr = rho * c, pim = rho * s;
-            er += Q.z * pr - Q.w * pim;
-            ei += Q.z * pim + Q.w * pr;
-        }
+        if (P.kernel == kKernelRsi)
+            fwd_tile64<true>(tile, n, fx, fy, z, k, er, ei);
+        else
+            fwd_tile64<false>(tile, n, fx, fy, z, k, er, ei);
     }
     if (active) partial[B.part + pix] = make_double2(er, ei);
 }
@@ -950,18 +1053,11 @@ __global__ __launch_bounds__(kThreads) void adj_fp32(
                     tile[tid] = make_float2(static_cast<float>(w.x), static_cast<float>(w.y));
                 }
                 __syncthreads();
-                for (int p = 0; p < n; ++p) {
-                    const float2 U = tile[p];
-                    const float fx = static_cast<float>(P.x0) +
-                                     static_cast<float>(base + p) * static_cast<float>(P.sx);
-                    const float dx = fx - v.x;
-                    const float R = sqrtf(dx * dx + dy * dy + z * z);
-                    float s, c;
-                    sincosf(k * R, &s, &c);
-                    const float inv = 1.0f / R;
-                    gr += (c * U.x + s * U.y) * inv;
-                    gi += (c * U.y - s * U.x) * inv;
-                }
+                const float x0 = static_cast<float>(P.x0), sx = static_cast<float>(P.sx);
+                if (P.kernel == kKernelRsi)
+                    adj_tile32<true>(tile, n, x0, sx, base, v.x, dy, z, k, gr, gi);
+                else
+                    adj_tile32<false>(tile, n, x0, sx, base, v.x, dy, z, k, gr, gi);
             }
         }
     }
@@ -992,18 +1088,10 @@ __global__ __launch_bounds__(kThreads) void adj_fp64(
                 __syncthreads();
                 if (tid < n) tile[tid] = urow[base + tid];
                 __syncthreads();
-                for (int p = 0; p < n; ++p) {
-                    const double2 U = tile[p];
-                    const double fx = P.x0 + (base + p) * P.sx;
-                    const double dx = fx - v.x;
-                    const double R = sqrt(dx * dx + dy * dy + z * z);
-                    double s, c;
-                    sincos(P.k * R, &s, &c);
-                    const double rho = 1.0 / R;
-                    const double pr = rho * c, pim = rho * s;
-                    gr += pr * U.x + pim * U.y;
-                    gi += pr * U.y - pim * U.x;
-                }
+                if (P.kernel == kKernelRsi)
+                    adj_tile64<true>(tile, n, P.x0, P.sx, base, v.x, dy, z, P.k, gr, gi);
+                else
+                    adj_tile64<false>(tile, n, P.x0, P.sx, base, v.x, dy, z, P.k, gr, gi);
             }
         }
     }

@@ -25,6 +25,12 @@ from . import _lib
 from ._lib import MODE_FP32, MODE_FP32C, MODE_FP64, MODES, crs_plane
 
 
+# Plane kernels (crs_kernel): the reference's exp(ikR)/R, and Rayleigh-Sommerfeld I,
+# exp(ikR)/R (z/R)(1/R - ik) (FP64 / FP32 modes only).
+KERNEL_REF = 0
+KERNEL_RSI = 1
+
+
 def _mode(mode) -> int:
     if isinstance(mode, str):
         try:
@@ -49,6 +55,7 @@ class Plane:
     ny: int
     tset: int = 0
     offset: int = -1  # -1: placed after the previous plane by pack_planes
+    kernel: int = 0   # KERNEL_REF: exp(ikR)/R (the reference's); KERNEL_RSI: RS-I, opt-in
 
     @property
     def npix(self) -> int:
@@ -90,7 +97,7 @@ def pack_planes(planes: Sequence[Plane]):
     total = 0
     for idx, p in enumerate(planes):
         off = cursor if p.offset < 0 else p.offset
-        arr[idx] = crs_plane(p.k, p.x0, p.sx, p.y0, p.sy, p.z0, p.sz, p.nx, p.ny, p.tset, 0, off)
+        arr[idx] = crs_plane(p.k, p.x0, p.sx, p.y0, p.sy, p.z0, p.sz, p.nx, p.ny, p.tset, p.kernel, off)
         cursor = off + p.npix
         total = max(total, off + p.npix)
     return arr, total

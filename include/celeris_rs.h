@@ -59,13 +59,25 @@ typedef enum crs_mode {
     CRS_MODE_FP32 = 2
 } crs_mode;
 
+/* Propagation kernels G_pq (per plane). */
+typedef enum crs_kernel {
+    /* exp(i k R) / R: the reference's kernel (propagate.hpp:4-5, focal.cpp:232).
+     * The default, and the only one the drop-in layer uses. */
+    CRS_KERNEL_REF = 0,
+    /* Rayleigh-Sommerfeld I: exp(i k R) / R * (z / R) * (1 / R - i k), the z-derivative
+     * form -d/dz [exp(i k R) / R] (no reference counterpart; opt-in).  CRS_MODE_FP64
+     * and CRS_MODE_FP32 only: an FP32C call with such a plane returns CRS_ERR_INVALID. */
+    CRS_KERNEL_RSI = 1
+} crs_kernel;
+
 /* One focal plane (an "instance": one wavelength, one pixel lattice).
  * Pixel (i, j), 0 <= i < nx, 0 <= j < ny, sits at
  *     fx = x0 + i*sx,   fy = y0 + j*sy,   z = z0 + j*sz
  * so a psf window is (x0 = cx-W, sx = 2W/(n-1), y0 = cy-W, sy = sx, sz = 0) and a
  * y=0 caustic slice is (x0 = xmin, sx = xstep, y0 = sy = 0, z0 = zmin, sz = zstep).
  * Its field occupies elements [offset, offset + nx*ny) of the output arrays, row
- * j at offset + j*nx.  tset selects the transmission set (0 <= tset < nsets). */
+ * j at offset + j*nx.  tset selects the transmission set (0 <= tset < nsets);
+ * kernel is a crs_kernel (0 for the reference's). */
 typedef struct crs_plane {
     double k;
     double x0, sx;
@@ -73,7 +85,7 @@ typedef struct crs_plane {
     double z0, sz;
     int32_t nx, ny;
     int32_t tset;
-    int32_t reserved; /* must be 0 */
+    int32_t kernel;
     int64_t offset;
 } crs_plane;
 

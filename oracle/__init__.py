@@ -30,7 +30,7 @@ class orc_plane(ctypes.Structure):
     _fields_ = [
         ("k", c_double), ("x0", c_double), ("sx", c_double), ("y0", c_double),
         ("sy", c_double), ("z0", c_double), ("sz", c_double), ("nx", c_int32),
-        ("ny", c_int32), ("tset", c_int32), ("reserved", c_int32), ("offset", c_int64),
+        ("ny", c_int32), ("tset", c_int32), ("kernel", c_int32), ("offset", c_int64),
     ]
 
 
@@ -66,7 +66,8 @@ def _planes(planes):
     cursor, total = 0, 0
     for i, p in enumerate(planes):
         off = cursor if p.offset < 0 else p.offset
-        arr[i] = orc_plane(p.k, p.x0, p.sx, p.y0, p.sy, p.z0, p.sz, p.nx, p.ny, p.tset, 0, off)
+        arr[i] = orc_plane(p.k, p.x0, p.sx, p.y0, p.sy, p.z0, p.sz, p.nx, p.ny, p.tset,
+                           getattr(p, "kernel", 0), off)
         cursor = off + p.nx * p.ny
         total = max(total, cursor)
     return arr, total

@@ -16,6 +16,8 @@
  *       fx = xmin + ix*xstep, z = zmin + iz*zstep, dy = y_pillar
  *   - the adjoint g_q = sum_p conj(G_pq) u_p has no reference counterpart; it is
  *     the conjugate transpose of the sum above, evaluated with the same G_pq.
+ *   - the opt-in Rayleigh-Sommerfeld I kernel (plane kernel 1) has no reference
+ *     counterpart either: G = exp(ikR)/R (z/R)(1/R - ik) = -d/dz [exp(ikR)/R].
  *   - the pupil autocorrelation of analyze_mtf     src/celeris/analysis/mtf.cpp:40-49
  *       acc(s) = sum_iy sum_{ix+s<n} P[iy][ix] conj(P[iy][ix+s]), rows then columns
  *   - the fill jitter of analyze_tolerance         src/celeris/analysis/tolerance.cpp:20-31
@@ -34,7 +36,9 @@
  * committed golden vectors produced by that build, against closed forms (single
  * pillar, in-phase focus, linearity, mirror symmetry), against the long-double
  * brute force below, and the adjoint against the dot-product identity and
- * finite differences of the forward.
+ * finite differences of the forward.  The RS-I kernel is pinned by the on-axis
+ * single-pillar closed form exp(ikz)(1/z - ik)/z, by central differences in z of
+ * the reference kernel (G_RSI = -dG/dz), and by the long-double brute force.
  */
 #include <math.h>
 #include <pthread.h>
@@ -50,7 +54,7 @@ typedef struct orc_plane {
     double z0, sz;
     int32_t nx, ny;
     int32_t tset;
-    int32_t reserved;
+    int32_t kernel; /* 0: exp(ikR)/R; 1: RS-I, exp(ikR)/R (z/R)(1/R - ik) */
     int64_t offset;
 } orc_plane;
 
@@ -59,14 +63,25 @@ static int online_cpus(void) {
     return n > 0 ? (int)n : 1;
 }
 
-/* Pair kernel of the reference: G = polar(1/R, k R), R from the plane point. */
+/* Pair kernel of the reference: G = polar(1/R, k R), R from the plane point
+ * (focal.cpp:230-232).  kernel 1, which the reference does not have: the
+ * Rayleigh-Sommerfeld I kernel G = exp(ikR)/R (z/R)(1/R - ik), evaluated as
+ * polar(1/R, k R) times the factor (z/R^2)(1/R - ik). */
 static inline void pair_g(double fx, double fy, double z, double k, double qx,
-                          double qy, double *gr, double *gi) {
+                          double qy, int kernel, double *gr, double *gi) {
     const double dx = fx - qx, dy = fy - qy;
     const double R = sqrt(dx * dx + dy * dy + z * z);
     const double rho = 1.0 / R, th = k * R;
-    *gr = rho * cos(th);
-    *gi = rho * sin(th);
+    const double pr = rho * cos(th), pi_ = rho * sin(th);
+    if (kernel == 1) {
+        const double a = z * rho;                 /* z / R */
+        const double fr = a * rho, fi = -a * k;   /* (z/R)(1/R - ik) */
+        *gr = pr * fr - pi_ * fi;
+        *gi = pr * fi + pi_ * fr;
+    } else {
+        *gr = pr;
+        *gi = pi_;
+    }
 }
 
 /* ------------------------------------------------------------------------- */
@@ -126,7 +141,7 @@ static void fwd_row(fwd_job *J, int64_t r) {
         double er = 0.0, ei = 0.0;
         for (int q = 0; q < J->npil; ++q) {
             double pr, pim;
-            pair_g(fx, fy, z, k, J->px[q], J->py[q], &pr, &pim);
+            pair_g(fx, fy, z, k, J->px[q], J->py[q], P->kernel, &pr, &pim);
             const double tr = t[2 * q], ti = t[2 * q + 1];
             er += tr * pr - ti * pim;
             ei += tr * pim + ti * pr;
@@ -213,7 +228,7 @@ static void *adj_worker(void *arg) {
                     for (int i = 0; i < P->nx; ++i) {
                         const double fx = P->x0 + i * P->sx;
                         double cr, ci;
-                        pair_g(fx, fy, z, P->k, qx, qy, &cr, &ci);
+                        pair_g(fx, fy, z, P->k, qx, qy, P->kernel, &cr, &ci);
                         const double ur = u[2 * i], ui = u[2 * i + 1];
                         /* conj(G) * u = (cr - i ci)(ur + i ui) */
                         gr += cr * ur + ci * ui;
@@ -276,7 +291,13 @@ int orc_forward_ld(const double *px, const double *py, const double *t, int npil
                     const long double dx = fx - px[q], dy = fy - py[q];
                     const long double R = sqrtl(dx * dx + dy * dy + z * z);
                     const long double th = (long double)P->k * R;
-                    const long double pr = cosl(th) / R, pim = sinl(th) / R;
+                    long double pr = cosl(th) / R, pim = sinl(th) / R;
+                    if (P->kernel == 1) {
+                        const long double fr = z / (R * R), fi = -(long double)P->k * z / R;
+                        const long double gr = pr * fr - pim * fi;
+                        pim = pr * fi + pim * fr;
+                        pr = gr;
+                    }
                     er += tt[2 * q] * pr - tt[2 * q + 1] * pim;
                     ei += tt[2 * q] * pim + tt[2 * q + 1] * pr;
                 }

@@ -14,12 +14,13 @@ transposed index, a wrong pixel step, a missing conjugate) fails at least one:
 """
 import math
 import os
+from dataclasses import replace
 
 import numpy as np
 import pytest
 
 import oracle
-from celeris_amd import Plane, configs
+from celeris_amd import KERNEL_RSI, Plane, configs
 from celeris_amd.design import MetalensDesign, UnitCellLibrary, aperture_pillars
 from conftest import achromatic_golden, dispersive_golden, rel_l2
 
@@ -412,3 +413,49 @@ def test_oracle_chromatic_dispersive_matches_reference():
     # the fixed-transmission scan differs (the libraries really disperse)
     fixed = oracle.chromatic(px, py, t0[None], 0, [0] * n, f, lam, lmin, lmax, n)
     assert np.max(np.abs(fixed[:, 2] - ref[:, 2])) > 1e-3
+
+
+# ---- the opt-in RS-I kernel exp(ikR)/R (z/R)(1/R - ik) (no reference counterpart) ----
+
+def _rsi(p):
+    return replace(p, kernel=KERNEL_RSI)
+
+
+def test_rsi_single_pillar_on_axis_closed_form():
+    # Pixel straight above the pillar: R = z, z/R = 1, so G = exp(ikz) (1/z - ik) / z.
+    k = 2 * math.pi / 0.633
+    x, y, z, t = 1.3, -0.7, 37.0, 0.8 * np.exp(0.3j)
+    plane = _rsi(Plane(k, x, 0.0, y, 0.0, z, 0.0, 1, 1))
+    e = oracle.forward(np.array([x]), np.array([y]), np.array([[t]]), [plane])
+    exact = t * np.exp(1j * k * z) * (1.0 / z - 1j * k) / z
+    assert abs(e[0] - exact) <= 1e-14 * abs(exact)
+    # Far from the axis the factor tends to -ik z/R: |G| R / (k z/R) -> 1.
+    far = _rsi(Plane(k, x + 4000.0, 0.0, y, 0.0, z, 0.0, 1, 1))
+    g = oracle.forward(np.array([x]), np.array([y]), np.array([[1.0]]), [far])[0]
+    R = math.hypot(4000.0, z)
+    assert abs(abs(g) * R / (k * z / R) - 1.0) <= 1e-6
+
+
+def test_rsi_is_minus_z_derivative_of_reference_kernel():
+    # G_RSI = -d/dz [exp(ikR)/R]: central differences of the reference-kernel sum in the
+    # plane depth (tilted plane: every row moves by the same dz).
+    L = configs.small_lens(n_side=7, n=5)
+    base = Plane(2 * math.pi / 0.55, -1.0, 0.5, -0.8, 0.4, 20.0, 0.3, 5, 4)
+    h = 1e-4
+    ep = oracle.forward(L.px, L.py, L.t, [replace(base, z0=base.z0 + h)])
+    em = oracle.forward(L.px, L.py, L.t, [replace(base, z0=base.z0 - h)])
+    rsi = oracle.forward(L.px, L.py, L.t, [_rsi(base)])
+    assert rel_l2(rsi, -(ep - em) / (2 * h)) <= 1e-6
+
+
+def test_rsi_long_double_brute_force_and_adjoint():
+    L = configs.small_lens(n_side=9, n=5)
+    planes = [_rsi(p) for p in L.planes]
+    f = oracle.forward(L.px, L.py, L.t, planes)
+    fl = oracle.forward_ld(L.px, L.py, L.t, planes)
+    assert rel_l2(f, fl) <= 1e-12
+    # dot-product identity with the adjoint on the same RS-I planes
+    rng = np.random.default_rng(11)
+    u = rng.standard_normal(f.size) + 1j * rng.standard_normal(f.size)
+    g = oracle.adjoint(L.px, L.py, u, planes)[0]
+    assert abs(np.vdot(u, f) - np.vdot(g, L.t[0])) <= 1e-12 * abs(np.vdot(u, f))
