@@ -191,9 +191,15 @@ struct crs_context {
     int last_groups[2] = {0, 0};
     // Completion of the last call's work and its stream: a call on another stream first
     // waits for it (the scratch and tables are shared), and scratch is freed only after it.
+    // A call on a created stream records done_ev as it ends; a call on the null stream,
+    // which lives as long as the process, leaves null_pending set instead, and the event
+    // is recorded on the null stream only when a later call or a scratch release needs it
+    // (an event record costs ~4 us of GPU time per call, 12% of a 4,096-pillar x 81^2
+    // forward + adjoint step).
     hipEvent_t done_ev = nullptr;
     hipStream_t last_stream = nullptr;
     bool done_valid = false;
+    bool null_pending = false;
 };
 
 namespace {
@@ -219,8 +225,17 @@ constexpr int kAdjRounds = 8;
 // 0.6x the time of 3,696).
 constexpr int64_t kMinPiecePairs = 65536;
 
+// Records done_ev for a previous call on the null stream that has not recorded it yet.
+void settle_null_pending(crs_context* ctx) {
+    if (!ctx || !ctx->null_pending) return;
+    ctx->null_pending = false;
+    if (hipEventRecord(ctx->done_ev, nullptr) == hipSuccess) ctx->done_valid = true;
+    else hipGetLastError();
+}
+
 // The previous call's work must be finished before its scratch is freed.
 void wait_previous(crs_context* ctx) {
+    settle_null_pending(ctx);
     if (ctx && ctx->done_valid) hipEventSynchronize(ctx->done_ev);
 }
 
@@ -362,17 +377,25 @@ crs_status check_common(crs_context* ctx, int mode, int npil, int nsets, const c
 // Orders a call on `stream` after the context's previous call when that ran on another
 // stream: the scratch buffers and descriptor tables it reads are shared by the calls.
 void order_after_previous(crs_context* ctx, hipStream_t stream) {
-    if (ctx->done_valid && ctx->last_stream != stream && !stream_capturing(stream))
-        hipStreamWaitEvent(stream, ctx->done_ev, 0);
+    if (ctx->last_stream == stream || stream_capturing(stream)) return;
+    settle_null_pending(ctx);
+    if (ctx->done_valid) hipStreamWaitEvent(stream, ctx->done_ev, 0);
 }
 
 // Marks the end of a call's device work on `stream` (outside capture).
 void record_done(crs_context* ctx, hipStream_t stream) {
     if (stream_capturing(stream)) return;
+    ctx->last_stream = stream;
+    if (stream == nullptr) {
+        ctx->null_pending = true;
+        ctx->done_valid = false;
+        return;
+    }
+    ctx->null_pending = false;
     if (hipEventRecord(ctx->done_ev, stream) == hipSuccess) {
-        ctx->last_stream = stream;
         ctx->done_valid = true;
     } else {
+        ctx->done_valid = false;
         hipGetLastError();
     }
 }

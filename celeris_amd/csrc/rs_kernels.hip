@@ -438,13 +438,14 @@ __global__ __launch_bounds__(kFwdThreads, fwd_waves_per_simd(KBX * KBY)) void fw
         if constexpr (KBX * KBY == 1) {
             fwd_segment<1, 1, 1, 1>(pos, aps, P, T, S.q0, S.q1, kP1A1, tile, run, slot);
         } else {
-            // Series order: class 0 from the segment's pillar extent (the staged extents
-            // of the kExtTile runs it covers); the smaller edge blocks take kP2A2, which
-            // the host admitted for the full block shape at any obliquity.
-            // The extent loop runs on uniform addresses (scalar loads, no lane shuffles:
-            // shuffle indices held across the segment loop used to spill to scratch).
+            // Series order from the segment's pillar extent (the staged extents of the
+            // kExtTile runs it covers) and the plane's obliquity limits for the full
+            // KBX x KBY block; the 1 x KBY and KBX x 1 edge blocks share every offset
+            // bound of the full block and take the same order.  The extent loop runs on
+            // uniform addresses (scalar loads, no lane shuffles: shuffle indices held
+            // across the segment loop used to spill to scratch).
             int order = kP2A2;
-            if (T.shape == 0) {
+            if (T.shape != 3) {
                 double mx = 0.0, my = 0.0;
                 const int e0 = S.q0 / kExtTile, e1 = (S.q1 - 1) / kExtTile;
                 for (int i = e0; i <= e1; ++i) {
@@ -459,10 +460,10 @@ __global__ __launch_bounds__(kFwdThreads, fwd_waves_per_simd(KBX * KBY)) void fw
                     fwd_segment<KBX, KBY, KBX, KBY>(pos, aps, P, T, S.q0, S.q1, order, tile, run, slot);
                     break;
                 case 1:
-                    fwd_segment<1, KBY, KBX, KBY>(pos, aps, P, T, S.q0, S.q1, kP2A2, tile, run, slot);
+                    fwd_segment<1, KBY, KBX, KBY>(pos, aps, P, T, S.q0, S.q1, order, tile, run, slot);
                     break;
                 case 2:
-                    fwd_segment<KBX, 1, KBX, KBY>(pos, aps, P, T, S.q0, S.q1, kP2A2, tile, run, slot);
+                    fwd_segment<KBX, 1, KBX, KBY>(pos, aps, P, T, S.q0, S.q1, order, tile, run, slot);
                     break;
                 default:
                     fwd_segment<1, 1, KBX, KBY>(pos, aps, P, T, S.q0, S.q1, kP1A1, tile, run, slot);
@@ -1132,41 +1133,54 @@ __global__ __launch_bounds__(kThreads) void adj_reduce(const T* __restrict__ par
 }
 
 // Adjoint reduction, FP32C: g[s npil + q] = sum over the slots listed for q's group
-// (slot_off[s ngroups + grp] .. [+1]) of partial[slot * kThreads + q - grp kThreads],
-// in list order (FP64).
+// (slot_off[s ngroups + grp] .. [+1]) of partial[slot * kThreads + q - grp kThreads]
+// (FP64).  A work-group takes 64 pillars (a quarter of a group); wave w sums the w-th
+// quarter of the group's slot list (eight loads in flight) and wave 0 adds the four
+// quarter sums in wave order, so the result does not depend on timing.
 __global__ __launch_bounds__(kThreads) void adj_reduce_fp32c(
     const float2* __restrict__ partial, const int32_t* __restrict__ slot_off,
-    const int32_t* __restrict__ slots, int npil, int ngroups, int64_t n,
-    double2* __restrict__ g) {
-    const int64_t idx = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
-    if (idx >= n) return;
-    const int s = static_cast<int>(idx / npil);
-    const int q = static_cast<int>(idx - static_cast<int64_t>(s) * npil);
-    const int grp = q / kThreads, lane = q - grp * kThreads;
-    const int k0 = slot_off[s * ngroups + grp], k1 = slot_off[s * ngroups + grp + 1];
+    const int32_t* __restrict__ slots, int npil, int ngroups, double2* __restrict__ g) {
+    __shared__ double2 red[kWaves][64];
+    const int quarter = blockIdx.x & 3;
+    const int sg = blockIdx.x >> 2;  // s * ngroups + grp
+    const int s = sg / ngroups, grp = sg - s * ngroups;
+    const int wave = wave_id(), lane = threadIdx.x & 63;
+    const int lq = quarter * 64 + lane;  // pillar within the group
+    const int k0 = slot_off[sg], n = slot_off[sg + 1] - k0;
+    const int a0 = k0 + (wave * n) / kWaves, a1 = k0 + ((wave + 1) * n) / kWaves;
     double gr = 0.0, gi = 0.0;
-    // Eight slot indices, then their eight values, in flight at once; summed in list order.
     constexpr int U = 8;
-    int k = k0;
-    for (; k + U <= k1; k += U) {
+    int k = a0;
+    for (; k + U <= a1; k += U) {
         int sl[U];
         float2 v[U];
 #pragma unroll
         for (int j = 0; j < U; ++j) sl[j] = slots[k + j];
 #pragma unroll
-        for (int j = 0; j < U; ++j) v[j] = partial[static_cast<int64_t>(sl[j]) * kThreads + lane];
+        for (int j = 0; j < U; ++j) v[j] = partial[static_cast<int64_t>(sl[j]) * kThreads + lq];
 #pragma unroll
         for (int j = 0; j < U; ++j) {
             gr += v[j].x;
             gi += v[j].y;
         }
     }
-    for (; k < k1; ++k) {
-        const float2 v = partial[static_cast<int64_t>(slots[k]) * kThreads + lane];
+    for (; k < a1; ++k) {
+        const float2 v = partial[static_cast<int64_t>(slots[k]) * kThreads + lq];
         gr += v.x;
         gi += v.y;
     }
-    g[idx] = make_double2(gr, gi);
+    red[wave][lane] = make_double2(gr, gi);
+    __syncthreads();
+    const int q = grp * kThreads + lq;
+    if (wave != 0 || q >= npil) return;
+    gr = red[0][lane].x;
+    gi = red[0][lane].y;
+#pragma unroll
+    for (int w = 1; w < kWaves; ++w) {
+        gr += red[w][lane].x;
+        gi += red[w][lane].y;
+    }
+    g[static_cast<int64_t>(s) * npil + q] = make_double2(gr, gi);
 }
 
 // ---------------------------------------------------------------------------------
@@ -1304,9 +1318,10 @@ hipError_t launch_adjoint_fp32c(int bx, int by, const Fp32cAdjArgs& a, hipStream
 hipError_t launch_adj_reduce_fp32c(const float2* partial, const int32_t* slot_off,
                                    const int32_t* slots, int npil, int ngroups, int64_t n,
                                    double2* g, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    adj_reduce_fp32c<<<grid_for(n), kThreads, 0, s>>>(partial, slot_off, slots, npil, ngroups, n,
-                                                      g);
+    if (n <= 0 || npil <= 0) return hipSuccess;
+    const int64_t nsets = n / npil;
+    adj_reduce_fp32c<<<static_cast<unsigned>(nsets * ngroups * 4), kThreads, 0, s>>>(
+        partial, slot_off, slots, npil, ngroups, g);
     return hipGetLastError();
 }
 

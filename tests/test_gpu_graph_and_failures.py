@@ -225,13 +225,16 @@ def test_one_context_alternating_streams(torch_cuda):
     torch.cuda.synchronize()
     s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
     out = []
-    for i in range(9):
+    # two created streams, then the null stream (whose ordering event the context
+    # records only when the next call on another stream needs it) against a created one
+    for i in range(18):
         pl = shapes[i % 3]
+        sa, sb = (s1, s2) if i < 9 else (None, s1)
         if i % 2 == 0:
-            f, _ = ctx.forward(dpx, dpy, dt, pl, mode="fp32c", stream=s1)
+            f, _ = ctx.forward(dpx, dpy, dt, pl, mode="fp32c", stream=sa)
             out.append(("fwd", i % 3, f))
         else:
-            g = ctx.adjoint(dpx, dpy, dus[i % 3], pl, nsets=1, mode="fp32c", stream=s2)
+            g = ctx.adjoint(dpx, dpy, dus[i % 3], pl, nsets=1, mode="fp32c", stream=sb)
             out.append(("adj", i % 3, g))
     torch.cuda.synchronize()
     for kind, si, res in out:
