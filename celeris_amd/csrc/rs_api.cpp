@@ -208,8 +208,9 @@ namespace {
 // x a few resident work-groups, several waves deep so the tail of uneven splits is short.
 int default_target() { return 4096; }
 
-// Built-in FP32C block shapes: 4 x 4 pixels per forward lane (4 x 8 needs ~220 VGPRs,
-// 2 waves per SIMD, and measured no faster), 4 x 8 per adjoint block.
+// Built-in FP32C block shapes: 4 x 4 pixels per forward lane (4 x 8 blocks, 3 waves per
+// SIMD, run as fast at 161^2 and 3% faster at 321^2, but their second-order loops reload
+// from scratch, so the autotuner, not the default, picks them), 4 x 8 per adjoint block.
 constexpr int kDefFwdBX = 4, kDefFwdBY = 4;
 constexpr int kDefAdjBX = 4, kDefAdjBY = 8;
 // Built-in FP32C launch sizes, in multiples of the resident work-groups (4 per CU for

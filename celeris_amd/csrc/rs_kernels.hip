@@ -178,13 +178,69 @@ __global__ void stage_pillars_fp64(const double* __restrict__ px, const double* 
 // the four waves' sums are added, wave by wave in order, into FP64 running sums in LDS,
 // which the segment finally stores as float pairs in its own partial slot.
 
-// Per-lane constants of a pixel block: its centre (turns, FP64) and the offsets of its
-// columns / rows, te = 2 e, e2 = e^2 (+ z_row'^2 - z_centre'^2 for the rows).
+// Constants of a pixel block.  Its centre (turns, FP64) is per lane; the offsets of its
+// columns and rows enter D = a[c] + b[r] as
+//     a[c] = m_c u + Ex(|m_c|),  u = sxt dx0,              m_c = 2c - (BX - 1)
+//     b[r] = m_r v + Ey(|m_r|),  v = syt dy0 + szt zc',    m_r = 2r - (BY - 1)
+// with Ex(m) = (m sxt / 2)^2 and Ey(m) = (m / 2)^2 (syt^2 + szt^2) (sxt = k sx / 2 pi etc.;
+// the rows' depth change z_r'^2 - zc'^2 = m_r szt zc' + (m_r szt / 2)^2 folds into v and
+// Ey).  sxt, syt and the Ex / Ey tables are the same for every lane (scalar registers),
+// so a block costs one multiply per distinct |m| and an add per column or row, and no
+// vector registers for its offsets.
 template <int BX, int BY>
 struct BlockGeom {
     double FXc, FYc, ZC2;
-    float tex[BX], ex2[BX], tey[BY], eyz[BY];
+    float wz;                    // szt zc' (per lane in the forward)
+    float sxt, syt;              // uniform
+    float ex[(BX + 1) / 2];      // uniform: Ex(|m|) for |m| = 2k + (BX + 1) % 2
+    float ey[(BY + 1) / 2];      // uniform: Ey(|m|)
 };
+
+__device__ __forceinline__ float uniform_f(float x) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+
+// The plane-wide offset constants of BX x BY blocks (sxt, syt, Ex, Ey), and wz for a
+// block centred at depth zc' (turns).
+template <int BX, int BY>
+__device__ __forceinline__ void block_offsets_setup(const DevPlane& P, double zc,
+                                                    BlockGeom<BX, BY>& G) {
+    const double kt = P.kturn;
+    const double sxt = kt * P.sx, syt = kt * P.sy, szt = kt * P.sz;
+    G.sxt = uniform_f(static_cast<float>(sxt));
+    G.syt = uniform_f(static_cast<float>(syt));
+    G.wz = static_cast<float>(szt * zc);
+#pragma unroll
+    for (int k = 0; k < (BX + 1) / 2; ++k) {
+        const double m = 2 * k + (BX + 1) % 2;
+        G.ex[k] = uniform_f(static_cast<float>(0.25 * m * m * sxt * sxt));
+    }
+#pragma unroll
+    for (int k = 0; k < (BY + 1) / 2; ++k) {
+        const double m = 2 * k + (BY + 1) % 2;
+        G.ey[k] = uniform_f(static_cast<float>(0.25 * m * m * (syt * syt + szt * szt)));
+    }
+}
+
+// a[c], b[r] of a block for a pillar at (dx0, dy0) turns from its centre.
+template <int BX, int BY>
+__device__ __forceinline__ void block_offsets(const BlockGeom<BX, BY>& G, float dx0, float dy0,
+                                              float (&a)[BX], float (&b)[BY]) {
+    const float u = G.sxt * dx0;
+    const float v = fmaf(G.syt, dy0, G.wz);
+#pragma unroll
+    for (int c = 0; c < BX; ++c) {
+        const int m = 2 * c - (BX - 1), am = m < 0 ? -m : m;
+        const float mu = am == 1 ? u : static_cast<float>(am) * u;
+        a[c] = m == 0 ? G.ex[0] : (m < 0 ? G.ex[am / 2] - mu : G.ex[am / 2] + mu);
+    }
+#pragma unroll
+    for (int r = 0; r < BY; ++r) {
+        const int m = 2 * r - (BY - 1), am = m < 0 ? -m : m;
+        const float mv = am == 1 ? v : static_cast<float>(am) * v;
+        b[r] = m == 0 ? G.ey[0] : (m < 0 ? G.ey[am / 2] - mv : G.ey[am / 2] + mv);
+    }
+}
 
 // One pillar against one block: th0 = frac(R0 - zeta), h = 1/(2 R0), c1 = |t| (k/2pi)
 // / R0', and the separable parts of D (orders kP1A1 / kP1A2) or of x1 = D h (kP2A2):
@@ -221,16 +277,12 @@ __device__ __forceinline__ void pair_state(const PillarRec& T, const BlockGeom<B
         if constexpr (ORD == kP1A2) S.r = 6.0f * c1 * (h2 * h2);
     }
     if constexpr (BX * BY > 1) {
-        const float dx0 = static_cast<float>(dX0), dy0 = static_cast<float>(dY0);
+        block_offsets(G, static_cast<float>(dX0), static_cast<float>(dY0), S.a, S.b);
+        if constexpr (ORD == kP2A2) {
 #pragma unroll
-        for (int c = 0; c < BX; ++c) {
-            S.a[c] = fmaf(G.tex[c], dx0, G.ex2[c]);
-            if constexpr (ORD == kP2A2) S.a[c] *= S.h;
-        }
+            for (int c = 0; c < BX; ++c) S.a[c] *= S.h;
 #pragma unroll
-        for (int r = 0; r < BY; ++r) {
-            S.b[r] = fmaf(G.tey[r], dy0, G.eyz[r]);
-            if constexpr (ORD == kP2A2) S.b[r] *= S.h;
+            for (int r = 0; r < BY; ++r) S.b[r] *= S.h;
         }
     }
 }
@@ -273,7 +325,8 @@ __device__ __forceinline__ void block_pairs(const PairState<BX, BY>& S, float (&
         // with scheduling barriers so that no sum waits on a transcendental issued just
         // before it (left to itself the compiler chains each pair's sin and cos straight
         // into its sums, measured 12% slower for 4 x 4 blocks).
-        constexpr int G = N < 8 ? N : 8;
+        // (4 x 8 blocks: groups of 4, so the kernel fits 168 VGPRs, 3 waves per SIMD.)
+        constexpr int G = N < 8 ? N : (BX == 4 && BY == 8 ? 4 : 8);
 #pragma unroll
         for (int g0 = 0; g0 < N; g0 += G) {
             float th[G], A[G], cs[G], sn[G];
@@ -331,19 +384,7 @@ __device__ __forceinline__ void fwd_segment(
         G.FYc = kt * (P.y0 + jc * P.sy);
         const double zc = kt * (P.z0 + jc * P.sz);
         G.ZC2 = zc * zc;
-#pragma unroll
-        for (int c = 0; c < BX; ++c) {
-            const float e = static_cast<float>(kt * P.sx * (c - (BX - 1) * 0.5));
-            G.tex[c] = 2.0f * e;
-            G.ex2[c] = e * e;
-        }
-#pragma unroll
-        for (int r = 0; r < BY; ++r) {
-            const float e = static_cast<float>(kt * P.sy * (r - (BY - 1) * 0.5));
-            const double zr = kt * (P.z0 + (oy + r) * P.sz);
-            G.tey[r] = 2.0f * e;
-            G.eyz[r] = e * e + static_cast<float>(zr * zr - G.ZC2);
-        }
+        block_offsets_setup(P, zc, G);
     }
     const float amp_scale = static_cast<float>(kt);  // 1/R = (k/2pi) / R'
 
@@ -413,15 +454,19 @@ __device__ __forceinline__ void fwd_segment(
 }
 
 // Waves per SIMD each forward instance is compiled for (a register budget of 512 /
-// waves VGPRs): the accumulators alone take 2 BX BY registers; 4 x 8 blocks need ~220
-// (forcing 168 spills), so they run 2 waves per SIMD.
-constexpr int fwd_waves_per_simd(int n) { return n <= 16 ? 4 : 2; }
+// waves VGPRs; the accumulators alone take 2 BX BY registers).  4 x 4: 4 waves (122
+// VGPRs).  4 x 8: 3 waves (168 VGPRs, groups of 4 pairs; its first-order loop keeps
+// every value in registers, the rarer higher orders reload three FP64 values per pillar
+// from scratch): 0.713 against 0.757 ms at 2 waves for 91,809 pillars x 161^2, 2.571
+// against 2.640 ms (the best 2-wave shape) at 321^2.  8 x 4: 2 waves (194 VGPRs); at 3
+// waves its loops spill and it ran 3-4% slower.
+constexpr int fwd_waves_per_simd(int bx, int by) { return bx * by <= 16 ? 4 : (bx == 4 ? 3 : 2); }
 
 // KBX x KBY: the launch's block shape; its work-groups take the four tile classes of
 // the planes blocked that way (and the 1 x 1 tiles of exact planes).  <1, 1>: calls
 // whose planes are all exact.
 template <int KBX, int KBY>
-__global__ __launch_bounds__(kFwdThreads, fwd_waves_per_simd(KBX * KBY)) void fwd_fp32c(
+__global__ __launch_bounds__(kFwdThreads, fwd_waves_per_simd(KBX, KBY)) void fwd_fp32c(
     const double2* __restrict__ pos, const float2* __restrict__ ap,
     const double2* __restrict__ ext, int npil, const DevPlane* __restrict__ planes,
     const FwdTile* __restrict__ tiles, const FwdSeg* __restrict__ segs,
@@ -839,11 +884,11 @@ __device__ __forceinline__ void adj_state(double dX0, double dY0, double zref,
     S.rf = rf;
     S.h = 0.5f * rf;
     if constexpr (BX * BY > 1) {
-        const float dx0 = static_cast<float>(dX0), dy0 = static_cast<float>(dY0);
+        block_offsets(G, static_cast<float>(dX0), static_cast<float>(dY0), S.a, S.b);
 #pragma unroll
-        for (int c = 0; c < BX; ++c) S.a[c] = fmaf(G.tex[c], dx0, G.ex2[c]) * S.h;
+        for (int c = 0; c < BX; ++c) S.a[c] *= S.h;
 #pragma unroll
-        for (int r = 0; r < BY; ++r) S.b[r] = fmaf(G.tey[r], dy0, G.eyz[r]) * S.h;
+        for (int r = 0; r < BY; ++r) S.b[r] *= S.h;
     }
 }
 
@@ -908,25 +953,13 @@ __device__ __forceinline__ void adj_tile(const float* sU2, const float* sUm, con
                                          double qy, float& ar, float& ai) {
     const double kt = P.kturn;
     BlockGeom<BX, BY> G;
-#pragma unroll
-    for (int c = 0; c < BX; ++c) {
-        const float e = static_cast<float>(kt * P.sx * (c - (BX - 1) * 0.5));
-        G.tex[c] = 2.0f * e;
-        G.ex2[c] = e * e;
-    }
     for (int sr0 = 0; sr0 < kStrip; sr0 += BY) {
         if (BY < kStrip && j0 + sr0 >= P.ny) break;  // all-padding rows (uniform)
         const double jc = j0 + sr0 + (BY - 1) * 0.5;
         G.FYc = kt * (P.y0 + jc * P.sy);
         const double zc = kt * (P.z0 + jc * P.sz);
         G.ZC2 = zc * zc;
-#pragma unroll
-        for (int r = 0; r < BY; ++r) {
-            const float e = static_cast<float>(kt * P.sy * (r - (BY - 1) * 0.5));
-            const double zr = kt * (P.z0 + (j0 + sr0 + r) * P.sz);
-            G.tey[r] = 2.0f * e;
-            G.eyz[r] = e * e + static_cast<float>(zr * zr - G.ZC2);
-        }
+        block_offsets_setup(P, zc, G);
         const double dY0 = fma(-kt, qy, G.FYc);
         for (int g = 0; g < ncol; g += BX) {
             G.FXc = kt * (P.x0 + (cb + g + (BX - 1) * 0.5) * P.sx);
