@@ -354,10 +354,6 @@ crs_status check_args(int mode, int npil, int nsets, const crs_plane* planes, in
             set_error("plane %d: unknown kernel %d", p, P.kernel);
             return CRS_ERR_INVALID;
         }
-        if (P.kernel == CRS_KERNEL_RSI && mode == CRS_MODE_FP32C) {
-            set_error("plane %d: the RS-I kernel runs in CRS_MODE_FP64 or CRS_MODE_FP32 only", p);
-            return CRS_ERR_INVALID;
-        }
     }
     return CRS_OK;
 }
@@ -705,6 +701,7 @@ crs_status plan_forward_fp32c(const crs_context* ctx, int npil, std::vector<DevP
     const int shapes[3][2] = {{want_bx, want_by}, {4, 4}, {1, 1}};
     std::vector<int> pick(nplanes, 2);
     for (int p = 0; p < nplanes; ++p) {
+        if (dp[p].kernel == CRS_KERNEL_RSI) continue;  // RS-I: every pair exact
         if (fp32c_block_admitted(dp[p], want_bx, want_by, dp[p].ny)) pick[p] = 0;
         else if (fp32c_block_admitted(dp[p], 4, 4, dp[p].ny)) pick[p] = 1;
     }
@@ -950,6 +947,7 @@ crs_status plan_adjoint_fp32c(const crs_context* ctx, int npil, int nsets,
     std::vector<int> pick(nplanes, 2);
     for (int p = 0; p < nplanes; ++p) {
         const int rows = dp[p].nquads * kStrip;
+        if (dp[p].kernel == CRS_KERNEL_RSI) continue;  // RS-I: every pair exact
         if (fp32c_block_admitted(dp[p], want_bx, want_by, rows)) pick[p] = 0;
         else if (fp32c_block_admitted(dp[p], 4, 4, rows)) pick[p] = 1;
     }

@@ -191,6 +191,7 @@ template <int BX, int BY>
 struct BlockGeom {
     double FXc, FYc, ZC2;
     float wz;                    // szt zc' (per lane in the forward)
+    float zk;                    // (k / 2 pi) zc', the RS-I factor's depth (1 x 1 blocks)
     float sxt, syt;              // uniform
     float ex[(BX + 1) / 2];      // uniform: Ex(|m|) for |m| = 2k + (BX + 1) % 2
     float ey[(BY + 1) / 2];      // uniform: Ey(|m|)
@@ -210,6 +211,7 @@ __device__ __forceinline__ void block_offsets_setup(const DevPlane& P, double zc
     G.sxt = uniform_f(static_cast<float>(sxt));
     G.syt = uniform_f(static_cast<float>(syt));
     G.wz = static_cast<float>(szt * zc);
+    G.zk = static_cast<float>(kt * zc);
 #pragma unroll
     for (int k = 0; k < (BX + 1) / 2; ++k) {
         const double m = 2 * k + (BX + 1) % 2;
@@ -360,6 +362,25 @@ __device__ __forceinline__ void pillar_run(const PillarRec* tile, int k0, int k1
     }
 }
 
+// Pillars tile[k0 .. k1) against one pixel with the RS-I kernel (exact planes only):
+// t exp(ikR)/R (z/R)(1/R - ik) = [A (cos, sin) th0] (zk/R')(1/R' - 2 pi i), with
+// A = |t| k/2pi / R', zk = (k/2pi) z', R' = R k/2pi.
+__device__ __forceinline__ void pillar_run_rsi(const PillarRec* tile, int k0, int k1,
+                                               const BlockGeom<1, 1>& G, float (&ar)[1],
+                                               float (&ai)[1]) {
+    for (int k = k0; k < k1; ++k) {
+        PairState<1, 1> st;
+        pair_state<kP1A1>(tile[k], G, st);
+        const float rf = st.h + st.h;
+        const float f = G.zk * rf;
+        const float fr = f * rf, fi = f * -6.28318530717958648f;
+        const float cr = st.c * __builtin_amdgcn_cosf(st.th0);
+        const float ci = st.c * __builtin_amdgcn_sinf(st.th0);
+        ar[0] = fmaf(cr, fr, fmaf(-ci, fi, ar[0]));
+        ai[0] = fmaf(ci, fr, fmaf(cr, fi, ai[0]));
+    }
+}
+
 // One segment: tile T (class of block shape BX x BY within a plane blocked KBX x KBY)
 // against pillars [q0, q1) at series order `order`; partial sums to `slot`
 // (element e * 64 + lane).
@@ -410,7 +431,10 @@ __device__ __forceinline__ void fwd_segment(
         __syncthreads();
         const int k0 = (wave * n) / kFwdWaves, k1 = ((wave + 1) * n) / kFwdWaves;
         if constexpr (N == 1) {
-            pillar_run<kP1A1>(tile, k0, k1, G, ar, ai);
+            if (P.kernel == kKernelRsi)
+                pillar_run_rsi(tile, k0, k1, G, ar, ai);
+            else
+                pillar_run<kP1A1>(tile, k0, k1, G, ar, ai);
         } else {
             if (order == kP1A1)
                 pillar_run<kP1A1>(tile, k0, k1, G, ar, ai);
@@ -872,6 +896,8 @@ __global__ __launch_bounds__(kThreads) void stage_pixels_fp32c(
 template <int BX, int BY>
 struct AdjState {
     float th0, h, rf;
+    float zk;  // RS-I planes: (k/2pi) z' of the pixel row (1 x 1 blocks)
+    bool rsi;
     float a[BX], b[BY];
 };
 
@@ -883,6 +909,8 @@ __device__ __forceinline__ void adj_state(double dX0, double dY0, double zref,
     S.th0 = static_cast<float>(__builtin_amdgcn_fract(R0 - zref));
     S.rf = rf;
     S.h = 0.5f * rf;
+    S.zk = 0.0f;
+    S.rsi = false;
     if constexpr (BX * BY > 1) {
         block_offsets(G, static_cast<float>(dX0), static_cast<float>(dY0), S.a, S.b);
 #pragma unroll
@@ -905,8 +933,17 @@ __device__ __forceinline__ void adj_pairs(const AdjState<BX, BY>& S, const float
     if constexpr (BX * BY == 1) {
         const float phi = sChi[k0] - S.th0;
         const float A = S.rf * (sU2[k0] + sUm[k0]);
-        gr = fmaf(A, __builtin_amdgcn_cosf(phi), gr);
-        gi = fmaf(A, __builtin_amdgcn_sinf(phi), gi);
+        if (S.rsi) {
+            // RS-I: conj(G) u carries conj((zk/R')(1/R' - 2 pi i)) = f (rf, +2 pi).
+            const float f = S.zk * S.rf;
+            const float fr = f * S.rf, fi = f * 6.28318530717958648f;
+            const float cr = A * __builtin_amdgcn_cosf(phi), ci = A * __builtin_amdgcn_sinf(phi);
+            gr = fmaf(cr, fr, fmaf(-ci, fi, gr));
+            gi = fmaf(ci, fr, fmaf(cr, fi, gi));
+        } else {
+            gr = fmaf(A, __builtin_amdgcn_cosf(phi), gr);
+            gi = fmaf(A, __builtin_amdgcn_sinf(phi), gi);
+        }
     } else {
         float br[BX], bi[BX];
 #pragma unroll
@@ -965,6 +1002,10 @@ __device__ __forceinline__ void adj_tile(const float* sU2, const float* sUm, con
             G.FXc = kt * (P.x0 + (cb + g + (BX - 1) * 0.5) * P.sx);
             AdjState<BX, BY> S;
             adj_state(fma(-kt, qx, G.FXc), dY0, P.zref, G, S);
+            if constexpr (BX * BY == 1) {
+                S.rsi = P.kernel == kKernelRsi;
+                S.zk = G.zk;
+            }
             adj_pairs<ORD>(S, sU2, sUm, sChi, g * kStrip + sr0, ar, ai);
         }
     }

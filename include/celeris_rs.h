@@ -65,8 +65,9 @@ typedef enum crs_kernel {
      * The default, and the only one the drop-in layer uses. */
     CRS_KERNEL_REF = 0,
     /* Rayleigh-Sommerfeld I: exp(i k R) / R * (z / R) * (1 / R - i k), the z-derivative
-     * form -d/dz [exp(i k R) / R] (no reference counterpart; opt-in).  CRS_MODE_FP64
-     * and CRS_MODE_FP32 only: an FP32C call with such a plane returns CRS_ERR_INVALID. */
+     * form -d/dz [exp(i k R) / R] (no reference counterpart; opt-in).  In CRS_MODE_FP32C
+     * such a plane is evaluated pair by pair, each pair at its own FP64 distance (no
+     * pixel blocks). */
     CRS_KERNEL_RSI = 1
 } crs_kernel;
 

@@ -5,8 +5,9 @@ closed form, -d/dz of the reference kernel and the long-double brute force in
 tests/test_oracle.py), forward and adjoint, alone and mixed with reference-kernel
 planes in one launch.
 
-Tolerances: FP64 <= 1e-12 relative L2 of the complex field / gradient; FP32 (float
-R, k R, sincos, sums: the reference GPU kernel's arithmetic) <= 2e-3.
+Tolerances: FP64 <= 1e-12 relative L2 of the complex field / gradient; FP32C (RS-I
+planes pair by pair at FP64 distances) <= 1e-5; FP32 (float R, k R, sincos, sums: the
+reference GPU kernel's arithmetic) <= 2e-3.
 """
 import math
 from dataclasses import replace
@@ -34,7 +35,7 @@ def planes_mixed():
     ]
 
 
-@pytest.mark.parametrize("mode,tol", [("fp64", 1e-12), ("fp32", 2e-3)])
+@pytest.mark.parametrize("mode,tol", [("fp64", 1e-12), ("fp32c", 1e-5), ("fp32", 2e-3)])
 def test_rsi_forward_and_adjoint_vs_oracle(torch_cuda, rsctx, mode, tol):
     torch = torch_cuda
     L = configs.config1()
@@ -62,12 +63,19 @@ def test_rsi_forward_and_adjoint_vs_oracle(torch_cuda, rsctx, mode, tol):
     assert rel_l2(g, gref) <= tol
 
 
-def test_rsi_refused_in_fp32c(torch_cuda, rsctx):
+def test_rsi_fp32c_zero_depth_row_and_unknown_kernel(torch_cuda, rsctx):
+    """A zx slice starting at z = 0: RS-I gives exactly 0 on that row (z/R = 0), which the
+    FP32C exact path must keep; an unknown kernel id is refused."""
     torch = torch_cuda
     L = configs.small_lens(n_side=8, n=9)
-    planes = [replace(L.planes[0], kernel=KERNEL_RSI)]
-    with pytest.raises(RuntimeError, match="RS-I"):
-        rsctx.forward(dev(torch, L.px), dev(torch, L.py), dev(torch, L.t), planes, mode="fp32c")
+    k = L.planes[0].k
+    plane = replace(Plane.zx(-2.0, 2.0, 9, 0.0, 8.0, 5, k), kernel=KERNEL_RSI)
+    f, _ = rsctx.forward(dev(torch, L.px), dev(torch, L.py), dev(torch, L.t), [plane], mode="fp32c")
+    torch.cuda.synchronize()
+    f = f.cpu().numpy().reshape(5, 9)
+    ref = oracle.forward(L.px, L.py, L.t, [plane]).reshape(5, 9)
+    assert np.all(f[0] == 0.0) and np.all(ref[0] == 0.0)
+    assert rel_l2(f[1:], ref[1:]) <= 1e-5
     with pytest.raises(RuntimeError, match="unknown kernel"):
         rsctx.forward(dev(torch, L.px), dev(torch, L.py), dev(torch, L.t),
                       [replace(L.planes[0], kernel=7)], mode="fp64")
