@@ -431,10 +431,17 @@ __device__ __forceinline__ void fwd_segment(
         __syncthreads();
         const int k0 = (wave * n) / kFwdWaves, k1 = ((wave + 1) * n) / kFwdWaves;
         if constexpr (N == 1) {
-            if (P.kernel == kKernelRsi)
-                pillar_run_rsi(tile, k0, k1, G, ar, ai);
-            else
+            // RS-I planes only ever run in the exact-plane launch (KBX = KBY = 1); keeping
+            // the branch out of the block kernels leaves their code as it was (a change
+            // there moved the 4 x 8 kernel's register assignment and cost it 6% at 321^2).
+            if constexpr (KBX * KBY == 1) {
+                if (P.kernel == kKernelRsi)
+                    pillar_run_rsi(tile, k0, k1, G, ar, ai);
+                else
+                    pillar_run<kP1A1>(tile, k0, k1, G, ar, ai);
+            } else {
                 pillar_run<kP1A1>(tile, k0, k1, G, ar, ai);
+            }
         } else {
             if (order == kP1A1)
                 pillar_run<kP1A1>(tile, k0, k1, G, ar, ai);
@@ -481,9 +488,8 @@ __device__ __forceinline__ void fwd_segment(
 // waves VGPRs; the accumulators alone take 2 BX BY registers).  4 x 4: 4 waves (122
 // VGPRs).  4 x 8: 3 waves (168 VGPRs, groups of 4 pairs; its first-order loop keeps
 // every value in registers, the rarer higher orders reload three FP64 values per pillar
-// from scratch): 0.713 against 0.757 ms at 2 waves for 91,809 pillars x 161^2, 2.571
-// against 2.640 ms (the best 2-wave shape) at 321^2.  8 x 4: 2 waves (194 VGPRs); at 3
-// waves its loops spill and it ran 3-4% slower.
+// from scratch): 2.59 against 2.65 ms for the best 2-wave shape (8 x 4) at 91,809 pillars
+// x 321^2.  8 x 4: 2 waves (194 VGPRs); at 3 waves its loops spill and it ran 4% slower.
 constexpr int fwd_waves_per_simd(int bx, int by) { return bx * by <= 16 ? 4 : (bx == 4 ? 3 : 2); }
 
 // KBX x KBY: the launch's block shape; its work-groups take the four tile classes of
