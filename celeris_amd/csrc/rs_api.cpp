@@ -439,13 +439,17 @@ bool fp32c_block_admitted(const DevPlane& d, int bx, int by, int ny_rows) {
 // (pair ops plus the per-pillar FP64 distance and state), for the stream-K cuts.
 double fwd_tile_rate(int nel) { return nel == 1 ? 48.0 : 7.0 * nel + 40.0; }
 
-// Work-groups of an FP32C launch: the tuned target, else `rounds` x every CU's resident
-// count.
-int64_t fp32c_target(const crs_context* ctx, int tuned, bool adjoint, int bx, int by, int rounds) {
-    if (tuned > 0) return tuned;
+// Work-groups of an FP32C launch resident on the whole chip at once.
+int64_t fp32c_slots(const crs_context* ctx, bool adjoint, int bx, int by) {
     int per_cu = fp32c_resident_per_cu(adjoint, bx, by);
     if (per_cu <= 0) per_cu = 2;
-    return static_cast<int64_t>(per_cu) * std::max(1, ctx->cus) * rounds;
+    return static_cast<int64_t>(per_cu) * std::max(1, ctx->cus);
+}
+
+// Work-groups of an FP32C launch: the tuned target, else `rounds` x the resident slots.
+int64_t fp32c_target(const crs_context* ctx, int tuned, bool adjoint, int bx, int by, int rounds) {
+    if (tuned > 0) return tuned;
+    return fp32c_slots(ctx, adjoint, bx, by) * rounds;
 }
 
 // Bytes of staged pillar data: FP32C keeps FP64 positions once plus {|t|, arg t} per
@@ -633,19 +637,51 @@ struct Scratch {  // where one call's device data lives (a context's or a graph'
     crs_context* owner;     // the context whose previous call may still use the buffers
 };
 
+// Piece costs of a launch of nwg work-groups with `slots` resident at a time.  The
+// hardware hands out work-groups in index order as slots free up, so a launch of equal
+// pieces ends with a ragged last round: at 91,809 pillars x 161^2, 4 x 4 blocks in 8,192
+// equal pieces, the slots held a work-group for 0.92 of the kernel's span (per-work-group
+// clock stamps), the gap spread over the last ~100 us.  With nwg >= 2 slots the costs
+// fall geometrically instead, c_i ~ exp(-x i / nwg), x fixed by making the last piece a
+// quarter of the mean (the first is 2.6 times the mean): the final round is short (0.96
+// busy, a 25 us tail; an eighth gave 0.95), and the piece count, hence the partial-sum
+// traffic, stays nwg.
+// Returns the cumulative cost at each of the nwg - 1 cuts.
+std::vector<double> piece_bounds(double total, int64_t nwg, int64_t slots) {
+    std::vector<double> b;
+    if (nwg <= 1) return b;
+    b.resize(static_cast<size_t>(nwg - 1));
+    if (slots <= 0 || nwg < 2 * slots) {
+        for (int64_t i = 1; i < nwg; ++i)
+            b[static_cast<size_t>(i - 1)] = total * static_cast<double>(i) / static_cast<double>(nwg);
+        return b;
+    }
+    // x / (e^x - 1) = 1/4: the last piece's cost over the mean.
+    const double x = 2.3366629822630536;
+    const double q = std::exp(-x / static_cast<double>(nwg));  // ratio of successive pieces
+    // cumulative share after i pieces: (1 - q^i) / (1 - q^nwg)
+    const double denom = 1.0 - std::pow(q, static_cast<double>(nwg));
+    for (int64_t i = 1; i < nwg; ++i)
+        b[static_cast<size_t>(i - 1)] =
+            total * (1.0 - std::pow(q, static_cast<double>(i))) / denom;
+    return b;
+}
+
 // Stream-K cut of a work line into work-groups.  The line is a sequence of runs (run r
-// has len[r] units of cost rate[r] each); `nwg` equal-cost pieces are cut at unit
-// positions that are multiples of `gran` within a run (no empty segment; a piece that
-// would be empty merges into its neighbour, so fewer pieces may result).  emit(run, u0,
-// u1) is called for each segment in order; wg_start receives the index of each piece's
-// first segment followed by the total segment count.
+// has len[r] units of cost rate[r] each); `nwg` pieces of the costs piece_bounds gives
+// (equal ones below 2 slots' worth) are cut at unit positions that are multiples of
+// `gran` within a run (no empty segment; a piece that would be empty merges into its
+// neighbour, so fewer pieces may result).  emit(run, u0, u1) is called for each segment
+// in order; wg_start receives the index of each piece's first segment followed by the
+// total segment count.
 template <class Emit>
 void streamk_cut(const std::vector<int64_t>& len, const std::vector<double>& rate, int64_t nwg,
-                 int64_t gran, Emit emit, std::vector<int32_t>& wg_start) {
+                 int64_t slots, int64_t gran, Emit emit, std::vector<int32_t>& wg_start) {
     double total = 0.0;
     for (size_t r = 0; r < len.size(); ++r) total += rate[r] * static_cast<double>(len[r]);
     nwg = std::max<int64_t>(1, nwg);
-    const double per = total / static_cast<double>(nwg);
+    const std::vector<double> bnd = piece_bounds(total, nwg, slots);
+    auto bound = [&](int64_t k) { return bnd[static_cast<size_t>(k - 1)]; };  // 1 <= k < nwg
     int64_t nseg = 0;
     wg_start.clear();
     wg_start.push_back(0);
@@ -661,7 +697,7 @@ void streamk_cut(const std::vector<int64_t>& len, const std::vector<double>& rat
             int64_t uend = n;
             bool cut = false;
             if (next < nwg) {
-                const double b = per * static_cast<double>(next);
+                const double b = bound(next);
                 if (b < end_cost) {
                     int64_t uc = static_cast<int64_t>(std::llround((b - cum) / rt / gran)) * gran;
                     if (uc <= u) uc = u + gran;
@@ -682,7 +718,7 @@ void streamk_cut(const std::vector<int64_t>& len, const std::vector<double>& rat
         cum = end_cost;
         // Boundaries at or before this run's end that fell on its last unit start the
         // next piece at the next run.
-        while (next < nwg && per * static_cast<double>(next) <= cum * (1.0 + 1e-12)) {
+        while (next < nwg && bound(next) <= cum * (1.0 + 1e-12)) {
             if (wg_start.back() != nseg) wg_start.push_back(static_cast<int32_t>(nseg));
             ++next;
         }
@@ -749,7 +785,7 @@ crs_status plan_forward_fp32c(const crs_context* ctx, int npil, std::vector<DevP
         std::vector<int32_t> wg_start;
         const int64_t seg_base = static_cast<int64_t>(segs.size());
         int last_tile = -1;
-        streamk_cut(len, rate, nwg, 16,
+        streamk_cut(len, rate, nwg, fp32c_slots(ctx, false, kbx, kby), 16,
                     [&](int r, int64_t u0, int64_t u1) {
                         const int t = static_cast<int>(tile0) + r;
                         if (t != last_tile) {
@@ -1010,7 +1046,7 @@ crs_status plan_adjoint_fp32c(const crs_context* ctx, int npil, int nsets,
             std::max<int64_t>(1, std::min(units, pairs / kMinPiecePairs)));
         std::vector<int32_t> wg_start;
         const int64_t seg_base = static_cast<int64_t>(segs.size());
-        streamk_cut(len, rate, nwg, 1,
+        streamk_cut(len, rate, nwg, fp32c_slots(ctx, true, bx, by), 1,
                     [&](int r, int64_t b0, int64_t b1) {
                         const int s = run_sg[r].first, g = run_sg[r].second;
                         const std::vector<int64_t>& pre = prefix[s];
