@@ -143,12 +143,12 @@ struct AdjBlock {
     int64_t part;
 };
 
-// ---- FP32C work decomposition (stream-K: equal work per work-group) ----------------
+// ---- FP32C work decomposition (stream-K: a cost-cut work line) ---------------------
 // Forward: a "tile" is 64 thread tiles of one class of one plane (class 0 = BX x BY
 // blocks, 1 = 1 x BY, 2 = BX x 1, 3 = 1 x 1).  A "segment" is one tile against a pillar
 // range; each segment writes its own partial slot (64 x BX*BY float pairs).  Work-group
 // w runs segments [wg_seg[w], wg_seg[w+1]), a contiguous run of the tile-major
-// (tile, pillar) work line cut into equal-cost pieces.
+// (tile, pillar) work line cut into pieces of falling cost.
 struct FwdTile {
     int32_t plane, shape, tile0;
     int32_t nel;    // pixels per lane of this class (BX*BY, BY, BX or 1)

@@ -35,8 +35,9 @@
 //
 // FP32C work is split stream-K style: the (pixel tile, pillar) work of a forward call
 // and the (pillar group, pixel strip column) work of an adjoint call are laid on one
-// line and cut into equal-cost pieces, one per work-group, so a launch of exactly the
-// resident work-group count keeps every CU busy to the end and writes few partials.
+// line and cut into pieces, one per work-group, of costs that fall along the line (the
+// host's piece_bounds), so the last work-groups the hardware dispatches are short and
+// every CU stays busy nearly to the end; each piece writes one partial slot.
 #include "rs_internal.h"
 
 #include <hip/hip_runtime.h>

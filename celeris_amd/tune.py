@@ -4,12 +4,12 @@ Two knobs decide how a call maps onto the 256 CUs:
 
 * the block shape (FP32C only): pixels per forward lane (4 x 4, 4 x 8 or 8 x 4) and per
   adjoint block (4 or 8 each way).  Larger blocks spread each pillar's FP64 distance
-  work over more pairs but hold more accumulators (the 4 x 8 forward runs 2 waves per
+  work over more pairs but hold more accumulators (the 4 x 8 forward runs 3 waves per
   SIMD instead of 4); a plane whose pair-error bound does not admit a shape falls back to
   4 x 4, then to pair-by-pair evaluation, so every candidate is exact to the same bound;
-* the work-group count: the pieces a call's work is cut into (FP32C: equal-cost pieces;
-  FP32 / FP64: pillar chunks or pixel-row runs).  Few pieces write few partial sums;
-  more pieces let the CUs that finish early take more work.
+* the work-group count: the pieces a call's work is cut into (FP32C: pieces of falling
+  cost; FP32 / FP64: pillar chunks or pixel-row runs).  Few pieces write few partial
+  sums; more pieces let the CUs that finish early take more work.
 
 Each candidate's whole call (staging, pair kernels, reduction) is timed with stream
 events over back-to-back calls.  Among the candidates within TIE of the fastest, the one
