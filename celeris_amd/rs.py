@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import threading
 from dataclasses import dataclass, replace
 from typing import Iterable, Optional, Sequence
 
@@ -181,19 +182,34 @@ class Context:
 
     Work is enqueued on torch's current stream of the device unless ``stream`` (a
     torch.cuda.Stream) is given; calls return without synchronizing, like any torch op.
+    A C context serves one host thread at a time; this wrapper takes a lock around every
+    library call, so one Context may be shared by several threads (the autograd engine's
+    backward thread and the caller's, say).
     """
 
     def __init__(self, device: int = 0):
         lib = _lib.load()
         h = ctypes.c_void_p()
         _lib.check(lib.crs_context_create(int(device), ctypes.byref(h)), "crs_context_create")
+        self._lock = threading.RLock()
         self._h = h
         self.device = int(device)
 
     def close(self) -> None:
-        if getattr(self, "_h", None):
-            _lib.load().crs_context_destroy(self._h)
-            self._h = None
+        lock = getattr(self, "_lock", None)
+        if lock is None:
+            return
+        with lock:
+            if getattr(self, "_h", None):
+                _lib.load().crs_context_destroy(self._h)
+                self._h = None
+
+    def _call(self, what: str, fn, *args) -> None:
+        """fn(handle, *args) under the context's lock; raises on a failure status."""
+        with self._lock:
+            if not getattr(self, "_h", None):
+                raise RuntimeError(f"{what}: the context is closed")
+            _lib.check(fn(self._h, *args), what)
 
     def __del__(self):
         try:
@@ -220,30 +236,30 @@ class Context:
             raise ValueError(f"{name} has {x.numel()} elements, {min_numel} needed")
 
     def set_tuning(self, mode, fwd_target_blocks: int = 0, adj_target_blocks: int = 0) -> None:
-        _lib.check(_lib.load().crs_set_tuning(self._h, _mode(mode), int(fwd_target_blocks),
-                                              int(adj_target_blocks)), "crs_set_tuning")
+        self._call("crs_set_tuning", _lib.load().crs_set_tuning, _mode(mode),
+                   int(fwd_target_blocks), int(adj_target_blocks))
 
     def set_block_shape(self, fwd=(0, 0), adj=(0, 0)) -> None:
         """FP32C pixels per forward lane and per adjoint block, (columns, rows); (0, 0)
         restores the built-in 4 x 8."""
-        _lib.check(_lib.load().crs_set_block_shape(self._h, int(fwd[0]), int(fwd[1]), int(adj[0]),
-                                                   int(adj[1])), "crs_set_block_shape")
+        self._call("crs_set_block_shape", _lib.load().crs_set_block_shape, int(fwd[0]),
+                   int(fwd[1]), int(adj[0]), int(adj[1]))
 
     def set_timing(self, enable: bool = True) -> None:
-        _lib.check(_lib.load().crs_set_timing(self._h, int(bool(enable))), "crs_set_timing")
+        self._call("crs_set_timing", _lib.load().crs_set_timing, int(bool(enable)))
 
     def last_kernel_ms(self, adjoint: bool = False) -> float:
         """Duration of the last pair-sum kernel (HIP events on its stream); waits for it."""
         ms = ctypes.c_float()
-        _lib.check(_lib.load().crs_last_kernel_ms(self._h, int(bool(adjoint)), ctypes.byref(ms)),
-                   "crs_last_kernel_ms")
+        self._call("crs_last_kernel_ms", _lib.load().crs_last_kernel_ms, int(bool(adjoint)),
+                   ctypes.byref(ms))
         return float(ms.value)
 
     def last_launch(self, adjoint: bool = False):
         """(kernel name as a profiler lists it, work-groups) of the last pair-sum launch."""
         name, groups = ctypes.c_char_p(), ctypes.c_int()
-        _lib.check(_lib.load().crs_last_launch(self._h, int(bool(adjoint)), ctypes.byref(name),
-                                               ctypes.byref(groups)), "crs_last_launch")
+        self._call("crs_last_launch", _lib.load().crs_last_launch, int(bool(adjoint)),
+                   ctypes.byref(name), ctypes.byref(groups))
         return name.value.decode(), int(groups.value)
 
     def forward(self, px, py, t, planes: Sequence[Plane], mode="fp32c", field: bool = True,
@@ -269,12 +285,11 @@ class Context:
             self._check_dev("out_intensity", it, torch.float64, total)
         elif intensity:
             it = torch.empty(total, dtype=torch.float64, device=dev)
-        st = _lib.load().crs_forward(
-            self._h, self._stream(stream), _mode(mode), ctypes.c_void_p(px.data_ptr()),
-            ctypes.c_void_p(py.data_ptr()), ctypes.c_void_p(t.data_ptr()), npil, nsets, arr,
-            len(planes), ctypes.c_void_p(f.data_ptr() if f is not None else None),
-            ctypes.c_void_p(it.data_ptr() if it is not None else None))
-        _lib.check(st, "crs_forward")
+        self._call("crs_forward", _lib.load().crs_forward, self._stream(stream), _mode(mode),
+                   ctypes.c_void_p(px.data_ptr()), ctypes.c_void_p(py.data_ptr()),
+                   ctypes.c_void_p(t.data_ptr()), npil, nsets, arr, len(planes),
+                   ctypes.c_void_p(f.data_ptr() if f is not None else None),
+                   ctypes.c_void_p(it.data_ptr() if it is not None else None))
         return f, it
 
     def adjoint(self, px, py, u, planes: Sequence[Plane], nsets: int = 1, mode="fp32c",
@@ -294,11 +309,10 @@ class Context:
             self._check_dev("out", out, torch.complex128, int(nsets) * npil)
         g = out if out is not None else torch.empty((nsets, npil), dtype=torch.complex128,
                                                     device=px.device)
-        st = _lib.load().crs_adjoint(
-            self._h, self._stream(stream), _mode(mode), ctypes.c_void_p(px.data_ptr()),
-            ctypes.c_void_p(py.data_ptr()), npil, int(nsets), arr, len(planes),
-            ctypes.c_void_p(u.data_ptr()), ctypes.c_void_p(g.data_ptr()))
-        _lib.check(st, "crs_adjoint")
+        self._call("crs_adjoint", _lib.load().crs_adjoint, self._stream(stream), _mode(mode),
+                   ctypes.c_void_p(px.data_ptr()), ctypes.c_void_p(py.data_ptr()), npil,
+                   int(nsets), arr, len(planes), ctypes.c_void_p(u.data_ptr()),
+                   ctypes.c_void_p(g.data_ptr()))
         return g
 
 
@@ -346,12 +360,12 @@ def graph_forward(ctx: "Context", px, py, t, planes: Sequence[Plane], field, int
         ctx._check_dev("intensity", intensity, torch.float64, total)
     nsets = t.numel() // max(npil, 1)
     h = ctypes.c_void_p()
-    _lib.check(_lib.load().crs_graph_forward(
-        ctx._h, _mode(mode), ctypes.c_void_p(px.data_ptr()), ctypes.c_void_p(py.data_ptr()),
-        ctypes.c_void_p(t.data_ptr()), px.numel(), nsets, arr, len(planes),
-        ctypes.c_void_p(field.data_ptr() if field is not None else None),
-        ctypes.c_void_p(intensity.data_ptr() if intensity is not None else None),
-        ctypes.byref(h)), "crs_graph_forward")
+    ctx._call("crs_graph_forward", _lib.load().crs_graph_forward, _mode(mode),
+              ctypes.c_void_p(px.data_ptr()), ctypes.c_void_p(py.data_ptr()),
+              ctypes.c_void_p(t.data_ptr()), px.numel(), nsets, arr, len(planes),
+              ctypes.c_void_p(field.data_ptr() if field is not None else None),
+              ctypes.c_void_p(intensity.data_ptr() if intensity is not None else None),
+              ctypes.byref(h))
     return Graph(h, (px, py, t, field, intensity))
 
 
@@ -366,10 +380,10 @@ def graph_adjoint(ctx: "Context", px, py, u, planes: Sequence[Plane], g, nsets: 
     ctx._check_dev("u", u, torch.complex128, total)
     ctx._check_dev("g", g, torch.complex128, int(nsets) * npil)
     h = ctypes.c_void_p()
-    _lib.check(_lib.load().crs_graph_adjoint(
-        ctx._h, _mode(mode), ctypes.c_void_p(px.data_ptr()), ctypes.c_void_p(py.data_ptr()),
-        px.numel(), int(nsets), arr, len(planes), ctypes.c_void_p(u.data_ptr()),
-        ctypes.c_void_p(g.data_ptr()), ctypes.byref(h)), "crs_graph_adjoint")
+    ctx._call("crs_graph_adjoint", _lib.load().crs_graph_adjoint, _mode(mode),
+              ctypes.c_void_p(px.data_ptr()), ctypes.c_void_p(py.data_ptr()), px.numel(),
+              int(nsets), arr, len(planes), ctypes.c_void_p(u.data_ptr()),
+              ctypes.c_void_p(g.data_ptr()), ctypes.byref(h))
     return Graph(h, (px, py, u, g))
 
 

@@ -247,3 +247,57 @@ def test_one_context_alternating_streams(torch_cuda):
         print(f"{kind} shape {si}: relL2 {e:.2e}")
         assert e <= 1e-5
     ctx.close()
+
+
+def test_one_context_shared_by_threads(torch_cuda):
+    """Four host threads share one Python Context (its lock serialises the library
+    calls), each running forwards and adjoints of its own plane list on its own stream;
+    every result equals, bit for bit, the same call made alone on one thread (the same
+    plan, the same cut, the same fixed-order sums)."""
+    import threading
+
+    torch = torch_cuda
+    L = configs.config1()
+    k = L.planes[0].k
+    dpx, dpy, dt = dev(torch, L.px), dev(torch, L.py), dev(torch, L.t)
+    shapes = [[Plane.psf(0.0, 0.0, 50.0, k, 33, 2.0)],
+              [Plane.psf(0.2, -0.1, 49.0, k, 57, 4.0)],
+              [Plane.psf(-0.5, 0.5, 51.0, k, 81, 5.0), Plane.psf(1.0, -1.0, 48.0, k, 41, 3.0)],
+              [Plane.zx(-3.0, 3.0, 45, 40.0, 60.0, 37, k)]]
+    rng = np.random.default_rng(33)
+    dus = [dev(torch, rng.standard_normal(sum(p.npix for p in pl))
+               + 1j * rng.standard_normal(sum(p.npix for p in pl))) for pl in shapes]
+    solo = Context(0)
+    want = [(solo.forward(dpx, dpy, dt, pl, mode="fp32c")[0].cpu(),
+             solo.adjoint(dpx, dpy, du, pl, nsets=1, mode="fp32c").cpu())
+            for pl, du in zip(shapes, dus)]
+    solo.close()
+    ctx = Context(0)
+    got = [[] for _ in shapes]
+    errors = []
+
+    def work(i):
+        try:
+            s = torch.cuda.Stream()
+            with torch.cuda.stream(s):
+                for _ in range(6):
+                    f, _ = ctx.forward(dpx, dpy, dt, shapes[i], mode="fp32c")
+                    g = ctx.adjoint(dpx, dpy, dus[i], shapes[i], nsets=1, mode="fp32c")
+                    got[i].append((f, g))
+            s.synchronize()
+        except Exception as e:  # reported on the main thread
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=work, args=(i,)) for i in range(len(shapes))]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    torch.cuda.synchronize()
+    assert not errors, errors
+    for i, runs in enumerate(got):
+        assert len(runs) == 6
+        for f, g in runs:
+            assert torch.equal(f.cpu(), want[i][0])
+            assert torch.equal(g.cpu(), want[i][1])
+    ctx.close()
