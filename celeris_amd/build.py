@@ -80,6 +80,19 @@ def build_issue_probe() -> str:
     return PROBE
 
 
+REREAD_SRC = os.path.join(CSRC, "reread_probe.hip")
+REREAD = os.path.join(LIBDIR, "reread_probe")
+
+
+def build_reread_probe() -> str:
+    """The work-layout probe: the plain FP32 pair sum with each work-group summing every
+    pillar (the reference's layout) against pillar chunks split over work-groups."""
+    os.makedirs(LIBDIR, exist_ok=True)
+    flags = [f for f in HIP_FLAGS if f not in ("-fPIC", "-shared")]
+    _run([HIPCC, *flags, REREAD_SRC, "-o", REREAD])
+    return REREAD
+
+
 def build_oracle() -> str:
     _run(["make", "-s", "-C", ORACLE])
     return os.path.join(ORACLE, "_build", "liboracle_rs.so")
@@ -98,6 +111,7 @@ def build_reference() -> bool:
 def build_all() -> None:
     build_library()
     build_issue_probe()
+    build_reread_probe()
     build_dropin_driver()
     build_oracle()
     build_reference()
