@@ -110,7 +110,7 @@ struct DevPlane {
     // series orders kP1A1 / kP1A2 meet the pair budget for the plane's block shape (-1:
     // never); beyond obl2, kP2A2, which the host admitted at any obliquity.
     double obl1, obl2;
-    int32_t kernel;  // crs_kernel: 0 exp(ikR)/R, 1 RS-I (FP32 / FP64 kernels only)
+    int32_t kernel;  // crs_kernel: 0 exp(ikR)/R, 1 RS-I (FP32C: pair by pair, 1 x 1 blocks)
     int32_t pad;
 };
 
