@@ -174,3 +174,20 @@ def test_mirror_through_lens_plane(torch_cuda, rsctx, kernel):
         assert np.max(np.abs(back64 - sign * front64)) <= 1e-12 * np.max(np.abs(front64))
         assert rel_l2(back32, sign * front32) <= 2e-6
         o += 2 * n
+
+
+def test_thin_planes(torch_cuda, rsctx):
+    """Planes thinner than a block in one call: a row, a column, a single pixel and a few
+    small rectangles (the forward's edge classes and the adjoint's padded strips alone)."""
+    px, py, t, k, lam, f = lens()
+    step = 0.07
+    planes = [
+        Plane(k, -2.0, step, 0.4, 0.0, f, 0.0, 57, 1),          # one row
+        Plane(k, 0.3, 0.0, -2.0, step, f, 0.0, 1, 57),          # one column
+        Plane(k, 0.1, step, -0.1, step, f, 0.0, 1, 1),          # one pixel
+        Plane(k, -0.2, step, -0.2, step, f, 0.0, 2, 2),
+        Plane(k, -0.5, step, 0.2, step, 0.9 * f, 0.0, 3, 7),
+        Plane(k, -1.0, step, -1.0, step, 1.1 * f, 0.0, 9, 5),
+        Plane(k, -1.0, step, 0.0, 0.0, 0.8 * f, 0.5, 5, 9),    # zx, 5 wide
+    ]
+    check_modes(rsctx, torch_cuda, px, py, t, planes, "thin")
