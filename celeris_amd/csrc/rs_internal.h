@@ -111,7 +111,7 @@ struct DevPlane {
     // never); beyond obl2, kP2A2, which the host admitted at any obliquity.
     double obl1, obl2;
     int32_t kernel;  // crs_kernel: 0 exp(ikR)/R, 1 RS-I (FP32C: pair by pair, 1 x 1 blocks)
-    int32_t pad;
+    int32_t ldx;     // output row stride (nx, or the parent plane's nx for an FP32C edge strip)
 };
 
 // ---- FP32 / FP64 work decomposition (split sums, uniform slots) --------------------
@@ -200,6 +200,41 @@ __host__ __device__ inline void tile_origin(const DevPlane& P, int shape, int t,
         default: ox = kbx * P.tx + t % rx; oy = kby * P.ty + t / rx; break;
     }
 }
+
+// ---- FP32C forward on the matrix cores -----------------------------------------------
+// A plane's interior is cut into kMfmaBX x kMfmaBY pixel blocks (its edge strips go to the
+// block kernels as planes of their own).  For a pillar q and a block with centre
+// distance R0, a pixel (c, r) at R^2 = R0^2 + a_c + b_r has
+//     t exp(2 pi i R') / R = U_c V_r K(a_c, b_r),
+//     U_c = c1 exp(2 pi i th0) F(a_c),  V_r = F(b_r),  F(D) = (R0 / R(D)) exp(2 pi i (R(D) - R0)),
+//     K(a, b) = F(a + b) / (F(a) F(b)) = 1 + a l1(b) + a^2 (l2(b) + l1(b)^2 / 2) + O(a^3),
+// l_j(b) = (psi^(j)(b) - psi^(j)(0)) / j!, psi = log F.  So the block's field is a sum
+// over pillars of three rank-one complex products X_m(c) Y_m(r) (X_m = U_c a_c^m), i.e.
+// a complex matrix product with the pillars (x 3) as the contraction: each rank term is
+// one v_mfma_f32_32x32x2_f32 (A = 16 columns x {Re, Im} rows, B = 32 rows, K = {Re, Im}).
+constexpr int kMfmaBX = 16;
+constexpr int kMfmaBY = 32;
+constexpr int kMfmaPix = kMfmaBX * kMfmaBY;
+
+// One block: pixel origin (ox, oy) of plane `plane`; its segments are partial slots
+// seg0 .. seg0 + nseg - 1 (kMfmaPix float pairs each, pixel r * kMfmaBX + c).
+struct MfmaBlock {
+    int32_t plane, ox, oy, seg0, nseg, pad;
+};
+struct MfmaSeg {
+    int32_t block, q0, q1, pad;
+};
+struct Fp32cMfmaArgs {
+    const double2* pos;
+    const float2* ap;
+    int npil;
+    const DevPlane* planes;
+    const MfmaBlock* blocks;
+    const MfmaSeg* segs;
+    const int32_t* wg_seg;  // nwg + 1 entries
+    int nwg;
+    float2* partial;
+};
 
 // Device arguments of one FP32C forward / adjoint pair-kernel launch.
 struct Fp32cFwdArgs {
@@ -337,6 +372,69 @@ __host__ __device__ inline int fp32c_series_order(const DevPlane& P, double ex, 
     if (sn <= P.obl1) return kP1A1;
     if (sn <= P.obl2) return kP1A2;
     return kP2A2;
+}
+
+// ---- Pair error of the matrix-core forward ---------------------------------------------
+// For a pair at obliquity s (from the block centre) a kMfmaBX x kMfmaBY block of plane P
+// gives a term within eps |t| / R of t exp(ikR) / R, eps the sum of
+//   * the dropped rank terms of K.  With h = 1/(2 R0), |a| <= A = 2 e_x rho + e_x^2,
+//     |b| <= B = 2 e_y rho + e_y^2 + 2 zc e_z + e_z^2 (rho = R0 s, zc = R0 sqrt(1 - s^2),
+//     e the block's half extents in turns) and y = b h^2, the series in y give
+//       |l1| <= |y| (8 h^2 + 4 pi h) / (1 - 4|y|),   |l2| <= |y| (32 h^4 + 12 pi h^3) / (1 - 6|y|),
+//       |l3| <= |y| (128 h^6 + 40 pi h^5) / (1 - 8|y|),
+//     so L = log K = sum_j a^j l_j has |L| <= ell = t1 + t2 + t3 (t1 = A|l1|, t2 = A^2|l2|,
+//     t3 = A^3|l3| (1 + 16 A h^2) with the tail j >= 4), and exp(L) minus its kept part
+//     1 + a l1 + a^2 (l2 + l1^2/2) is at most ell^3 e^ell / 6 + t3 + (t2 + t3)(ell + t1)/2;
+//   * the truncated series the kernel evaluates: l1 to y^2 (remainders 512 h^2 |y|^4 and
+//     140 pi h |y|^4, over 1 - 4|y|, times A), l2 to y (1024 h^4 |y|^3 and 280 pi h^3
+//     |y|^3 over 1 - 5|y|, times A^2), and each side's F to y^3 (phase 14 |D h| y^4,
+//     amplitude 70 y^4, over 1 - 4y);
+//   * rounding: each side's phase is formed in FP64 and rounded once after its reduction
+//     mod 1 (2^-25 turns each), its float correction term adds 3 2^-24 |D h| y; the
+//     amplitude product is 8 float roundings (2^-21);
+//   * the hardware sin / cos of the two factors, 2.5e-7 (as for the block kernels).
+// The host admits a plane where eps stays within kPairBudget at every obliquity.
+__host__ __device__ inline double mfma_pair_error(const DevPlane& P, double s) {
+    const double kt = P.kturn;
+    const double ex = 0.5 * (kMfmaBX - 1) * fabs(P.sx) * kt;
+    const double ey = 0.5 * (kMfmaBY - 1) * fabs(P.sy) * kt;
+    const double ez = 0.5 * (kMfmaBY - 1) * fabs(P.sz) * kt;
+    const double z = kt * plane_min_abs_z(P.z0, P.sz, P.ny);
+    if (!(z > 0.0) || !(s >= 0.0 && s < 1.0)) return INFINITY;
+    const double cz = sqrt(1.0 - s * s), R0 = z / cz, h = 0.5 / R0, rho = R0 * s;
+    const double A = 2.0 * ex * rho + ex * ex;
+    const double B = 2.0 * ey * rho + ey * ey + 2.0 * R0 * cz * ez + ez * ez;
+    const double h2 = h * h, h3 = h2 * h, ya = A * h2, yb = B * h2;
+    if (!(ya < 0.02 && yb < 0.02)) return INFINITY;
+    const double pi = 0.5 * kTwoPiD;
+    const double l1 = yb * (8.0 * h2 + 4.0 * pi * h) / (1.0 - 4.0 * yb);
+    const double l2 = yb * (32.0 * h2 * h2 + 12.0 * pi * h3) / (1.0 - 6.0 * yb);
+    const double l3 = yb * (128.0 * h3 * h3 + 40.0 * pi * h3 * h2) / (1.0 - 8.0 * yb);
+    const double t1 = A * l1, t2 = A * A * l2, t3 = A * A * A * l3 * (1.0 + 16.0 * ya);
+    const double ell = t1 + t2 + t3;
+    const double rank = ell * ell * ell * exp(ell) / 6.0 + t3 + (t2 + t3) * (ell + t1) / 2.0;
+    const double yb2 = yb * yb, ya2 = ya * ya;
+    const double dl1 = A * (512.0 * h2 + 140.0 * pi * h) * yb2 * yb2 / (1.0 - 4.0 * yb);
+    const double dl2 = A * A * (1024.0 * h2 * h2 + 280.0 * pi * h3) * yb2 * yb / (1.0 - 5.0 * yb);
+    const double side_phase =
+        14.0 * (A * h * ya2 * ya2 / (1.0 - 4.0 * ya) + B * h * yb2 * yb2 / (1.0 - 4.0 * yb));
+    const double side_amp = 70.0 * (ya2 * ya2 / (1.0 - 4.0 * ya) + yb2 * yb2 / (1.0 - 4.0 * yb));
+    const double round_phase = 2.0 * 2.98023223876953125e-8 +
+                               3.0 * 5.9604644775390625e-8 * (A * h * ya + B * h * yb);
+    return kTwoPiD * (side_phase + round_phase) + side_amp + 1.1 * (rank + dl1 + dl2) +
+           4.76837158203125e-7 + 2.5e-7;
+}
+
+// Whether the matrix-core forward may take plane P: the exp(ikR)/R kernel, room for at
+// least one block, and eps within the pair budget at every obliquity (a grid of 512
+// obliquities; eps varies slowly with s).
+inline bool mfma_plane_admitted(const DevPlane& P) {
+    if (P.kernel != 0 || P.nx < kMfmaBX || P.ny < kMfmaBY) return false;
+    for (int i = 0; i < 512; ++i) {
+        const double s = 0.999 * i / 511.0;
+        if (!(mfma_pair_error(P, s) <= kPairBudget)) return false;
+    }
+    return true;
 }
 
 }  // namespace crs
