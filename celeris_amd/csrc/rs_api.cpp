@@ -26,7 +26,6 @@
 #include <cstring>
 #include <mutex>
 #include <string>
-#include <type_traits>
 #include <vector>
 
 namespace crs {
@@ -43,11 +42,6 @@ hipError_t launch_adj_reduce_fp32c(const float2* partial, const int32_t* slot_of
                                    double2* g, hipStream_t s);
 int fp32c_resident_per_cu(bool adjoint, int bx, int by);
 const char* fp32c_kernel_name(bool adjoint, int bx, int by);
-hipError_t launch_forward_mfma(const Fp32cMfmaArgs& a, hipStream_t s);
-hipError_t launch_fwd_reduce_mfma(const DevPlane* planes, const MfmaBlock* blocks, int nblocks,
-                                  const float2* partial, double2* field, double* intensity,
-                                  hipStream_t s);
-int mfma_resident_per_cu();
 hipError_t launch_forward(int mode, const void* pil, int npil, const DevPlane* planes,
                           const FwdBlock* blocks, int nblocks, void* partial, hipStream_t s);
 hipError_t launch_fwd_reduce(const DevPlane* planes, const PixBlock* pblocks, int nblocks,
@@ -140,17 +134,7 @@ struct FwdPlan {
     int64_t nseg = 0, slot_elems = 0;
     int nred = 0;
     size_t o_tiles = 0, o_segs = 0, o_reds = 0;
-    // FP32C on the matrix cores: blocks and stream-K segments; their partial slots
-    // (kMfmaPix float pairs each) follow the block kernels' nseg * slot_elems.
-    int nmblocks = 0, nmwg = 0;
-    int64_t nmseg = 0;
-    size_t o_mblocks = 0, o_msegs = 0, o_mwg = 0;
 };
-
-// Float-pair partial elements of an FP32C forward plan.
-int64_t fp32c_partial_elems(const FwdPlan& plan) {
-    return plan.nseg * plan.slot_elems + plan.nmseg * kMfmaPix;
-}
 
 struct AdjPlan {
     int mode = 0;
@@ -226,11 +210,11 @@ int default_target() { return 4096; }
 
 // Built-in FP32C block shapes: 4 x 4 pixels per forward lane (4 x 8 blocks, 3 waves per
 // SIMD, run as fast at 161^2 and 3% faster at 321^2, but their second-order loops reload
-// from scratch, so the autotuner, not the default, picks them), 4 x 8 per adjoint block.
+// from scratch, so the autotuner, not the default, picks them), 8 x 8 per adjoint block
+// (the separable form where its bound holds: 36.8 against 43.1 cycles per 64 pairs for
+// the 4 x 8 series at 91,809 pillars x 321^2).
 constexpr int kDefFwdBX = 4, kDefFwdBY = 4;
-constexpr int kDefAdjBX = 4, kDefAdjBY = 8;
-// Block-kernel shape of the planes (and edge strips) a matrix-core forward leaves to them.
-constexpr int kDefVecFwdBX = 4, kDefVecFwdBY = 4;
+constexpr int kDefAdjBX = 8, kDefAdjBY = 8;
 // Built-in FP32C launch sizes, in multiples of the resident work-groups (4 per CU for
 // both default shapes).  Several rounds let the CUs that finish early take more work:
 // at 91,809 pillars x 161^2 (whole call, pair kernel and reduction) the forward takes
@@ -238,8 +222,6 @@ constexpr int kDefVecFwdBX = 4, kDefVecFwdBY = 4;
 // 0.693 / 0.687 ms.
 constexpr int kFwdRounds = 8;
 constexpr int kAdjRounds = 8;
-// Matrix-core forward launch size in rounds of resident work-groups.
-constexpr int kMfmaRounds = 4;
 // Fewest pairs per FP32C work-group piece: below this a piece's fixed costs (pillar
 // loads, state, the partial slot and its reduction) outweigh its pairs, so small calls
 // launch fewer, larger pieces (measured at 4,096 pillars x 81^2: 512 adjoint pieces take
@@ -440,7 +422,6 @@ DevPlane make_dev_plane(const crs_plane& P) {
     d.obl1 = 1.0;
     d.obl2 = 1.0;
     d.kernel = P.kernel;
-    d.ldx = P.nx;
     return d;
 }
 
@@ -574,7 +555,7 @@ extern "C" crs_status crs_set_tuning(crs_context* ctx, int mode, int fwd_target_
 namespace {
 bool fwd_shape_ok(int bx, int by) {
     return (bx == 0 && by == 0) || (bx == 4 && by == 4) || (bx == 4 && by == 8) ||
-           (bx == 8 && by == 4) || (bx == kMfmaBX && by == kMfmaBY);
+           (bx == 8 && by == 4);
 }
 bool adj_shape_ok(int bx, int by) {
     return (bx == 0 && by == 0) || ((bx == 4 || bx == 8) && (by == 4 || by == 8));
@@ -752,50 +733,12 @@ void streamk_cut(const std::vector<int64_t>& len, const std::vector<double>& rat
 // else 4 x 4, else exact), tiles per launch, stream-K segments per launch.
 crs_status plan_forward_fp32c(const crs_context* ctx, int npil, std::vector<DevPlane>& dp,
                               FwdPlan& plan) {
+    const int nplanes = static_cast<int>(dp.size());
     int want_bx = ctx->fwd_shape[0] ? ctx->fwd_shape[0] : kDefFwdBX;
     int want_by = ctx->fwd_shape[1] ? ctx->fwd_shape[1] : kDefFwdBY;
-    // Matrix-core blocks: the interior of every admitted plane; its edge strips (columns
-    // beyond the last whole block, then rows below the last whole block) become planes of
-    // their own for the block kernels, writing into the parent's rows (ldx).
-    const bool mfma = want_bx == kMfmaBX && want_by == kMfmaBY;
-    std::vector<MfmaBlock> mblocks;
-    std::vector<int> parent_of(dp.size(), -1);
-    if (mfma) {
-        const int n0 = static_cast<int>(dp.size());
-        for (int p = 0; p < n0; ++p) {
-            if (!mfma_plane_admitted(dp[p])) continue;
-            const DevPlane P = dp[p];
-            const int tx = P.nx / kMfmaBX, ty = P.ny / kMfmaBY;
-            for (int j = 0; j < ty; ++j)
-                for (int i = 0; i < tx; ++i)
-                    mblocks.push_back({p, i * kMfmaBX, j * kMfmaBY, 0, 0, 0});
-            parent_of[p] = p;
-            auto strip = [&](int i0, int j0, int nx, int ny) {
-                if (nx <= 0 || ny <= 0) return;
-                DevPlane d = P;
-                d.x0 = P.x0 + i0 * P.sx;
-                d.y0 = P.y0 + j0 * P.sy;
-                d.z0 = P.z0 + j0 * P.sz;
-                d.zref = d.kturn * d.z0;
-                d.nx = nx;
-                d.ny = ny;
-                d.nxp = nx;
-                d.nquads = static_cast<int32_t>(ceil_div(ny, kQuad));
-                d.offset = P.offset + static_cast<int64_t>(j0) * P.ldx + i0;
-                dp.push_back(d);
-                parent_of.push_back(-1);
-            };
-            strip(tx * kMfmaBX, 0, P.nx - tx * kMfmaBX, P.ny);
-            strip(0, ty * kMfmaBY, tx * kMfmaBX, P.ny - ty * kMfmaBY);
-        }
-        want_bx = kDefVecFwdBX;  // the strips and the planes not admitted
-        want_by = kDefVecFwdBY;
-    }
-    const int nplanes = static_cast<int>(dp.size());
     const int shapes[3][2] = {{want_bx, want_by}, {4, 4}, {1, 1}};
     std::vector<int> pick(nplanes, 2);
     for (int p = 0; p < nplanes; ++p) {
-        if (parent_of[p] >= 0) { pick[p] = -1; continue; }  // on the matrix cores
         if (dp[p].kernel == CRS_KERNEL_RSI) continue;  // RS-I: every pair exact
         if (fp32c_block_admitted(dp[p], want_bx, want_by, dp[p].ny)) pick[p] = 0;
         else if (fp32c_block_admitted(dp[p], 4, 4, dp[p].ny)) pick[p] = 1;
@@ -864,40 +807,9 @@ crs_status plan_forward_fp32c(const crs_context* ctx, int npil, std::vector<DevP
         wg_all.insert(wg_all.end(), wg_start.begin(), wg_start.end());
         plan.launches.push_back(La);
     }
-    // Matrix-core launch: the (block, pillar) work line cut stream-K style at 64 pillars.
-    std::vector<MfmaSeg> msegs;
-    std::vector<int32_t> mwg;
-    if (!mblocks.empty()) {
-        std::vector<int64_t> len(mblocks.size(), npil);
-        std::vector<double> rate(mblocks.size(), 1.0);
-        const int64_t units = static_cast<int64_t>(mblocks.size()) * ceil_div(npil, 64);
-        const int64_t pairs = static_cast<int64_t>(mblocks.size()) * npil * kMfmaPix;
-        int per_cu = mfma_resident_per_cu();
-        if (per_cu <= 0) per_cu = 2;
-        const int64_t slots = static_cast<int64_t>(per_cu) * std::max(1, ctx->cus);
-        const int tuned = ctx->fwd_target[CRS_MODE_FP32C];
-        const int64_t target = tuned > 0 ? tuned : slots * kMfmaRounds;
-        const int64_t nwg = std::min<int64_t>(
-            target, std::max<int64_t>(1, std::min(units, pairs / kMinPiecePairs)));
-        int last = -1;
-        streamk_cut(len, rate, nwg, slots, 64,
-                    [&](int r, int64_t u0, int64_t u1) {
-                        if (r != last) {
-                            mblocks[r].seg0 = static_cast<int32_t>(msegs.size());
-                            last = r;
-                        }
-                        ++mblocks[r].nseg;
-                        msegs.push_back({r, static_cast<int32_t>(u0), static_cast<int32_t>(u1), 0});
-                    },
-                    mwg);
-        plan.nmblocks = static_cast<int>(mblocks.size());
-        plan.nmseg = static_cast<int64_t>(msegs.size());
-        plan.nmwg = static_cast<int>(mwg.size()) - 1;
-    }
-    if (segs.size() > static_cast<size_t>(INT32_MAX) || msegs.size() > static_cast<size_t>(INT32_MAX) ||
-        static_cast<double>(segs.size()) * slot_elems * 8.0 +
-                static_cast<double>(msegs.size()) * kMfmaPix * 8.0 > 64e9) {
-        set_error("launch too large (%zu + %zu segments)", segs.size(), msegs.size());
+    if (segs.size() > static_cast<size_t>(INT32_MAX) ||
+        static_cast<double>(segs.size()) * slot_elems * 8.0 > 64e9) {
+        set_error("launch too large (%zu segments)", segs.size());
         return CRS_ERR_INVALID;
     }
     std::vector<FwdRed> reds;
@@ -913,11 +825,6 @@ crs_status plan_forward_fp32c(const crs_context* ctx, int npil, std::vector<DevP
     plan.o_reds = plan.tb.add(reds.data(), reds.size() * sizeof(FwdRed));
     const size_t o_wg = plan.tb.add(wg_all.data(), wg_all.size() * sizeof(int32_t));
     for (Fp32cLaunch& La : plan.launches) La.o_wg = o_wg + La.o_wg * sizeof(int32_t);
-    if (plan.nmwg > 0) {
-        plan.o_mblocks = plan.tb.add(mblocks.data(), mblocks.size() * sizeof(MfmaBlock));
-        plan.o_msegs = plan.tb.add(msegs.data(), msegs.size() * sizeof(MfmaSeg));
-        plan.o_mwg = plan.tb.add(mwg.data(), mwg.size() * sizeof(int32_t));
-    }
     return CRS_OK;
 }
 
@@ -1002,7 +909,7 @@ crs_status size_forward(const FwdPlan& plan, const Scratch& sc, hipStream_t stre
     crs_status st = ensure(sc.owner, *sc.pil, staged_pillar_bytes(plan.mode, plan.npil, plan.nsets),
                            stream, "staged pillars");
     if (st != CRS_OK) return st;
-    const int64_t elems = plan.mode == CRS_MODE_FP32C ? fp32c_partial_elems(plan) : plan.part_elems;
+    const int64_t elems = plan.mode == CRS_MODE_FP32C ? plan.nseg * plan.slot_elems : plan.part_elems;
     return ensure(sc.owner, *sc.partial, static_cast<size_t>(elems) * partial_bytes(plan.mode),
                   stream, "forward partials");
 }
@@ -1031,19 +938,6 @@ crs_status enqueue_forward(const FwdPlan& plan, const Scratch& sc, const double*
         a.segs = reinterpret_cast<const FwdSeg*>(sc.tables + plan.o_segs);
         a.partial = static_cast<float2*>(sc.partial->ptr);
         a.slot_elems = plan.slot_elems;
-        Fp32cMfmaArgs m;
-        m.pos = a.pos;
-        m.ap = a.ap;
-        m.npil = plan.npil;
-        m.planes = d_planes;
-        m.blocks = reinterpret_cast<const MfmaBlock*>(sc.tables + plan.o_mblocks);
-        m.segs = reinterpret_cast<const MfmaSeg*>(sc.tables + plan.o_msegs);
-        m.wg_seg = reinterpret_cast<const int32_t*>(sc.tables + plan.o_mwg);
-        m.nwg = plan.nmwg;
-        m.partial = a.partial + plan.nseg * plan.slot_elems;
-        if (plan.nmwg > 0 &&
-            (st = launch_status(launch_forward_mfma(m, stream), "forward (matrix cores)")) != CRS_OK)
-            return st;
         for (const Fp32cLaunch& La : plan.launches) {
             a.wg_seg = reinterpret_cast<const int32_t*>(sc.tables + La.o_wg);
             a.nwg = La.nwg;
@@ -1051,18 +945,12 @@ crs_status enqueue_forward(const FwdPlan& plan, const Scratch& sc, const double*
                 return st;
         }
         if (ev1) hipEventRecord(ev1, stream);
-        if ((st = launch_status(
-                 launch_fwd_reduce_fp32c(d_planes, a.tiles,
-                                         reinterpret_cast<const FwdRed*>(sc.tables + plan.o_reds),
-                                         plan.nred, a.partial, plan.slot_elems,
-                                         reinterpret_cast<double2*>(field), intensity, stream),
-                 "forward reduce")) != CRS_OK)
-            return st;
-        if (plan.nmwg <= 0) return CRS_OK;
-        return launch_status(launch_fwd_reduce_mfma(d_planes, m.blocks, plan.nmblocks, m.partial,
-                                                    reinterpret_cast<double2*>(field), intensity,
-                                                    stream),
-                             "forward reduce (matrix cores)");
+        return launch_status(
+            launch_fwd_reduce_fp32c(d_planes, a.tiles,
+                                    reinterpret_cast<const FwdRed*>(sc.tables + plan.o_reds),
+                                    plan.nred, a.partial, plan.slot_elems,
+                                    reinterpret_cast<double2*>(field), intensity, stream),
+            "forward reduce");
     }
     if ((st = launch_status(launch_forward(mode, sc.pil->ptr, plan.npil, d_planes,
                                            reinterpret_cast<const FwdBlock*>(sc.tables + plan.o_blocks),
@@ -1098,8 +986,18 @@ crs_status plan_adjoint_fp32c(const crs_context* ctx, int npil, int nsets,
     for (int p = 0; p < nplanes; ++p) {
         const int rows = dp[p].nquads * kStrip;
         if (dp[p].kernel == CRS_KERNEL_RSI) continue;  // RS-I: every pair exact
-        if (fp32c_block_admitted(dp[p], want_bx, want_by, rows)) pick[p] = 0;
-        else if (fp32c_block_admitted(dp[p], 4, 4, rows)) pick[p] = 1;
+        DevPlane q = dp[p];
+        q.ny = rows;
+        // 8 x 8 blocks take the separable form where its bound holds at every obliquity
+        // (the staging then writes the plane's records in its layout), else the series.
+        if (want_bx * want_by == 64 && sep_plane_admitted(q, want_bx, want_by)) {
+            pick[p] = 0;
+            dp[p].sep = 1;
+        } else if (fp32c_block_admitted(dp[p], want_bx, want_by, rows)) {
+            pick[p] = 0;
+        } else if (fp32c_block_admitted(dp[p], 4, 4, rows)) {
+            pick[p] = 1;
+        }
     }
     if (want_bx == 4 && want_by == 4)
         for (int& k : pick) k = k == 1 ? 0 : k;
@@ -1371,13 +1269,6 @@ crs_status enqueue_adjoint(const AdjPlan& plan, const Scratch& sc, const double*
 template <class Plan>
 void note_launch(crs_context* ctx, const Plan& plan, bool adjoint) {
     const int a = adjoint ? 1 : 0;
-    if constexpr (std::is_same<Plan, FwdPlan>::value) {
-        if (plan.mode == CRS_MODE_FP32C && plan.nmwg > 0) {
-            ctx->last_kernel[a] = "crs::fwd_mfma";
-            ctx->last_groups[a] = plan.nmwg;
-            return;
-        }
-    }
     if (plan.mode == CRS_MODE_FP32C) {
         if (plan.launches.empty()) return;
         ctx->last_kernel[a] = fp32c_kernel_name(adjoint, plan.launches[0].bx, plan.launches[0].by);
@@ -1685,15 +1576,12 @@ extern "C" crs_status crs_forward_noalloc(void* stream_, int mode, const double*
         plan = FwdPlan();
         if ((st = plan_forward(&pc, mode, npil, nsets, planes, nplanes, plan)) != CRS_OK) return st;
         staged = align256(staged_pillar_bytes(mode, npil, nsets));
-        const int64_t elems = mode == CRS_MODE_FP32C ? fp32c_partial_elems(plan) : plan.part_elems;
+        const int64_t elems = mode == CRS_MODE_FP32C ? plan.nseg * plan.slot_elems : plan.part_elems;
         partial = align256(static_cast<size_t>(elems) * partial_bytes(mode));
         tables = align256(plan.tb.bytes.size());
         if (staged + partial + tables <= kNoAllocScratchBytes) break;
-        int groups = plan.nblocks;
-        if (mode == CRS_MODE_FP32C) {
-            groups = plan.nmwg;
-            for (const Fp32cLaunch& La : plan.launches) groups = std::max(groups, La.nwg);
-        }
+        const int groups = mode == CRS_MODE_FP32C ? (plan.launches.empty() ? 0 : plan.launches[0].nwg)
+                                                  : plan.nblocks;
         if (attempt >= 12 || groups <= 64) {
             set_error("crs_forward_noalloc: the call needs %zu bytes of scratch, more than the "
                       "%zu-byte allocation-free maximum (use crs_forward)",

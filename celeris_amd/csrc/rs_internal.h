@@ -61,6 +61,8 @@ constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
 constexpr int kP1A1 = 0;
 constexpr int kP1A2 = 1;
 constexpr int kP2A2 = 2;
+// The adjoint's separable block form (8 x 8 blocks of planes the host marks `sep`).
+constexpr int kSep = 3;
 
 // Lower bound on |z| of every row z0 + j sz (0 <= j < ny) of a plane and of every
 // pixel block's centre: the nearer end when all rows lie on one side of the lens
@@ -111,7 +113,9 @@ struct DevPlane {
     // never); beyond obl2, kP2A2, which the host admitted at any obliquity.
     double obl1, obl2;
     int32_t kernel;  // crs_kernel: 0 exp(ikR)/R, 1 RS-I (FP32C: pair by pair, 1 x 1 blocks)
-    int32_t ldx;     // output row stride (nx, or the parent plane's nx for an FP32C edge strip)
+    // FP32C adjoint: 1 if the plane's blocks take the separable form (its staged pixel
+    // records then hold u (k/2pi) exp(-2 pi i z'_ref) as {Re, Im, -Re}).
+    int32_t sep;
 };
 
 // ---- FP32 / FP64 work decomposition (split sums, uniform slots) --------------------
@@ -200,41 +204,6 @@ __host__ __device__ inline void tile_origin(const DevPlane& P, int shape, int t,
         default: ox = kbx * P.tx + t % rx; oy = kby * P.ty + t / rx; break;
     }
 }
-
-// ---- FP32C forward on the matrix cores -----------------------------------------------
-// A plane's interior is cut into kMfmaBX x kMfmaBY pixel blocks (its edge strips go to the
-// block kernels as planes of their own).  For a pillar q and a block with centre
-// distance R0, a pixel (c, r) at R^2 = R0^2 + a_c + b_r has
-//     t exp(2 pi i R') / R = U_c V_r K(a_c, b_r),
-//     U_c = c1 exp(2 pi i th0) F(a_c),  V_r = F(b_r),  F(D) = (R0 / R(D)) exp(2 pi i (R(D) - R0)),
-//     K(a, b) = F(a + b) / (F(a) F(b)) = 1 + a l1(b) + a^2 (l2(b) + l1(b)^2 / 2) + O(a^3),
-// l_j(b) = (psi^(j)(b) - psi^(j)(0)) / j!, psi = log F.  So the block's field is a sum
-// over pillars of three rank-one complex products X_m(c) Y_m(r) (X_m = U_c a_c^m), i.e.
-// a complex matrix product with the pillars (x 3) as the contraction: each rank term is
-// one v_mfma_f32_32x32x2_f32 (A = 16 columns x {Re, Im} rows, B = 32 rows, K = {Re, Im}).
-constexpr int kMfmaBX = 16;
-constexpr int kMfmaBY = 32;
-constexpr int kMfmaPix = kMfmaBX * kMfmaBY;
-
-// One block: pixel origin (ox, oy) of plane `plane`; its segments are partial slots
-// seg0 .. seg0 + nseg - 1 (kMfmaPix float pairs each, pixel r * kMfmaBX + c).
-struct MfmaBlock {
-    int32_t plane, ox, oy, seg0, nseg, pad;
-};
-struct MfmaSeg {
-    int32_t block, q0, q1, pad;
-};
-struct Fp32cMfmaArgs {
-    const double2* pos;
-    const float2* ap;
-    int npil;
-    const DevPlane* planes;
-    const MfmaBlock* blocks;
-    const MfmaSeg* segs;
-    const int32_t* wg_seg;  // nwg + 1 entries
-    int nwg;
-    float2* partial;
-};
 
 // Device arguments of one FP32C forward / adjoint pair-kernel launch.
 struct Fp32cFwdArgs {
@@ -374,36 +343,46 @@ __host__ __device__ inline int fp32c_series_order(const DevPlane& P, double ex, 
     return kP2A2;
 }
 
-// ---- Pair error of the matrix-core forward ---------------------------------------------
-// For a pair at obliquity s (from the block centre) a kMfmaBX x kMfmaBY block of plane P
-// gives a term within eps |t| / R of t exp(ikR) / R, eps the sum of
-//   * the dropped rank terms of K.  With h = 1/(2 R0), |a| <= A = 2 e_x rho + e_x^2,
-//     |b| <= B = 2 e_y rho + e_y^2 + 2 zc e_z + e_z^2 (rho = R0 s, zc = R0 sqrt(1 - s^2),
-//     e the block's half extents in turns) and y = b h^2, the series in y give
-//       |l1| <= |y| (8 h^2 + 4 pi h) / (1 - 4|y|),   |l2| <= |y| (32 h^4 + 12 pi h^3) / (1 - 6|y|),
-//       |l3| <= |y| (128 h^6 + 40 pi h^5) / (1 - 8|y|),
-//     so L = log K = sum_j a^j l_j has |L| <= ell = t1 + t2 + t3 (t1 = A|l1|, t2 = A^2|l2|,
-//     t3 = A^3|l3| (1 + 16 A h^2) with the tail j >= 4), and exp(L) minus its kept part
-//     1 + a l1 + a^2 (l2 + l1^2/2) is at most ell^3 e^ell / 6 + t3 + (t2 + t3)(ell + t1)/2;
-//   * the truncated series the kernel evaluates: l1 to y^2 (remainders 512 h^2 |y|^4 and
-//     140 pi h |y|^4, over 1 - 4|y|, times A), l2 to y (1024 h^4 |y|^3 and 280 pi h^3
-//     |y|^3 over 1 - 5|y|, times A^2), and each side's F to y^3 (phase 14 |D h| y^4,
-//     amplitude 70 y^4, over 1 - 4y);
-//   * rounding: each side's phase is formed in FP64 and rounded once after its reduction
-//     mod 1 (2^-25 turns each), its float correction term adds 3 2^-24 |D h| y; the
-//     amplitude product is 8 float roundings (2^-21);
-//   * the hardware sin / cos of the two factors, 2.5e-7 (as for the block kernels).
-// The host admits a plane where eps stays within kPairBudget at every obliquity.
-__host__ __device__ inline double mfma_pair_error(const DevPlane& P, double s) {
+// ---- Separable block evaluation ----------------------------------------------------------
+// For a pillar and a block with centre distance R0 (turns), pixel (c, r) has
+// R^2 = R0^2 + a_c + b_r, and with F(D) = (R0 / R(D)) exp(2 pi i (R(D) - R0)),
+//     t exp(2 pi i R') / R = c1 exp(2 pi i th0) F(a_c + b_r) = U_c V_r K(a_c, b_r),
+//     U_c = c1 exp(2 pi i th0) F(a_c),  V_r = F(b_r),  K(a, b) = F(a + b) / (F(a) F(b)).
+// log K = sum_{j>=1} a^j l_j(b), l_j(b) = (psi^(j)(b) - psi^(j)(0)) / j!, psi = log F; with
+// h = 1/(2 R0) and y = b h^2 the leading terms are
+//     l1 = b (8 h^4 (1 - 4y + ...) - 4 pi i h^3 (1 - 3y + ...)),   l2 = y (-32 h^4 - ... + 12 pi i h^3 ...).
+// The separable form keeps rank 2: pair (c, r) = U_c (V_r + a_c W_r), W_r = V_r l1(b_r),
+// with F to second order (phase D h (1 - y + 2y^2), amplitude 1 - 2y + 6y^2, y = D h^2) and
+// l1 to first order in y.  Per pair that is 6 FP32 FMAs and no transcendental; per pillar
+// and block one sin / cos per column and per row.
+//
+// Pair error, as a fraction of |t| / R, for a pillar at obliquity s and azimuth phi from
+// the block centre (|dX0| = rho |cos phi|, |dY0| = rho |sin phi|, rho = R0 s, block depth
+// zc = R0 sqrt(1 - s^2), R0 >= z' / sqrt(1 - s^2)): with |a| <= A = 2 e_x |dX0| + e_x^2,
+// |b| <= B = 2 e_y |dY0| + e_y^2 + 2 zc e_z + e_z^2 (e the block's half extents in turns),
+// y_a = A h^2, y_b = B h^2, the series in y give
+//     |l1| <= y_b (8 h^2 + 4 pi h) / (1 - 4 y_b),   |l2| <= y_b (32 h^4 + 12 pi h^3) / (1 - 6 y_b),
+//     |l3| <= y_b (128 h^6 + 40 pi h^5) / (1 - 8 y_b),
+// so |log K| <= ell = t1 + t2 + t3 (t1 = A|l1|, t2 = A^2|l2|, t3 = A^3|l3|(1 + 16 y_a), the
+// tail j >= 4 included) and eps is the sum of
+//   * the dropped rank terms, |K - 1 - a l1| <= ell^2 e^ell / 2 + t2 + t3;
+//   * the series remainders: F, phase 5 y^3 |D h| and amplitude 20 y^3 per side over
+//     1 - 4y; l1, A (128 h^2 + 40 pi h) y_b^3 over 1 - 4 y_b;
+//   * float rounding: phase 2^-24 (3 + 8 (A + B) h) turns (th0 and the two phase sums;
+//     a and b from rounded offsets, and h, each a few ulp of |a h| or |b h|); amplitude
+//     1.2 2^-21 (c1, the two series amplitudes, the two phasor scalings, the products);
+//   * the hardware sin / cos of the two factors, 2.5e-7.
+__host__ __device__ inline double sep_pair_error(const DevPlane& P, double s, double cphi,
+                                                 double sphi, int bx, int by) {
     const double kt = P.kturn;
-    const double ex = 0.5 * (kMfmaBX - 1) * fabs(P.sx) * kt;
-    const double ey = 0.5 * (kMfmaBY - 1) * fabs(P.sy) * kt;
-    const double ez = 0.5 * (kMfmaBY - 1) * fabs(P.sz) * kt;
+    const double ex = 0.5 * (bx - 1) * fabs(P.sx) * kt;
+    const double ey = 0.5 * (by - 1) * fabs(P.sy) * kt;
+    const double ez = 0.5 * (by - 1) * fabs(P.sz) * kt;
     const double z = kt * plane_min_abs_z(P.z0, P.sz, P.ny);
     if (!(z > 0.0) || !(s >= 0.0 && s < 1.0)) return INFINITY;
     const double cz = sqrt(1.0 - s * s), R0 = z / cz, h = 0.5 / R0, rho = R0 * s;
-    const double A = 2.0 * ex * rho + ex * ex;
-    const double B = 2.0 * ey * rho + ey * ey + 2.0 * R0 * cz * ez + ez * ez;
+    const double A = 2.0 * ex * rho * cphi + ex * ex;
+    const double B = 2.0 * ey * rho * sphi + ey * ey + 2.0 * R0 * cz * ez + ez * ez;
     const double h2 = h * h, h3 = h2 * h, ya = A * h2, yb = B * h2;
     if (!(ya < 0.02 && yb < 0.02)) return INFINITY;
     const double pi = 0.5 * kTwoPiD;
@@ -412,29 +391,33 @@ __host__ __device__ inline double mfma_pair_error(const DevPlane& P, double s) {
     const double l3 = yb * (128.0 * h3 * h3 + 40.0 * pi * h3 * h2) / (1.0 - 8.0 * yb);
     const double t1 = A * l1, t2 = A * A * l2, t3 = A * A * A * l3 * (1.0 + 16.0 * ya);
     const double ell = t1 + t2 + t3;
-    const double rank = ell * ell * ell * exp(ell) / 6.0 + t3 + (t2 + t3) * (ell + t1) / 2.0;
-    const double yb2 = yb * yb, ya2 = ya * ya;
-    const double dl1 = A * (512.0 * h2 + 140.0 * pi * h) * yb2 * yb2 / (1.0 - 4.0 * yb);
-    const double dl2 = A * A * (1024.0 * h2 * h2 + 280.0 * pi * h3) * yb2 * yb / (1.0 - 5.0 * yb);
-    const double side_phase =
-        14.0 * (A * h * ya2 * ya2 / (1.0 - 4.0 * ya) + B * h * yb2 * yb2 / (1.0 - 4.0 * yb));
-    const double side_amp = 70.0 * (ya2 * ya2 / (1.0 - 4.0 * ya) + yb2 * yb2 / (1.0 - 4.0 * yb));
-    const double round_phase = 2.0 * 2.98023223876953125e-8 +
-                               3.0 * 5.9604644775390625e-8 * (A * h * ya + B * h * yb);
-    return kTwoPiD * (side_phase + round_phase) + side_amp + 1.1 * (rank + dl1 + dl2) +
-           4.76837158203125e-7 + 2.5e-7;
+    const double rank = ell * ell * exp(ell) / 2.0 + t2 + t3;
+    const double ya3 = ya * ya * ya, yb3 = yb * yb * yb;
+    const double dl1 = A * (128.0 * h2 + 40.0 * pi * h) * yb3 / (1.0 - 4.0 * yb);
+    const double side_phase = 5.0 * (A * h * ya3 / (1.0 - 4.0 * ya) + B * h * yb3 / (1.0 - 4.0 * yb));
+    const double side_amp = 20.0 * (ya3 / (1.0 - 4.0 * ya) + yb3 / (1.0 - 4.0 * yb));
+    const double round_phase = 5.9604644775390625e-8 * (3.0 + 8.0 * (A + B) * h);
+    return kTwoPiD * (side_phase + round_phase) + side_amp + 1.1 * (rank + dl1) +
+           1.2 * 4.76837158203125e-7 + 2.5e-7;
 }
 
-// Whether the matrix-core forward may take plane P: the exp(ikR)/R kernel, room for at
-// least one block, and eps within the pair budget at every obliquity (a grid of 512
-// obliquities; eps varies slowly with s).
-inline bool mfma_plane_admitted(const DevPlane& P) {
-    if (P.kernel != 0 || P.nx < kMfmaBX || P.ny < kMfmaBY) return false;
-    for (int i = 0; i < 512; ++i) {
-        const double s = 0.999 * i / 511.0;
-        if (!(mfma_pair_error(P, s) <= kPairBudget)) return false;
+// Largest eps of bx x by separable blocks of P over pillars at every obliquity and azimuth
+// (a grid of 256 obliquities x 33 azimuths in [0, pi/2]; eps varies slowly with both).
+inline double sep_plane_error(const DevPlane& P, int bx, int by) {
+    double worst = 0.0;
+    for (int j = 0; j <= 32; ++j) {
+        const double phi = 0.25 * kTwoPiD * j / 32.0, cphi = cos(phi), sphi = sin(phi);
+        for (int i = 0; i < 256; ++i) {
+            const double e = sep_pair_error(P, 0.999 * i / 255.0, cphi, sphi, bx, by);
+            if (!(e <= worst)) worst = e;
+        }
     }
-    return true;
+    return worst;
+}
+
+// Whether bx x by blocks of P take the separable form (the exp(ikR)/R kernel only).
+inline bool sep_plane_admitted(const DevPlane& P, int bx, int by) {
+    return P.kernel == 0 && sep_plane_error(P, bx, by) <= kPairBudget;
 }
 
 }  // namespace crs

@@ -550,220 +550,6 @@ __global__ __launch_bounds__(kFwdThreads, fwd_waves_per_simd(KBX, KBY)) void fwd
 }
 
 // ---------------------------------------------------------------------------------
-// Forward, FP32C on the matrix cores (the expansion and its error bound: rs_internal.h).
-// A work-group runs its segments (one kMfmaBX x kMfmaBY block against a pillar range) in
-// turn.  Per LDS tile of 256 pillars each wave takes 64: first one lane per pillar forms
-// the pillar's block state (FP64 distance, phase, direction cosines) into LDS, then the
-// whole wave walks the 64 states, each lane forming one element of the A operand (a
-// column factor: one v_sin_f32) and of the B operand (a row factor: v_cos_f32 and
-// v_sin_f32) for each of the three rank terms, and issues three v_mfma_f32_32x32x2_f32.
-// The matrix unit runs while other waves form their operands.
-
-typedef float floatx16 __attribute__((ext_vector_type(16)));
-
-// Pillar state of one block: u = dX0/R0, v = dY0/R0, th0 = frac(R0 - zeta), h = 1/(2 R0)
-// (FP64, turns); c1 = |t| (k/2pi) / R0', hf = h, twoR0 = 2 R0, p1r = 8 h^4, p1i = -4 pi h^3.
-struct MfmaState {
-    double u, v, th0, h;
-    float c1, hf, twoR0, p1r, p1i, pad0, pad1, pad2;
-};
-
-// Pixel (r, c) of the D registers: lane j + 32 * half holds row r = j and, in registers g
-// and g + 8 (g < 8), Re and Im of column 8 (g / 4) + 4 half + g % 4.
-__device__ __forceinline__ int mfma_col(int g, int half) { return 8 * (g >> 2) + 4 * half + (g & 3); }
-
-__device__ __forceinline__ void mfma_segment(const double2* __restrict__ pos,
-                                             const float2* __restrict__ aps, const DevPlane& P,
-                                             const MfmaBlock& Bk, int q0, int q1, PillarRec* tile,
-                                             MfmaState (*st)[64], double2* runl,
-                                             float2* __restrict__ slot) {
-    const int tid = threadIdx.x, wave = wave_id(), lane = tid & 63;
-    const double kt = P.kturn, zref = P.zref;
-    const double jc = Bk.oy + 0.5 * (kMfmaBY - 1);
-    const double FXc = kt * (P.x0 + (Bk.ox + 0.5 * (kMfmaBX - 1)) * P.sx);
-    const double FYc = kt * (P.y0 + jc * P.sy);
-    const double zc = kt * (P.z0 + jc * P.sz);
-    const double ZC2 = zc * zc;
-    // A operand element of this lane: row i = lane % 32 (Re of column i, or Im of column
-    // i - 16), K index kk = lane / 32 (0: Re part of the rank term, 1: Im part), so the
-    // element is Re X (cos), -Im X (-sin) or Im X (sin): sin(2 pi (phase + shA)).
-    const int ia = lane & 31, kk = lane >> 5;
-    const int col = ia & 15, part = ia >> 4;
-    const double ex = (col - 0.5 * (kMfmaBX - 1)) * (kt * P.sx);
-    const double ex2 = ex * ex;
-    const double shA = part == kk ? 0.25 : (part == 0 ? 0.5 : 0.0);
-    // B operand element: row r = lane % 32 of the block, kk = 0: Re Y, kk = 1: Im Y =
-    // Re(-i Y), the row factor turned by -1/4 turn.
-    const double mr = ia - 0.5 * (kMfmaBY - 1);
-    const double ey = mr * (kt * P.sy), ez = mr * (kt * P.sz);
-    const double Cb = fma(ey, ey, ez * (2.0 * zc + ez));  // b = 2 ey dY0 + Cb
-    const double shB = kk ? -0.25 : 0.0;
-
-    floatx16 acc0, acc1, acc2;
-#pragma unroll
-    for (int g = 0; g < 16; ++g) acc0[g] = acc1[g] = acc2[g] = 0.0f;
-    double run[16];
-#pragma unroll
-    for (int g = 0; g < 16; ++g) run[g] = 0.0;
-    int since_fold = 0;
-    for (int base = q0; base < q1; base += kFwdThreads) {
-        const int n = min(kFwdThreads, q1 - base);
-        __syncthreads();
-        if (tid < n) {
-            const double2 xy = pos[base + tid];
-            const float2 a = aps[base + tid];
-            PillarRec rec;
-            rec.x = kt * xy.x;
-            rec.y = kt * xy.y;
-            rec.zeta = zref - static_cast<double>(a.y);
-            rec.amp = a.x;  // |t|
-            rec.pad = 0.0f;
-            tile[tid] = rec;
-        }
-        __syncthreads();
-        const int k0 = wave * 64, nw = max(0, min(64, n - k0));
-        if (lane < nw) {
-            const PillarRec T = tile[k0 + lane];
-            const double dX0 = FXc - T.x, dY0 = FYc - T.y;
-            const double R0sq = fma(dX0, dX0, fma(dY0, dY0, ZC2));
-            float rf;
-            const double R0 = block_distance(R0sq, rf);
-            double inv = static_cast<double>(rf);
-            inv = inv * fma(-0.5 * R0sq, inv * inv, 1.5);  // 1/R0 to FP64 accuracy
-            MfmaState Q;
-            Q.u = dX0 * inv;
-            Q.v = dY0 * inv;
-            Q.th0 = __builtin_amdgcn_fract(R0 - T.zeta);
-            Q.h = 0.5 * inv;
-            Q.c1 = static_cast<float>(static_cast<double>(T.amp) * kt * inv);
-            Q.hf = static_cast<float>(Q.h);
-            Q.twoR0 = static_cast<float>(R0 + R0);
-            const float h2 = Q.hf * Q.hf;
-            Q.p1r = 8.0f * h2 * h2;
-            Q.p1i = -12.5663706143591729f * h2 * Q.hf;
-            Q.pad0 = Q.pad1 = Q.pad2 = 0.0f;
-            st[wave][lane] = Q;
-        }
-        __syncthreads();
-        for (int k = 0; k < nw; ++k) {
-            const MfmaState Q = st[wave][k];
-            // Column factor X_m = c1 F(a) exp(2 pi i th0) a^m: a h = ex u + ex^2 h, y = a h^2,
-            // phase th0 + a h (1 - y + 2y^2 - 5y^3) in FP64, amplitude 1 - 2y + 6y^2 - 20y^3.
-            const double ah = fma(ex, Q.u, ex2 * Q.h);
-            const float ahf = static_cast<float>(ah);
-            const float ya = ahf * Q.hf;
-            const float da = ya * fmaf(ya, fmaf(-5.0f, ya, 2.0f), -1.0f);
-            const float ga = fmaf(ya, fmaf(ya, fmaf(-20.0f, ya, 6.0f), -2.0f), 1.0f);
-            const double pa = fma(ah, static_cast<double>(da), ah) + (Q.th0 + shA);
-            const float sa = __builtin_amdgcn_sinf(static_cast<float>(__builtin_amdgcn_fract(pa)));
-            const float a = ahf * Q.twoR0;
-            const float x0 = Q.c1 * ga * sa;
-            const float x1 = x0 * a;
-            const float x2 = x1 * a;
-            // Row factors Y_m = F(b) k_m(b): k_0 = 1, k_1 = l1, k_2 = l2 + l1^2 / 2 with
-            // l1 = b (p1r (1 - 4y + 16y^2) + i p1i (1 - 3y + 10y^2)),
-            // l2 = y (-4 p1r (1 - 6y) - 3 i p1i (1 - 5y)).
-            const double bh = fma(ey, Q.v, Cb * Q.h);
-            const float bhf = static_cast<float>(bh);
-            const float yb = bhf * Q.hf;
-            const float db = yb * fmaf(yb, fmaf(-5.0f, yb, 2.0f), -1.0f);
-            const float gb = fmaf(yb, fmaf(yb, fmaf(-20.0f, yb, 6.0f), -2.0f), 1.0f);
-            const double pb = fma(bh, static_cast<double>(db), bh) + shB;
-            const float phb = static_cast<float>(__builtin_amdgcn_fract(pb));
-            const float cb = gb * __builtin_amdgcn_cosf(phb);
-            const float sb = gb * __builtin_amdgcn_sinf(phb);
-            const float b = bhf * Q.twoR0;
-            const float l1r = b * Q.p1r * fmaf(yb, fmaf(16.0f, yb, -4.0f), 1.0f);
-            const float l1i = b * Q.p1i * fmaf(yb, fmaf(10.0f, yb, -3.0f), 1.0f);
-            const float l2r = -4.0f * Q.p1r * yb * fmaf(-6.0f, yb, 1.0f);
-            const float l2i = -3.0f * Q.p1i * yb * fmaf(-5.0f, yb, 1.0f);
-            const float k2r = fmaf(0.5f * l1r, l1r, fmaf(-0.5f * l1i, l1i, l2r));
-            const float k2i = fmaf(l1r, l1i, l2i);
-            const float y1 = fmaf(cb, l1r, -sb * l1i);
-            const float y2 = fmaf(cb, k2r, -sb * k2i);
-            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(x0, cb, acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(x1, y1, acc1, 0, 0, 0);
-            acc2 = __builtin_amdgcn_mfma_f32_32x32x2f32(x2, y2, acc2, 0, 0, 0);
-        }
-        since_fold += 64;
-        if (since_fold >= kWaveFold || base + kFwdThreads >= q1) {
-#pragma unroll
-            for (int g = 0; g < 16; ++g) {
-                run[g] += static_cast<double>(acc0[g]) + static_cast<double>(acc1[g]) +
-                          static_cast<double>(acc2[g]);
-                acc0[g] = acc1[g] = acc2[g] = 0.0f;
-            }
-            since_fold = 0;
-        }
-    }
-    // The four waves' sums, added in wave order, then the segment's slot.
-#pragma unroll
-    for (int w = 0; w < kFwdWaves; ++w) {
-        __syncthreads();
-        if (wave == w) {
-#pragma unroll
-            for (int g = 0; g < 8; ++g) {
-                const int p = ia * kMfmaBX + mfma_col(g, kk);
-                double2 v = w == 0 ? make_double2(0.0, 0.0) : runl[p];
-                v.x += run[g];
-                v.y += run[g + 8];
-                runl[p] = v;
-            }
-        }
-    }
-    __syncthreads();
-    for (int i = tid; i < kMfmaPix; i += kFwdThreads) {
-        const double2 v = runl[i];
-        slot[i] = make_float2(static_cast<float>(v.x), static_cast<float>(v.y));
-    }
-}
-
-__global__ __launch_bounds__(kFwdThreads, 3) void fwd_mfma(
-    const double2* __restrict__ pos, const float2* __restrict__ ap, int npil,
-    const DevPlane* __restrict__ planes, const MfmaBlock* __restrict__ blocks,
-    const MfmaSeg* __restrict__ segs, const int32_t* __restrict__ wg_seg,
-    float2* __restrict__ partial) {
-    __shared__ PillarRec tile[kFwdThreads];
-    __shared__ MfmaState st[kFwdWaves][64];
-    __shared__ double2 runl[kMfmaPix];
-    const int s0 = wg_seg[blockIdx.x], s1 = wg_seg[blockIdx.x + 1];
-    for (int s = s0; s < s1; ++s) {
-        const MfmaSeg S = segs[s];
-        const MfmaBlock Bk = blocks[S.block];
-        const DevPlane& P = planes[Bk.plane];
-        mfma_segment(pos, ap + static_cast<int64_t>(P.tset) * npil, P, Bk, S.q0, S.q1, tile, st,
-                     runl, partial + static_cast<int64_t>(s) * kMfmaPix);
-    }
-}
-
-// Reduction of the matrix-core forward: one work-group per block adds the block's slots
-// in FP64 in slot order (each thread two pixels), turns by exp(2 pi i z'_ref) and writes
-// field and / or |E|^2.
-__global__ __launch_bounds__(kThreads) void fwd_reduce_mfma(
-    const DevPlane* __restrict__ planes, const MfmaBlock* __restrict__ blocks,
-    const float2* __restrict__ partial, double2* __restrict__ field, double* __restrict__ intensity) {
-    const MfmaBlock Bk = blocks[blockIdx.x];
-    const DevPlane& P = planes[Bk.plane];
-    double sn, cs;
-    sincos(kTwoPi * (P.zref - floor(P.zref)), &sn, &cs);
-    for (int p = threadIdx.x; p < kMfmaPix; p += kThreads) {
-        const float2* src = partial + static_cast<int64_t>(Bk.seg0) * kMfmaPix + p;
-        double er = 0.0, ei = 0.0;
-        for (int k = 0; k < Bk.nseg; ++k) {
-            const float2 v = src[static_cast<int64_t>(k) * kMfmaPix];
-            er += v.x;
-            ei += v.y;
-        }
-        const double xr = er * cs - ei * sn;
-        const double xi = er * sn + ei * cs;
-        const int r = p / kMfmaBX, c = p % kMfmaBX;
-        const int64_t o = P.offset + static_cast<int64_t>(Bk.oy + r) * P.ldx + (Bk.ox + c);
-        if (field) field[o] = make_double2(xr, xi);
-        if (intensity) intensity[o] = xr * xr + xi * xi;
-    }
-}
-
-// ---------------------------------------------------------------------------------
 // Pair kernels G(R) of the FP32 / FP64 paths.  RSI = false: exp(ikR)/R, the reference's
 // (focal.cpp:232, propagate.cu:71-77).  RSI = true: Rayleigh-Sommerfeld I,
 // exp(ikR)/R * (z/R) * (1/R - ik) = exp(ikR) * (z/R^2) * (1/R - ik), opt-in per plane.
@@ -1068,7 +854,7 @@ __global__ __launch_bounds__(kThreads) void fwd_reduce_fp32c(
     sincos(kTwoPi * (P.zref - floor(P.zref)), &sn, &cs);
     const double xr = er * cs - ei * sn;
     const double xi = er * sn + ei * cs;
-    const int64_t o = P.offset + static_cast<int64_t>(oy + r) * P.ldx + (ox + c);
+    const int64_t o = P.offset + static_cast<int64_t>(oy + r) * P.nx + (ox + c);
     if (field) field[o] = make_double2(xr, xi);
     if (intensity) intensity[o] = xr * xr + xi * xi;
 }
@@ -1077,8 +863,10 @@ __global__ __launch_bounds__(kThreads) void fwd_reduce_fp32c(
 // Adjoint staging (FP32C).  Pixel records are laid out strip by strip (kStrip rows),
 // column by column, kStrip records per column: record (strip u, column i, row r) sits at
 // sbase + (u*nxp + i)*kStrip + r, so an LDS tile of 64 columns x kStrip rows is one
-// contiguous run.  Record = {2U, -U, arg(u)/2pi - z'_ref mod 1, 0} with U = |u| k/2pi;
-// padding (rows >= ny, columns >= nx) has U = 0 and contributes nothing.
+// contiguous run.  Record = {2U, -U, arg(u)/2pi - z'_ref mod 1, 0} with U = |u| k/2pi, or
+// on planes whose blocks take the separable form {Re w, Im w, -Re w, 0} with
+// w = u (k/2pi) exp(-2 pi i z'_ref); padding (rows >= ny, columns >= nx) is all zero and
+// contributes nothing.
 
 __global__ __launch_bounds__(kThreads) void stage_pixels_fp32c(
     const DevPlane* __restrict__ planes, const PixBlock* __restrict__ pblocks,
@@ -1094,10 +882,20 @@ __global__ __launch_bounds__(kThreads) void stage_pixels_fp32c(
     float4 rec = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (i < P.nx && j < P.ny) {
         const double2 v = u[P.offset + static_cast<int64_t>(j) * P.nx + i];
-        double chi = atan2(v.y, v.x) * kInv2Pi - P.zref;
-        chi -= rint(chi);
-        const float U = static_cast<float>(hypot(v.x, v.y) * P.kturn);
-        rec = make_float4(2.0f * U, -U, static_cast<float>(chi), 0.0f);
+        if (P.sep) {
+            // Separable planes: u (k/2pi) exp(-2 pi i z'_ref) as {Re, Im, -Re}.
+            double sn, cs;
+            sincos(kTwoPi * (P.zref - floor(P.zref)), &sn, &cs);
+            const double wr = P.kturn * (v.x * cs + v.y * sn);
+            const double wi = P.kturn * (v.y * cs - v.x * sn);
+            rec = make_float4(static_cast<float>(wr), static_cast<float>(wi),
+                              static_cast<float>(-wr), 0.0f);
+        } else {
+            double chi = atan2(v.y, v.x) * kInv2Pi - P.zref;
+            chi -= rint(chi);
+            const float U = static_cast<float>(hypot(v.x, v.y) * P.kturn);
+            rec = make_float4(2.0f * U, -U, static_cast<float>(chi), 0.0f);
+        }
     }
     pix[P.sbase + rix] = rec;
 }
@@ -1232,8 +1030,119 @@ __device__ __forceinline__ void adj_tile(const float* sU2, const float* sUm, con
     }
 }
 
+// Separable form (planes the host marks `sep`; the expansion and its bound: rs_internal.h).
+// For one pillar and one block whose first record is k0,
+//     sum_(c,r) conj(G) u = sum_c Ubar_c S_c,   S_c = sum_r conj(V_r + a_c W_r) w_cr,
+//     Ubar_c = rf (1 - 2y + 6y^2) exp(-2 pi i (th0 + a_c h (1 - y + 2y^2))),  y = a_c h^2,
+// V_r = (1 - 2y + 6y^2) exp(2 pi i b_r h (1 - y + 2y^2)) (y = b_r h^2), W_r = V_r l1(b_r) and
+// w the staged records {Re, Im, -Re} of u (k/2pi) exp(-2 pi i z'_ref): per pair two FMAs
+// for V_r + a_c W_r and four for S_c; per pillar and block one v_cos_f32 / v_sin_f32 pair
+// per row and per column.  RAG: a block cut by the plane's last column or row, whose
+// columns c >= nc and rows r >= nr (all-zero padding records) are skipped, at the cost of
+// uniform branches the full blocks do not take.
+//
+// The series factors are written with inline constants (+-1, +-2, +-4 and literals in
+// two-operand forms): a constant the three-operand FMA encoding cannot hold goes to a
+// scalar register, and an FMA with a scalar operand issues at ~4.5 cycles per wave64
+// instead of ~3.1.
+
+// (1 - 2y + 6y^2, 1 - y + 2y^2): amplitude and phase factors of F at y = D h^2.
+__device__ __forceinline__ void sep_factors(float y, float& amp, float& ph) {
+#pragma clang fp contract(off)  // keep y * 6 a multiply by a literal, not an FMA taking 6 from an SGPR
+    const float y6 = y * 6.0f;
+    amp = fmaf(y, y6 - 2.0f, 1.0f);
+    ph = fmaf(y, fmaf(2.0f, y, -1.0f), 1.0f);
+}
+
+template <int BX, int BY, bool RAG>
+__device__ __forceinline__ void adj_pairs_sep(double dX0, double dY0, double zref,
+                                              const BlockGeom<BX, BY>& G, const float* sRe,
+                                              const float* sIm, const float* sNRe, int k0,
+                                              int nc, int nr, float& gr, float& gi) {
+    float rf;
+    const double R0 = block_distance(fma(dX0, dX0, fma(dY0, dY0, G.ZC2)), rf);
+    const float th0 = static_cast<float>(__builtin_amdgcn_fract(R0 - zref));
+    const float h = 0.5f * rf, h2 = h * h;
+    float a[BX], b[BY];
+    block_offsets(G, static_cast<float>(dX0), static_cast<float>(dY0), a, b);
+    const float p1r = 8.0f * h2 * h2, p1i = -12.5663706143591729f * h2 * h;
+    float vr[BY], vi[BY], wr[BY], wi[BY];
+#pragma unroll
+    for (int r = 0; r < BY; ++r) {
+        if (RAG && r >= nr) continue;
+        const float y = b[r] * h2;
+        float g, pf;
+        sep_factors(y, g, pf);
+        const float ph = b[r] * h * pf;
+        const float cs = g * __builtin_amdgcn_cosf(ph), sn = g * __builtin_amdgcn_sinf(ph);
+        const float l1r = b[r] * p1r * fmaf(-4.0f, y, 1.0f);
+        const float l1i = b[r] * p1i * fmaf(-2.0f, y, 1.0f - y);  // (1 - 3y)
+        vr[r] = cs;
+        vi[r] = sn;
+        wr[r] = fmaf(cs, l1r, -sn * l1i);
+        wi[r] = fmaf(cs, l1i, sn * l1r);
+    }
+#pragma unroll
+    for (int c = 0; c < BX; ++c) {
+        if (RAG && c >= nc) continue;
+        float sr = 0.0f, si = 0.0f;
+#pragma unroll
+        for (int r = 0; r < BY; ++r) {
+            if (RAG && r >= nr) continue;
+            const int k = k0 + c * kStrip + r;
+            const float ur = sRe[k], ui = sIm[k], nur = sNRe[k];
+            const float yr = fmaf(a[c], wr[r], vr[r]);
+            const float yi = fmaf(a[c], wi[r], vi[r]);
+            sr = fmaf(yr, ur, sr);
+            sr = fmaf(yi, ui, sr);
+            si = fmaf(yr, ui, si);
+            si = fmaf(yi, nur, si);
+        }
+        const float y = a[c] * h2;
+        float g, pf;
+        sep_factors(y, g, pf);
+        const float ph = fmaf(a[c] * h, pf, th0);
+        g *= rf;
+        const float cu = g * __builtin_amdgcn_cosf(ph), su = g * __builtin_amdgcn_sinf(ph);
+        gr = fmaf(cu, sr, fmaf(su, si, gr));
+        gi = fmaf(cu, si, fmaf(-su, sr, gi));
+    }
+}
+
+// Columns [0, ncol) of one LDS tile in the separable form (BY = kStrip: one block row per
+// strip).
 template <int BX, int BY>
-__global__ __launch_bounds__(kThreads) void adj_fp32c(
+__device__ __forceinline__ void adj_tile_sep(const float* sRe, const float* sIm, const float* sNRe,
+                                             int ncol, const DevPlane& P, int j0, int cb,
+                                             double qx, double qy, float& ar, float& ai) {
+    static_assert(BY == kStrip, "separable adjoint blocks span a whole strip");
+    const double kt = P.kturn;
+    BlockGeom<BX, BY> G;
+    const double jc = j0 + (BY - 1) * 0.5;
+    G.FYc = kt * (P.y0 + jc * P.sy);
+    const double zc = kt * (P.z0 + jc * P.sz);
+    G.ZC2 = zc * zc;
+    block_offsets_setup(P, zc, G);
+    const double dY0 = fma(-kt, qy, G.FYc);
+    const int nr = min(BY, P.ny - j0);  // real rows of the strip (uniform)
+    for (int g = 0; g < ncol; g += BX) {
+        G.FXc = kt * (P.x0 + (cb + g + (BX - 1) * 0.5) * P.sx);
+        const double dX0 = fma(-kt, qx, G.FXc);
+        const int nc = min(BX, P.nx - (cb + g));  // real columns of the block (uniform)
+        if (nc == BX && nr == BY)
+            adj_pairs_sep<BX, BY, false>(dX0, dY0, P.zref, G, sRe, sIm, sNRe, g * kStrip, BX, BY,
+                                         ar, ai);
+        else
+            adj_pairs_sep<BX, BY, true>(dX0, dY0, P.zref, G, sRe, sIm, sNRe, g * kStrip, nc, nr,
+                                        ar, ai);
+    }
+}
+
+// Every adjoint instance is held to 128 VGPRs, 4 waves per SIMD: VALU issue slows with
+// fewer waves (FP32 FMAs measured 3.1 cycles per wave64 instruction per SIMD at 4 waves,
+// 3.5 at 2 and 3, 5.3 at 1).
+template <int BX, int BY>
+__global__ __launch_bounds__(kThreads, 4) void adj_fp32c(
     const double* __restrict__ px, const double* __restrict__ py, int npil,
     const DevPlane* __restrict__ planes, const AdjStrip* __restrict__ strips,
     const AdjSeg* __restrict__ segs, const int32_t* __restrict__ wg_seg,
@@ -1281,6 +1190,7 @@ __global__ __launch_bounds__(kThreads) void adj_fp32c(
                     DevPlane Q = P;
                     Q.ny = P.nquads * kStrip;
                     order = __builtin_amdgcn_readfirstlane(fp32c_series_order(Q, E.x, E.y));
+                    if (BX * BY == 64 && P.sep) order = kSep;
                     prev_plane = st.plane;
                 }
             }
@@ -1297,9 +1207,16 @@ __global__ __launch_bounds__(kThreads) void adj_fp32c(
                 }
                 __syncthreads();
                 float ar = 0.0f, ai = 0.0f;
-                if (order == kP1A1)
+                int ord = order;
+                if constexpr (BX * BY == 64) {
+                    if (order == kSep) {
+                        adj_tile_sep<BX, BY>(sU2, sUm, sChi, ncol, P, st.j0, cb, qx, qy, ar, ai);
+                        ord = -1;
+                    }
+                }
+                if (ord == kP1A1)
                     adj_tile<kP1A1, BX, BY>(sU2, sUm, sChi, ncol, P, st.j0, cb, qx, qy, ar, ai);
-                else
+                else if (ord >= 0)
                     adj_tile<kP2A2, BX, BY>(sU2, sUm, sChi, ncol, P, st.j0, cb, qx, qy, ar, ai);
                 const float yr = ar - cr;
                 const float tr = sr + yr;
@@ -1597,32 +1514,6 @@ hipError_t launch_fwd_reduce_fp32c(const DevPlane* planes, const FwdTile* tiles,
     fwd_reduce_fp32c<<<nred, kThreads, 0, s>>>(planes, tiles, reds, partial, slot_elems, field,
                                                intensity);
     return hipGetLastError();
-}
-
-hipError_t launch_forward_mfma(const Fp32cMfmaArgs& a, hipStream_t s) {
-    if (a.nwg <= 0) return hipSuccess;
-    fwd_mfma<<<a.nwg, kFwdThreads, 0, s>>>(a.pos, a.ap, a.npil, a.planes, a.blocks, a.segs,
-                                            a.wg_seg, a.partial);
-    return hipGetLastError();
-}
-
-hipError_t launch_fwd_reduce_mfma(const DevPlane* planes, const MfmaBlock* blocks, int nblocks,
-                                  const float2* partial, double2* field, double* intensity,
-                                  hipStream_t s) {
-    if (nblocks <= 0) return hipSuccess;
-    fwd_reduce_mfma<<<nblocks, kThreads, 0, s>>>(planes, blocks, partial, field, intensity);
-    return hipGetLastError();
-}
-
-// Work-groups of the matrix-core forward resident per CU (0 if the query fails).
-int mfma_resident_per_cu() {
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(fwd_mfma),
-                                                     kFwdThreads, 0) != hipSuccess) {
-        (void)hipGetLastError();
-        return 0;
-    }
-    return n;
 }
 
 hipError_t launch_adjoint_fp32c(int bx, int by, const Fp32cAdjArgs& a, hipStream_t s) {

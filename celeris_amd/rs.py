@@ -241,7 +241,7 @@ class Context:
 
     def set_block_shape(self, fwd=(0, 0), adj=(0, 0)) -> None:
         """FP32C pixels per forward lane and per adjoint block, (columns, rows); (0, 0)
-        restores the built-in 4 x 8."""
+        restores the built-in shape (forward 4 x 4, adjoint 8 x 8)."""
         self._call("crs_set_block_shape", _lib.load().crs_set_block_shape, int(fwd[0]),
                    int(fwd[1]), int(adj[0]), int(adj[1]))
 

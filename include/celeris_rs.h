@@ -361,9 +361,11 @@ crs_status crs_set_tuning(crs_context *ctx, int mode, int fwd_target_blocks,
 
 /* FP32C block shape (no reference counterpart): pixels per lane of the forward
  * (fwd_bx columns x fwd_by rows: 4 x 4, 4 x 8 or 8 x 4) and per pixel block of the
- * adjoint (adj_bx x adj_by, each 4 or 8); 0, 0 restores the built-in 4 x 8.  A plane
- * whose pair-error bound does not admit the shape falls back to 4 x 4, then to pair by
- * pair.  CRS_ERR_INVALID for other shapes.  Applies to the context's later calls. */
+ * adjoint (adj_bx x adj_by, each 4 or 8); 0, 0 restores the built-in forward 4 x 4 and
+ * adjoint 8 x 8.  Adjoint 8 x 8 blocks take the separable form (rank-2 column x row
+ * factors) on planes whose separable pair-error bound holds, else the series orders.  A
+ * plane whose pair-error bound does not admit the shape falls back to 4 x 4, then to pair
+ * by pair.  CRS_ERR_INVALID for other shapes.  Applies to the context's later calls. */
 crs_status crs_set_block_shape(crs_context *ctx, int fwd_bx, int fwd_by, int adj_bx,
                                int adj_by);
 
