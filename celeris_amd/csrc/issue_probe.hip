@@ -34,10 +34,12 @@ constexpr int kThreadsPerWg = 256;
         }                                                                           \
     } while (0)
 
-// MODE 0: 48 FP32 FMAs per iteration (16 chains of 3); 1: 16 v_sin_f32 (16 chains);
-// 2: 16 v_sin_f32 then 48 FP32 FMAs (grouped, as the pair loops group them); 3: 48 FP64
-// FMAs (8 chains of 6); 4: 48 FP32 adds; 5: 48 FP32 multiplies.  At most 64 VGPRs, so
-// that kW waves fit on a SIMD.
+// MODE 0: 48 FP32 FMAs per iteration (16 chains of 3), each taking the kernel argument s
+// (a scalar register) as an operand; 1: 16 v_sin_f32 (16 chains); 2: 16 v_sin_f32 then 48
+// FP32 FMAs (grouped, as the pair loops group them); 3: 48 FP64 FMAs (8 chains of 6);
+// 4: 48 FP32 adds; 5: 48 FP32 multiplies; 6: 48 FP32 FMAs on vector operands only (as in
+// the pair loops: acc += x y with x, y per lane).  At most 64 VGPRs, so that kW waves fit
+// on a SIMD.
 template <int MODE>
 __global__ __launch_bounds__(kThreadsPerWg, kW) void probe(long long* cycles, float* sink,
                                                            int iters, float s) {
@@ -93,6 +95,15 @@ __global__ __launch_bounds__(kThreadsPerWg, kW) void probe(long long* cycles, fl
                 acc[i] = acc[i] * x[i];
                 x[i] = x[i] * s;
                 acc[i] = acc[i] * x[i];
+            }
+        } else if constexpr (MODE == 6) {
+            // Every operand a vector register; x changes each iteration, so no product
+            // is loop-invariant.
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                acc[i] = fmaf(x[i], x[(i + 1) & 15], acc[i]);
+                acc[i] = fmaf(x[(i + 2) & 15], x[(i + 3) & 15], acc[i]);
+                x[i] = fmaf(x[i], x[(i + 5) & 15], x[(i + 7) & 15]);
             }
         } else {
 #pragma unroll
@@ -183,6 +194,11 @@ int main(int argc, char** argv) {
     const Result fma64 = run<3>(nwg, iters, 48, d_cyc, d_sink);
     const Result add32 = run<4>(nwg, iters, 48, d_cyc, d_sink);
     const Result mul32 = run<5>(nwg, iters, 48, d_cyc, d_sink);
+    // Vector-operand FMAs with 1, 2, 3, 4 and kW waves on every SIMD (one round each).
+    const int occ[5] = {1, 2, 3, 4, kW};
+    Result fmav[5];
+    for (int i = 0; i < 5; ++i)
+        fmav[i] = run<6>(simds * occ[i] * 64 / kThreadsPerWg, iters, 48, d_cyc, d_sink);
     CHECK(hipFree(d_cyc));
     CHECK(hipFree(d_sink));
     // mix: cycles per 64-instruction iteration against the sum of its parts
@@ -193,8 +209,14 @@ int main(int argc, char** argv) {
     };
     std::printf("{\"arch\": \"%s\", \"cus\": %d, \"launch_waves_per_simd\": %d, \"iterations\": %d, ",
                 prop.gcnArchName, prop.multiProcessorCount, kW, iters);
-    item("v_fma_f32", fma32);
+    item("v_fma_f32_sgpr_operand", fma32);
     std::printf(", ");
+    for (int i = 0; i < 5; ++i) {
+        char name[48];
+        std::snprintf(name, sizeof(name), "v_fma_f32_vgpr_%d_waves", occ[i]);
+        item(name, fmav[i]);
+        std::printf(", ");
+    }
     item("v_sin_f32", sin32);
     std::printf(", ");
     item("v_fma_f64", fma64);
