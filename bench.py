@@ -11,10 +11,10 @@ Prints one JSON line.  Throughput = pillar-pixel pairs / (forward + adjoint time
 in G pairs/s.  Roofline: 21 + 21 = 42 FLOP per pair (dx, dy, R^2, sqrt, 1/R, k R,
 sin, cos, complex multiply, 1/R scale, accumulate; same count transposed) against
 the MI355X FP32 vector peak of 157.3 TFLOP/s, using the two pair-sum kernels'
-durations measured with HIP events on their stream.  That peak counts packed FP32
-FMAs; with packed FP32 switched off in this build a wave64 FMA issues every 4 cycles
-per SIMD (celeris_amd/_lib/issue_probe), so 78.6 TFLOP/s is the FMA ceiling the
-kernels can reach, reported beside the peak.
+durations measured with HIP events on their stream.  That peak is one wave64 FMA per
+2 cycles per SIMD; without packed FP32 (switched off in this build) an FMA whose two
+multiplicands sit in different VGPR banks issues at 2.2 cycles per SIMD
+(celeris_amd/_lib/issue_probe), so the peak is within reach of unpacked code.
 """
 from __future__ import annotations
 
@@ -34,7 +34,6 @@ sys.path.insert(0, ROOT)
 
 FLOP_PER_PAIR = 42.0
 PEAK_FP32_TFLOPS = 157.3
-NONPACKED_FMA_TFLOPS = 78.6  # 1,024 SIMDs x 16 FMA lanes per cycle x 2 FLOP x 2.4 GHz
 PEAK_HBM_GBS = 8000.0
 
 
@@ -422,8 +421,6 @@ def main():
             "workgroups": {launches["fwd"][0]: launches["fwd"][1],
                            launches["adj"][0]: launches["adj"][1]},
             "flop_per_pair": FLOP_PER_PAIR,
-            "nonpacked_fma_ceiling": NONPACKED_FMA_TFLOPS,
-            "frac_of_nonpacked_ceiling": achieved / NONPACKED_FMA_TFLOPS,
         },
         "cpu_baseline": cpu,
         **extra,

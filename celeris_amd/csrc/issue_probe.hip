@@ -135,6 +135,180 @@ __global__ __launch_bounds__(kThreadsPerWg, kW) void probe(long long* cycles, fl
     }
 }
 
+// Operand placement, with fixed registers (inline asm, VALU only): 48 instructions per
+// iteration on 16 accumulators v40..v55 (each updated every 16th instruction); the first
+// multiplicands are four registers of one bank (register number mod 4) in v8..v23, the
+// second four of one bank in v24..v39.  BANKED 0: v_fmac_f32 with the multiplicands in
+// different banks; 1: in the same bank; 2: the first multiplicand an SGPR; 3: v_mul_f32
+// with both operands in one bank; 4: v_mov_b32_dpp quad_perm [1, 0, 3, 2]; 5: one
+// dependent chain (every FMA updates v40).  The registers stay below v64, so 8 waves fit
+// on a SIMD.  One round: 16 instructions, accumulator i (v40 + i) taking v(a + 4j) x
+// v(b + 4k).
+#define CRS_FMAC_DIFFER \
+    "v_fmac_f32 v40, v9, v26\n" \
+    "v_fmac_f32 v41, v13, v30\n" \
+    "v_fmac_f32 v42, v17, v34\n" \
+    "v_fmac_f32 v43, v21, v38\n" \
+    "v_fmac_f32 v44, v9, v30\n" \
+    "v_fmac_f32 v45, v13, v34\n" \
+    "v_fmac_f32 v46, v17, v38\n" \
+    "v_fmac_f32 v47, v21, v26\n" \
+    "v_fmac_f32 v48, v9, v34\n" \
+    "v_fmac_f32 v49, v13, v38\n" \
+    "v_fmac_f32 v50, v17, v26\n" \
+    "v_fmac_f32 v51, v21, v30\n" \
+    "v_fmac_f32 v52, v9, v38\n" \
+    "v_fmac_f32 v53, v13, v26\n" \
+    "v_fmac_f32 v54, v17, v30\n" \
+    "v_fmac_f32 v55, v21, v34\n"
+#define CRS_FMAC_SAME \
+    "v_fmac_f32 v40, v10, v26\n" \
+    "v_fmac_f32 v41, v14, v30\n" \
+    "v_fmac_f32 v42, v18, v34\n" \
+    "v_fmac_f32 v43, v22, v38\n" \
+    "v_fmac_f32 v44, v10, v30\n" \
+    "v_fmac_f32 v45, v14, v34\n" \
+    "v_fmac_f32 v46, v18, v38\n" \
+    "v_fmac_f32 v47, v22, v26\n" \
+    "v_fmac_f32 v48, v10, v34\n" \
+    "v_fmac_f32 v49, v14, v38\n" \
+    "v_fmac_f32 v50, v18, v26\n" \
+    "v_fmac_f32 v51, v22, v30\n" \
+    "v_fmac_f32 v52, v10, v38\n" \
+    "v_fmac_f32 v53, v14, v26\n" \
+    "v_fmac_f32 v54, v18, v30\n" \
+    "v_fmac_f32 v55, v22, v34\n"
+#define CRS_FMAC_SGPR \
+    "v_fmac_f32 v40, s8, v26\n" \
+    "v_fmac_f32 v41, s8, v30\n" \
+    "v_fmac_f32 v42, s8, v34\n" \
+    "v_fmac_f32 v43, s8, v38\n" \
+    "v_fmac_f32 v44, s8, v30\n" \
+    "v_fmac_f32 v45, s8, v34\n" \
+    "v_fmac_f32 v46, s8, v38\n" \
+    "v_fmac_f32 v47, s8, v26\n" \
+    "v_fmac_f32 v48, s8, v34\n" \
+    "v_fmac_f32 v49, s8, v38\n" \
+    "v_fmac_f32 v50, s8, v26\n" \
+    "v_fmac_f32 v51, s8, v30\n" \
+    "v_fmac_f32 v52, s8, v38\n" \
+    "v_fmac_f32 v53, s8, v26\n" \
+    "v_fmac_f32 v54, s8, v30\n" \
+    "v_fmac_f32 v55, s8, v34\n"
+#define CRS_MUL_SAME \
+    "v_mul_f32 v40, v10, v26\n" \
+    "v_mul_f32 v41, v14, v30\n" \
+    "v_mul_f32 v42, v18, v34\n" \
+    "v_mul_f32 v43, v22, v38\n" \
+    "v_mul_f32 v44, v10, v30\n" \
+    "v_mul_f32 v45, v14, v34\n" \
+    "v_mul_f32 v46, v18, v38\n" \
+    "v_mul_f32 v47, v22, v26\n" \
+    "v_mul_f32 v48, v10, v34\n" \
+    "v_mul_f32 v49, v14, v38\n" \
+    "v_mul_f32 v50, v18, v26\n" \
+    "v_mul_f32 v51, v22, v30\n" \
+    "v_mul_f32 v52, v10, v38\n" \
+    "v_mul_f32 v53, v14, v26\n" \
+    "v_mul_f32 v54, v18, v30\n" \
+    "v_mul_f32 v55, v22, v34\n"
+#define CRS_DPP \
+    "v_mov_b32_dpp v40, v9 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+    "v_mov_b32_dpp v41, v13 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+    "v_mov_b32_dpp v42, v17 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+    "v_mov_b32_dpp v43, v21 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+    "v_mov_b32_dpp v44, v9 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+    "v_mov_b32_dpp v45, v13 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+    "v_mov_b32_dpp v46, v17 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+    "v_mov_b32_dpp v47, v21 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+    "v_mov_b32_dpp v48, v9 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+    "v_mov_b32_dpp v49, v13 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+    "v_mov_b32_dpp v50, v17 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+    "v_mov_b32_dpp v51, v21 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+    "v_mov_b32_dpp v52, v9 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+    "v_mov_b32_dpp v53, v13 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+    "v_mov_b32_dpp v54, v17 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+    "v_mov_b32_dpp v55, v21 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+#define CRS_CHAIN \
+    "v_fmac_f32 v40, v9, v26\n" \
+    "v_fmac_f32 v40, v13, v30\n" \
+    "v_fmac_f32 v40, v17, v34\n" \
+    "v_fmac_f32 v40, v21, v38\n" \
+    "v_fmac_f32 v40, v9, v30\n" \
+    "v_fmac_f32 v40, v13, v34\n" \
+    "v_fmac_f32 v40, v17, v38\n" \
+    "v_fmac_f32 v40, v21, v26\n" \
+    "v_fmac_f32 v40, v9, v34\n" \
+    "v_fmac_f32 v40, v13, v38\n" \
+    "v_fmac_f32 v40, v17, v26\n" \
+    "v_fmac_f32 v40, v21, v30\n" \
+    "v_fmac_f32 v40, v9, v38\n" \
+    "v_fmac_f32 v40, v13, v26\n" \
+    "v_fmac_f32 v40, v17, v30\n" \
+    "v_fmac_f32 v40, v21, v34\n"
+#define CRS_CLOBBER                                                                         \
+    "v8", "v9", "v10", "v11", "v12", "v13", "v14", "v15", "v16", "v17", "v18", "v19", "v20", \
+        "v21", "v22", "v23", "v24", "v25", "v26", "v27", "v28", "v29", "v30", "v31", "v32",  \
+        "v33", "v34", "v35", "v36", "v37", "v38", "v39", "v40", "v41", "v42", "v43", "v44",  \
+        "v45", "v46", "v47", "v48", "v49", "v50", "v51", "v52", "v53", "v54", "v55", "s8"
+
+template <int BANKED>
+__global__ __launch_bounds__(kThreadsPerWg, kW) void probe_banked(long long* cycles, float* sink,
+                                                                  int iters, float s) {
+    unsigned hw_id, xcc_id;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_HW_ID)" : "=s"(hw_id));
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc_id));
+    const float seed = threadIdx.x * 1e-3f + s;
+    // Every register the blocks read gets a value first.
+    asm volatile(
+        "v_mov_b32 v8, %0\n v_mov_b32 v9, %0\n v_mov_b32 v10, %0\n v_mov_b32 v11, %0\n"
+        "v_mov_b32 v12, %0\n v_mov_b32 v13, %0\n v_mov_b32 v14, %0\n v_mov_b32 v15, %0\n"
+        "v_mov_b32 v16, %0\n v_mov_b32 v17, %0\n v_mov_b32 v18, %0\n v_mov_b32 v19, %0\n"
+        "v_mov_b32 v20, %0\n v_mov_b32 v21, %0\n v_mov_b32 v22, %0\n v_mov_b32 v23, %0\n"
+        "v_mov_b32 v24, %0\n v_mov_b32 v25, %0\n v_mov_b32 v26, %0\n v_mov_b32 v27, %0\n"
+        "v_mov_b32 v28, %0\n v_mov_b32 v29, %0\n v_mov_b32 v30, %0\n v_mov_b32 v31, %0\n"
+        "v_mov_b32 v32, %0\n v_mov_b32 v33, %0\n v_mov_b32 v34, %0\n v_mov_b32 v35, %0\n"
+        "v_mov_b32 v36, %0\n v_mov_b32 v37, %0\n v_mov_b32 v38, %0\n v_mov_b32 v39, %0\n"
+        "v_mov_b32 v40, 0\n v_mov_b32 v41, 0\n v_mov_b32 v42, 0\n v_mov_b32 v43, 0\n"
+        "v_mov_b32 v44, 0\n v_mov_b32 v45, 0\n v_mov_b32 v46, 0\n v_mov_b32 v47, 0\n"
+        "v_mov_b32 v48, 0\n v_mov_b32 v49, 0\n v_mov_b32 v50, 0\n v_mov_b32 v51, 0\n"
+        "v_mov_b32 v52, 0\n v_mov_b32 v53, 0\n v_mov_b32 v54, 0\n v_mov_b32 v55, 0\n"
+        "s_mov_b32 s8, 0x3f7fbe77\n" ::"v"(seed)
+        : CRS_CLOBBER);
+    const long long t0 = clock64();
+    for (int it = 0; it < iters; ++it) {
+        if constexpr (BANKED == 0)  // banks: multiplicands 1 and 2
+            asm volatile(CRS_FMAC_DIFFER CRS_FMAC_DIFFER
+                             CRS_FMAC_DIFFER ::: CRS_CLOBBER);
+        else if constexpr (BANKED == 1)  // both multiplicands in bank 2
+            asm volatile(CRS_FMAC_SAME CRS_FMAC_SAME
+                             CRS_FMAC_SAME ::: CRS_CLOBBER);
+        else if constexpr (BANKED == 2)
+            asm volatile(CRS_FMAC_SGPR CRS_FMAC_SGPR
+                             CRS_FMAC_SGPR ::: CRS_CLOBBER);
+        else if constexpr (BANKED == 3)
+            asm volatile(CRS_MUL_SAME CRS_MUL_SAME
+                             CRS_MUL_SAME ::: CRS_CLOBBER);
+        else if constexpr (BANKED == 4)
+            asm volatile(CRS_DPP CRS_DPP
+                             CRS_DPP ::: CRS_CLOBBER);
+        else
+            asm volatile(CRS_CHAIN CRS_CHAIN
+                             CRS_CHAIN ::: CRS_CLOBBER);
+    }
+    const long long t1 = clock64();
+    float r;
+    asm volatile("v_add_f32 %0, v40, v41" : "=v"(r) : : CRS_CLOBBER);
+    const int gid = blockIdx.x * kThreadsPerWg + threadIdx.x;
+    sink[gid] = r;
+    if ((threadIdx.x & 63) == 0) {
+        const long long key = (static_cast<long long>(xcc_id & 0xf) << 12) | ((hw_id >> 4) & 0xfff);
+        cycles[3 * (gid >> 6) + 0] = t0;
+        cycles[3 * (gid >> 6) + 1] = t1;
+        cycles[3 * (gid >> 6) + 2] = key;
+    }
+}
+
 struct Result {
     double med, lo, hi;  // over the fullest SIMDs
     int waves, simds;    // waves on those SIMDs, and how many SIMDs held that many
@@ -143,9 +317,16 @@ struct Result {
 template <int MODE>
 Result run(int nwg, int iters, int ninstr, long long* d_rec, float* d_sink) {
     const int nwaves = nwg * kThreadsPerWg / 64;
-    probe<MODE><<<nwg, kThreadsPerWg>>>(d_rec, d_sink, 8, 0.999f);  // warm-up
-    CHECK(hipGetLastError());
-    probe<MODE><<<nwg, kThreadsPerWg>>>(d_rec, d_sink, iters, 0.999f);
+    // MODE >= 100: probe_banked<MODE - 100>.
+    if constexpr (MODE >= 100) {
+        probe_banked<MODE - 100><<<nwg, kThreadsPerWg>>>(d_rec, d_sink, 8, 0.5f);  // warm-up
+        CHECK(hipGetLastError());
+        probe_banked<MODE - 100><<<nwg, kThreadsPerWg>>>(d_rec, d_sink, iters, 0.5f);
+    } else {
+        probe<MODE><<<nwg, kThreadsPerWg>>>(d_rec, d_sink, 8, 0.999f);  // warm-up
+        CHECK(hipGetLastError());
+        probe<MODE><<<nwg, kThreadsPerWg>>>(d_rec, d_sink, iters, 0.999f);
+    }
     CHECK(hipGetLastError());
     CHECK(hipDeviceSynchronize());
     std::vector<long long> r(3 * static_cast<size_t>(nwaves));
@@ -199,6 +380,21 @@ int main(int argc, char** argv) {
     Result fmav[5];
     for (int i = 0; i < 5; ++i)
         fmav[i] = run<6>(simds * occ[i] * 64 / kThreadsPerWg, iters, 48, d_cyc, d_sink);
+    // Operand placement (probe_banked), at 4 waves per SIMD (the pair kernels' occupancy)
+    // and kW.
+    const char* bnames[6] = {"fmac_banks_differ", "fmac_same_bank", "fmac_sgpr",
+                             "mul_same_bank", "dpp_mov", "fmac_one_chain"};
+    Result banked[6][2];
+    for (int o = 0; o < 2; ++o) {
+        const int w = o == 0 ? 4 : kW;
+        const int g = simds * w * 64 / kThreadsPerWg;
+        banked[0][o] = run<100>(g, iters, 48, d_cyc, d_sink);
+        banked[1][o] = run<101>(g, iters, 48, d_cyc, d_sink);
+        banked[2][o] = run<102>(g, iters, 48, d_cyc, d_sink);
+        banked[3][o] = run<103>(g, iters, 48, d_cyc, d_sink);
+        banked[4][o] = run<104>(g, iters, 48, d_cyc, d_sink);
+        banked[5][o] = run<105>(g, iters, 48, d_cyc, d_sink);
+    }
     CHECK(hipFree(d_cyc));
     CHECK(hipFree(d_sink));
     // mix: cycles per 64-instruction iteration against the sum of its parts
@@ -226,6 +422,14 @@ int main(int argc, char** argv) {
     item("v_mul_f32", mul32);
     std::printf(", ");
     item("mix_16sin_48fma", mix);
-    std::printf(", \"mix_iteration_cycles\": %.1f, \"sum_of_parts_cycles\": %.1f}\n", mix_iter, parts);
+    std::printf(", \"mix_iteration_cycles\": %.1f, \"sum_of_parts_cycles\": %.1f", mix_iter, parts);
+    for (int b = 0; b < 6; ++b)
+        for (int o = 0; o < 2; ++o) {
+            char name[64];
+            std::snprintf(name, sizeof(name), "%s_%d_waves", bnames[b], banked[b][o].waves);
+            std::printf(", ");
+            item(name, banked[b][o]);
+        }
+    std::printf("}\n");
     return 0;
 }
