@@ -309,6 +309,55 @@ __global__ __launch_bounds__(kThreadsPerWg, kW) void probe_banked(long long* cyc
     }
 }
 
+// Matrix cores beside the VALU.  MX 0: v_mfma_f32_32x32x2_f32, MX 1:
+// v_mfma_f32_16x16x1_4b_f32 (accumulators in AGPRs, one dependent chain), NM of them per
+// iteration, then NV x 16 vector-operand FP32 FMAs (16 chains); with both, each wave
+// alternates the two, so the SIMD overlaps them only if one wave's MFMAs can run beside
+// another's VALU work.
+typedef float crs_f16v __attribute__((ext_vector_type(16)));
+template <int MX, int NM, int NV>
+__global__ __launch_bounds__(kThreadsPerWg, 4) void probe_mfma(long long* cycles, float* sink,
+                                                              int iters, float s) {
+    unsigned hw_id, xcc_id;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_HW_ID)" : "=s"(hw_id));
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc_id));
+    const int l = threadIdx.x;
+    crs_f16v acc = {};
+    const float a = l * 1e-3f + s, b = 0.5f + l * 1e-4f;
+    float x[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) x[i] = l * 1e-3f + i;
+    const long long t0 = clock64();
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int m = 0; m < NM; ++m) {
+            if constexpr (MX == 0)
+                asm volatile("v_mfma_f32_32x32x2_f32 %0, %1, %2, %0" : "+a"(acc) : "v"(a), "v"(b));
+            else
+                asm volatile("v_mfma_f32_16x16x1_4b_f32 %0, %1, %2, %0" : "+a"(acc) : "v"(a), "v"(b));
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) x[i] = fmaf(x[i], x[(i + 5) & 15], x[(i + 2) & 15]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    const long long t1 = clock64();
+    float r = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) r += acc[i] + x[i];
+    const int gid = blockIdx.x * kThreadsPerWg + threadIdx.x;
+    sink[gid] = r;
+    if ((threadIdx.x & 63) == 0) {
+        const long long key = (static_cast<long long>(xcc_id & 0xf) << 12) | ((hw_id >> 4) & 0xfff);
+        cycles[3 * (gid >> 6) + 0] = t0;
+        cycles[3 * (gid >> 6) + 1] = t1;
+        cycles[3 * (gid >> 6) + 2] = key;
+    }
+}
+
 struct Result {
     double med, lo, hi;  // over the fullest SIMDs
     int waves, simds;    // waves on those SIMDs, and how many SIMDs held that many
@@ -317,8 +366,14 @@ struct Result {
 template <int MODE>
 Result run(int nwg, int iters, int ninstr, long long* d_rec, float* d_sink) {
     const int nwaves = nwg * kThreadsPerWg / 64;
-    // MODE >= 100: probe_banked<MODE - 100>.
-    if constexpr (MODE >= 100) {
+    // MODE >= 200: probe_mfma<MX, NM, NV> with MODE = 200 + 100 MX + 10 (NM / 16) + NV / 8
+    // (NM 0 or 16, NV 0 or 8); MODE >= 100: probe_banked<MODE - 100>.
+    if constexpr (MODE >= 200) {
+        constexpr int MX = (MODE - 200) / 100, NM = ((MODE / 10) % 10) * 16, NV = (MODE % 10) * 8;
+        probe_mfma<MX, NM, NV><<<nwg, kThreadsPerWg>>>(d_rec, d_sink, 8, 0.5f);  // warm-up
+        CHECK(hipGetLastError());
+        probe_mfma<MX, NM, NV><<<nwg, kThreadsPerWg>>>(d_rec, d_sink, iters, 0.5f);
+    } else if constexpr (MODE >= 100) {
         probe_banked<MODE - 100><<<nwg, kThreadsPerWg>>>(d_rec, d_sink, 8, 0.5f);  // warm-up
         CHECK(hipGetLastError());
         probe_banked<MODE - 100><<<nwg, kThreadsPerWg>>>(d_rec, d_sink, iters, 0.5f);
@@ -395,6 +450,13 @@ int main(int argc, char** argv) {
         banked[4][o] = run<104>(g, iters, 48, d_cyc, d_sink);
         banked[5][o] = run<105>(g, iters, 48, d_cyc, d_sink);
     }
+    // Matrix cores beside the VALU, at 4 waves per SIMD: cycles per wave-iteration per SIMD
+    // (16 MFMAs; 128 FMAs; both).
+    const int g4 = simds * 4 * 64 / kThreadsPerWg, mi = iters / 20 > 0 ? iters / 20 : 1;
+    const Result mx32 = run<210>(g4, mi, 1, d_cyc, d_sink), vx32 = run<201>(g4, mi, 1, d_cyc, d_sink),
+                 bx32 = run<211>(g4, mi, 1, d_cyc, d_sink);
+    const Result mx16 = run<310>(g4, mi, 1, d_cyc, d_sink), vx16 = run<301>(g4, mi, 1, d_cyc, d_sink),
+                 bx16 = run<311>(g4, mi, 1, d_cyc, d_sink);
     CHECK(hipFree(d_cyc));
     CHECK(hipFree(d_sink));
     // mix: cycles per 64-instruction iteration against the sum of its parts
@@ -430,6 +492,10 @@ int main(int argc, char** argv) {
             std::printf(", ");
             item(name, banked[b][o]);
         }
+    std::printf(", \"mfma_32x32x2_f32\": {\"cycles_each\": %.1f, \"with_128_fma\": %.1f, \"sum_of_parts\": %.1f}",
+                mx32.med / 16, bx32.med, mx32.med + vx32.med);
+    std::printf(", \"mfma_16x16x1_4b_f32\": {\"cycles_each\": %.1f, \"with_128_fma\": %.1f, \"sum_of_parts\": %.1f}",
+                mx16.med / 16, bx16.med, mx16.med + vx16.med);
     std::printf("}\n");
     return 0;
 }
