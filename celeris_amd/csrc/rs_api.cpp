@@ -31,17 +31,17 @@
 namespace crs {
 hipError_t launch_stage_pillars(int mode, const double* px, const double* py, const double* t,
                                 int npil, int nsets, void* out, hipStream_t s);
-hipError_t launch_forward_fp32c(int bx, int by, const Fp32cFwdArgs& a, hipStream_t s);
+hipError_t launch_forward_fp32c(int bx, int by, bool rsi, const Fp32cFwdArgs& a, hipStream_t s);
 hipError_t launch_fwd_reduce_fp32c(const DevPlane* planes, const FwdTile* tiles,
                                    const FwdRed* reds, int nred, const float2* partial,
                                    int64_t slot_elems, double2* field, double* intensity,
                                    hipStream_t s);
-hipError_t launch_adjoint_fp32c(int bx, int by, const Fp32cAdjArgs& a, hipStream_t s);
+hipError_t launch_adjoint_fp32c(int bx, int by, bool rsi, const Fp32cAdjArgs& a, hipStream_t s);
 hipError_t launch_adj_reduce_fp32c(const float2* partial, const int32_t* slot_off,
                                    const int32_t* slots, int npil, int ngroups, int64_t n,
                                    double2* g, hipStream_t s);
-int fp32c_resident_per_cu(bool adjoint, int bx, int by);
-const char* fp32c_kernel_name(bool adjoint, int bx, int by);
+int fp32c_resident_per_cu(bool adjoint, int bx, int by, bool rsi);
+const char* fp32c_kernel_name(bool adjoint, int bx, int by, bool rsi);
 hipError_t launch_forward(int mode, const void* pil, int npil, const DevPlane* planes,
                           const FwdBlock* blocks, int nblocks, void* partial, hipStream_t s);
 hipError_t launch_fwd_reduce(const DevPlane* planes, const PixBlock* pblocks, int nblocks,
@@ -116,6 +116,7 @@ struct TableBuilder {
 // One FP32C pair-kernel launch of a plan: block shape and its work-group table.
 struct Fp32cLaunch {
     int bx = 1, by = 1;
+    bool rsi = false;  // RS-I planes in blocks (their own kernel)
     int nwg = 0;
     size_t o_wg = 0;  // int32 wg_seg[nwg + 1]
 };
@@ -434,7 +435,7 @@ bool fp32c_block_admitted(const DevPlane& d, int bx, int by, int ny_rows) {
     q.ny = ny_rows;
     const SeriesGeom g = series_geom(q, bx, by);
     if (!std::isfinite(g.a) || !std::isfinite(g.c) || !(g.z > 0.0)) return false;
-    return fp32c_pair_error(g, 1.0, kP2A2) <= kPairBudget;
+    return fp32c_pair_error(g, 1.0, kP2A2, d.kernel == CRS_KERNEL_RSI) <= kPairBudget;
 }
 
 // FP32C relative cost of one pillar against one lane of a thread tile of nel pixels
@@ -442,16 +443,17 @@ bool fp32c_block_admitted(const DevPlane& d, int bx, int by, int ny_rows) {
 double fwd_tile_rate(int nel) { return nel == 1 ? 48.0 : 7.0 * nel + 40.0; }
 
 // Work-groups of an FP32C launch resident on the whole chip at once.
-int64_t fp32c_slots(const crs_context* ctx, bool adjoint, int bx, int by) {
-    int per_cu = fp32c_resident_per_cu(adjoint, bx, by);
+int64_t fp32c_slots(const crs_context* ctx, bool adjoint, int bx, int by, bool rsi = false) {
+    int per_cu = fp32c_resident_per_cu(adjoint, bx, by, rsi);
     if (per_cu <= 0) per_cu = 2;
     return static_cast<int64_t>(per_cu) * std::max(1, ctx->cus);
 }
 
 // Work-groups of an FP32C launch: the tuned target, else `rounds` x the resident slots.
-int64_t fp32c_target(const crs_context* ctx, int tuned, bool adjoint, int bx, int by, int rounds) {
+int64_t fp32c_target(const crs_context* ctx, int tuned, bool adjoint, int bx, int by, int rounds,
+                     bool rsi = false) {
     if (tuned > 0) return tuned;
-    return fp32c_slots(ctx, adjoint, bx, by) * rounds;
+    return fp32c_slots(ctx, adjoint, bx, by, rsi) * rounds;
 }
 
 // Bytes of staged pillar data: FP32C keeps FP64 positions once plus {|t|, arg t} per
@@ -736,10 +738,14 @@ crs_status plan_forward_fp32c(const crs_context* ctx, int npil, std::vector<DevP
     const int nplanes = static_cast<int>(dp.size());
     int want_bx = ctx->fwd_shape[0] ? ctx->fwd_shape[0] : kDefFwdBX;
     int want_by = ctx->fwd_shape[1] ? ctx->fwd_shape[1] : kDefFwdBY;
-    const int shapes[3][2] = {{want_bx, want_by}, {4, 4}, {1, 1}};
-    std::vector<int> pick(nplanes, 2);
+    // Launches: the tuned shape, 4 x 4, RS-I planes in 4 x 4 blocks, exact planes.
+    const int shapes[4][2] = {{want_bx, want_by}, {4, 4}, {4, 4}, {1, 1}};
+    std::vector<int> pick(nplanes, 3);
     for (int p = 0; p < nplanes; ++p) {
-        if (dp[p].kernel == CRS_KERNEL_RSI) continue;  // RS-I: every pair exact
+        if (dp[p].kernel == CRS_KERNEL_RSI) {
+            if (fp32c_block_admitted(dp[p], 4, 4, dp[p].ny)) pick[p] = 2;
+            continue;
+        }
         if (fp32c_block_admitted(dp[p], want_bx, want_by, dp[p].ny)) pick[p] = 0;
         else if (fp32c_block_admitted(dp[p], 4, 4, dp[p].ny)) pick[p] = 1;
     }
@@ -749,8 +755,9 @@ crs_status plan_forward_fp32c(const crs_context* ctx, int npil, std::vector<DevP
     std::vector<FwdSeg> segs;
     std::vector<int32_t> wg_all;
     int64_t slot_elems = 64;
-    for (int L = 0; L < 3; ++L) {
+    for (int L = 0; L < 4; ++L) {
         const int kbx = shapes[L][0], kby = shapes[L][1];
+        const bool rsi = L == 2;
         std::vector<int64_t> len;
         std::vector<double> rate;
         const size_t tile0 = tiles.size();
@@ -782,12 +789,12 @@ crs_status plan_forward_fp32c(const crs_context* ctx, int npil, std::vector<DevP
             pairs += len[r] * 64 * tiles[tile0 + r].nel;
         }
         const int64_t nwg = std::min<int64_t>(
-            fp32c_target(ctx, ctx->fwd_target[CRS_MODE_FP32C], false, kbx, kby, kFwdRounds),
+            fp32c_target(ctx, ctx->fwd_target[CRS_MODE_FP32C], false, kbx, kby, kFwdRounds, rsi),
             std::max<int64_t>(1, std::min(units, pairs / kMinPiecePairs)));
         std::vector<int32_t> wg_start;
         const int64_t seg_base = static_cast<int64_t>(segs.size());
         int last_tile = -1;
-        streamk_cut(len, rate, nwg, fp32c_slots(ctx, false, kbx, kby), 16,
+        streamk_cut(len, rate, nwg, fp32c_slots(ctx, false, kbx, kby, rsi), 16,
                     [&](int r, int64_t u0, int64_t u1) {
                         const int t = static_cast<int>(tile0) + r;
                         if (t != last_tile) {
@@ -801,6 +808,7 @@ crs_status plan_forward_fp32c(const crs_context* ctx, int npil, std::vector<DevP
         Fp32cLaunch La;
         La.bx = kbx;
         La.by = kby;
+        La.rsi = rsi;
         La.nwg = static_cast<int>(wg_start.size()) - 1;
         for (int32_t& w : wg_start) w += static_cast<int32_t>(seg_base);
         La.o_wg = wg_all.size();
@@ -941,7 +949,7 @@ crs_status enqueue_forward(const FwdPlan& plan, const Scratch& sc, const double*
         for (const Fp32cLaunch& La : plan.launches) {
             a.wg_seg = reinterpret_cast<const int32_t*>(sc.tables + La.o_wg);
             a.nwg = La.nwg;
-            if ((st = launch_status(launch_forward_fp32c(La.bx, La.by, a, stream), "forward")) != CRS_OK)
+            if ((st = launch_status(launch_forward_fp32c(La.bx, La.by, La.rsi, a, stream), "forward")) != CRS_OK)
                 return st;
         }
         if (ev1) hipEventRecord(ev1, stream);
@@ -981,11 +989,15 @@ crs_status plan_adjoint_fp32c(const crs_context* ctx, int npil, int nsets,
     if (srec > (int64_t(1) << 31)) { set_error("too many pixels"); return CRS_ERR_INVALID; }
     const int want_bx = ctx->adj_shape[0] ? ctx->adj_shape[0] : kDefAdjBX;
     const int want_by = ctx->adj_shape[1] ? ctx->adj_shape[1] : kDefAdjBY;
-    const int shapes[3][2] = {{want_bx, want_by}, {4, 4}, {1, 1}};
-    std::vector<int> pick(nplanes, 2);
+    // Launches: the tuned shape, 4 x 4, RS-I planes in 4 x 4 blocks, exact planes.
+    const int shapes[4][2] = {{want_bx, want_by}, {4, 4}, {4, 4}, {1, 1}};
+    std::vector<int> pick(nplanes, 3);
     for (int p = 0; p < nplanes; ++p) {
         const int rows = dp[p].nquads * kStrip;
-        if (dp[p].kernel == CRS_KERNEL_RSI) continue;  // RS-I: every pair exact
+        if (dp[p].kernel == CRS_KERNEL_RSI) {
+            if (fp32c_block_admitted(dp[p], 4, 4, rows)) pick[p] = 2;
+            continue;
+        }
         DevPlane q = dp[p];
         q.ny = rows;
         // 8 x 8 blocks take the separable form where its bound holds at every obliquity
@@ -1016,8 +1028,9 @@ crs_status plan_adjoint_fp32c(const crs_context* ctx, int npil, int nsets,
     std::vector<int32_t> wg_all;
     // Slots of each (set, group), in launch order then segment order.
     std::vector<std::vector<int32_t>> gslots(static_cast<size_t>(nsets) * ngroups);
-    for (int L = 0; L < 3; ++L) {
+    for (int L = 0; L < 4; ++L) {
         const int bx = shapes[L][0], by = shapes[L][1];
+        const bool rsi = L == 2;
         // Per set: its strips (planes in order, strips in order) and their block columns.
         std::vector<int64_t> set_u0(nsets), set_nb(nsets, 0);
         std::vector<std::vector<int64_t>> prefix(nsets);  // block-column prefix per set's strips
@@ -1054,11 +1067,11 @@ crs_status plan_adjoint_fp32c(const crs_context* ctx, int npil, int nsets,
         for (int64_t l : len) units += l;
         const int64_t pairs = units * bx * kStrip * kThreads;
         const int64_t nwg = std::min<int64_t>(
-            fp32c_target(ctx, ctx->adj_target[CRS_MODE_FP32C], true, bx, by, kAdjRounds),
+            fp32c_target(ctx, ctx->adj_target[CRS_MODE_FP32C], true, bx, by, kAdjRounds, rsi),
             std::max<int64_t>(1, std::min(units, pairs / kMinPiecePairs)));
         std::vector<int32_t> wg_start;
         const int64_t seg_base = static_cast<int64_t>(segs.size());
-        streamk_cut(len, rate, nwg, fp32c_slots(ctx, true, bx, by), 1,
+        streamk_cut(len, rate, nwg, fp32c_slots(ctx, true, bx, by, rsi), 1,
                     [&](int r, int64_t b0, int64_t b1) {
                         const int s = run_sg[r].first, g = run_sg[r].second;
                         const std::vector<int64_t>& pre = prefix[s];
@@ -1083,6 +1096,7 @@ crs_status plan_adjoint_fp32c(const crs_context* ctx, int npil, int nsets,
         Fp32cLaunch La;
         La.bx = bx;
         La.by = by;
+        La.rsi = rsi;
         La.nwg = static_cast<int>(wg_start.size()) - 1;
         for (int32_t& w : wg_start) w += static_cast<int32_t>(seg_base);
         La.o_wg = wg_all.size();
@@ -1235,7 +1249,7 @@ crs_status enqueue_adjoint(const AdjPlan& plan, const Scratch& sc, const double*
         for (const Fp32cLaunch& La : plan.launches) {
             a.wg_seg = reinterpret_cast<const int32_t*>(sc.tables + La.o_wg);
             a.nwg = La.nwg;
-            if ((st = launch_status(launch_adjoint_fp32c(La.bx, La.by, a, stream), "adjoint")) != CRS_OK)
+            if ((st = launch_status(launch_adjoint_fp32c(La.bx, La.by, La.rsi, a, stream), "adjoint")) != CRS_OK)
                 return st;
         }
         if (ev1) hipEventRecord(ev1, stream);
@@ -1271,7 +1285,8 @@ void note_launch(crs_context* ctx, const Plan& plan, bool adjoint) {
     const int a = adjoint ? 1 : 0;
     if (plan.mode == CRS_MODE_FP32C) {
         if (plan.launches.empty()) return;
-        ctx->last_kernel[a] = fp32c_kernel_name(adjoint, plan.launches[0].bx, plan.launches[0].by);
+        ctx->last_kernel[a] = fp32c_kernel_name(adjoint, plan.launches[0].bx, plan.launches[0].by,
+                                                plan.launches[0].rsi);
         ctx->last_groups[a] = plan.launches[0].nwg;
     } else {
         ctx->last_kernel[a] = plain_kernel(plan.mode, adjoint);

@@ -250,6 +250,18 @@ struct Fp32cAdjArgs {
 // A series order is admitted where eps <= kPairBudget.  The float accumulation of the
 // admitted terms (at most kWaveFold per float sum, then FP64) has its own error, ~1e-6
 // of sum |t|/R at worst for a coherent pixel, so the total stays below 1e-5 of it.
+//
+// RS-I planes (G = exp(ikR)/R (z/R)(1/R - ik)): in turns the pair is
+//     t (k/2pi)^2 z' exp(2 pi i R') (1/R' - 2 pi i) / R'^2
+//   = c1 2 pi zk rf (1 + 4y)^-1 sqrt(1 + e'^2) exp(2 pi i (R' - 1/4 + atan(e') / 2 pi)),
+// e = 1/(2 pi R0') = h / pi, e' = e (1 + 4y)^-1/2, zk = (k/2pi) z' of the pixel's row;
+// so the same pair loop serves with th0 - 1/4 + e / 2 pi, the phase's D-linear
+// coefficient h (1 - h^2/pi^2) (the -2 e y of atan(e')), the amplitude series of
+// (1 + 4y)^-1 (1 - 4y, then + 16y^2) times (1 + e^2/2) and the row's z'_r / z'_c.  The
+// bound's amplitude terms become 16 y^2 <= B^2 / R0^2 and 64 y^3 <= B^3 / R0^3; the
+// dropped e terms add e^3/3 + 6 e y^2 radians of phase and e^4/8 + 2.5 e^2 |y| of
+// amplitude (e <= 1/(2 pi z')); the extra products double the amplitude rounding, and
+// th0's shift adds 2^-24 of phase rounding.
 constexpr double kPairBudget = 8e-6;
 constexpr double kTwoPiD = 6.28318530717958647693;
 
@@ -286,8 +298,10 @@ __host__ __device__ inline double series_peak(double a, double c, double sn, int
     return pow(B, k) * f;
 }
 
-// eps (fraction of |t|/R) of series order `order` for pairs of obliquity <= sn.
-__host__ __device__ inline double fp32c_pair_error(const SeriesGeom& g, double sn, int order) {
+// eps (fraction of the pair's magnitude) of series order `order` for pairs of obliquity
+// <= sn; rsi: the plane's pairs take the RS-I kernel.
+__host__ __device__ inline double fp32c_pair_error(const SeriesGeom& g, double sn, int order,
+                                                   bool rsi = false) {
     if (!(g.z > 0.0)) return INFINITY;
     const double z2 = g.z * g.z, z3 = z2 * g.z;
     double phase, amp;
@@ -296,13 +310,22 @@ __host__ __device__ inline double fp32c_pair_error(const SeriesGeom& g, double s
     } else {
         phase = series_peak(g.a, g.c, sn, 3, 2) / (16.0 * z2);
     }
+    const double amp_c1 = rsi ? 1.0 : 3.0 / 8.0, amp_c2 = rsi ? 1.0 : 5.0 / 16.0;
     if (order == kP1A1) {
-        amp = 3.0 * series_peak(g.a, g.c, sn, 2, 2) / (8.0 * z2);
+        amp = amp_c1 * series_peak(g.a, g.c, sn, 2, 2) / z2;
     } else {
-        amp = 5.0 * series_peak(g.a, g.c, sn, 3, 3) / (16.0 * z3);
+        amp = amp_c2 * series_peak(g.a, g.c, sn, 3, 3) / z3;
     }
-    const double round_phase = 5.9604644775390625e-8 * (2.0 + 6.0 * g.e);
-    const double round_amp = 4.76837158203125e-7;
+    double round_phase = 5.9604644775390625e-8 * (2.0 + 6.0 * g.e);
+    double round_amp = 4.76837158203125e-7;
+    if (rsi) {
+        const double e = 1.0 / (kTwoPiD * g.z);             // >= 1 / (2 pi R0')
+        const double ymax = (g.a * sn + g.c) / (4.0 * g.z);  // >= |y|
+        phase += (e * e * e / 3.0 + 6.0 * e * ymax * ymax) * (1.0 + 4.0 * ymax) / kTwoPiD;
+        amp += e * e * e * e / 8.0 + 2.5 * e * e * ymax;
+        round_phase += 5.9604644775390625e-8;
+        round_amp *= 2.0;
+    }
     const double trig = 2.5e-7;
     return kTwoPiD * (phase + round_phase) + amp + round_amp + trig;
 }
@@ -323,12 +346,13 @@ __host__ __device__ inline double plane_obliquity(const DevPlane& P, double ex, 
 // meets the pair budget (the error grows with sn), or -1 if it never does.
 inline double fp32c_order_obliquity(const DevPlane& P, int bx, int by, int order) {
     const SeriesGeom g = series_geom(P, bx, by);
-    if (!(fp32c_pair_error(g, 0.0, order) <= kPairBudget)) return -1.0;
-    if (fp32c_pair_error(g, 1.0, order) <= kPairBudget) return 1.0;
+    const bool rsi = P.kernel == kKernelRsi;
+    if (!(fp32c_pair_error(g, 0.0, order, rsi) <= kPairBudget)) return -1.0;
+    if (fp32c_pair_error(g, 1.0, order, rsi) <= kPairBudget) return 1.0;
     double lo = 0.0, hi = 1.0;
     for (int i = 0; i < 60; ++i) {
         const double mid = 0.5 * (lo + hi);
-        if (fp32c_pair_error(g, mid, order) <= kPairBudget) lo = mid;
+        if (fp32c_pair_error(g, mid, order, rsi) <= kPairBudget) lo = mid;
         else hi = mid;
     }
     return lo;

@@ -192,7 +192,8 @@ template <int BX, int BY>
 struct BlockGeom {
     double FXc, FYc, ZC2;
     float wz;                    // szt zc' (per lane in the forward)
-    float zk;                    // (k / 2 pi) zc', the RS-I factor's depth (1 x 1 blocks)
+    float zk;                    // (k / 2 pi) zc', the RS-I factor's depth
+    float mz;                    // szt / (2 zc'): RS-I rows' depth ratio z'_r / zc' = 1 + m_r mz
     float sxt, syt;              // uniform
     float ex[(BX + 1) / 2];      // uniform: Ex(|m|) for |m| = 2k + (BX + 1) % 2
     float ey[(BY + 1) / 2];      // uniform: Ey(|m|)
@@ -213,6 +214,7 @@ __device__ __forceinline__ void block_offsets_setup(const DevPlane& P, double zc
     G.syt = uniform_f(static_cast<float>(syt));
     G.wz = static_cast<float>(szt * zc);
     G.zk = static_cast<float>(kt * zc);
+    G.mz = static_cast<float>(0.5 * szt / zc);
 #pragma unroll
     for (int k = 0; k < (BX + 1) / 2; ++k) {
         const double m = 2 * k + (BX + 1) % 2;
@@ -252,13 +254,19 @@ __device__ __forceinline__ void block_offsets(const BlockGeom<BX, BY>& G, float 
 //   kP1A2:  as kP1A1, and r = 6 c1 h^4
 //   kP2A2:  p = 2 c1, q = -c1
 //   1 x 1 blocks: c = c1 only (R = R0).
+//   RSI (RS-I planes, the expansion in rs_internal.h): th0 - 1/4 + h / 2pi^2, c1 times
+//   2 pi zk rf (1 + e^2/2), the phase's D-linear coefficient h (1 - e^2) (hp; w0 = 1 - e^2
+//   for kP2A2), e^2 = h^2 / pi^2, the amplitude series of (1 + 4y)^-1 (p = 4 c1 for kP2A2,
+//   q = -4 c1 h^2 and r = 16 c1 h^4 for the first-order phase) and each row's depth ratio mu.
 template <int BX, int BY>
 struct PairState {
     float th0, h, p, q, c, r;
+    float hp, w0;  // RS-I only
     float a[BX], b[BY];
+    float mu[BY];  // RS-I only
 };
 
-template <int ORD, int BX, int BY>
+template <int ORD, int BX, int BY, bool RSI = false>
 __device__ __forceinline__ void pair_state(const PillarRec& T, const BlockGeom<BX, BY>& G,
                                            PairState<BX, BY>& S) {
     const double dX0 = G.FXc - T.x, dY0 = G.FYc - T.y;
@@ -266,18 +274,27 @@ __device__ __forceinline__ void pair_state(const PillarRec& T, const BlockGeom<B
     const double R0 = block_distance(fma(dX0, dX0, fma(dY0, dY0, G.ZC2)), rf);
     S.th0 = static_cast<float>(__builtin_amdgcn_fract(R0 - T.zeta));
     S.h = 0.5f * rf;
-    const float c1 = T.amp * rf;
+    float c1 = T.amp * rf;
+    if constexpr (RSI && BX * BY > 1) {
+        const float e2 = S.h * S.h * 0.10132118364233778f;  // h^2 / pi^2
+        S.th0 += fmaf(S.h, 0.050660591821168886f, -0.25f);  // + h / (2 pi^2) - 1/4
+        c1 *= 6.28318530717958648f * G.zk * rf * fmaf(0.5f, e2, 1.0f);
+        S.hp = S.h * (1.0f - e2);
+        S.w0 = 1.0f - e2;
+#pragma unroll
+        for (int r = 0; r < BY; ++r) S.mu[r] = fmaf(static_cast<float>(2 * r - (BY - 1)), G.mz, 1.0f);
+    }
     if constexpr (BX * BY == 1) {
         S.c = c1;
     } else if constexpr (ORD == kP2A2) {
-        S.p = c1 + c1;
+        S.p = RSI ? 4.0f * c1 : c1 + c1;
         S.q = -c1;
     } else {
         const float h2 = S.h * S.h;
         S.p = h2 * S.h;
-        S.q = -2.0f * c1 * h2;
+        S.q = (RSI ? -4.0f : -2.0f) * c1 * h2;
         S.c = c1;
-        if constexpr (ORD == kP1A2) S.r = 6.0f * c1 * (h2 * h2);
+        if constexpr (ORD == kP1A2) S.r = (RSI ? 16.0f : 6.0f) * c1 * (h2 * h2);
     }
     if constexpr (BX * BY > 1) {
         block_offsets(G, static_cast<float>(dX0), static_cast<float>(dY0), S.a, S.b);
@@ -297,25 +314,32 @@ __device__ __forceinline__ void pair_state(const PillarRec& T, const BlockGeom<B
 // and E += A (cos 2 pi th, sin 2 pi th): 6, 7 and 9 FP32 ops per pair besides the two
 // transcendentals.
 // Phase and amplitude of pair e of a block (ORD as above).
-template <int ORD, int BX, int BY>
+template <int ORD, int BX, int BY, bool RSI = false>
 __device__ __forceinline__ void pair_phase(const PairState<BX, BY>& S, int e, float& th, float& A) {
     const int r = e / BX, c = e % BX;
     const float d = S.a[c] + S.b[r];
     if constexpr (ORD == kP2A2) {
         const float y = d * S.h;
-        const float w = fmaf(-y, fmaf(-2.0f, y, 1.0f), 1.0f);
-        A = fmaf(fmaf(3.0f * S.p, y, -S.p), y, -S.q);
-        th = fmaf(d, w, S.th0);
+        if constexpr (RSI) {
+            const float w = fmaf(-y, fmaf(-2.0f, y, 1.0f), S.w0);
+            A = fmaf(fmaf(4.0f * S.p, y, -S.p), y, -S.q);
+            th = fmaf(d, w, S.th0);
+        } else {
+            const float w = fmaf(-y, fmaf(-2.0f, y, 1.0f), 1.0f);
+            A = fmaf(fmaf(3.0f * S.p, y, -S.p), y, -S.q);
+            th = fmaf(d, w, S.th0);
+        }
     } else {
-        th = fmaf(d, fmaf(-d, S.p, S.h), S.th0);
+        th = fmaf(d, fmaf(-d, S.p, RSI ? S.hp : S.h), S.th0);
         if constexpr (ORD == kP1A2)
             A = fmaf(fmaf(S.r, d, S.q), d, S.c);
         else
             A = fmaf(S.q, d, S.c);
     }
+    if constexpr (RSI) A *= S.mu[r];
 }
 
-template <int ORD, int BX, int BY>
+template <int ORD, int BX, int BY, bool RSI = false>
 __device__ __forceinline__ void block_pairs(const PairState<BX, BY>& S, float (&ar)[BX * BY],
                                             float (&ai)[BX * BY]) {
     constexpr int N = BX * BY;
@@ -334,7 +358,7 @@ __device__ __forceinline__ void block_pairs(const PairState<BX, BY>& S, float (&
         for (int g0 = 0; g0 < N; g0 += G) {
             float th[G], A[G], cs[G], sn[G];
 #pragma unroll
-            for (int k = 0; k < G; ++k) pair_phase<ORD>(S, g0 + k, th[k], A[k]);
+            for (int k = 0; k < G; ++k) pair_phase<ORD, BX, BY, RSI>(S, g0 + k, th[k], A[k]);
 #pragma unroll
             for (int k = 0; k < G; ++k) {
                 cs[k] = __builtin_amdgcn_cosf(th[k]);
@@ -352,14 +376,14 @@ __device__ __forceinline__ void block_pairs(const PairState<BX, BY>& S, float (&
 }
 
 // Pillars tile[k0 .. k1) against one block, one pillar state at a time.
-template <int ORD, int BX, int BY>
+template <int ORD, int BX, int BY, bool RSI = false>
 __device__ __forceinline__ void pillar_run(const PillarRec* tile, int k0, int k1,
                                            const BlockGeom<BX, BY>& G, float (&ar)[BX * BY],
                                            float (&ai)[BX * BY]) {
     for (int k = k0; k < k1; ++k) {
         PairState<BX, BY> st;
-        pair_state<ORD>(tile[k], G, st);
-        block_pairs<ORD>(st, ar, ai);
+        pair_state<ORD, BX, BY, RSI>(tile[k], G, st);
+        block_pairs<ORD, BX, BY, RSI>(st, ar, ai);
     }
 }
 
@@ -385,7 +409,7 @@ __device__ __forceinline__ void pillar_run_rsi(const PillarRec* tile, int k0, in
 // One segment: tile T (class of block shape BX x BY within a plane blocked KBX x KBY)
 // against pillars [q0, q1) at series order `order`; partial sums to `slot`
 // (element e * 64 + lane).
-template <int BX, int BY, int KBX, int KBY>
+template <int BX, int BY, int KBX, int KBY, bool RSI = false>
 __device__ __forceinline__ void fwd_segment(
     const double2* __restrict__ pos, const float2* __restrict__ aps, const DevPlane& P,
     const FwdTile& T, int q0, int q1, int order, PillarRec* tile, double2* run,
@@ -435,7 +459,7 @@ __device__ __forceinline__ void fwd_segment(
             // RS-I planes only ever run in the exact-plane launch (KBX = KBY = 1); keeping
             // the branch out of the block kernels leaves their code as it was (a change
             // there moved the 4 x 8 kernel's register assignment and cost it 6% at 321^2).
-            if constexpr (KBX * KBY == 1) {
+            if constexpr (KBX * KBY == 1 || RSI) {
                 if (P.kernel == kKernelRsi)
                     pillar_run_rsi(tile, k0, k1, G, ar, ai);
                 else
@@ -445,11 +469,11 @@ __device__ __forceinline__ void fwd_segment(
             }
         } else {
             if (order == kP1A1)
-                pillar_run<kP1A1>(tile, k0, k1, G, ar, ai);
+                pillar_run<kP1A1, BX, BY, RSI>(tile, k0, k1, G, ar, ai);
             else if (order == kP1A2)
-                pillar_run<kP1A2>(tile, k0, k1, G, ar, ai);
+                pillar_run<kP1A2, BX, BY, RSI>(tile, k0, k1, G, ar, ai);
             else
-                pillar_run<kP2A2>(tile, k0, k1, G, ar, ai);
+                pillar_run<kP2A2, BX, BY, RSI>(tile, k0, k1, G, ar, ai);
         }
         since_fold += kFwdThreads / kFwdWaves;
         if (since_fold >= kWaveFold || base + kFwdThreads >= q1) {
@@ -496,8 +520,8 @@ constexpr int fwd_waves_per_simd(int bx, int by) { return bx * by <= 16 ? 4 : (b
 // KBX x KBY: the launch's block shape; its work-groups take the four tile classes of
 // the planes blocked that way (and the 1 x 1 tiles of exact planes).  <1, 1>: calls
 // whose planes are all exact.
-template <int KBX, int KBY>
-__global__ __launch_bounds__(kFwdThreads, fwd_waves_per_simd(KBX, KBY)) void fwd_fp32c(
+template <int KBX, int KBY, bool RSI>
+__device__ __forceinline__ void fwd_fp32c_body(
     const double2* __restrict__ pos, const float2* __restrict__ ap,
     const double2* __restrict__ ext, int npil, const DevPlane* __restrict__ planes,
     const FwdTile* __restrict__ tiles, const FwdSeg* __restrict__ segs,
@@ -533,20 +557,42 @@ __global__ __launch_bounds__(kFwdThreads, fwd_waves_per_simd(KBX, KBY)) void fwd
             }
             switch (T.shape) {
                 case 0:
-                    fwd_segment<KBX, KBY, KBX, KBY>(pos, aps, P, T, S.q0, S.q1, order, tile, run, slot);
+                    fwd_segment<KBX, KBY, KBX, KBY, RSI>(pos, aps, P, T, S.q0, S.q1, order, tile, run, slot);
                     break;
                 case 1:
-                    fwd_segment<1, KBY, KBX, KBY>(pos, aps, P, T, S.q0, S.q1, order, tile, run, slot);
+                    fwd_segment<1, KBY, KBX, KBY, RSI>(pos, aps, P, T, S.q0, S.q1, order, tile, run, slot);
                     break;
                 case 2:
-                    fwd_segment<KBX, 1, KBX, KBY>(pos, aps, P, T, S.q0, S.q1, order, tile, run, slot);
+                    fwd_segment<KBX, 1, KBX, KBY, RSI>(pos, aps, P, T, S.q0, S.q1, order, tile, run, slot);
                     break;
                 default:
-                    fwd_segment<1, 1, KBX, KBY>(pos, aps, P, T, S.q0, S.q1, kP1A1, tile, run, slot);
+                    fwd_segment<1, 1, KBX, KBY, RSI>(pos, aps, P, T, S.q0, S.q1, kP1A1, tile, run, slot);
                     break;
             }
         }
     }
+}
+
+template <int KBX, int KBY>
+__global__ __launch_bounds__(kFwdThreads, fwd_waves_per_simd(KBX, KBY)) void fwd_fp32c(
+    const double2* __restrict__ pos, const float2* __restrict__ ap,
+    const double2* __restrict__ ext, int npil, const DevPlane* __restrict__ planes,
+    const FwdTile* __restrict__ tiles, const FwdSeg* __restrict__ segs,
+    const int32_t* __restrict__ wg_seg, float2* __restrict__ partial, int64_t slot_elems) {
+    fwd_fp32c_body<KBX, KBY, false>(pos, ap, ext, npil, planes, tiles, segs, wg_seg, partial,
+                                    slot_elems);
+}
+
+// RS-I planes in KBX x KBY blocks (a launch of their own, so the reference kernel's
+// instances keep their code).
+template <int KBX, int KBY>
+__global__ __launch_bounds__(kFwdThreads, fwd_waves_per_simd(KBX, KBY)) void fwd_fp32c_rsi(
+    const double2* __restrict__ pos, const float2* __restrict__ ap,
+    const double2* __restrict__ ext, int npil, const DevPlane* __restrict__ planes,
+    const FwdTile* __restrict__ tiles, const FwdSeg* __restrict__ segs,
+    const int32_t* __restrict__ wg_seg, float2* __restrict__ partial, int64_t slot_elems) {
+    fwd_fp32c_body<KBX, KBY, true>(pos, ap, ext, npil, planes, tiles, segs, wg_seg, partial,
+                                   slot_elems);
 }
 
 // ---------------------------------------------------------------------------------
@@ -917,10 +963,14 @@ struct AdjState {
     float th0, h, rf;
     float zk;  // RS-I planes: (k/2pi) z' of the pixel row (1 x 1 blocks)
     bool rsi;
+    float w0;      // RS-I blocks: 1 - h^2/pi^2
+    float mu[BY];  // RS-I blocks: the rows' depth ratios z'_r / zc'
     float a[BX], b[BY];
 };
 
-template <int BX, int BY>
+// RSI: RS-I planes in blocks (the expansion in rs_internal.h): th0 - 1/4 + h / 2pi^2, the
+// block's scale rf times 2 pi zk rf (1 + e^2/2), e^2 = h^2 / pi^2.
+template <int BX, int BY, bool RSI = false>
 __device__ __forceinline__ void adj_state(double dX0, double dY0, double zref,
                                           const BlockGeom<BX, BY>& G, AdjState<BX, BY>& S) {
     float rf;
@@ -930,6 +980,14 @@ __device__ __forceinline__ void adj_state(double dX0, double dY0, double zref,
     S.h = 0.5f * rf;
     S.zk = 0.0f;
     S.rsi = false;
+    if constexpr (RSI && BX * BY > 1) {
+        const float e2 = S.h * S.h * 0.10132118364233778f;  // h^2 / pi^2
+        S.th0 += fmaf(S.h, 0.050660591821168886f, -0.25f);  // + h / (2 pi^2) - 1/4
+        S.rf = rf * (6.28318530717958648f * G.zk * rf * fmaf(0.5f, e2, 1.0f));
+        S.w0 = 1.0f - e2;
+#pragma unroll
+        for (int r = 0; r < BY; ++r) S.mu[r] = fmaf(static_cast<float>(2 * r - (BY - 1)), G.mz, 1.0f);
+    }
     if constexpr (BX * BY > 1) {
         block_offsets(G, static_cast<float>(dX0), static_cast<float>(dY0), S.a, S.b);
 #pragma unroll
@@ -944,8 +1002,9 @@ __device__ __forceinline__ void adj_state(double dX0, double dY0, double zref,
 // block's turned sum to (gr, gi).
 //   kP1A1:  w = 1 - x1 h,             A = 2U w - U,                phase chi - x1 w
 //   kP2A2:  w = 1 - y (1 - 2 y),      A = U (1 + y (6 y - 2)),     phase chi - x1 w
-// (y = x1 h); 1 x 1 blocks use chi - th0 and A = U directly.
-template <int ORD, int BX, int BY>
+// (y = x1 h); 1 x 1 blocks use chi - th0 and A = U directly.  RSI: w starts from w0, and
+// A = U (1 - 4y) (kP1A1) or U (1 + y (16 y - 4)) (kP2A2), times the row's mu.
+template <int ORD, int BX, int BY, bool RSI = false>
 __device__ __forceinline__ void adj_pairs(const AdjState<BX, BY>& S, const float* sU2,
                                           const float* sUm, const float* sChi, int k0,
                                           float& gr, float& gi) {
@@ -975,7 +1034,17 @@ __device__ __forceinline__ void adj_pairs(const AdjState<BX, BY>& S, const float
                 const float U2 = sU2[k], Um = sUm[k], chi = sChi[k];
                 const float x1 = S.a[c] + S.b[r];
                 float w, A;
-                if constexpr (ORD == kP2A2) {
+                if constexpr (RSI) {
+                    if constexpr (ORD == kP2A2) {
+                        const float y = x1 * S.h;
+                        w = fmaf(-y, fmaf(-2.0f, y, 1.0f), S.w0);
+                        A = fmaf(fmaf(8.0f * U2, y, -2.0f * U2), y, -Um);
+                    } else {
+                        w = fmaf(-x1, S.h, S.w0);
+                        A = fmaf(U2, fmaf(-x1, S.h + S.h, 1.0f), Um);
+                    }
+                    A *= S.mu[r];
+                } else if constexpr (ORD == kP2A2) {
                     const float y = x1 * S.h;
                     w = fmaf(-y, fmaf(-2.0f, y, 1.0f), 1.0f);
                     A = fmaf(fmaf(3.0f * U2, y, -U2), y, -Um);
@@ -1003,7 +1072,7 @@ __device__ __forceinline__ void adj_pairs(const AdjState<BX, BY>& S, const float
 
 // Columns [0, ncol) of one LDS tile (strip starting at row j0, tile starting at plane
 // column cb) against the thread's pillar (qx, qy in um).
-template <int ORD, int BX, int BY>
+template <int ORD, int BX, int BY, bool RSI = false>
 __device__ __forceinline__ void adj_tile(const float* sU2, const float* sUm, const float* sChi,
                                          int ncol, const DevPlane& P, int j0, int cb, double qx,
                                          double qy, float& ar, float& ai) {
@@ -1020,12 +1089,12 @@ __device__ __forceinline__ void adj_tile(const float* sU2, const float* sUm, con
         for (int g = 0; g < ncol; g += BX) {
             G.FXc = kt * (P.x0 + (cb + g + (BX - 1) * 0.5) * P.sx);
             AdjState<BX, BY> S;
-            adj_state(fma(-kt, qx, G.FXc), dY0, P.zref, G, S);
+            adj_state<BX, BY, RSI>(fma(-kt, qx, G.FXc), dY0, P.zref, G, S);
             if constexpr (BX * BY == 1) {
                 S.rsi = P.kernel == kKernelRsi;
                 S.zk = G.zk;
             }
-            adj_pairs<ORD>(S, sU2, sUm, sChi, g * kStrip + sr0, ar, ai);
+            adj_pairs<ORD, BX, BY, RSI>(S, sU2, sUm, sChi, g * kStrip + sr0, ar, ai);
         }
     }
 }
@@ -1138,11 +1207,11 @@ __device__ __forceinline__ void adj_tile_sep(const float* sRe, const float* sIm,
     }
 }
 
-// Every adjoint instance is held to 128 VGPRs, 4 waves per SIMD: VALU issue slows with
-// fewer waves (FP32 FMAs measured 3.1 cycles per wave64 instruction per SIMD at 4 waves,
-// 3.5 at 2 and 3, 5.3 at 1).
-template <int BX, int BY>
-__global__ __launch_bounds__(kThreads, 4) void adj_fp32c(
+// The adjoint kernels' body (RSI: RS-I planes in blocks).  Every adjoint kernel below is
+// held to 128 VGPRs, 4 waves per SIMD: VALU issue slows with fewer waves (FP32 FMAs
+// measured 3.1 cycles per wave64 instruction per SIMD at 4 waves, 3.5 at 2 and 3, 5.3 at 1).
+template <int BX, int BY, bool RSI>
+__device__ __forceinline__ void adj_fp32c_body(
     const double* __restrict__ px, const double* __restrict__ py, int npil,
     const DevPlane* __restrict__ planes, const AdjStrip* __restrict__ strips,
     const AdjSeg* __restrict__ segs, const int32_t* __restrict__ wg_seg,
@@ -1208,16 +1277,16 @@ __global__ __launch_bounds__(kThreads, 4) void adj_fp32c(
                 __syncthreads();
                 float ar = 0.0f, ai = 0.0f;
                 int ord = order;
-                if constexpr (BX * BY == 64) {
+                if constexpr (BX * BY == 64 && !RSI) {
                     if (order == kSep) {
                         adj_tile_sep<BX, BY>(sU2, sUm, sChi, ncol, P, st.j0, cb, qx, qy, ar, ai);
                         ord = -1;
                     }
                 }
                 if (ord == kP1A1)
-                    adj_tile<kP1A1, BX, BY>(sU2, sUm, sChi, ncol, P, st.j0, cb, qx, qy, ar, ai);
+                    adj_tile<kP1A1, BX, BY, RSI>(sU2, sUm, sChi, ncol, P, st.j0, cb, qx, qy, ar, ai);
                 else if (ord >= 0)
-                    adj_tile<kP2A2, BX, BY>(sU2, sUm, sChi, ncol, P, st.j0, cb, qx, qy, ar, ai);
+                    adj_tile<kP2A2, BX, BY, RSI>(sU2, sUm, sChi, ncol, P, st.j0, cb, qx, qy, ar, ai);
                 const float yr = ar - cr;
                 const float tr = sr + yr;
                 cr = (tr - sr) - yr;
@@ -1232,6 +1301,25 @@ __global__ __launch_bounds__(kThreads, 4) void adj_fp32c(
             make_float2(static_cast<float>(static_cast<double>(sr) - static_cast<double>(cr)),
                         static_cast<float>(static_cast<double>(si) - static_cast<double>(ci)));
     }
+}
+
+template <int BX, int BY>
+__global__ __launch_bounds__(kThreads, 4) void adj_fp32c(
+    const double* __restrict__ px, const double* __restrict__ py, int npil,
+    const DevPlane* __restrict__ planes, const AdjStrip* __restrict__ strips,
+    const AdjSeg* __restrict__ segs, const int32_t* __restrict__ wg_seg,
+    const float4* __restrict__ pix, float2* __restrict__ partial) {
+    adj_fp32c_body<BX, BY, false>(px, py, npil, planes, strips, segs, wg_seg, pix, partial);
+}
+
+// RS-I planes in BX x BY blocks (a launch of their own).
+template <int BX, int BY>
+__global__ __launch_bounds__(kThreads, 4) void adj_fp32c_rsi(
+    const double* __restrict__ px, const double* __restrict__ py, int npil,
+    const DevPlane* __restrict__ planes, const AdjStrip* __restrict__ strips,
+    const AdjSeg* __restrict__ segs, const int32_t* __restrict__ wg_seg,
+    const float4* __restrict__ pix, float2* __restrict__ partial) {
+    adj_fp32c_body<BX, BY, true>(px, py, npil, planes, strips, segs, wg_seg, pix, partial);
 }
 
 // ---------------------------------------------------------------------------------
@@ -1461,16 +1549,26 @@ __global__ __launch_bounds__(kThreads) void mtf_finish(const double2* __restrict
 namespace {
 inline unsigned grid_for(int64_t n) { return static_cast<unsigned>((n + kThreads - 1) / kThreads); }
 
-template <int KBX, int KBY>
+template <int KBX, int KBY, bool RSI = false>
 void launch_fwd_shape(const Fp32cFwdArgs& a, hipStream_t s) {
-    fwd_fp32c<KBX, KBY><<<a.nwg, kFwdThreads, 0, s>>>(a.pos, a.ap, a.ext, a.npil, a.planes, a.tiles,
-                                                   a.segs, a.wg_seg, a.partial, a.slot_elems);
+    if constexpr (RSI)
+        fwd_fp32c_rsi<KBX, KBY><<<a.nwg, kFwdThreads, 0, s>>>(a.pos, a.ap, a.ext, a.npil, a.planes,
+                                                           a.tiles, a.segs, a.wg_seg, a.partial,
+                                                           a.slot_elems);
+    else
+        fwd_fp32c<KBX, KBY><<<a.nwg, kFwdThreads, 0, s>>>(a.pos, a.ap, a.ext, a.npil, a.planes,
+                                                       a.tiles, a.segs, a.wg_seg, a.partial,
+                                                       a.slot_elems);
 }
 
-template <int BX, int BY>
+template <int BX, int BY, bool RSI = false>
 void launch_adj_shape(const Fp32cAdjArgs& a, hipStream_t s) {
-    adj_fp32c<BX, BY><<<a.nwg, kThreads, 0, s>>>(a.px, a.py, a.npil, a.planes, a.strips, a.segs,
-                                                 a.wg_seg, a.pix, a.partial);
+    if constexpr (RSI)
+        adj_fp32c_rsi<BX, BY><<<a.nwg, kThreads, 0, s>>>(a.px, a.py, a.npil, a.planes, a.strips,
+                                                      a.segs, a.wg_seg, a.pix, a.partial);
+    else
+        adj_fp32c<BX, BY><<<a.nwg, kThreads, 0, s>>>(a.px, a.py, a.npil, a.planes, a.strips,
+                                                  a.segs, a.wg_seg, a.pix, a.partial);
 }
 }  // namespace
 
@@ -1495,10 +1593,13 @@ hipError_t launch_stage_pillars(int mode, const double* px, const double* py, co
 }
 
 // FP32C forward pair kernel of block shape bx x by: 4 x 4, 4 x 8, 8 x 4, or 1 x 1
-// (exact planes only).
-hipError_t launch_forward_fp32c(int bx, int by, const Fp32cFwdArgs& a, hipStream_t s) {
+// (exact planes only); rsi: RS-I planes in 4 x 4 blocks.
+hipError_t launch_forward_fp32c(int bx, int by, bool rsi, const Fp32cFwdArgs& a, hipStream_t s) {
     if (a.nwg <= 0) return hipSuccess;
-    if (bx == 4 && by == 4) launch_fwd_shape<4, 4>(a, s);
+    if (rsi) {
+        if (bx != 4 || by != 4) return hipErrorInvalidValue;
+        launch_fwd_shape<4, 4, true>(a, s);
+    } else if (bx == 4 && by == 4) launch_fwd_shape<4, 4>(a, s);
     else if (bx == 4 && by == 8) launch_fwd_shape<4, 8>(a, s);
     else if (bx == 8 && by == 4) launch_fwd_shape<8, 4>(a, s);
     else if (bx == 1 && by == 1) launch_fwd_shape<1, 1>(a, s);
@@ -1516,9 +1617,12 @@ hipError_t launch_fwd_reduce_fp32c(const DevPlane* planes, const FwdTile* tiles,
     return hipGetLastError();
 }
 
-hipError_t launch_adjoint_fp32c(int bx, int by, const Fp32cAdjArgs& a, hipStream_t s) {
+hipError_t launch_adjoint_fp32c(int bx, int by, bool rsi, const Fp32cAdjArgs& a, hipStream_t s) {
     if (a.nwg <= 0) return hipSuccess;
-    if (bx == 4 && by == 4) launch_adj_shape<4, 4>(a, s);
+    if (rsi) {
+        if (bx != 4 || by != 4) return hipErrorInvalidValue;
+        launch_adj_shape<4, 4, true>(a, s);
+    } else if (bx == 4 && by == 4) launch_adj_shape<4, 4>(a, s);
     else if (bx == 4 && by == 8) launch_adj_shape<4, 8>(a, s);
     else if (bx == 8 && by == 4) launch_adj_shape<8, 4>(a, s);
     else if (bx == 8 && by == 8) launch_adj_shape<8, 8>(a, s);
@@ -1539,13 +1643,16 @@ hipError_t launch_adj_reduce_fp32c(const float2* partial, const int32_t* slot_of
 
 // Work-groups of the FP32C pair kernel of this shape that one CU holds at once (0 if
 // unknown): the stream-K planner sizes its launches from it.
-int fp32c_resident_per_cu(bool adjoint, int bx, int by) {
+int fp32c_resident_per_cu(bool adjoint, int bx, int by, bool rsi) {
     int n = 0;
     hipError_t e = hipErrorInvalidValue;
     auto occ = [&](const void* fn, int threads) {
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, threads, 0);
     };
-    if (!adjoint) {
+    if (rsi) {
+        if (adjoint) occ(reinterpret_cast<const void*>(adj_fp32c_rsi<4, 4>), kThreads);
+        else occ(reinterpret_cast<const void*>(fwd_fp32c_rsi<4, 4>), kFwdThreads);
+    } else if (!adjoint) {
         if (bx == 4 && by == 4) occ(reinterpret_cast<const void*>(fwd_fp32c<4, 4>), kFwdThreads);
         else if (bx == 4 && by == 8) occ(reinterpret_cast<const void*>(fwd_fp32c<4, 8>), kFwdThreads);
         else if (bx == 8 && by == 4) occ(reinterpret_cast<const void*>(fwd_fp32c<8, 4>), kFwdThreads);
@@ -1565,7 +1672,8 @@ int fp32c_resident_per_cu(bool adjoint, int bx, int by) {
 }
 
 // Profiler names of the FP32C pair kernels.
-const char* fp32c_kernel_name(bool adjoint, int bx, int by) {
+const char* fp32c_kernel_name(bool adjoint, int bx, int by, bool rsi) {
+    if (rsi) return adjoint ? "crs::adj_fp32c_rsi<4, 4>" : "crs::fwd_fp32c_rsi<4, 4>";
     static const char* fwd[] = {"crs::fwd_fp32c<4, 4>", "crs::fwd_fp32c<4, 8>",
                                 "crs::fwd_fp32c<8, 4>", "crs::fwd_fp32c<8, 8>",
                                 "crs::fwd_fp32c<1, 1>"};  // <8, 8>: adjoint only

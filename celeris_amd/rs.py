@@ -27,7 +27,8 @@ from ._lib import MODE_FP32, MODE_FP32C, MODE_FP64, MODES, crs_plane
 
 
 # Plane kernels (crs_kernel): the reference's exp(ikR)/R, and Rayleigh-Sommerfeld I,
-# exp(ikR)/R (z/R)(1/R - ik) (in FP32C evaluated pair by pair at FP64 distances).
+# exp(ikR)/R (z/R)(1/R - ik) (in FP32C in 4 x 4 blocks where its series bound admits
+# them, else pair by pair at FP64 distances).
 KERNEL_REF = 0
 KERNEL_RSI = 1
 

@@ -66,8 +66,9 @@ typedef enum crs_kernel {
     CRS_KERNEL_REF = 0,
     /* Rayleigh-Sommerfeld I: exp(i k R) / R * (z / R) * (1 / R - i k), the z-derivative
      * form -d/dz [exp(i k R) / R] (no reference counterpart; opt-in).  In CRS_MODE_FP32C
-     * such a plane is evaluated pair by pair, each pair at its own FP64 distance (no
-     * pixel blocks). */
+     * such a plane runs in 4 x 4 pixel blocks where its series bound admits them (kernels
+     * crs::fwd_fp32c_rsi<4, 4> / crs::adj_fp32c_rsi<4, 4>), else pair by pair, each pair
+     * at its own FP64 distance. */
     CRS_KERNEL_RSI = 1
 } crs_kernel;
 

@@ -260,3 +260,104 @@ def test_host_admission_keeps_highest_order_within_budget():
     print(f"{allowed} planes admitted; worst P2A2 pair error {worst:.2f} of the budget")
     assert allowed > 150
     assert worst <= 1.0
+
+
+# ---- RS-I planes ----------------------------------------------------------------------
+# The RS-I pair t exp(ikR)/R (z/R)(1/R - ik) is, relative to its block-centre magnitude
+# c1 2 pi zk rf and phase th0 - 1/4,
+#     Z = (1 + 4y)^-1 (1 + i e (1 + 4y)^-1/2) exp(2 pi i (R' - R0')),   e = h / pi,
+# and the kernels evaluate it as (1 + e^2/2) A(y) exp(2 pi i (e / 2 pi + P(D))) with
+#     P1: P = D h (1 - h^2/pi^2) - D^2 h^3,   P2: P = x1 (1 - h^2/pi^2 - y (1 - 2y)),
+#     A1: A = 1 - 4y,                         A2: A = 1 - 4y + 16y^2.
+# rs_internal.h (fp32c_pair_error, rsi) bounds |Z - series| by its truncation terms; the
+# rounding and sin/cos terms are separate.  Restated here and checked against the exact Z.
+
+
+def rsi_truncation_bound(g, sn, order):
+    """2 pi (phase) + amplitude truncation terms of fp32c_pair_error for an RS-I plane."""
+    z2, z3 = g["z"] ** 2, g["z"] ** 3
+    if order == 2:
+        ph = 5 * series_peak(g["a"], g["c"], sn, 4, 3) / (128 * z3)
+    else:
+        ph = series_peak(g["a"], g["c"], sn, 3, 2) / (16 * z2)
+    if order == 0:
+        am = series_peak(g["a"], g["c"], sn, 2, 2) / z2
+    else:
+        am = series_peak(g["a"], g["c"], sn, 3, 3) / z3
+    e = 1 / (2 * math.pi * g["z"])
+    ymax = (g["a"] * sn + g["c"]) / (4 * g["z"])
+    ph += (e ** 3 / 3 + 6 * e * ymax * ymax) * (1 + 4 * ymax) / (2 * math.pi)
+    am += e ** 4 / 8 + 2.5 * e * e * ymax
+    return 2 * math.pi * ph + am, ymax
+
+
+def rsi_remainders(P, px, py, ox, oy, bx, by):
+    """max over the block's pairs of |Z - series| for orders P1A1, P1A2, P2A2."""
+    kt = P["k"] / (2 * math.pi)
+    ic, jc = ox + 0.5 * (bx - 1), oy + 0.5 * (by - 1)
+    FXc, FYc = kt * (P["x0"] + ic * P["sx"]), kt * (P["y0"] + jc * P["sy"])
+    zc = kt * (P["z0"] + jc * P["sz"])
+    dX0, dY0 = FXc - kt * px, FYc - kt * py
+    R0 = np.sqrt(dX0 * dX0 + dY0 * dY0 + zc * zc)
+    h = 0.5 / R0
+    e = h / math.pi
+    out = np.zeros(3)
+    for r in range(by):
+        ey = kt * P["sy"] * (r - 0.5 * (by - 1))
+        zr = kt * (P["z0"] + (oy + r) * P["sz"])
+        for c in range(bx):
+            ex = kt * P["sx"] * (c - 0.5 * (bx - 1))
+            D = ex * (2 * dX0 + ex) + ey * (2 * dY0 + ey) + (zr * zr - zc * zc)
+            R = np.sqrt(R0 * R0 + D)
+            x1, y = D * h, D * h * h
+            dR = D / (R + R0)
+            exact = (1 + 1j * e * R0 / R) * (R0 / R) ** 2 * np.exp(2j * math.pi * dR)
+            w1 = D * h * (1 - h * h / math.pi ** 2) - D * D * h ** 3
+            w2 = x1 * (1 - h * h / math.pi ** 2 - y * (1 - 2 * y))
+            a1, a2 = 1 - 4 * y, 1 + y * (16 * y - 4)
+            f = (1 + e * e / 2) * np.exp(1j * e)
+            for i, (ph, am) in enumerate(((w1, a1), (w1, a2), (w2, a2))):
+                ser = f * am * np.exp(2j * math.pi * ph)
+                out[i] = max(out[i], float(np.abs(exact - ser).max()))
+    return out
+
+
+@pytest.mark.parametrize("bx,by", [(4, 4), (8, 8)])
+def test_rsi_series_bound_covers_every_pair(bx, by):
+    rng = np.random.default_rng(20261204 + 10 * bx + by)
+    worst = np.zeros(3)
+    for _ in range(120):
+        lam, n, P = random_plane(rng, bx, by)
+        half = rng.uniform(2.0, 80.0)
+        P["x0"] = rng.uniform(-1.5 * half, 1.5 * half) - 0.5 * (n - 1) * P["sx"]
+        P["y0"] = rng.uniform(-1.5 * half, 1.5 * half) - 0.5 * (n - 1) * P["sy"]
+        g = series_geom(P, bx, by)
+        sn = obliquity(P, half, half)
+        px = np.concatenate([rng.uniform(-half, half, 200), [half, -half, half, -half]])
+        py = np.concatenate([rng.uniform(-half, half, 200), [half, half, -half, -half]])
+        bounds = [rsi_truncation_bound(g, sn, o) for o in range(3)]
+        for ox, oy in [(0, 0), (n - bx, n - by), (int(rng.integers(0, n - bx + 1)), 0)]:
+            rem = rsi_remainders(P, px, py, ox, oy, bx, by)
+            for o in range(3):
+                b, ymax = bounds[o]
+                assert rem[o] <= b * (1 + 4 * ymax) + 1e-13, (bx, by, o, P, rem[o], b)
+                worst[o] = max(worst[o], rem[o] / b)
+    print(f"RS-I {bx}x{by}: worst |Z - series| / bound per order {np.round(worst, 3)}")
+    assert worst[0] > 0.3  # not vacuous
+
+
+def test_rsi_bound_in_the_library_matches():
+    """fp32c_pair_error(rsi) = the truncation terms above + rounding + sin/cos, and RS-I
+    planes admit 4 x 4 blocks at the headline geometry."""
+    lam, f = 0.633, 100.0
+    k = 2 * math.pi / lam
+    W = 5.404873920333813
+    step = 2 * W / 160
+    P = dict(k=k, x0=-W, sx=step, y0=-W, sy=step, z0=f, sz=0.0, nx=161, ny=161)
+    g = series_geom(P, 4, 4)
+    sn = obliquity(P, 151 * 0.35, 151 * 0.35)
+    b, _ = rsi_truncation_bound(g, sn, 0)
+    full = b + 2 * math.pi * 2.0 ** -24 * (3 + 6 * g["e"]) + 2.0 ** -20 + 2.5e-7
+    assert full <= PAIR_BUDGET
+    b2, _ = rsi_truncation_bound(g, 1.0, 2)
+    assert b2 + 2 * math.pi * 2.0 ** -24 * (3 + 6 * g["e"]) + 2.0 ** -20 + 2.5e-7 <= PAIR_BUDGET
