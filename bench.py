@@ -15,10 +15,16 @@ durations measured with HIP events on their stream.  That peak is one wave64 FMA
 2 cycles per SIMD; without packed FP32 (switched off in this build) an FMA whose two
 multiplicands sit in different VGPR banks issues at 2.2 cycles per SIMD
 (celeris_amd/_lib/issue_probe), so the peak is within reach of unpacked code.
+
+`--kernel rsi` runs the same step with the Rayleigh-Sommerfeld I kernel
+exp(ikR)/R (z/R)(1/R - ik), which the reference does not have: 28 FLOP per pair each
+way (the obliquity z/R, the complex factor (1/R - ik) and a complex instead of a real
+scale), 56 per pair; no CPU baseline is reported for it.
 """
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import math
 import os
@@ -33,6 +39,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FLOP_PER_PAIR = 42.0
+FLOP_PER_PAIR_RSI = 56.0
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
@@ -44,6 +51,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--grid", type=int, default=161)
     ap.add_argument("--mode", default="fp32c", choices=["fp32c", "fp64", "fp32"])
+    ap.add_argument("--kernel", default="ref", choices=["ref", "rsi"],
+                    help="ref: the reference's exp(ikR)/R (default); rsi: Rayleigh-Sommerfeld I "
+                         "exp(ikR)/R (z/R)(1/R - ik), no CPU baseline")
     ap.add_argument("--no-compare", action="store_true", help="skip FP64/FP32 GPU comparison legs")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
@@ -248,7 +258,7 @@ def main():
         sys.exit(2)
     import torch
 
-    from celeris_amd import Context, configs
+    from celeris_amd import KERNEL_RSI, Context, configs
     from celeris_amd.autograd import encircled_energy
 
     if not torch.cuda.is_available():
@@ -257,6 +267,10 @@ def main():
     dev = torch.device("cuda:0")
     lens = configs.config2(n=args.grid) if args.lens == "headline" else configs.config1()
     plane = lens.planes[0]
+    flop_per_pair = FLOP_PER_PAIR
+    if args.kernel == "rsi":
+        plane = dataclasses.replace(plane, kernel=KERNEL_RSI)
+        flop_per_pair = FLOP_PER_PAIR_RSI
     n_grid = plane.nx
     npix = plane.npix
     pairs = lens.npil * npix
@@ -317,7 +331,7 @@ def main():
     launches = {"fwd": ctx.last_launch(False), "adj": ctx.last_launch(True)}
     traffic, traffic_src = profiled_traffic(launches)
     value = pairs / (ms * 1e-3) / 1e9
-    achieved = pairs * FLOP_PER_PAIR / ((k_fwd + k_adj) * 1e-3) / 1e12
+    achieved = pairs * flop_per_pair / ((k_fwd + k_adj) * 1e-3) / 1e12
     print(f"[bench] {args.mode}: {ms:.3f} ms/step, kernels fwd {k_fwd:.3f} ms adj {k_adj:.3f} ms, "
           f"{value:.1f} G pairs/s", file=sys.stderr, flush=True)
 
@@ -344,7 +358,7 @@ def main():
               file=sys.stderr, flush=True)
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and args.kernel == "ref":
         port = cpu_baseline(lens, u.cpu().numpy(), args.cpu_seconds)
         cpu = port
         ref = reference_cpu_forward(lens, 0.5 * args.cpu_seconds)
@@ -398,11 +412,13 @@ def main():
             "workload": (f"RS far field fwd+adjoint, {lens.npil} pillars "
                          + ("(303x303 @0.35um, lambda 0.633, f 100)" if args.lens == "headline"
                             else "(64x64 @0.4um, lambda 0.532, f 50)")
-                         + f" x {plane.nx}x{plane.ny} focal grid, mode {args.mode}"),
+                         + f" x {plane.nx}x{plane.ny} focal grid, mode {args.mode}"
+                         + (", RS-I kernel" if args.kernel == "rsi" else "")),
             "pillars": lens.npil,
             "grid": plane.nx,
             "pairs_per_step": pairs,
             "mode": args.mode,
+            "kernel": args.kernel,
             "parallelism": "single GPU",
             "tuning": tuning and {"fwd_target_blocks": tuning["fwd"],
                                   "adj_target_blocks": tuning.get("adj"),
@@ -420,7 +436,7 @@ def main():
             "kernels": {launches["fwd"][0]: k_fwd, launches["adj"][0]: k_adj},
             "workgroups": {launches["fwd"][0]: launches["fwd"][1],
                            launches["adj"][0]: launches["adj"][1]},
-            "flop_per_pair": FLOP_PER_PAIR,
+            "flop_per_pair": flop_per_pair,
         },
         "cpu_baseline": cpu,
         **extra,
